@@ -1,0 +1,276 @@
+/*
+ * mlxpbd.h -- C ABI of the MI355X-native multi-layer XPBD substep.
+ *
+ * Drop-in boundary for the hot path of AlexandreMercierAubin/MultiLayerSolver2024:
+ * the entry points below replace, one for one, the Matlab functions and MEX
+ * files cited next to them, so that a MEX gateway (or ctypes) can route the
+ * reference's simulate/simulate3D scenes through the GPU without touching the
+ * scene scripts, SimulationSettings or the layer logic around them.
+ *
+ * Conventions (those of the Matlab call sites):
+ *   - arrays are column-major; per-element matrices are pages (M(i,j,e) at
+ *     i + n*j + n*n*e);
+ *   - index arrays are int32 and 1-based (vertex, DOF, color, body ids);
+ *   - DOFs are xyz (xy in 2D) interleaved: vertex v owns DOFs dim*v-dim+1..dim*v;
+ *   - logical arrays are uint8 (0/1);
+ *   - all pointers are host pointers unless the name says "dev".
+ * Every function returns MLX_OK (0) or a negative status; mlx_last_error()
+ * returns a message for the last failure on the calling thread.  No function
+ * calls back into Matlab.  Calls on one mlx_mesh are not reentrant; each
+ * mesh owns one HIP stream and every call returns after that stream is idle.
+ */
+#ifndef MLXPBD_H
+#define MLXPBD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLX_OK 0
+#define MLX_ERR_ARG -1          /* bad argument (null, size, range) */
+#define MLX_ERR_HIP -2          /* HIP runtime failure */
+#define MLX_ERR_NOMEM -3        /* allocation failure */
+#define MLX_ERR_UNSUPPORTED -4  /* input outside what the GPU path implements */
+#define MLX_ERR_NODEVICE -5     /* no gfx950 device visible */
+#define MLX_ERR_STATE -6        /* call sequence error */
+
+#define MLX_FP64 0              /* all arithmetic in double (parity build) */
+#define MLX_FP32 1              /* bulk projection in float, rigid layer and
+                                   residual reductions in double */
+
+#define MLX_MAX_LAYERS 16
+
+const char *mlx_last_error(void);
+int mlx_version(void);
+
+/* Selects the HIP device used by meshes created afterwards (default 0) and
+ * checks that it is a gfx950 part. */
+int mlx_init(int32_t device);
+
+/* ------------------------------------------------------------------ */
+/* Scene precompute (host)                                              */
+/* ------------------------------------------------------------------ */
+
+/* Replaces naiveGraphColoring(T) (2d/tools/functionalities/naiveGraphColoring.m:1-19).
+ * t: nT x vpe int32 1-based, column-major (mesh.t).  elementColor: nT, 1-based.
+ * numColors: max color.  Identical output to the reference on identical input. */
+int mlx_naive_graph_coloring(const int32_t *t, int64_t nT, int32_t vpe,
+                             int64_t nV, int32_t *elementColor,
+                             int32_t *numColors);
+
+/* Replaces the precompute of the Mesh3D constructor on the XPBD path:
+ * computeB3D (3d/tools/functionalities/computeB3D.m:18-29) -> DmInv,
+ * makeElements3D (3d/@Mesh3D/makeElements3D.m:11) -> elV (signed volume
+ * det(Dm)/6), computeMass (3d/@Mesh3D/computeMass.m:50-65) -> mass,
+ * updateMaterials (3d/@Mesh3D/Mesh3D.m:445-460) -> alpha = inv(K),
+ * beta = alpha1*K.  p: 3nV positions; per-element rho, mu, lambda, alpha1.
+ * Outputs: DmInv 3x3xnT, elV nT, mass 3nV, alpha/beta 6x6xnT. */
+int mlx_mesh3d_precompute(const double *p, int64_t nV, const int32_t *t,
+                          int64_t nT, const double *rho, const double *mu,
+                          const double *lambda, const double *alpha1,
+                          double *DmInv, double *elV, double *mass,
+                          double *alpha, double *beta);
+
+/* Replaces the 2D Mesh constructor precompute: computeB2D (2d/@Mesh/Mesh.m:439-469),
+ * makeElements area (Mesh.m:481-492), computeMass (2d/@Mesh/computeMass.m:1-18),
+ * updateMaterials (Mesh.m:247-258).  Outputs DmInv 2x2xnT, elA nT, mass 2nV,
+ * alpha/beta 3x3xnT. */
+int mlx_mesh2d_precompute(const double *p, int64_t nV, const int32_t *t,
+                          int64_t nT, const double *rho, const double *mu,
+                          const double *lambda, const double *alpha1,
+                          double *DmInv, double *elA, double *mass,
+                          double *alpha, double *beta);
+
+/* ------------------------------------------------------------------ */
+/* Layer building natives                                               */
+/* ------------------------------------------------------------------ */
+
+/* Replaces mexEdiffNorm3D(Fa, Fb, h) (3d/mex/mexEdiffNorm3D.cpp:25-61).
+ * Fa, Fb: 9nT (mesh3D.B*p layout).  out: nT.  Runs on the GPU; bit-identical
+ * to the same expression evaluated without fused multiply-adds. */
+int mlx_ediff_norm3d(const double *Fa, const double *Fb, int64_t nT, double h,
+                     double *out);
+/* Replaces mexEdiffNorm2D(Fa, Fb, h) (2d/mex/mexEdiffNorm2D.cpp:25-52). */
+int mlx_ediff_norm2d(const double *Fa, const double *Fb, int64_t nT, double h,
+                     double *out);
+
+/* Replaces mexUnionFindByVertice(strainInds, sortedLayers, forcedElastic, t, N)
+ * (2d/mex/mexUnionFindByVertice.cpp:117-178).  strainInds: nT 1-based element
+ * ids in strain-rate order; sortedLayers: L ascending percentages;
+ * forcedElastic: nT logical; t: nT x vpe 1-based column-major.  Outputs:
+ * rigidBodySetsPerLayer nT x L column-major (0 = elastic, else 1-based body id
+ * in ascending element order), numRigids L.  Bit-identical maps. */
+int mlx_union_find_by_vertice(const int32_t *strainInds, int64_t nT,
+                              const double *sortedLayers, int32_t L,
+                              const uint8_t *forcedElastic, const int32_t *t,
+                              int32_t vpe, int64_t nV,
+                              int32_t *rigidBodySetsPerLayer,
+                              int32_t *numRigids);
+
+/* Replaces xpbdLayer2D.findBoundaryVertices (2d/integrator/xpbdLayer2D.m:551-569),
+ * also used by 3D (3d/integrator/xpbdLayer3D.m:652).  Outputs nV x L column-major. */
+int mlx_find_boundary_vertices(const int32_t *t, int64_t nT, int32_t vpe,
+                               int64_t nV, int32_t L,
+                               const int32_t *rigidBodySetsPerLayer,
+                               uint8_t *isBoundaryVertex,
+                               uint8_t *isRigidVertex,
+                               int32_t *vertexRigidBody);
+
+/* ------------------------------------------------------------------ */
+/* Device-resident scene                                                */
+/* ------------------------------------------------------------------ */
+
+typedef struct mlx_mesh mlx_mesh;
+
+/* The static arrays layerSolve receives (3d/integrator/xpbdLayer3D.m:44-93,
+ * 2d/integrator/xpbdLayer2D.m:606-641).  alpha and beta must have the
+ * isotropic StVK-Voigt form that Mesh3D.updateMaterials / Mesh.updateMaterials
+ * build (alpha = inv(K), beta = alpha1*K); anything else is rejected with
+ * MLX_ERR_UNSUPPORTED.  Masses must be equal on the dim DOFs of a vertex. */
+typedef struct {
+  int32_t dim;                 /* 2 or 3 */
+  int64_t nV, nT;
+  const int32_t *t;            /* nT x (dim+1), 1-based */
+  const double *DmInv;         /* dim x dim x nT */
+  const double *alpha;         /* nc x nc x nT (nc = 3 in 2D, 6 in 3D) */
+  const double *beta;          /* nc x nc x nT */
+  const double *volume;        /* nT: mesh3D.elV / mesh2d.elA */
+  const double *mass;          /* dim*nV */
+  const int32_t *elementColor; /* nT, 1-based */
+  const int32_t *pinnedDOFs;   /* 1-based, ascending */
+  int64_t nPinnedDOFs;
+  const uint8_t *forcedElastic;/* nT: pinned elements (pinnedTets) */
+} mlx_mesh_desc;
+
+int mlx_mesh_create(const mlx_mesh_desc *desc, int32_t precision,
+                    mlx_mesh **out);
+int mlx_mesh_destroy(mlx_mesh *mesh);
+/* Number of graph colors and color bucket sizes (for tuning and tests). */
+int mlx_mesh_info(const mlx_mesh *mesh, int32_t *numColors,
+                  int64_t *bucketSizes /* numColors, may be NULL */);
+
+/* Replaces xpbdLayer3D.stvkVoigtConstraints(numColors, elementDOFs, elementColor,
+ * DmInv, alpha/h^2, mass, volume, lambdai, xi, elasticElements, beta*h^2, oldp, h)
+ * (3d/integrator/xpbdLayer3D.m:347-379) and its 2D twin
+ * (2d/integrator/xpbdLayer2D.m:61-89): one Gauss-Seidel sweep over all colors.
+ * isElasticElement: nT logical.  lambdai: nT x nc column-major, in/out.
+ * xi: dim*nV in/out.  oldp: dim*nV. */
+int mlx_stvk_voigt_constraints(mlx_mesh *mesh, const uint8_t *isElasticElement,
+                               double h, double *lambdai, double *xi,
+                               const double *oldp);
+
+/* Arguments of layerSolve (3d/integrator/xpbdLayer3D.m:44-93; 2D
+ * 2d/integrator/xpbdLayer2D.m:606-641), minus those already in the mesh. */
+typedef struct {
+  int32_t nLayers;                /* numel(layers) */
+  const int32_t *layersMap;       /* 1-based map column per layer (setLayers) */
+  const int32_t *iterations;      /* per layer */
+  int32_t nSorted;                /* columns of the map arrays below */
+  const uint8_t *isElasticElement;/* nT x nSorted */
+  const int32_t *vertexRigidBody; /* nV x nSorted (0 = not rigid) */
+  const uint8_t *isBoundaryVertex;/* nV x nSorted */
+  const uint8_t *isRigidVertex;   /* nV x nSorted */
+  const int32_t *numRigids;       /* nSorted */
+  double h;
+  double boundaryCompliance;
+  double contactResistance;
+  int32_t computeResiduals;
+  int32_t hangingStop;
+  int32_t giveUpEnabled;
+  double giveUpThreshold;
+  int32_t lastLayerGiveUpEnabled;
+  int32_t useRigidContacts;
+  double stopAtImprovementPercent;
+  double runUntilResSmallerThan;  /* value on entry */
+  double residualArrayLast;       /* residualArray(end) on entry (0) */
+  double *xi;                     /* dim*nV in/out */
+  double *lambdai;                /* nT x nc in/out */
+  const double *oldp;             /* dim*nV (cache.oldp / mesh2d.prevp) */
+  const double *pinnedx;          /* nPinnedDOFs (3D; 2D pins to oldp) */
+  double *residualv;              /* 3nV in/out (3D only) */
+  int32_t nContacts;
+  const int32_t *contactVertexID; /* K, 1-based */
+  const double *contactNormals;   /* K x dim */
+  const double *pointColliders;   /* K x dim */
+  const double *contactCompliance;/* K */
+} mlx_layer_solve_args;
+
+typedef struct {
+  double *residualArray;          /* optional, capacity residualCap: appended
+                                     residuals (entry 0 = residualArrayLast) */
+  int32_t residualCap;
+  int32_t residualCount;          /* entries produced, incl. entry 0 */
+  double runUntilResSmallerThan;  /* value on exit */
+  int32_t itersPerLayer[MLX_MAX_LAYERS];
+  int32_t layersRun;
+  int64_t elasticProjections;     /* sum over iterations of elastic elements */
+  double *contactLambdas;         /* optional K: contact lambdas of the last
+                                     layer run (the reference discards them) */
+} mlx_layer_solve_out;
+
+/* Replaces xpbdLayer3D.layerSolve / xpbdLayer2D.layerSolve.  Returns final xi
+ * and lambdai (the reference discards lambdai; it is returned here). */
+int mlx_layer_solve(mlx_mesh *mesh, const mlx_layer_solve_args *args,
+                    mlx_layer_solve_out *out);
+
+/* ------------------------------------------------------------------ */
+/* Whole substep with device-resident state                             */
+/* ------------------------------------------------------------------ */
+
+typedef struct mlx_state mlx_state;
+
+/* Integrator properties (xpbdLayer3D.m:5-40, xpbdLayer2D.m:7-44) used by
+ * integrate(). */
+typedef struct {
+  int32_t nLayers;
+  double layers[MLX_MAX_LAYERS];  /* as passed to setLayers */
+  int32_t iterations[MLX_MAX_LAYERS];
+  double gravity;                 /* Integrator.Gravity */
+  double airDrag;                 /* 3D only */
+  double boundaryCompliance;
+  double contactResistance;
+  double restitution;             /* contactVelocityUpdate: 1e-2 (3D), 1e-5 (2D) */
+  int32_t computeResiduals;
+  int32_t hangingStop;
+  int32_t giveUpEnabled;
+  double giveUpThreshold;
+  int32_t lastLayerGiveUpEnabled;
+  int32_t useRigidContacts;
+  double stopAtImprovementPercent;
+  double runUntilResSmallerThan;  /* in/out, persists like the property */
+} mlx_settings;
+
+/* Plane contact finder (3d/contact/PlaneContactFinder3D.m:34-75,
+ * 2d/contact/PlaneContactFinder.m:22-59), evaluated on the device each substep
+ * at the start-of-step positions (3d/simulate3D.m:163). normal need not be
+ * unit length; D = dot(normal, position) as in the reference constructor. */
+typedef struct {
+  int32_t enabled;
+  double normal[3];
+  double position[3];
+  double compliance;              /* ContactFinder.xpbdContactCompliance (1e-4) */
+} mlx_plane;
+
+/* p, v: dim*nV initial state (mesh.p, mesh.v). */
+int mlx_state_create(mlx_mesh *mesh, const double *p, const double *v,
+                     mlx_state **out);
+int mlx_state_destroy(mlx_state *state);
+/* Copies state to the host; any pointer may be NULL.  lambdai: nT x nc of the
+ * last substep; contacts: number of active contacts of the last substep. */
+int mlx_state_download(mlx_state *state, double *p, double *v,
+                       double *lambdai, int32_t *nContacts);
+
+/* One xpbdLayer3D.integrate (3d/integrator/xpbdLayer3D.m:621-745) or
+ * xpbdLayer2D.integrate (2d/integrator/xpbdLayer2D.m:793-917) substep, with
+ * the simulate3D/simulate bookkeeping around it (contacts at start of step,
+ * position history shift).  frame is 1-based; frame 1 freezes pinnedx and
+ * cache.oldp like the reference. */
+int mlx_integrate(mlx_state *state, mlx_settings *settings, double h,
+                  int32_t frame, const mlx_plane *plane,
+                  mlx_layer_solve_out *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

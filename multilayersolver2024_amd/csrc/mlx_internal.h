@@ -1,0 +1,141 @@
+// Internal declarations of libmlxpbd: device data layout, kernel launchers
+// and the host-side scene/state objects behind the C ABI in mlxpbd.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mlxpbd.h"
+
+namespace mlx {
+
+void set_error(const char *fmt, ...);
+
+#define MLX_CHECK_HIP(call)                                                   \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::mlx::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                   \
+      return MLX_ERR_HIP;                                                     \
+    }                                                                         \
+  } while (0)
+
+#ifndef MLX_TRY
+#define MLX_TRY(expr)            \
+  do {                           \
+    int r_ = (expr);             \
+    if (r_ != 0) return r_;      \
+  } while (0)
+#endif
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+template <typename T> using vec4_t = typename Vec4<T>::type;
+
+// Control block shared by the kernels of one layerSolve.  Iteration kernels
+// return immediately once endLayer or stopSolve is set, so the host can
+// enqueue iterations ahead of the monitor's decision.
+struct Ctl {
+  int stopSolve;
+  int endLayer;
+  int iters;          // iterations run in the current layer
+  int errorChangeDecreased;
+  int residualCount;  // entries in the residual array (entry 0 included)
+  int stopLayer;      // layer index where stopSolve was raised, else -1
+  int isLastLayer;
+  int layer;
+  double runUntil;
+  double setPercentStop;
+  double normresv;
+  double lastResidual;
+};
+
+// Iteration-invariant parameters of the monitor (simpleMonitorError).
+struct MonitorParams {
+  int is3d;
+  int computeResiduals;
+  int hangingStop;
+  int giveUpEnabled;
+  int lastLayerGiveUpEnabled;
+  double giveUpThreshold;
+  double h;
+};
+
+// Device buffer with grow-only capacity.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t n);
+  void release();
+  template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+}  // namespace mlx
+
+// Scene: static data, laid out color-major ("slots") so that each color's
+// elements are contiguous and every per-element field is read coalesced.
+struct mlx_mesh {
+  int device = 0;
+  int dim = 3, vpe = 4, nc = 6, precision = MLX_FP64;
+  int64_t nV = 0, nT = 0, ld = 0;  // ld: padded slot count (multiple of 64)
+  int numColors = 0;
+  hipStream_t stream = nullptr;
+
+  // host copies (element order, 0-based)
+  std::vector<int32_t> T;           // nT*vpe
+  std::vector<double> DmInv;        // nT*dim*dim pages
+  std::vector<double> massV;        // per vertex
+  std::vector<int32_t> pinnedDOF;   // 0-based
+  std::vector<uint8_t> forcedElastic;
+  std::vector<int32_t> color;       // 1-based
+  std::vector<int64_t> colorOff;    // numColors+1 padded slot offsets
+  std::vector<int64_t> colorCnt;    // elements per color
+  std::vector<int32_t> perm;        // slot -> element (-1 for padding)
+  std::vector<int32_t> slotOf;      // element -> slot
+  // per-element material in element order: alpha_d, alpha_o, alpha_s, alpha1
+  std::vector<double> mat;
+
+  // device: slot-major SoA arrays (precision T)
+  mlx::DevBuf d_vid;      // int32  vpe*ld
+  mlx::DevBuf d_dminv;    // T      dim*dim*ld
+  mlx::DevBuf d_vol;      // T      ld
+  mlx::DevBuf d_mat;      // T      4*ld
+  mlx::DevBuf d_lam;      // T      nc*ld
+  mlx::DevBuf d_dminv64;  // double nT*dim*dim, element order (layer building)
+  mlx::DevBuf d_T;        // int32  nT*vpe element order
+  mlx::DevBuf d_minv64;   // double nV (1/mass per vertex)
+  mlx::DevBuf d_mass64;   // double nV
+  mlx::DevBuf d_pinned;   // int32  nPinnedDOF
+  mlx::DevBuf d_slotOf;   // int32  nT
+
+  // work buffers (sized on demand)
+  mlx::DevBuf d_x, d_xold;        // vec4<T> nV: position (w = 1/mass), oldp
+  mlx::DevBuf d_rv;               // T 3nV residual velocity
+  mlx::DevBuf d_pinnedx;          // T nPinnedDOF
+  mlx::DevBuf d_flags;            // uint8 nSorted*ld (slot order)
+  mlx::DevBuf d_vbody;            // int32 nSorted*nV
+  mlx::DevBuf d_ri;               // double 3nV
+  mlx::DevBuf d_bodies;           // double per body state
+  mlx::DevBuf d_csr;              // int32 per-layer CSR lists
+  mlx::DevBuf d_lamB;             // double boundary lambdas
+  mlx::DevBuf d_contacts;         // contact arrays
+  mlx::DevBuf d_partials;         // double reduction partials
+  mlx::DevBuf d_ctl;              // Ctl
+  mlx::DevBuf d_resid;            // double residual array
+  mlx::DevBuf d_tmp;              // misc
+  mlx::Ctl *h_ctl = nullptr;      // pinned host mirror
+};
+
+struct mlx_state {
+  mlx_mesh *mesh = nullptr;
+  int frame_seen = 0;
+  mlx::DevBuf d_v;        // T dim*nV velocity
+  mlx::DevBuf d_ediff;    // double nT
+  std::vector<double> h_ediff;
+  int32_t lastContacts = 0;
+};
+

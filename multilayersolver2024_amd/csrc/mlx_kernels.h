@@ -1,0 +1,1205 @@
+// Device kernels of libmlxpbd (gfx950).  Included once, by mlx_solver.hip.
+//
+// Layout: per-element data lives in color-major "slots" as structure-of-
+// arrays (field f of slot s at f*ld + s), so the elements of one graph color
+// are contiguous and every field load of a wavefront is one coalesced
+// transaction.  Vertex positions are vec4 (x, y, z, 1/mass) so one 16- or
+// 32-byte load per vertex brings the position and the inverse mass used by
+// the projection.  Rigid-body, boundary and contact arithmetic is always
+// double; the element projection runs in T (float or double).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mlx_internal.h"
+
+namespace mlx {
+
+constexpr int kBlock = 256;
+
+// Per-body state (double), one record per rigid body of the current layer.
+struct Body {
+  double M, invM;
+  double COM[3], COMDOT[3], omega[3];
+  double R[9], I0[9], I[9], Rstep[9];  // column-major
+};
+
+__device__ __forceinline__ bool iteration_stopped(const Ctl *ctl) {
+  return (ctl->stopSolve | ctl->endLayer) != 0;
+}
+
+// ---------------------------------------------------------------------------
+// small dense helpers (double)
+// ---------------------------------------------------------------------------
+
+// A x = b, 3x3 column-major, Gaussian elimination with partial pivoting
+// (the method of Matlab's mldivide on a general square page).
+__device__ __forceinline__ void solve3_pivot(const double *Ain, const double *bin,
+                                             double *x) {
+  double A[9], b[3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) A[k] = Ain[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) b[k] = bin[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int piv = k;
+    double best = fabs(A[k + 3 * k]);
+    for (int i = k + 1; i < 3; ++i) {
+      double v = fabs(A[i + 3 * k]);
+      if (v > best) { best = v; piv = i; }
+    }
+    if (piv != k) {
+      for (int j = 0; j < 3; ++j) {
+        double t = A[k + 3 * j]; A[k + 3 * j] = A[piv + 3 * j]; A[piv + 3 * j] = t;
+      }
+      double t = b[k]; b[k] = b[piv]; b[piv] = t;
+    }
+    double d = A[k + 3 * k];
+    for (int i = k + 1; i < 3; ++i) {
+      double l = A[i + 3 * k] / d;
+      for (int j = k + 1; j < 3; ++j) A[i + 3 * j] = A[i + 3 * j] - l * A[k + 3 * j];
+      b[i] = b[i] - l * b[k];
+    }
+  }
+  for (int i = 2; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < 3; ++j) s = s - A[i + 3 * j] * x[j];
+    x[i] = s / A[i + 3 * i];
+  }
+}
+
+__device__ __forceinline__ void matmul3(const double *A, const double *B, double *C) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      C[i + 3 * j] = A[i] * B[3 * j] + A[i + 3] * B[1 + 3 * j] + A[i + 6] * B[2 + 3 * j];
+}
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// R = exp(h [omega]x) by Rodrigues' formula; omega = 0 gives I exactly.
+__device__ __forceinline__ void exp_rodrigues(const double *om, double h, double *R) {
+  double mag = sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+  double w[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    w[i] = om[i] / mag;
+    if (w[i] != w[i]) w[i] = 0;
+  }
+  double th = h * mag;
+  double K[9] = {0, w[2], -w[1], -w[2], 0, w[0], w[1], -w[0], 0};
+  double K2[9];
+  matmul3(K, K, K2);
+  double s = sin(th), c1 = 1 - cos(th);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = ((k % 4) == 0 ? 1.0 : 0.0) + s * K[k] + c1 * K2[k];
+}
+
+// pos = R r + c
+__device__ __forceinline__ void rigid_pos(const double *R, const double *r,
+                                          const double *c, double *out) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out[i] = (R[i] * r[0] + R[i + 3] * r[1] + R[i + 6] * r[2]) + c[i];
+}
+
+// I = R (I0 R^T)
+__device__ __forceinline__ void inertia_update(const double *R, const double *I0, double *I) {
+  double Rt[9], tmp[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Rt[i + 3 * j] = R[j + 3 * i];
+  matmul3(I0, Rt, tmp);
+  matmul3(R, tmp, I);
+}
+
+// w = minv + (r x n) . (I \ (r x n))
+__device__ __forceinline__ double gen_inv_mass(double minv, const double *r,
+                                               const double *I, const double *n) {
+  double ax[3], ic[3];
+  cross3(r, n, ax);
+  solve3_pivot(I, ax, ic);
+  return minv + (ax[0] * ic[0] + ax[1] * ic[1] + ax[2] * ic[2]);
+}
+
+// Block-wide sum of NV doubles per thread into out[0..NV) (valid in all
+// threads after the call).  Fixed tree: deterministic.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *sh, double *out) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) sh[k * kBlock + t] = v[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) sh[k * kBlock + t] += sh[k * kBlock + t + s];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out[k] = sh[k * kBlock];
+  __syncthreads();
+}
+
+// M x = n for a 3x3 matrix by cofactors (x_i = sum_j cof_ji n_j / det).
+template <typename T>
+__device__ __forceinline__ void solve3_cof(const T (&M)[3][3], const T (&n)[3], T (&x)[3]) {
+  const T c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+  const T c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+  const T c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+  const T c10 = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+  const T c11 = M[0][0] * M[2][2] - M[0][2] * M[2][0];
+  const T c12 = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+  const T c20 = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+  const T c21 = M[0][2] * M[1][0] - M[0][0] * M[1][2];
+  const T c22 = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+  const T idet = T(1) / (M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02);
+  x[0] = (c00 * n[0] + c10 * n[1] + c20 * n[2]) * idet;
+  x[1] = (c01 * n[0] + c11 * n[1] + c21 * n[2]) * idet;
+  x[2] = (c02 * n[0] + c12 * n[1] + c22 * n[2]) * idet;
+}
+
+// ---------------------------------------------------------------------------
+// element projection: one Gauss-Seidel pass over one color
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct SweepParams {
+  const int32_t *__restrict__ vid;
+  const T *__restrict__ dminv;
+  const T *__restrict__ vol;
+  const T *__restrict__ mat;  // alpha_d, alpha_o, alpha_s, alpha1
+  T *__restrict__ lam;
+  const uint8_t *__restrict__ flags;  // elastic flags (slot order) or null
+  vec4_t<T> *__restrict__ x;
+  const vec4_t<T> *__restrict__ xold;
+  const Ctl *ctl;
+  int64_t ld, begin, end;
+  T ih2, ih;  // 1/h^2, 1/h
+};
+
+// StVK-Voigt tetrahedron (matStvkVoigtConstraint3D.m:1-45).  With F the
+// deformation gradient, Cg = F^T F, E = (Cg - I)/2, constraint i = vol*E_jk
+// and its gradient w.r.t. vertex a is vol * F P_i g_a, g_a the rows of
+// [-1^T DmInv; DmInv] and P_i the symmetric selector of (j,k).  Hence
+//   gradC_i . gradC_j weighted by 1/m  = vol^2 tr(P_i Cg P_j W),
+//   W = sum_a (1/m_a) g_a g_a^T,  gradC_i . (x - x_old) = vol tr(P_i F^T Fd),
+// and the displacement is (1/m_a) vol F S g_a with S = sum_i dlambda_i P_i.
+// gamma = alpha~ beta~ / h is alpha1/h times I for the isotropic material.
+// The normal block is solved as a 3x3 system, the three shear rows as
+// scalars with the (4,4) compliance, as decomposeSolve does.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep3d(SweepParams<T> P) {
+  if (P.ctl && iteration_stopped(P.ctl)) return;
+  const int64_t s = P.begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (s >= P.end) return;
+  if (P.flags && !P.flags[s]) return;
+  const int64_t ld = P.ld;
+  const int v0 = P.vid[s], v1 = P.vid[ld + s], v2 = P.vid[2 * ld + s], v3 = P.vid[3 * ld + s];
+  const vec4_t<T> X0 = P.x[v0], X1 = P.x[v1], X2 = P.x[v2], X3 = P.x[v3];
+  const vec4_t<T> O0 = P.xold[v0], O1 = P.xold[v1], O2 = P.xold[v2], O3 = P.xold[v3];
+  T D[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) D[k] = P.dminv[k * ld + s];
+  const T vol = P.vol[s];
+  const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
+  const T g = P.mat[3 * ld + s] * P.ih;
+
+  // edge vectors (current and displacement since x_old)
+  const T e[3][3] = {{X1.x - X0.x, X1.y - X0.y, X1.z - X0.z},
+                     {X2.x - X0.x, X2.y - X0.y, X2.z - X0.z},
+                     {X3.x - X0.x, X3.y - X0.y, X3.z - X0.z}};
+  const T d0[3] = {X0.x - O0.x, X0.y - O0.y, X0.z - O0.z};
+  const T f[3][3] = {{(X1.x - O1.x) - d0[0], (X1.y - O1.y) - d0[1], (X1.z - O1.z) - d0[2]},
+                     {(X2.x - O2.x) - d0[0], (X2.y - O2.y) - d0[1], (X2.z - O2.z) - d0[2]},
+                     {(X3.x - O3.x) - d0[0], (X3.y - O3.y) - d0[1], (X3.z - O3.z) - d0[2]}};
+  T F[3][3], Fd[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      F[r][c] = e[0][r] * D[3 * c] + e[1][r] * D[1 + 3 * c] + e[2][r] * D[2 + 3 * c];
+      Fd[r][c] = f[0][r] * D[3 * c] + f[1][r] * D[1 + 3 * c] + f[2][r] * D[2 + 3 * c];
+    }
+  // g_a: g1..g3 = rows of DmInv, g0 = -(g1+g2+g3)
+  T G[4][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    G[1][c] = D[3 * c];
+    G[2][c] = D[1 + 3 * c];
+    G[3][c] = D[2 + 3 * c];
+    G[0][c] = -(G[1][c] + G[2][c] + G[3][c]);
+  }
+  const T w[4] = {X0.w, X1.w, X2.w, X3.w};
+  T Cg[3][3], H[3][3], W[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Cg[j][k] = F[0][j] * F[0][k] + F[1][j] * F[1][k] + F[2][j] * F[2][k];
+      H[j][k] = F[0][j] * Fd[0][k] + F[1][j] * Fd[1][k] + F[2][j] * Fd[2][k];
+      W[j][k] = w[0] * G[0][j] * G[0][k] + w[1] * G[1][j] * G[1][k] +
+                w[2] * G[2][j] * G[2][k] + w[3] * G[3][j] * G[3][k];
+    }
+  const T half = T(0.5), vol2 = vol * vol;
+  T lam[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) lam[i] = P.lam[i * ld + s];
+  // normal block
+  T C[3], num[3], M[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    C[j] = vol * (half * (Cg[j][j] - T(1)));
+    const int k = (j + 1) % 3, l = (j + 2) % 3;
+    num[j] = -C[j] - (tad * lam[j] + tao * lam[k] + tao * lam[l]) - g * (vol * H[j][j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      M[j][k] = (T(1) + g) * (vol2 * Cg[j][k] * W[j][k]) + (j == k ? tad : tao);
+  T dl[6];
+  {
+    T xs[3];
+    solve3_cof(M, num, xs);
+    dl[0] = xs[0];
+    dl[1] = xs[1];
+    dl[2] = xs[2];
+  }
+  // shear rows (0,1), (0,2), (1,2)
+  const int sj[3] = {0, 0, 1}, sk[3] = {1, 2, 2};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int j = sj[q], k = sk[q];
+    const T Cs = vol * (half * Cg[j][k]);
+    const T gd = vol * (half * (H[j][k] + H[k][j]));
+    const T A = vol2 * (T(0.25) * (T(2) * Cg[j][k] * W[j][k] + Cg[k][k] * W[j][j] + Cg[j][j] * W[k][k]));
+    dl[3 + q] = (-Cs - tas * lam[3 + q] - g * gd) / ((T(1) + g) * A + tas);
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) P.lam[i * ld + s] = lam[i] + dl[i];
+  // S = sum_i dl_i P_i, FS = F S
+  const T S[3][3] = {{dl[0], half * dl[3], half * dl[4]},
+                     {half * dl[3], dl[1], half * dl[5]},
+                     {half * dl[4], half * dl[5], dl[2]}};
+  T FS[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) FS[r][c] = F[r][0] * S[0][c] + F[r][1] * S[1][c] + F[r][2] * S[2][c];
+  const vec4_t<T> Xs[4] = {X0, X1, X2, X3};
+  const int vs[4] = {v0, v1, v2, v3};
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const T sc = w[a] * vol;
+    vec4_t<T> o = Xs[a];
+    o.x += sc * (FS[0][0] * G[a][0] + FS[0][1] * G[a][1] + FS[0][2] * G[a][2]);
+    o.y += sc * (FS[1][0] * G[a][0] + FS[1][1] * G[a][1] + FS[1][2] * G[a][2]);
+    o.z += sc * (FS[2][0] * G[a][0] + FS[2][1] * G[a][1] + FS[2][2] * G[a][2]);
+    P.x[vs[a]] = o;
+  }
+}
+
+// StVK-Voigt triangle (matStvkVoigtConstraint.m:1-34): same algebra in 2D
+// with constraints (E11, E22, E12) and the full 3x3 solve of justSolve.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep2d(SweepParams<T> P) {
+  if (P.ctl && iteration_stopped(P.ctl)) return;
+  const int64_t s = P.begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (s >= P.end) return;
+  if (P.flags && !P.flags[s]) return;
+  const int64_t ld = P.ld;
+  const int v0 = P.vid[s], v1 = P.vid[ld + s], v2 = P.vid[2 * ld + s];
+  const vec4_t<T> X0 = P.x[v0], X1 = P.x[v1], X2 = P.x[v2];
+  const vec4_t<T> O0 = P.xold[v0], O1 = P.xold[v1], O2 = P.xold[v2];
+  const T D[4] = {P.dminv[s], P.dminv[ld + s], P.dminv[2 * ld + s], P.dminv[3 * ld + s]};
+  const T vol = P.vol[s];
+  const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
+  const T g = P.mat[3 * ld + s] * P.ih;
+  // Ds = [p0 - p2, p1 - p2], F = Ds DmInv
+  const T e[2][2] = {{X0.x - X2.x, X0.y - X2.y}, {X1.x - X2.x, X1.y - X2.y}};
+  const T d2[2] = {X2.x - O2.x, X2.y - O2.y};
+  const T f[2][2] = {{(X0.x - O0.x) - d2[0], (X0.y - O0.y) - d2[1]},
+                     {(X1.x - O1.x) - d2[0], (X1.y - O1.y) - d2[1]}};
+  T F[2][2], Fd[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      F[r][c] = e[0][r] * D[2 * c] + e[1][r] * D[1 + 2 * c];
+      Fd[r][c] = f[0][r] * D[2 * c] + f[1][r] * D[1 + 2 * c];
+    }
+  T G[3][2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    G[0][c] = D[2 * c];
+    G[1][c] = D[1 + 2 * c];
+    G[2][c] = -(G[0][c] + G[1][c]);
+  }
+  const T w[3] = {X0.w, X1.w, X2.w};
+  T Cg[2][2], H[2][2], W[2][2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      Cg[j][k] = F[0][j] * F[0][k] + F[1][j] * F[1][k];
+      H[j][k] = F[0][j] * Fd[0][k] + F[1][j] * Fd[1][k];
+      W[j][k] = w[0] * G[0][j] * G[0][k] + w[1] * G[1][j] * G[1][k] + w[2] * G[2][j] * G[2][k];
+    }
+  const T half = T(0.5), vol2 = vol * vol;
+  const int pa[3] = {0, 1, 0}, pb[3] = {0, 1, 1};
+  T lam[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) lam[i] = P.lam[i * ld + s];
+  T C[3], gd[3];
+  C[0] = vol * (half * (Cg[0][0] - T(1)));
+  C[1] = vol * (half * (Cg[1][1] - T(1)));
+  C[2] = vol * (half * Cg[0][1]);
+  gd[0] = vol * H[0][0];
+  gd[1] = vol * H[1][1];
+  gd[2] = vol * (half * (H[0][1] + H[1][0]));
+  const T at[3][3] = {{tad, tao, T(0)}, {tao, tad, T(0)}, {T(0), T(0), tas}};
+  T M[3][3], num[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    num[i] = -C[i] - (at[i][0] * lam[0] + at[i][1] * lam[1] + at[i][2] * lam[2]) - g * gd[i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int a = pa[i], b = pb[i], c = pa[j], d = pb[j];
+      const T A = vol2 * (T(0.25) * (Cg[b][c] * W[d][a] + Cg[b][d] * W[c][a] +
+                                     Cg[a][c] * W[d][b] + Cg[a][d] * W[c][b]));
+      M[i][j] = (T(1) + g) * A + at[i][j];
+    }
+  }
+  T dl[3];
+  solve3_cof(M, num, dl);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) P.lam[i * ld + s] = lam[i] + dl[i];
+  const T S[2][2] = {{dl[0], half * dl[2]}, {half * dl[2], dl[1]}};
+  T FS[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) FS[r][c] = F[r][0] * S[0][c] + F[r][1] * S[1][c];
+  const vec4_t<T> Xs[3] = {X0, X1, X2};
+  const int vs[3] = {v0, v1, v2};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T sc = w[a] * vol;
+    vec4_t<T> o = Xs[a];
+    o.x += sc * (FS[0][0] * G[a][0] + FS[0][1] * G[a][1]);
+    o.y += sc * (FS[1][0] * G[a][0] + FS[1][1] * G[a][1]);
+    P.x[vs[a]] = o;
+  }
+}
+
+}  // namespace mlx
+
+namespace mlx {
+
+// ---------------------------------------------------------------------------
+// rigid layer: restriction (per body), step, prolongation
+// ---------------------------------------------------------------------------
+
+// Per-layer lists, CSR by body (0-based body k owns items off[k]..off[k+1]).
+struct LayerLists {
+  const int32_t *rig_off, *rig_vert;  // rigid vertices of each body
+  const int32_t *bnd_off, *bnd_vert;  // boundary vertices of each body
+  const int32_t *rc_off, *rc_idx;     // rigid contacts of each body
+  const int32_t *ec_idx;              // elastic contacts
+  int nb, nbnd, nrc, nec;
+};
+
+// makeRigidsFromConnectivity / matComputeRigidBodyProperties3D
+// (AdaptiveMesh3D.m:32-43, RigidBody3D.m:198-265): mass, COM, COM velocity,
+// inertia sum(-m rhat rhat), angular momentum sum(m r x (v - vbar)),
+// omega = I \ L, R = I, r_i = x - COM.  One workgroup per body, fixed
+// reduction tree.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict(LayerLists L, const vec4_t<T> *x,
+                                                     const T *rv, const double *mass,
+                                                     Body *bodies, double *ri) {
+  __shared__ double sh[12 * kBlock];
+  const int k = blockIdx.x;
+  const int off = L.rig_off[k], n = L.rig_off[k + 1] - off;
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = L.rig_vert[off + q];
+    const double m = mass[v];
+    const vec4_t<T> p = x[v];
+    a[0] += m;
+    a[1] += (double)p.x * m;
+    a[2] += (double)p.y * m;
+    a[3] += (double)p.z * m;
+    a[4] += (double)rv[3 * v] * m;
+    a[5] += (double)rv[3 * v + 1] * m;
+    a[6] += (double)rv[3 * v + 2] * m;
+  }
+  double s7[7];
+  block_sum<7>(a, sh, s7);
+  const double M = s7[0];
+  const double com[3] = {s7[1] / M, s7[2] / M, s7[3] / M};
+  const double cdot[3] = {s7[4] / M, s7[5] / M, s7[6] / M};
+  double b[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = L.rig_vert[off + q];
+    const double m = mass[v];
+    const vec4_t<T> p = x[v];
+    const double r[3] = {(double)p.x - com[0], (double)p.y - com[1], (double)p.z - com[2]};
+    const double vd[3] = {(double)rv[3 * v] - cdot[0], (double)rv[3 * v + 1] - cdot[1],
+                          (double)rv[3 * v + 2] - cdot[2]};
+    // -m * (rhat rhat) = m (|r|^2 I - r r^T)
+    const double rh[9] = {0, r[2], -r[1], -r[2], 0, r[0], r[1], -r[0], 0};
+    double rr[9];
+    matmul3(rh, rh, rr);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) b[i] += (-m) * rr[i];
+    double c[3];
+    cross3(r, vd, c);
+    b[9] += m * c[0];
+    b[10] += m * c[1];
+    b[11] += m * c[2];
+    ri[3 * v] = r[0];
+    ri[3 * v + 1] = r[1];
+    ri[3 * v + 2] = r[2];
+  }
+  double s12[12];
+  block_sum<12>(b, sh, s12);
+  if (threadIdx.x == 0) {
+    Body &B = bodies[k];
+    B.M = M;
+    B.invM = 1.0 / M;
+    for (int d = 0; d < 3; ++d) {
+      B.COM[d] = com[d];
+      B.COMDOT[d] = cdot[d];
+    }
+    for (int i = 0; i < 9; ++i) {
+      B.I0[i] = s12[i];
+      B.I[i] = s12[i];
+      B.R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+      B.Rstep[i] = B.R[i];
+    }
+    solve3_pivot(s12, s12 + 9, B.omega);
+  }
+}
+
+// rigidSymplecticImperativeStep (RigidBody3D.m:174-179).
+__global__ void k_body_step(Body *bodies, int nb, double h) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  Body &B = bodies[k];
+  for (int d = 0; d < 3; ++d) B.COM[d] = B.COM[d] + h * B.COMDOT[d];
+  double dR[9], Rn[9], tmp[9], Rt[9];
+  exp_rodrigues(B.omega, h, dR);
+  matmul3(dR, B.R, Rn);
+  for (int i = 0; i < 9; ++i) B.R[i] = Rn[i];
+  matmul3(B.R, B.I0, tmp);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Rt[i + 3 * j] = B.R[j + 3 * i];
+  matmul3(tmp, Rt, B.I);
+  for (int i = 0; i < 9; ++i) B.Rstep[i] = B.R[i];
+}
+
+// Layer start on the vertices (xpbdLayer3D.m:138-158): elastic vertices are
+// released (x += h v_res, v_res = 0); when the layer has rigid elements,
+// rigid vertices move with their stepped body and their residual velocity
+// becomes R v_res - (x_new - x_prev)/h.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_layer_vertices(int64_t nV, const int32_t *vbody,
+                                                           vec4_t<T> *x, T *rv,
+                                                           const Body *bodies,
+                                                           const double *ri, double h,
+                                                           int stepRigid) {
+  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nV) return;
+  const int b = vbody[v];
+  vec4_t<T> p = x[v];
+  if (b == 0) {
+    p.x = p.x + T(h) * rv[3 * v];
+    p.y = p.y + T(h) * rv[3 * v + 1];
+    p.z = p.z + T(h) * rv[3 * v + 2];
+    rv[3 * v] = T(0);
+    rv[3 * v + 1] = T(0);
+    rv[3 * v + 2] = T(0);
+    x[v] = p;
+    return;
+  }
+  if (!stepRigid) return;
+  const Body &B = bodies[b - 1];
+  double xn[3];
+  rigid_pos(B.R, ri + 3 * v, B.COM, xn);
+  const double prev[3] = {(double)p.x, (double)p.y, (double)p.z};
+  p.x = (T)xn[0];
+  p.y = (T)xn[1];
+  p.z = (T)xn[2];
+  x[v] = p;
+  const double vr[3] = {(double)rv[3 * v], (double)rv[3 * v + 1], (double)rv[3 * v + 2]};
+  const double zero[3] = {0, 0, 0};
+  double rot[3];
+  rigid_pos(B.R, vr, zero, rot);
+  rv[3 * v] = (T)(rot[0] - ((double)p.x - prev[0]) / h);
+  rv[3 * v + 1] = (T)(rot[1] - ((double)p.y - prev[1]) / h);
+  rv[3 * v + 2] = (T)(rot[2] - ((double)p.z - prev[2]) / h);
+}
+
+// rotateAllVerts (xpbdLayer3D.m:508-513): rigid vertices to R r_i + COM.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rotate_all(int64_t nV, const int32_t *vbody,
+                                                       vec4_t<T> *x, const Body *bodies,
+                                                       const double *ri) {
+  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nV) return;
+  const int b = vbody[v];
+  if (b == 0) return;
+  const Body &B = bodies[b - 1];
+  double xn[3];
+  rigid_pos(B.R, ri + 3 * v, B.COM, xn);
+  vec4_t<T> p = x[v];
+  p.x = (T)xn[0];
+  p.y = (T)xn[1];
+  p.z = (T)xn[2];
+  x[v] = p;
+}
+
+// rotateResidual with deltaR = R_projected R_stepped^T (xpbdLayer3D.m:217-241).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rotate_residual(int64_t nV, const int32_t *vbody,
+                                                            T *rv, const Body *bodies) {
+  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nV) return;
+  const int b = vbody[v];
+  if (b == 0) return;
+  const Body &B = bodies[b - 1];
+  double Rt[9], dR[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Rt[i + 3 * j] = B.Rstep[j + 3 * i];
+  matmul3(B.R, Rt, dR);
+  const double vr[3] = {(double)rv[3 * v], (double)rv[3 * v + 1], (double)rv[3 * v + 2]};
+  const double zero[3] = {0, 0, 0};
+  double o[3];
+  rigid_pos(dR, vr, zero, o);
+  rv[3 * v] = (T)o[0];
+  rv[3 * v + 1] = (T)o[1];
+  rv[3 * v + 2] = (T)o[2];
+}
+
+// Apply accumulated dCOM / dOmega to body k (xpbdLayer3D.m:400-414):
+// COM += dCOM; w = I \ dOmega; R = exp(h w) R; I = R I0 R^T.
+__device__ __forceinline__ void body_apply(Body &B, const double *dCOM, const double *dOm,
+                                           double h) {
+  for (int d = 0; d < 3; ++d) B.COM[d] = B.COM[d] + dCOM[d];
+  double w[3], dR[9], Rn[9];
+  solve3_pivot(B.I, dOm, w);
+  exp_rodrigues(w, h, dR);
+  matmul3(dR, B.R, Rn);
+  for (int i = 0; i < 9; ++i) B.R[i] = Rn[i];
+  inertia_update(B.R, B.I0, B.I);
+}
+
+// rigidElasticBoundaryAccumulationConstraints (xpbdLayer3D.m:381-420) with
+// rigidBoundaryConstraint.project (rigidBoundaryConstraint.m:4-35).  One
+// workgroup per body: project its boundary vertices, reduce, apply, snap.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_boundary(LayerLists L, vec4_t<T> *x,
+                                                     const double *minv, Body *bodies,
+                                                     const double *ri, double *lamB,
+                                                     double compliance, double h,
+                                                     const Ctl *ctl) {
+  if (iteration_stopped(ctl)) return;
+  __shared__ double sh[6 * kBlock];
+  const int k = blockIdx.x;
+  Body &B = bodies[k];
+  const int off = L.bnd_off[k], n = L.bnd_off[k + 1] - off;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = L.bnd_vert[off + q];
+    double rx[3], r[3], dir[3], nn[3];
+    rigid_pos(B.R, ri + 3 * v, B.COM, rx);
+    const vec4_t<T> p = x[v];
+    const double xe[3] = {(double)p.x, (double)p.y, (double)p.z};
+    for (int d = 0; d < 3; ++d) {
+      r[d] = rx[d] - B.COM[d];
+      dir[d] = rx[d] - xe[d];
+    }
+    const double C = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    for (int d = 0; d < 3; ++d) {
+      nn[d] = dir[d] / C;
+      if (nn[d] != nn[d]) nn[d] = 0;
+    }
+    const double w = gen_inv_mass(B.invM, r, B.I, nn);
+    const double dl = (-C - compliance * lamB[off + q]) / ((minv[v] + w) + compliance);
+    lamB[off + q] += dl;
+    const double pp[3] = {dl * nn[0], dl * nn[1], dl * nn[2]};
+    const double ph[3] = {pp[0] / h, pp[1] / h, pp[2] / h};
+    double dom[3];
+    cross3(r, ph, dom);
+    for (int d = 0; d < 3; ++d) {
+      acc[d] += pp[d] * B.invM;
+      acc[3 + d] += dom[d];
+    }
+  }
+  double s6[6];
+  block_sum<6>(acc, sh, s6);
+  __shared__ double sR[9], sC[3];
+  if (threadIdx.x == 0) {
+    body_apply(B, s6, s6 + 3, h);
+    for (int i = 0; i < 9; ++i) sR[i] = B.R[i];
+    for (int i = 0; i < 3; ++i) sC[i] = B.COM[i];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = L.bnd_vert[off + q];
+    double xn[3];
+    rigid_pos(sR, ri + 3 * v, sC, xn);
+    vec4_t<T> p = x[v];
+    p.x = (T)xn[0];
+    p.y = (T)xn[1];
+    p.z = (T)xn[2];
+    x[v] = p;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// contacts
+// ---------------------------------------------------------------------------
+
+struct ContactArrays {
+  const int32_t *vert;      // K, 0-based
+  const double *normal;     // 3K (xyz per contact)
+  const double *point;      // 3K
+  const double *compliance; // K
+  double *lambda;           // K
+  int K;
+};
+
+// penaltyConstraint.projectElastic (penaltyConstraint.m:4-28), 3D;
+// penaltyConstraint2D.projectElastic (penaltyConstraint2D.m:4-26) in 2D.
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void k_contacts_elastic(LayerLists L, ContactArrays c,
+                                                             vec4_t<T> *x, const double *minv,
+                                                             double resistance, double h,
+                                                             const Ctl *ctl) {
+  if (iteration_stopped(ctl)) return;
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= L.nec) return;
+  const int i = L.ec_idx[q];
+  const int v = c.vert[i];
+  const double *n = c.normal + 3 * i, *pc = c.point + 3 * i;
+  vec4_t<T> p = x[v];
+  const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
+  double phi = n[0] * (xv[0] - pc[0]) + n[1] * (xv[1] - pc[1]);
+  if (DIM == 3) phi = phi + n[2] * (xv[2] - pc[2]);
+  if (phi > 0) phi = 0;
+  const double C = resistance * phi;
+  const double aT = c.compliance[i] / (h * h);
+  const double mi = minv[v];
+  double dl;
+  if (DIM == 3) {
+    dl = (-C - aT * c.lambda[i]) / (2 * mi + aT);
+    p.x = (T)(xv[0] + mi * (dl * n[0]));
+    p.y = (T)(xv[1] + mi * (dl * n[1]));
+    p.z = (T)(xv[2] + mi * (dl * n[2]));
+  } else {
+    const double g0 = n[0] * 2 * mi, g1 = n[1] * 2 * mi;
+    dl = -(C + aT * c.lambda[i]) / ((g0 * n[0] + g1 * n[1]) + aT);
+    p.x = (T)(xv[0] + g0 * dl);
+    p.y = (T)(xv[1] + g1 * dl);
+  }
+  c.lambda[i] += dl;
+  x[v] = p;
+}
+
+// penaltyConstraint.projectRigid (penaltyConstraint.m:30-63) with the
+// per-body accumulation of contactConstraints (xpbdLayer3D.m:445-484).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_contacts_rigid(LayerLists L, ContactArrays c,
+                                                           vec4_t<T> *x, Body *bodies,
+                                                           const double *ri,
+                                                           double resistance, double h,
+                                                           const Ctl *ctl) {
+  if (iteration_stopped(ctl)) return;
+  __shared__ double sh[6 * kBlock];
+  const int k = blockIdx.x;
+  Body &B = bodies[k];
+  const int off = L.rc_off[k], n = L.rc_off[k + 1] - off;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int i = L.rc_idx[off + q];
+    const int v = c.vert[i];
+    const double *nn = c.normal + 3 * i, *pc = c.point + 3 * i;
+    double rx[3], r[3];
+    rigid_pos(B.R, ri + 3 * v, B.COM, rx);
+    for (int d = 0; d < 3; ++d) r[d] = rx[d] - B.COM[d];
+    const vec4_t<T> p = x[v];
+    double phi = nn[0] * ((double)p.x - pc[0]) + nn[1] * ((double)p.y - pc[1]) +
+                 nn[2] * ((double)p.z - pc[2]);
+    if (phi > 0) phi = 0;
+    const double C = resistance * phi;
+    const double w = gen_inv_mass(B.invM, r, B.I, nn);
+    const double aT = (2 * c.compliance[i]) / (h * h);
+    const double dl = (-C - aT * c.lambda[i]) / (2 * w + aT);
+    c.lambda[i] += dl;
+    const double pp[3] = {dl * nn[0], dl * nn[1], dl * nn[2]};
+    double cr[3];
+    cross3(r, pp, cr);
+    for (int d = 0; d < 3; ++d) {
+      acc[d] += pp[d] * B.invM;
+      acc[3 + d] += (1 / h) * cr[d];
+    }
+  }
+  double s6[6];
+  block_sum<6>(acc, sh, s6);
+  __shared__ double sR[9], sC[3];
+  if (threadIdx.x == 0) {
+    body_apply(B, s6, s6 + 3, h);
+    for (int i = 0; i < 9; ++i) sR[i] = B.R[i];
+    for (int i = 0; i < 3; ++i) sC[i] = B.COM[i];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = c.vert[L.rc_idx[off + q]];
+    double xn[3];
+    rigid_pos(sR, ri + 3 * v, sC, xn);
+    vec4_t<T> p = x[v];
+    p.x = (T)xn[0];
+    p.y = (T)xn[1];
+    p.z = (T)xn[2];
+    x[v] = p;
+  }
+}
+
+// xi(pinnedDOFs) = pinnedx (xpbdLayer3D.m:333; 2D pins to oldp, :187).
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void k_pins(int64_t n, const int32_t *dof, const T *val,
+                                                 vec4_t<T> *x, const Ctl *ctl) {
+  if (iteration_stopped(ctl)) return;
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= n) return;
+  const int d = dof[q];
+  const int v = d / DIM, c = d % DIM;
+  T *px = reinterpret_cast<T *>(&x[v]);
+  px[c] = val[q];
+}
+
+// ---------------------------------------------------------------------------
+// residual and monitor
+// ---------------------------------------------------------------------------
+
+// Element part of simpleEvaluateAllConstraintResiduals (xpbdLayer3D.m:546-568;
+// 2D xpbdLayer2D.m:477-496): per element sum_i (|C_i| - min(|C_i|,
+// |(alpha~ lambda)_i|))^2, reduced per workgroup in double.  lamElem gives,
+// for the 2D quirk, the slot whose lambda column feeds each page entry.
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void k_residual_elems(SweepParams<T> P, int64_t nSlots,
+                                                           const int32_t *lamSlot2d,
+                                                           double *partials) {
+  if (iteration_stopped(P.ctl)) return;
+  __shared__ double sh[kBlock];
+  const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double acc = 0;
+  if (s < nSlots && P.vid[s] >= 0) {
+    const int64_t ld = P.ld;
+    const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
+    const T vol = P.vol[s];
+    if (DIM == 3) {
+      const vec4_t<T> X0 = P.x[P.vid[s]], X1 = P.x[P.vid[ld + s]], X2 = P.x[P.vid[2 * ld + s]],
+                      X3 = P.x[P.vid[3 * ld + s]];
+      T D[9];
+      for (int k = 0; k < 9; ++k) D[k] = P.dminv[k * ld + s];
+      const T e[3][3] = {{X1.x - X0.x, X1.y - X0.y, X1.z - X0.z},
+                         {X2.x - X0.x, X2.y - X0.y, X2.z - X0.z},
+                         {X3.x - X0.x, X3.y - X0.y, X3.z - X0.z}};
+      T F[3][3];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          F[r][c] = e[0][r] * D[3 * c] + e[1][r] * D[1 + 3 * c] + e[2][r] * D[2 + 3 * c];
+      T Cg[3][3];
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) Cg[j][k] = F[0][j] * F[0][k] + F[1][j] * F[1][k] + F[2][j] * F[2][k];
+      T lam[6];
+      for (int i = 0; i < 6; ++i) lam[i] = P.lam[i * ld + s];
+      const T half = T(0.5);
+      T C[6], al[6];
+      for (int j = 0; j < 3; ++j) {
+        C[j] = vol * (half * (Cg[j][j] - T(1)));
+        const int k = (j + 1) % 3, l = (j + 2) % 3;
+        al[j] = tad * lam[j] + tao * lam[k] + tao * lam[l];
+      }
+      C[3] = vol * (half * Cg[0][1]);
+      C[4] = vol * (half * Cg[0][2]);
+      C[5] = vol * (half * Cg[1][2]);
+      al[3] = tas * lam[3];
+      al[4] = tas * lam[4];
+      al[5] = tas * lam[5];
+      for (int i = 0; i < 6; ++i) {
+        const double a = fabs((double)C[i]), cr = fabs((double)al[i]);
+        const double t = a - (a < cr ? a : cr);
+        acc += t * t;
+      }
+    } else {
+      const vec4_t<T> X0 = P.x[P.vid[s]], X1 = P.x[P.vid[ld + s]], X2 = P.x[P.vid[2 * ld + s]];
+      const T D[4] = {P.dminv[s], P.dminv[ld + s], P.dminv[2 * ld + s], P.dminv[3 * ld + s]};
+      const T e[2][2] = {{X0.x - X2.x, X0.y - X2.y}, {X1.x - X2.x, X1.y - X2.y}};
+      T F[2][2];
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) F[r][c] = e[0][r] * D[2 * c] + e[1][r] * D[1 + 2 * c];
+      const T c00 = F[0][0] * F[0][0] + F[1][0] * F[1][0];
+      const T c11 = F[0][1] * F[0][1] + F[1][1] * F[1][1];
+      const T c01 = F[0][0] * F[0][1] + F[1][0] * F[1][1];
+      const T half = T(0.5);
+      const T C[3] = {vol * (half * (c00 - T(1))), vol * (half * (c11 - T(1))), vol * (half * c01)};
+      // lambda page e mixes elements: entry r is lambdai(L mod nT, L div nT)
+      // with L = 3e + r (reshape of the nT x 3 array, xpbdLayer2D.m:489).
+      T lp[3];
+      for (int r = 0; r < 3; ++r) {
+        const int32_t code = lamSlot2d[3 * s + r];  // slot*3 + component
+        lp[r] = P.lam[(code % 3) * ld + code / 3];
+      }
+      const T al[3] = {tad * lp[0] + tao * lp[1], tao * lp[0] + tad * lp[1], tas * lp[2]};
+      for (int i = 0; i < 3; ++i) {
+        const double a = fabs((double)C[i]), cr = fabs((double)al[i]);
+        const double t = a - (a < cr ? a : cr);
+        acc += t * t;
+      }
+    }
+  }
+  double o[1];
+  double v1[1] = {acc};
+  block_sum<1>(v1, sh, o);
+  if (threadIdx.x == 0) partials[blockIdx.x] = o[0];
+}
+
+// Contact part of the residual (xpbdLayer3D.m:573-583).  In 3D the
+// compliance enters as a 1x1xK page array times lambda(:), which Matlab
+// expands to K x 1 x K: every contact i contributes once per compliance
+// value j.  uniqAlpha/uniqCount list the distinct compliance values with
+// multiplicities (3D); in 2D each contact counts once with its own value.
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void k_residual_contacts(ContactArrays c, const vec4_t<T> *x,
+                                                              double resistance, double h,
+                                                              const double *uniqAlpha,
+                                                              const double *uniqCount, int nUniq,
+                                                              double *partials, const Ctl *ctl) {
+  if (iteration_stopped(ctl)) return;
+  __shared__ double sh[kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double acc = 0;
+  if (i < c.K) {
+    const int v = c.vert[i];
+    const double *n = c.normal + 3 * i, *pc = c.point + 3 * i;
+    const vec4_t<T> p = x[v];
+    double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]);
+    if (DIM == 3) phi = phi + n[2] * ((double)p.z - pc[2]);
+    if (phi > 0) phi = 0;
+    const double a = fabs(resistance * phi);
+    const double h2 = h * h;
+    if (DIM == 3) {
+      for (int u = 0; u < nUniq; ++u) {
+        const double cr = fabs((uniqAlpha[u] / h2) * c.lambda[i]);
+        const double t = a - (a < cr ? a : cr);
+        acc += uniqCount[u] * (t * t);
+      }
+    } else {
+      const double cr = fabs((c.compliance[i] / h2) * c.lambda[i]);
+      const double t = a - (a < cr ? a : cr);
+      acc += t * t;
+    }
+  }
+  double o[1];
+  double v1[1] = {acc};
+  block_sum<1>(v1, sh, o);
+  if (threadIdx.x == 0) partials[blockIdx.x] = o[0];
+}
+
+// End of one projectGS iteration: when residuals are monitored, sum the
+// partials, append the residual and apply simpleMonitorError
+// (xpbdLayer3D.m:515-544; 2D xpbdLayer2D.m:520-549); count the iteration.
+__global__ __launch_bounds__(kBlock) void k_iter_end(Ctl *ctl, MonitorParams mp,
+                                                     const double *partials, int nPartials,
+                                                     double *resid, int residCap) {
+  if (iteration_stopped(ctl)) return;
+  __shared__ double sh[kBlock];
+  double acc = 0;
+  if (mp.computeResiduals)
+    for (int q = threadIdx.x; q < nPartials; q += kBlock) acc += partials[q];
+  double o[1];
+  double v1[1] = {acc};
+  block_sum<1>(v1, sh, o);
+  if (threadIdx.x != 0) return;
+  ctl->iters += 1;
+  if (!mp.computeResiduals) return;
+  const double newRes = sqrt(o[0]);
+  const int idx = ctl->residualCount;
+  if (idx < residCap) resid[idx] = newRes;
+  ctl->residualCount = idx + 1;
+  const double last = fabs(ctl->lastResidual);
+  ctl->lastResidual = newRes;
+  const double nr = fabs(newRes);
+  const double change = nr - last;
+  if (ctl->setPercentStop > 0) {
+    double ru = (1 - ctl->setPercentStop / 100.0) * nr;
+    if (mp.is3d) ru = ru > 1e-8 ? ru : 1e-8;
+    ctl->runUntil = ru;
+    ctl->setPercentStop = 0;
+  }
+  if (change < 0) ctl->errorChangeDecreased = 1;
+  if (mp.giveUpEnabled && change > mp.giveUpThreshold && ctl->errorChangeDecreased &&
+      (!ctl->isLastLayer || mp.lastLayerGiveUpEnabled))
+    ctl->endLayer = 1;
+  const int stopHanging = ctl->residualCount > mp.hangingStop && mp.hangingStop >= 0;
+  int stop;
+  if (mp.is3d)
+    stop = stopHanging || (ctl->runUntil >= 0 && ctl->runUntil >= nr &&
+                           (ctl->isLastLayer || ctl->normresv < ctl->runUntil));
+  else
+    stop = (ctl->runUntil >= 0 && ctl->runUntil >= nr &&
+            (ctl->isLastLayer || ctl->normresv < mp.h * mp.h)) || stopHanging;
+  if (stop) {
+    ctl->stopSolve = 1;
+    ctl->stopLayer = ctl->layer;
+  }
+}
+
+// Sum of squares of n doubles-or-floats into partials (grid-stride, fixed grid).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sumsq(const T *a, int64_t n, double *partials) {
+  __shared__ double sh[kBlock];
+  double acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const double v = (double)a[i];
+    acc += v * v;
+  }
+  double o[1];
+  double v1[1] = {acc};
+  block_sum<1>(v1, sh, o);
+  if (threadIdx.x == 0) partials[blockIdx.x] = o[0];
+}
+
+// ctl->normresv = sqrt(sum(partials)); layer bookkeeping.
+__global__ void k_layer_begin(Ctl *ctl, const double *partials, int nPartials, int layer,
+                              int isLast, double setPercentStop) {
+  __shared__ double sh[kBlock];
+  double acc = 0;
+  for (int q = threadIdx.x; q < nPartials; q += kBlock) acc += partials[q];
+  double o[1];
+  double v1[1] = {acc};
+  block_sum<1>(v1, sh, o);
+  if (threadIdx.x != 0) return;
+  ctl->normresv = sqrt(o[0]);
+  ctl->endLayer = 0;
+  ctl->iters = 0;
+  ctl->layer = layer;
+  ctl->isLastLayer = isLast;
+  ctl->setPercentStop = setPercentStop;
+}
+
+// ---------------------------------------------------------------------------
+// layer building: strain-rate proxy
+// ---------------------------------------------------------------------------
+
+// Fa = B p and Fb = B p_old in the layout of computeB3D.m:31-41 (F row by
+// row), summed over the element's vertices in ascending vertex index (the
+// column order of the sparse product), then mexEdiffNorm3D.cpp:57-60.  No
+// contraction into FMA, so the result is bit-identical to the same
+// expression on the host.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ediff3d(int64_t nT, const int32_t *T0,
+                                                    const double *dminv, const vec4_t<T> *pa,
+                                                    const vec4_t<T> *pb, double h,
+                                                    double *out) {
+#pragma clang fp contract(off)
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nT) return;
+  int t[4] = {T0[4 * e], T0[4 * e + 1], T0[4 * e + 2], T0[4 * e + 3]};
+  const double *Mi = dminv + 9 * e;
+  double Dc[4][3];
+  for (int j = 0; j < 3; ++j) {
+    Dc[0][j] = -((Mi[3 * j] + Mi[1 + 3 * j]) + Mi[2 + 3 * j]);
+    for (int a = 1; a < 4; ++a) Dc[a][j] = Mi[(a - 1) + 3 * j];
+  }
+  int ord[4] = {0, 1, 2, 3};
+  for (int i = 1; i < 4; ++i)
+    for (int k = i; k > 0 && t[ord[k - 1]] > t[ord[k]]; --k) {
+      int tmp = ord[k];
+      ord[k] = ord[k - 1];
+      ord[k - 1] = tmp;
+    }
+  double A[9], B[9];
+  for (int d = 0; d < 3; ++d)
+    for (int j = 0; j < 3; ++j) {
+      double sa = 0.0, sb = 0.0;
+      for (int q = 0; q < 4; ++q) {
+        const int a = ord[q];
+        const vec4_t<T> xa = pa[t[a]], xb = pb[t[a]];
+        const double ca = d == 0 ? (double)xa.x : (d == 1 ? (double)xa.y : (double)xa.z);
+        const double cb = d == 0 ? (double)xb.x : (d == 1 ? (double)xb.y : (double)xb.z);
+        sa = sa + Dc[a][j] * ca;
+        sb = sb + Dc[a][j] * cb;
+      }
+      A[3 * d + j] = sa;
+      B[3 * d + j] = sb;
+    }
+  // mexEdiffNorm3D reads the 9 values column-major
+  const double a11 = A[0], a21 = A[1], a31 = A[2], a12 = A[3], a22 = A[4], a32 = A[5],
+               a13 = A[6], a23 = A[7], a33 = A[8];
+  const double b11 = B[0], b21 = B[1], b31 = B[2], b12 = B[3], b22 = B[4], b32 = B[5],
+               b13 = B[6], b23 = B[7], b33 = B[8];
+  const double t2 = 1.0 / (h * h);
+  double q1 = (a11 * a12) / 2.0 + (a21 * a22) / 2.0 + (a31 * a32) / 2.0 - (b11 * b12) / 2.0 - (b21 * b22) / 2.0 - (b31 * b32) / 2.0;
+  double q2 = (a11 * a13) / 2.0 + (a21 * a23) / 2.0 + (a31 * a33) / 2.0 - (b11 * b13) / 2.0 - (b21 * b23) / 2.0 - (b31 * b33) / 2.0;
+  double q3 = (a12 * a13) / 2.0 + (a22 * a23) / 2.0 + (a32 * a33) / 2.0 - (b12 * b13) / 2.0 - (b22 * b23) / 2.0 - (b32 * b33) / 2.0;
+  double q4 = (a11 * a11) / 2.0 + (a21 * a21) / 2.0 + (a31 * a31) / 2.0 - (b11 * b11) / 2.0 - (b21 * b21) / 2.0 - (b31 * b31) / 2.0;
+  double q5 = (a12 * a12) / 2.0 + (a22 * a22) / 2.0 + (a32 * a32) / 2.0 - (b12 * b12) / 2.0 - (b22 * b22) / 2.0 - (b32 * b32) / 2.0;
+  double q6 = (a13 * a13) / 2.0 + (a23 * a23) / 2.0 + (a33 * a33) / 2.0 - (b13 * b13) / 2.0 - (b23 * b23) / 2.0 - (b33 * b33) / 2.0;
+  out[e] = t2 * (q1 * q1) * 2.0 + t2 * (q2 * q2) * 2.0 + t2 * (q3 * q3) * 2.0 + t2 * (q4 * q4) +
+           t2 * (q5 * q5) + t2 * (q6 * q6);
+}
+
+// 2D: F = Ds DmInv column-major (Mesh.m:458-467, mult2x2Stack.m), then
+// mexEdiffNorm2D.cpp:47-51.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ediff2d(int64_t nT, const int32_t *T0,
+                                                    const double *dminv, const vec4_t<T> *pa,
+                                                    const vec4_t<T> *pb, double h,
+                                                    double *out) {
+#pragma clang fp contract(off)
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nT) return;
+  int t[3] = {T0[3 * e], T0[3 * e + 1], T0[3 * e + 2]};
+  const double *Mi = dminv + 4 * e;
+  double Dc[3][2];
+  for (int j = 0; j < 2; ++j) {
+    Dc[0][j] = Mi[2 * j];
+    Dc[1][j] = Mi[1 + 2 * j];
+    Dc[2][j] = (-1 * Mi[2 * j]) + (-1 * Mi[1 + 2 * j]);
+  }
+  int ord[3] = {0, 1, 2};
+  for (int i = 1; i < 3; ++i)
+    for (int k = i; k > 0 && t[ord[k - 1]] > t[ord[k]]; --k) {
+      int tmp = ord[k];
+      ord[k] = ord[k - 1];
+      ord[k - 1] = tmp;
+    }
+  double A[4], B[4];
+  for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < 2; ++i) {
+      double sa = 0.0, sb = 0.0;
+      for (int q = 0; q < 3; ++q) {
+        const int a = ord[q];
+        const vec4_t<T> xa = pa[t[a]], xb = pb[t[a]];
+        sa = sa + Dc[a][j] * (i == 0 ? (double)xa.x : (double)xa.y);
+        sb = sb + Dc[a][j] * (i == 0 ? (double)xb.x : (double)xb.y);
+      }
+      A[i + 2 * j] = sa;
+      B[i + 2 * j] = sb;
+    }
+  const double a = A[0] * A[0] + A[1] * A[1] - B[0] * B[0] - B[1] * B[1];
+  const double b = A[0] * A[2] + A[1] * A[3] - B[0] * B[2] - B[1] * B[3];
+  const double c = A[2] * A[2] + A[3] * A[3] - B[2] * B[2] - B[3] * B[3];
+  out[e] = (0.25 * a * a + 0.5 * b * b + 0.25 * c * c) / h / h;
+}
+
+// ---------------------------------------------------------------------------
+// integrate() prologue / epilogue
+// ---------------------------------------------------------------------------
+
+// frame-1 / NaN reset of cache.oldp and pinnedx (xpbdLayer3D.m:629-632).
+template <typename T>
+__global__ void k_any_nan(int64_t nV, const vec4_t<T> *xold, int *flag) {
+  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nV) return;
+  const vec4_t<T> p = xold[v];
+  if (p.x != p.x || p.y != p.y || p.z != p.z) *flag = 1;
+}
+
+// 3D: residualVelocity = v; += h f / m on unpinned DOFs with f = m g on z;
+// *= (1 - airDrag) (xpbdLayer3D.m:638-664).
+template <typename T>
+__global__ void k_residual_velocity3d(int64_t nV, const T *v, T *rv, const double *mass,
+                                      const uint8_t *pinnedDof, double h, double g,
+                                      double airDrag) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= 3 * nV) return;
+  double r = (double)v[i];
+  if (!pinnedDof[i]) {
+    const double m = mass[i / 3];
+    const double f = (i % 3 == 2) ? 0.0 + m * g : 0.0;
+    r = r + h * f / m;
+  }
+  rv[i] = (T)((1 - airDrag) * r);
+}
+
+// 2D explicit predictor v += h f/m; x += h v on unpinned DOFs with f = m g on
+// y (xpbdLayer2D.m:812-835).
+template <typename T>
+__global__ void k_predict2d(int64_t nV, T *v, vec4_t<T> *x, const double *mass,
+                            const uint8_t *pinnedDof, double h, double g) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nV) return;
+  vec4_t<T> p = x[i];
+  const double m = mass[i];
+  for (int d = 0; d < 2; ++d) {
+    if (pinnedDof[2 * i + d]) continue;
+    const double f = (d == 1) ? 0.0 + m * g : 0.0;
+    const double vn = (double)v[2 * i + d] + h * f / m;
+    v[2 * i + d] = (T)vn;
+    T *px = reinterpret_cast<T *>(&p);
+    px[d] = (T)((double)px[d] + h * (double)(T)vn);
+  }
+  x[i] = p;
+}
+
+// v = (xi - oldp)/h (xpbdLayer3D.m:737; 2D BDF1 :907).
+template <typename T, int DIM>
+__global__ void k_velocity(int64_t nV, const vec4_t<T> *x, const vec4_t<T> *xold, T *v,
+                           double h) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nV) return;
+  const vec4_t<T> p = x[i], o = xold[i];
+  v[DIM * i] = (T)(((double)p.x - (double)o.x) / h);
+  v[DIM * i + 1] = (T)(((double)p.y - (double)o.y) / h);
+  if (DIM == 3) v[DIM * i + 2] = (T)(((double)p.z - (double)o.z) / h);
+}
+
+// contactVelocityUpdate (xpbdLayer2D.m:577-603).
+template <typename T, int DIM>
+__global__ void k_contact_velocity(ContactArrays c, T *v, double h, double g, double e) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= c.K) return;
+  const int id = c.vert[i];
+  double vn[3] = {0, 0, 0}, nrm = 0;
+  for (int d = 0; d < DIM; ++d) {
+    vn[d] = (double)v[DIM * id + d];
+    nrm = nrm + vn[d] * vn[d];
+  }
+  nrm = sqrt(nrm);
+  double ei = e;
+  if (nrm <= 2 * g * h) ei = 0;
+  double vt = 0;
+  for (int d = 0; d < DIM; ++d) vt = vt + c.normal[3 * i + d] * vn[d];
+  double mn = -ei * vt;
+  if (mn > 0) mn = 0;
+  for (int d = 0; d < DIM; ++d) v[DIM * id + d] = (T)(vn[d] + c.normal[3 * i + d] * (-vn[d] + mn));
+}
+
+// Plane contact test (PlaneContactFinder3D.m:40-43; 2D strict).
+template <typename T, int DIM>
+__global__ void k_plane_flags(int64_t nV, const vec4_t<T> *x, double n0, double n1, double n2,
+                              double D, int strict, int32_t *flag) {
+  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nV) return;
+  const vec4_t<T> p = x[v];
+  double dist = (double)p.x * n0 + (double)p.y * n1;
+  if (DIM == 3) dist = dist + (double)p.z * n2;
+  dist = dist - D;
+  flag[v] = strict ? (dist < 0) : (dist <= 0);
+}
+
+}  // namespace mlx

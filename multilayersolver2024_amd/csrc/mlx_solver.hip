@@ -1,0 +1,1431 @@
+// GPU orchestration of the multi-layer XPBD substep (gfx950): scene upload
+// into color-major slots, layerSolve (layers x iterations x colors), the
+// integrate() prologue/epilogue and the C ABI entry points that touch the
+// device.  Kernels are in mlx_kernels.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "mlx_host.h"
+#include "mlx_internal.h"
+#include "mlx_kernels.h"
+
+using namespace mlx;
+
+namespace mlx {
+
+int DevBuf::reserve(size_t n) {
+  if (n <= bytes && ptr) return MLX_OK;
+  release();
+  size_t want = n < 256 ? 256 : n;
+  if (hipMalloc(&ptr, want) != hipSuccess) {
+    ptr = nullptr;
+    bytes = 0;
+    set_error("hipMalloc(%zu) failed", want);
+    return MLX_ERR_NOMEM;
+  }
+  bytes = want;
+  return MLX_OK;
+}
+
+void DevBuf::release() {
+  // a buffer may still be read by queued kernels: drain the device first
+  if (ptr) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(ptr);
+  }
+  ptr = nullptr;
+  bytes = 0;
+}
+
+static int g_device = 0;
+
+static int check_device(int dev) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    set_error("no HIP device visible");
+    return MLX_ERR_NODEVICE;
+  }
+  if (dev < 0 || dev >= n) {
+    set_error("device %d out of range (0..%d)", dev, n - 1);
+    return MLX_ERR_NODEVICE;
+  }
+  hipDeviceProp_t prop;
+  MLX_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error("device %d is %s; this library is built for gfx950", dev, prop.gcnArchName);
+    return MLX_ERR_NODEVICE;
+  }
+  return MLX_OK;
+}
+
+static inline unsigned blocks_for(int64_t n) {
+  return (unsigned)((n + kBlock - 1) / kBlock);
+}
+
+// ---------------------------------------------------------------------------
+// host-side layer data
+// ---------------------------------------------------------------------------
+
+struct HostMaps {
+  int nSorted = 0;
+  std::vector<uint8_t> elastic;   // nSorted x nT (element order)
+  std::vector<int32_t> vbody;     // nSorted x nV
+  std::vector<uint8_t> boundary;  // nSorted x nV
+  std::vector<int32_t> numRigids; // nSorted
+};
+
+struct HostContacts {
+  int K = 0;
+  std::vector<int32_t> vert;        // 0-based
+  std::vector<double> compliance;
+};
+
+struct SolveParams {
+  int dim = 3;
+  int nLayers = 1;
+  int32_t layersMap0[MLX_MAX_LAYERS];
+  int32_t iterations[MLX_MAX_LAYERS];
+  double h = 0.01;
+  double boundaryCompliance = 1e-8;
+  double contactResistance = 0.5;
+  int computeResiduals = 0, hangingStop = 1000, giveUpEnabled = 0;
+  int lastLayerGiveUpEnabled = 0, useRigidContacts = 1;
+  double giveUpThreshold = 0, stopAtImprovementPercent = 0;
+  double runUntil = -1, residualLast = 0;
+};
+
+struct SolveResult {
+  double runUntil = -1;
+  int residualCount = 1;
+  std::vector<double> residuals;
+  int itersPerLayer[MLX_MAX_LAYERS] = {0};
+  int layersRun = 0;
+  int64_t projections = 0;
+};
+
+static constexpr int kMaxPartials = 65536;
+
+// Upload the 1-based column-major contact arrays of a layerSolve call.
+static int upload_contacts(mlx_mesh &m, int dim, int K, const int32_t *vid1,
+                           const double *normals, const double *points,
+                           const double *compl_, HostContacts &hc) {
+  hc.K = K;
+  hc.vert.resize(K);
+  hc.compliance.resize(K);
+  std::vector<double> nrm(3 * (size_t)K, 0.0), pts(3 * (size_t)K, 0.0);
+  std::vector<uint8_t> seen(m.nV, 0);
+  for (int i = 0; i < K; ++i) {
+    int v = vid1[i] - 1;
+    if (v < 0 || v >= m.nV) {
+      set_error("contactVertexID(%d) = %d out of range", i + 1, vid1[i]);
+      return MLX_ERR_ARG;
+    }
+    if (seen[v]) {
+      set_error("vertex %d has more than one contact; the GPU path requires one contact per vertex",
+                v + 1);
+      return MLX_ERR_UNSUPPORTED;
+    }
+    seen[v] = 1;
+    hc.vert[i] = v;
+    hc.compliance[i] = compl_[i];
+    for (int d = 0; d < dim; ++d) {
+      nrm[3 * i + d] = normals[i + (size_t)K * d];
+      pts[3 * i + d] = points[i + (size_t)K * d];
+    }
+  }
+  size_t bytes = 8 * (size_t)K * 9 + 4 * (size_t)K + 256;
+  MLX_TRY(m.d_contacts.reserve(bytes));
+  char *base = m.d_contacts.as<char>();
+  if (K > 0) {
+    MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+    MLX_CHECK_HIP(hipMemcpy(base, nrm.data(), 24 * (size_t)K, hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemcpy(base + 24 * (size_t)K, pts.data(), 24 * (size_t)K, hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemcpy(base + 48 * (size_t)K, hc.compliance.data(), 8 * (size_t)K,
+                            hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemcpy(base + 64 * (size_t)K, hc.vert.data(), 4 * (size_t)K,
+                            hipMemcpyHostToDevice));
+  }
+  return MLX_OK;
+}
+
+static ContactArrays contact_arrays(mlx_mesh &m, int K) {
+  char *base = m.d_contacts.as<char>();
+  ContactArrays c;
+  c.normal = reinterpret_cast<const double *>(base);
+  c.point = reinterpret_cast<const double *>(base + 24 * (size_t)K);
+  c.compliance = reinterpret_cast<const double *>(base + 48 * (size_t)K);
+  c.lambda = reinterpret_cast<double *>(base + 56 * (size_t)K);
+  c.vert = reinterpret_cast<const int32_t *>(base + 64 * (size_t)K);
+  c.K = K;
+  return c;
+}
+
+template <typename T>
+static SweepParams<T> sweep_params(mlx_mesh &m, double h, const Ctl *ctl) {
+  SweepParams<T> P;
+  P.vid = m.d_vid.as<int32_t>();
+  P.dminv = m.d_dminv.as<T>();
+  P.vol = m.d_vol.as<T>();
+  P.mat = m.d_mat.as<T>();
+  P.lam = m.d_lam.as<T>();
+  P.flags = nullptr;
+  P.x = m.d_x.as<vec4_t<T>>();
+  P.xold = m.d_xold.as<vec4_t<T>>();
+  P.ctl = ctl;
+  P.ld = m.ld;
+  P.begin = 0;
+  P.end = 0;
+  P.ih2 = (T)(1.0 / (h * h));
+  P.ih = (T)(1.0 / h);
+  return P;
+}
+
+template <typename T>
+static void enqueue_sweeps(mlx_mesh &m, SweepParams<T> P, const uint8_t *flags) {
+  P.flags = flags;
+  for (int c = 0; c < m.numColors; ++c) {
+    P.begin = m.colorOff[c];
+    P.end = m.colorOff[c] + m.colorCnt[c];
+    if (P.end <= P.begin) continue;
+    if (m.dim == 3)
+      hipLaunchKernelGGL(k_sweep3d<T>, dim3(blocks_for(P.end - P.begin)), dim3(kBlock), 0,
+                         m.stream, P);
+    else
+      hipLaunchKernelGGL(k_sweep2d<T>, dim3(blocks_for(P.end - P.begin)), dim3(kBlock), 0,
+                         m.stream, P);
+  }
+}
+
+// Per-layer CSR lists, built on the host from the layer maps.
+struct LayerCSR {
+  int nb = 0, nbnd = 0, nrc = 0, nec = 0;
+  std::vector<int32_t> buf;  // rig_off | rig_vert | bnd_off | bnd_vert | rc_off | rc_idx | ec_idx
+  size_t o_rig_off, o_rig_vert, o_bnd_off, o_bnd_vert, o_rc_off, o_rc_idx, o_ec;
+};
+
+static void build_csr(const mlx_mesh &m, const HostMaps &maps, int col, const HostContacts &hc,
+                      LayerCSR &L) {
+  const int64_t nV = m.nV;
+  const int32_t *vb = maps.vbody.data() + (size_t)nV * col;
+  const uint8_t *bd = maps.boundary.data() + (size_t)nV * col;
+  const int nb = maps.numRigids[col];
+  L.nb = nb;
+  std::vector<int32_t> cr(nb + 1, 0), cb(nb + 1, 0), cc(nb + 1, 0);
+  int nrig = 0;
+  for (int64_t v = 0; v < nV; ++v)
+    if (vb[v] > 0) {
+      cr[vb[v]]++;
+      nrig++;
+      if (bd[v]) cb[vb[v]]++;
+    }
+  int nec = 0, nrc = 0;
+  for (int i = 0; i < hc.K; ++i) {
+    int b = vb[hc.vert[i]];
+    if (b > 0) {
+      cc[b]++;
+      nrc++;
+    } else {
+      nec++;
+    }
+  }
+  int nbnd = 0;
+  for (int k = 0; k < nb; ++k) {
+    cr[k + 1] += cr[k];
+    cb[k + 1] += cb[k];
+    cc[k + 1] += cc[k];
+  }
+  nbnd = cb[nb];
+  L.nbnd = nbnd;
+  L.nrc = nrc;
+  L.nec = nec;
+  L.o_rig_off = 0;
+  L.o_rig_vert = L.o_rig_off + nb + 1;
+  L.o_bnd_off = L.o_rig_vert + nrig;
+  L.o_bnd_vert = L.o_bnd_off + nb + 1;
+  L.o_rc_off = L.o_bnd_vert + nbnd;
+  L.o_rc_idx = L.o_rc_off + nb + 1;
+  L.o_ec = L.o_rc_idx + nrc;
+  L.buf.assign(L.o_ec + nec + 1, 0);
+  int32_t *B = L.buf.data();
+  for (int k = 0; k <= nb; ++k) {
+    B[L.o_rig_off + k] = cr[k];
+    B[L.o_bnd_off + k] = cb[k];
+    B[L.o_rc_off + k] = cc[k];
+  }
+  std::vector<int32_t> fr(cr.begin(), cr.end() - 1), fb(cb.begin(), cb.end() - 1),
+      fc(cc.begin(), cc.end() - 1);
+  for (int64_t v = 0; v < nV; ++v)
+    if (vb[v] > 0) {
+      const int k = vb[v] - 1;
+      B[L.o_rig_vert + fr[k]++] = (int32_t)v;
+      if (bd[v]) B[L.o_bnd_vert + fb[k]++] = (int32_t)v;
+    }
+  int e = 0;
+  for (int i = 0; i < hc.K; ++i) {
+    int b = vb[hc.vert[i]];
+    if (b > 0)
+      B[L.o_rc_idx + fc[b - 1]++] = i;
+    else
+      B[L.o_ec + e++] = i;
+  }
+}
+
+static int read_ctl(mlx_mesh &m, Ctl &out) {
+  MLX_CHECK_HIP(hipMemcpyAsync(m.h_ctl, m.d_ctl.ptr, sizeof(Ctl), hipMemcpyDeviceToHost, m.stream));
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  out = *m.h_ctl;
+  return MLX_OK;
+}
+
+// layerSolve on the device (xpbdLayer3D.m:44-232, xpbdLayer2D.m:606-738).
+// Expects d_x (xi), d_xold (oldp), d_lam, d_rv (3D), d_pinnedx and the
+// contact arrays to be on the device already.
+template <typename T>
+static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams &sp,
+                           const HostContacts &hc, SolveResult &res) {
+  const int64_t nV = m.nV, nT = m.nT, ld = m.ld;
+  const int ns = maps.nSorted;
+  hipStream_t st = m.stream;
+  // elastic flags in slot order, one column per sorted layer
+  std::vector<uint8_t> flags((size_t)ns * ld, 0);
+  std::vector<int64_t> nElastic(ns, 0);
+  for (int l = 0; l < ns; ++l)
+    for (int64_t s = 0; s < ld; ++s) {
+      const int32_t e = m.perm[s];
+      if (e < 0) continue;
+      const uint8_t f = maps.elastic[(size_t)nT * l + e] ? 1 : 0;
+      flags[(size_t)ld * l + s] = f;
+      nElastic[l] += f;
+    }
+  MLX_TRY(m.d_flags.reserve(flags.size()));
+  MLX_TRY(m.d_vbody.reserve(sizeof(int32_t) * maps.vbody.size()));
+  MLX_CHECK_HIP(hipMemcpyAsync(m.d_flags.ptr, flags.data(), flags.size(), hipMemcpyHostToDevice, st));
+  MLX_CHECK_HIP(hipMemcpyAsync(m.d_vbody.ptr, maps.vbody.data(), sizeof(int32_t) * maps.vbody.size(),
+                               hipMemcpyHostToDevice, st));
+  int maxNb = 1;
+  for (int l = 0; l < ns; ++l) maxNb = std::max(maxNb, maps.numRigids[l]);
+  MLX_TRY(m.d_bodies.reserve(sizeof(Body) * maxNb));
+  MLX_TRY(m.d_ri.reserve(sizeof(double) * 3 * nV));
+  MLX_TRY(m.d_partials.reserve(sizeof(double) * kMaxPartials));
+  const int residCap = 1 << 16;
+  MLX_TRY(m.d_resid.reserve(sizeof(double) * residCap));
+  MLX_TRY(m.d_ctl.reserve(sizeof(Ctl)));
+  const int K = hc.K;
+  ContactArrays ca = contact_arrays(m, K);
+
+  // distinct contact compliances and their multiplicities (3D residual)
+  std::vector<double> uniq, cnt;
+  for (int i = 0; i < K; ++i) {
+    auto it = std::find(uniq.begin(), uniq.end(), hc.compliance[i]);
+    if (it == uniq.end()) {
+      uniq.push_back(hc.compliance[i]);
+      cnt.push_back(1.0);
+    } else {
+      cnt[it - uniq.begin()] += 1.0;
+    }
+  }
+  const int nUniq = (int)uniq.size();
+  MLX_TRY(m.d_tmp.reserve(sizeof(double) * (2 * nUniq + 2) + sizeof(int32_t) * 3 * (m.dim == 2 ? ld : 1)));
+  double *d_uniq = m.d_tmp.as<double>();
+  double *d_cnt = d_uniq + nUniq + 1;
+  int32_t *d_lamSlot2d = reinterpret_cast<int32_t *>(d_cnt + nUniq + 1);
+  if (nUniq > 0) {
+    MLX_CHECK_HIP(hipMemcpyAsync(d_uniq, uniq.data(), 8 * nUniq, hipMemcpyHostToDevice, st));
+    MLX_CHECK_HIP(hipMemcpyAsync(d_cnt, cnt.data(), 8 * nUniq, hipMemcpyHostToDevice, st));
+  }
+  std::vector<int32_t> lamSlot;
+  if (m.dim == 2 && sp.computeResiduals) {
+    // page e, entry r of reshape(lambdai,3,1,[]) is lambdai(L mod nT, L div nT)
+    lamSlot.assign(3 * (size_t)ld, 0);
+    for (int64_t s = 0; s < ld; ++s) {
+      const int32_t e = m.perm[s];
+      if (e < 0) continue;
+      for (int r = 0; r < 3; ++r) {
+        const int64_t Lin = 3 * (int64_t)e + r;
+        const int64_t el = Lin % nT, comp = Lin / nT;
+        lamSlot[3 * s + r] = (int32_t)(m.slotOf[el] * 3 + comp);
+      }
+    }
+    MLX_CHECK_HIP(hipMemcpyAsync(d_lamSlot2d, lamSlot.data(), sizeof(int32_t) * lamSlot.size(),
+                                 hipMemcpyHostToDevice, st));
+  }
+
+  Ctl c0;
+  std::memset(&c0, 0, sizeof c0);
+  c0.residualCount = 1;
+  c0.stopLayer = -1;
+  c0.runUntil = sp.runUntil;
+  c0.lastResidual = sp.residualLast;
+  MLX_CHECK_HIP(hipMemcpyAsync(m.d_ctl.ptr, &c0, sizeof c0, hipMemcpyHostToDevice, st));
+  MLX_CHECK_HIP(hipMemcpyAsync(m.d_resid.ptr, &sp.residualLast, 8, hipMemcpyHostToDevice, st));
+  Ctl *ctl = m.d_ctl.as<Ctl>();
+  double *partials = m.d_partials.as<double>();
+  Body *bodies = m.d_bodies.as<Body>();
+  double *ri = m.d_ri.as<double>();
+  vec4_t<T> *x = m.d_x.as<vec4_t<T>>();
+  T *rv = m.d_rv.as<T>();
+  const double *minv = m.d_minv64.as<double>();
+  const double *mass = m.d_mass64.as<double>();
+  SweepParams<T> P = sweep_params<T>(m, sp.h, ctl);
+  MonitorParams mp;
+  mp.is3d = m.dim == 3;
+  mp.computeResiduals = sp.computeResiduals;
+  mp.hangingStop = sp.hangingStop;
+  mp.giveUpEnabled = sp.giveUpEnabled;
+  mp.lastLayerGiveUpEnabled = sp.lastLayerGiveUpEnabled;
+  mp.giveUpThreshold = sp.giveUpThreshold;
+  mp.h = sp.h;
+  const int nResElemBlocks = (int)blocks_for(ld);
+  const int nResContactBlocks = K > 0 ? (int)blocks_for(K) : 0;
+  if (nResElemBlocks + nResContactBlocks > kMaxPartials) {
+    set_error("scene too large for the residual partial buffer");
+    return MLX_ERR_UNSUPPORTED;
+  }
+
+  const int64_t nPin = (int64_t)m.pinnedDOF.size();
+  res.projections = 0;
+  res.layersRun = 0;
+  Ctl hc_ctl;
+  for (int layer = 0; layer < sp.nLayers; ++layer) {
+    const int col = sp.layersMap0[layer];
+    const bool isLast = layer == sp.nLayers - 1;
+    const uint8_t *fl = nElastic[col] == nT ? nullptr : m.d_flags.as<uint8_t>() + (size_t)ld * col;
+    const int32_t *vb = m.d_vbody.as<int32_t>() + (size_t)nV * col;
+    LayerCSR L;
+    build_csr(m, maps, col, hc, L);
+    const bool hasRigidEl = nElastic[col] < nT;
+    if (m.dim == 2 && hasRigidEl) {
+      set_error("2D layers with rigid elements are not implemented on the GPU path");
+      return MLX_ERR_UNSUPPORTED;
+    }
+    MLX_TRY(m.d_csr.reserve(sizeof(int32_t) * L.buf.size()));
+    MLX_CHECK_HIP(hipMemcpyAsync(m.d_csr.ptr, L.buf.data(), sizeof(int32_t) * L.buf.size(),
+                                 hipMemcpyHostToDevice, st));
+    const int32_t *cb = m.d_csr.as<int32_t>();
+    LayerLists LL;
+    LL.rig_off = cb + L.o_rig_off;
+    LL.rig_vert = cb + L.o_rig_vert;
+    LL.bnd_off = cb + L.o_bnd_off;
+    LL.bnd_vert = cb + L.o_bnd_vert;
+    LL.rc_off = cb + L.o_rc_off;
+    LL.rc_idx = cb + L.o_rc_idx;
+    LL.ec_idx = cb + L.o_ec;
+    LL.nb = L.nb;
+    LL.nbnd = L.nbnd;
+    LL.nrc = L.nrc;
+    LL.nec = L.nec;
+    MLX_TRY(m.d_lamB.reserve(sizeof(double) * (L.nbnd + 1)));
+    if (L.nbnd > 0)
+      MLX_CHECK_HIP(hipMemsetAsync(m.d_lamB.ptr, 0, sizeof(double) * L.nbnd, st));
+    if (K > 0) MLX_CHECK_HIP(hipMemsetAsync(ca.lambda, 0, sizeof(double) * K, st));
+
+    if (m.dim == 3) {
+      if (L.nb > 0)
+        hipLaunchKernelGGL(k_restrict<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, rv, mass,
+                           bodies, ri);
+      const int step = hasRigidEl && L.nb > 0;
+      if (step)
+        hipLaunchKernelGGL(k_body_step, dim3(blocks_for(L.nb)), dim3(kBlock), 0, st, bodies,
+                           L.nb, sp.h);
+      hipLaunchKernelGGL(k_layer_vertices<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV, vb,
+                         x, rv, bodies, ri, sp.h, step);
+      hipLaunchKernelGGL(k_sumsq<T>, dim3(256), dim3(kBlock), 0, st, rv, 3 * nV, partials);
+      hipLaunchKernelGGL(k_layer_begin, dim3(1), dim3(kBlock), 0, st, ctl, partials, 256, layer,
+                         (int)isLast, layer == 0 ? sp.stopAtImprovementPercent : 0.0);
+    } else {
+      hipLaunchKernelGGL(k_layer_begin, dim3(1), dim3(kBlock), 0, st, ctl, partials, 0, layer,
+                         (int)isLast, layer == 0 ? sp.stopAtImprovementPercent : 0.0);
+      // 2D layerSolve passes resvnorm = inf to projectGS (xpbdLayer2D.m:735)
+      static const double inf = INFINITY;
+      MLX_CHECK_HIP(hipMemcpyAsync(reinterpret_cast<char *>(m.d_ctl.ptr) + offsetof(Ctl, normresv),
+                                   &inf, sizeof(double), hipMemcpyHostToDevice, st));
+    }
+
+    auto enqueue_iteration = [&]() {
+      enqueue_sweeps<T>(m, P, fl);
+      if (m.dim == 3 && L.nbnd > 0)
+        hipLaunchKernelGGL(k_boundary<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, minv, bodies,
+                           ri, m.d_lamB.as<double>(), sp.boundaryCompliance, sp.h, ctl);
+      if (L.nec > 0) {
+        if (m.dim == 3)
+          hipLaunchKernelGGL((k_contacts_elastic<T, 3>), dim3(blocks_for(L.nec)), dim3(kBlock), 0,
+                             st, LL, ca, x, minv, sp.contactResistance, sp.h, ctl);
+        else
+          hipLaunchKernelGGL((k_contacts_elastic<T, 2>), dim3(blocks_for(L.nec)), dim3(kBlock), 0,
+                             st, LL, ca, x, minv, sp.contactResistance, sp.h, ctl);
+      }
+      if (m.dim == 3 && L.nrc > 0 && sp.useRigidContacts)
+        hipLaunchKernelGGL(k_contacts_rigid<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, ca, x, bodies,
+                           ri, sp.contactResistance, sp.h, ctl);
+      if (nPin > 0) {
+        if (m.dim == 3)
+          hipLaunchKernelGGL((k_pins<T, 3>), dim3(blocks_for(nPin)), dim3(kBlock), 0, st, nPin,
+                             m.d_pinned.as<int32_t>(), m.d_pinnedx.as<T>(), x, ctl);
+        else
+          hipLaunchKernelGGL((k_pins<T, 2>), dim3(blocks_for(nPin)), dim3(kBlock), 0, st, nPin,
+                             m.d_pinned.as<int32_t>(), m.d_pinnedx.as<T>(), x, ctl);
+      }
+      int nPart = 0;
+      if (sp.computeResiduals) {
+        if (m.dim == 3)
+          hipLaunchKernelGGL((k_residual_elems<T, 3>), dim3(nResElemBlocks), dim3(kBlock), 0, st,
+                             P, ld, (const int32_t *)nullptr, partials);
+        else
+          hipLaunchKernelGGL((k_residual_elems<T, 2>), dim3(nResElemBlocks), dim3(kBlock), 0, st,
+                             P, ld, (const int32_t *)d_lamSlot2d, partials);
+        nPart = nResElemBlocks;
+        if (K > 0) {
+          if (m.dim == 3)
+            hipLaunchKernelGGL((k_residual_contacts<T, 3>), dim3(nResContactBlocks), dim3(kBlock),
+                               0, st, ca, x, sp.contactResistance, sp.h, d_uniq, d_cnt, nUniq,
+                               partials + nResElemBlocks, ctl);
+          else
+            hipLaunchKernelGGL((k_residual_contacts<T, 2>), dim3(nResContactBlocks), dim3(kBlock),
+                               0, st, ca, x, sp.contactResistance, sp.h, d_uniq, d_cnt, nUniq,
+                               partials + nResElemBlocks, ctl);
+          nPart += nResContactBlocks;
+        }
+      }
+      hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(kBlock), 0, st, ctl, mp, partials, nPart,
+                         m.d_resid.as<double>(), residCap);
+    };
+
+    for (int it = 0; it < sp.iterations[layer]; ++it) enqueue_iteration();
+    MLX_TRY(read_ctl(m, hc_ctl));
+    if (isLast) {
+      // while it <= iterations || (~stopSolve && runUntil >= 0 && isLastLayer)
+      int64_t guard = 0;
+      int chunk = 2;
+      while (!hc_ctl.stopSolve && !hc_ctl.endLayer && hc_ctl.runUntil >= 0) {
+        for (int it = 0; it < chunk; ++it) enqueue_iteration();
+        MLX_TRY(read_ctl(m, hc_ctl));
+        guard += chunk;
+        chunk = std::min(chunk * 2, 16);
+        if (guard > 10000000) {
+          set_error("runUntil loop exceeded 1e7 iterations without converging (hangingStop < 0?)");
+          return MLX_ERR_STATE;
+        }
+      }
+    }
+    if (m.dim == 3 && L.nb > 0) {
+      hipLaunchKernelGGL(k_rotate_all<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV, vb, x,
+                         bodies, ri);
+      if (hasRigidEl)
+        hipLaunchKernelGGL(k_rotate_residual<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV, vb,
+                           rv, bodies);
+    }
+    MLX_CHECK_HIP(hipGetLastError());
+    res.itersPerLayer[layer] = hc_ctl.iters;
+    res.projections += (int64_t)hc_ctl.iters * nElastic[col];
+    res.layersRun = layer + 1;
+    if (hc_ctl.stopSolve) break;
+  }
+  MLX_TRY(read_ctl(m, hc_ctl));
+  res.runUntil = hc_ctl.runUntil;
+  res.residualCount = hc_ctl.residualCount;
+  const int n = std::min(hc_ctl.residualCount, residCap);
+  res.residuals.resize(n);
+  MLX_CHECK_HIP(hipMemcpy(res.residuals.data(), m.d_resid.ptr, sizeof(double) * n,
+                          hipMemcpyDeviceToHost));
+  return MLX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// uploads of per-call arrays
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static int upload_positions(mlx_mesh &m, DevBuf &dst, const double *p) {
+  std::vector<vec4_t<T>> h(m.nV);
+  for (int64_t v = 0; v < m.nV; ++v) {
+    vec4_t<T> q;
+    q.x = (T)p[m.dim * v];
+    q.y = (T)p[m.dim * v + 1];
+    q.z = m.dim == 3 ? (T)p[m.dim * v + 2] : T(0);
+    q.w = (T)(1.0 / m.massV[v]);
+    h[v] = q;
+  }
+  MLX_TRY(dst.reserve(sizeof(vec4_t<T>) * m.nV));
+  MLX_CHECK_HIP(hipMemcpy(dst.ptr, h.data(), sizeof(vec4_t<T>) * m.nV, hipMemcpyHostToDevice));
+  return MLX_OK;
+}
+
+template <typename T>
+static int download_positions(mlx_mesh &m, const DevBuf &src, double *p) {
+  std::vector<vec4_t<T>> h(m.nV);
+  MLX_CHECK_HIP(hipMemcpy(h.data(), src.ptr, sizeof(vec4_t<T>) * m.nV, hipMemcpyDeviceToHost));
+  for (int64_t v = 0; v < m.nV; ++v) {
+    p[m.dim * v] = (double)h[v].x;
+    p[m.dim * v + 1] = (double)h[v].y;
+    if (m.dim == 3) p[m.dim * v + 2] = (double)h[v].z;
+  }
+  return MLX_OK;
+}
+
+// lambdai nT x nc column-major (element order) <-> slots
+template <typename T>
+static int upload_lambda(mlx_mesh &m, const double *lam) {
+  std::vector<T> h((size_t)m.nc * m.ld, T(0));
+  if (lam)
+    for (int64_t e = 0; e < m.nT; ++e)
+      for (int i = 0; i < m.nc; ++i) h[(size_t)i * m.ld + m.slotOf[e]] = (T)lam[e + m.nT * i];
+  MLX_CHECK_HIP(hipMemcpy(m.d_lam.ptr, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return MLX_OK;
+}
+
+template <typename T>
+static int download_lambda(mlx_mesh &m, double *lam) {
+  std::vector<T> h((size_t)m.nc * m.ld);
+  MLX_CHECK_HIP(hipMemcpy(h.data(), m.d_lam.ptr, sizeof(T) * h.size(), hipMemcpyDeviceToHost));
+  for (int64_t e = 0; e < m.nT; ++e)
+    for (int i = 0; i < m.nc; ++i) lam[e + m.nT * i] = (double)h[(size_t)i * m.ld + m.slotOf[e]];
+  return MLX_OK;
+}
+
+template <typename T>
+static int upload_scalars(DevBuf &dst, const double *src, size_t n) {
+  std::vector<T> h(n);
+  for (size_t i = 0; i < n; ++i) h[i] = (T)src[i];
+  MLX_TRY(dst.reserve(sizeof(T) * (n > 0 ? n : 1)));
+  if (n) MLX_CHECK_HIP(hipMemcpy(dst.ptr, h.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+  return MLX_OK;
+}
+
+template <typename T>
+static int download_scalars(const DevBuf &src, double *dst, size_t n) {
+  std::vector<T> h(n);
+  if (n) MLX_CHECK_HIP(hipMemcpy(h.data(), src.ptr, sizeof(T) * n, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) dst[i] = (double)h[i];
+  return MLX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mesh creation
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static int upload_mesh_arrays(mlx_mesh &m, const mlx_mesh_desc &d) {
+  const int64_t ld = m.ld, nT = m.nT;
+  const int dd = m.dim * m.dim;
+  std::vector<int32_t> vid((size_t)m.vpe * ld, -1);
+  std::vector<T> dm((size_t)dd * ld, T(0)), vol(ld, T(0)), mat(4 * (size_t)ld, T(0));
+  for (int64_t s = 0; s < ld; ++s) {
+    const int32_t e = m.perm[s];
+    if (e < 0) continue;
+    for (int k = 0; k < m.vpe; ++k) vid[(size_t)k * ld + s] = m.T[(size_t)e * m.vpe + k];
+    for (int k = 0; k < dd; ++k) dm[(size_t)k * ld + s] = (T)d.DmInv[(size_t)dd * e + k];
+    vol[s] = (T)d.volume[e];
+    for (int k = 0; k < 4; ++k) mat[(size_t)k * ld + s] = (T)m.mat[4 * (size_t)e + k];
+  }
+  MLX_TRY(m.d_vid.reserve(sizeof(int32_t) * vid.size()));
+  MLX_TRY(m.d_dminv.reserve(sizeof(T) * dm.size()));
+  MLX_TRY(m.d_vol.reserve(sizeof(T) * vol.size()));
+  MLX_TRY(m.d_mat.reserve(sizeof(T) * mat.size()));
+  MLX_TRY(m.d_lam.reserve(sizeof(T) * m.nc * ld));
+  MLX_CHECK_HIP(hipMemcpy(m.d_vid.ptr, vid.data(), sizeof(int32_t) * vid.size(), hipMemcpyHostToDevice));
+  MLX_CHECK_HIP(hipMemcpy(m.d_dminv.ptr, dm.data(), sizeof(T) * dm.size(), hipMemcpyHostToDevice));
+  MLX_CHECK_HIP(hipMemcpy(m.d_vol.ptr, vol.data(), sizeof(T) * vol.size(), hipMemcpyHostToDevice));
+  MLX_CHECK_HIP(hipMemcpy(m.d_mat.ptr, mat.data(), sizeof(T) * mat.size(), hipMemcpyHostToDevice));
+  MLX_CHECK_HIP(hipMemset(m.d_lam.ptr, 0, sizeof(T) * m.nc * ld));
+  (void)nT;
+  return MLX_OK;
+}
+
+}  // namespace mlx
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int mlx_init(int32_t device) {
+  MLX_TRY(check_device(device));
+  g_device = device;
+  MLX_CHECK_HIP(hipSetDevice(device));
+  return MLX_OK;
+}
+
+int mlx_mesh_create(const mlx_mesh_desc *d, int32_t precision, mlx_mesh **out) {
+  if (!d || !out || (d->dim != 2 && d->dim != 3) || d->nV <= 0 || d->nT <= 0 || !d->t ||
+      !d->DmInv || !d->alpha || !d->beta || !d->volume || !d->mass || !d->elementColor ||
+      !d->forcedElastic || (d->nPinnedDOFs > 0 && !d->pinnedDOFs) ||
+      (precision != MLX_FP64 && precision != MLX_FP32)) {
+    set_error("mlx_mesh_create: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  MLX_TRY(check_device(g_device));
+  MLX_CHECK_HIP(hipSetDevice(g_device));
+  mlx_mesh *m = new mlx_mesh();
+  auto fail = [&](int code) {
+    mlx_mesh_destroy(m);
+    return code;
+  };
+  m->device = g_device;
+  m->dim = d->dim;
+  m->vpe = d->dim + 1;
+  m->nc = d->dim == 3 ? 6 : 3;
+  m->precision = precision;
+  m->nV = d->nV;
+  m->nT = d->nT;
+  const int64_t nV = d->nV, nT = d->nT;
+  const int vpe = m->vpe, dim = m->dim, nc = m->nc, dd = dim * dim;
+  m->T.resize((size_t)nT * vpe);
+  for (int64_t e = 0; e < nT; ++e)
+    for (int k = 0; k < vpe; ++k) {
+      const int32_t v = d->t[e + nT * k] - 1;
+      if (v < 0 || v >= nV) {
+        set_error("t(%lld,%d) out of range", (long long)e + 1, k + 1);
+        return fail(MLX_ERR_ARG);
+      }
+      m->T[(size_t)e * vpe + k] = v;
+    }
+  m->massV.resize(nV);
+  for (int64_t v = 0; v < nV; ++v) {
+    const double mv = d->mass[dim * v];
+    for (int c = 1; c < dim; ++c)
+      if (d->mass[dim * v + c] != mv) {
+        set_error("vertex %lld has unequal masses on its DOFs", (long long)v + 1);
+        return fail(MLX_ERR_UNSUPPORTED);
+      }
+    if (!(mv > 0)) {
+      set_error("vertex %lld has non-positive mass", (long long)v + 1);
+      return fail(MLX_ERR_ARG);
+    }
+    m->massV[v] = mv;
+  }
+  // material: alpha = inv(K) of the isotropic Voigt form, beta = alpha1 K
+  m->mat.resize(4 * (size_t)nT);
+  const int nn = dim == 3 ? 3 : 2;
+  for (int64_t e = 0; e < nT; ++e) {
+    const double *A = d->alpha + (size_t)nc * nc * e, *B = d->beta + (size_t)nc * nc * e;
+    const double ad = A[0], ao = A[nc], as = A[nn + nc * nn];
+    double a1 = 0;
+    for (int k = 0; k < nc; ++k) a1 += A[nc * k] * B[k];
+    double scale = std::max(std::fabs(ad), std::fabs(as));
+    bool ok = std::isfinite(ad) && std::isfinite(as) && scale > 0;
+    for (int i = 0; i < nc && ok; ++i)
+      for (int j = 0; j < nc && ok; ++j) {
+        double expect = 0;
+        if (i < nn && j < nn) expect = (i == j) ? ad : ao;
+        else if (i == j) expect = as;
+        if (std::fabs(A[i + nc * j] - expect) > 1e-9 * scale) ok = false;
+        double ab = 0;
+        for (int k = 0; k < nc; ++k) ab += A[i + nc * k] * B[k + nc * j];
+        if (std::fabs(ab - (i == j ? a1 : 0.0)) > 1e-9 * std::max(std::fabs(a1), 1e-300)) ok = false;
+      }
+    if (!ok) {
+      set_error("element %lld: alpha/beta are not the isotropic StVK-Voigt pair "
+                "(alpha = inv(K), beta = alpha1*K) that updateMaterials builds",
+                (long long)e + 1);
+      return fail(MLX_ERR_UNSUPPORTED);
+    }
+    m->mat[4 * e] = ad;
+    m->mat[4 * e + 1] = ao;
+    m->mat[4 * e + 2] = as;
+    m->mat[4 * e + 3] = a1;
+  }
+  m->DmInv.assign(d->DmInv, d->DmInv + (size_t)dd * nT);
+  // colors -> slots, each color padded to a multiple of 64 slots
+  int32_t nColors = 0;
+  m->color.resize(nT);
+  for (int64_t e = 0; e < nT; ++e) {
+    const int32_t c = d->elementColor[e];
+    if (c < 1) {
+      set_error("elementColor(%lld) = %d < 1", (long long)e + 1, c);
+      return fail(MLX_ERR_ARG);
+    }
+    m->color[e] = c;
+    nColors = std::max(nColors, c);
+  }
+  // check the coloring is valid: no vertex twice in one color
+  {
+    std::vector<int64_t> byColor(nColors + 1, 0);
+    for (int64_t e = 0; e < nT; ++e) byColor[m->color[e]]++;
+    std::vector<int64_t> start(nColors + 2, 0);
+    for (int c = 1; c <= nColors; ++c) start[c + 1] = start[c] + byColor[c];
+    std::vector<int32_t> order(nT);
+    std::vector<int64_t> fill(start);
+    for (int64_t e = 0; e < nT; ++e) order[fill[m->color[e]]++] = (int32_t)e;
+    std::vector<int32_t> mark((size_t)nV, 0);
+    for (int c = 1; c <= nColors; ++c)
+      for (int64_t q = start[c]; q < start[c + 1]; ++q) {
+        const int32_t e = order[q];
+        for (int k = 0; k < vpe; ++k) {
+          const int32_t v = m->T[(size_t)e * vpe + k];
+          if (mark[v] == c) {
+            set_error("elementColor is not a valid coloring: vertex %d appears twice in color %d",
+                      v + 1, c);
+            return fail(MLX_ERR_ARG);
+          }
+          mark[v] = c;
+        }
+      }
+    m->numColors = nColors;
+    m->colorOff.assign(nColors + 1, 0);
+    m->colorCnt.assign(nColors, 0);
+    int64_t off = 0;
+    for (int c = 1; c <= nColors; ++c) {
+      m->colorOff[c - 1] = off;
+      m->colorCnt[c - 1] = byColor[c];
+      off += (byColor[c] + 63) / 64 * 64;
+    }
+    m->colorOff[nColors] = off;
+    // colorOff marks padded starts; the sweep range ends at start + count
+    m->ld = std::max<int64_t>(off, 64);
+    m->perm.assign(m->ld, -1);
+    m->slotOf.assign(nT, -1);
+    std::vector<int64_t> cur(m->colorOff.begin(), m->colorOff.end());
+    for (int c = 1; c <= nColors; ++c)
+      for (int64_t q = start[c]; q < start[c + 1]; ++q) {
+        const int32_t e = order[q];
+        const int64_t s = cur[c - 1]++;
+        m->perm[s] = e;
+        m->slotOf[e] = (int32_t)s;
+      }
+  }
+  m->pinnedDOF.resize(d->nPinnedDOFs);
+  for (int64_t q = 0; q < d->nPinnedDOFs; ++q) {
+    const int32_t dof = d->pinnedDOFs[q] - 1;
+    if (dof < 0 || dof >= dim * nV) {
+      set_error("pinnedDOFs(%lld) out of range", (long long)q + 1);
+      return fail(MLX_ERR_ARG);
+    }
+    m->pinnedDOF[q] = dof;
+  }
+  m->forcedElastic.assign(d->forcedElastic, d->forcedElastic + nT);
+
+  if (hipStreamCreate(&m->stream) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(MLX_ERR_HIP);
+  }
+  if (hipHostMalloc((void **)&m->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) {
+    m->h_ctl = nullptr;
+    set_error("hipHostMalloc failed");
+    return fail(MLX_ERR_HIP);
+  }
+  int r = precision == MLX_FP64 ? upload_mesh_arrays<double>(*m, *d) : upload_mesh_arrays<float>(*m, *d);
+  if (r != MLX_OK) return fail(r);
+  // element-order arrays for layer building, per-vertex masses, pins
+  if ((r = m->d_T.reserve(sizeof(int32_t) * m->T.size())) != MLX_OK) return fail(r);
+  if ((r = m->d_dminv64.reserve(sizeof(double) * m->DmInv.size())) != MLX_OK) return fail(r);
+  if ((r = m->d_mass64.reserve(sizeof(double) * nV)) != MLX_OK) return fail(r);
+  if ((r = m->d_minv64.reserve(sizeof(double) * nV)) != MLX_OK) return fail(r);
+  if ((r = m->d_pinned.reserve(sizeof(int32_t) * (m->pinnedDOF.size() + 1))) != MLX_OK) return fail(r);
+  if ((r = m->d_slotOf.reserve(sizeof(int32_t) * nT)) != MLX_OK) return fail(r);
+  std::vector<double> minv(nV);
+  for (int64_t v = 0; v < nV; ++v) minv[v] = 1.0 / m->massV[v];
+  if (hipMemcpy(m->d_T.ptr, m->T.data(), sizeof(int32_t) * m->T.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_dminv64.ptr, m->DmInv.data(), sizeof(double) * m->DmInv.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_mass64.ptr, m->massV.data(), sizeof(double) * nV, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_minv64.ptr, minv.data(), sizeof(double) * nV, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_slotOf.ptr, m->slotOf.data(), sizeof(int32_t) * nT, hipMemcpyHostToDevice) != hipSuccess ||
+      (!m->pinnedDOF.empty() &&
+       hipMemcpy(m->d_pinned.ptr, m->pinnedDOF.data(), sizeof(int32_t) * m->pinnedDOF.size(),
+                 hipMemcpyHostToDevice) != hipSuccess)) {
+    set_error("upload of mesh arrays failed");
+    return fail(MLX_ERR_HIP);
+  }
+  const size_t tsz = precision == MLX_FP64 ? 8 : 4;
+  if ((r = m->d_x.reserve(4 * tsz * nV)) != MLX_OK) return fail(r);
+  if ((r = m->d_xold.reserve(4 * tsz * nV)) != MLX_OK) return fail(r);
+  if ((r = m->d_rv.reserve(tsz * 3 * nV)) != MLX_OK) return fail(r);
+  if ((r = m->d_pinnedx.reserve(tsz * (m->pinnedDOF.size() + 1))) != MLX_OK) return fail(r);
+  *out = m;
+  return MLX_OK;
+}
+
+int mlx_mesh_destroy(mlx_mesh *m) {
+  if (!m) return MLX_OK;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  mlx::DevBuf *bufs[] = {&m->d_vid, &m->d_dminv, &m->d_vol, &m->d_mat, &m->d_lam,
+                         &m->d_dminv64, &m->d_T, &m->d_minv64, &m->d_mass64, &m->d_pinned,
+                         &m->d_slotOf, &m->d_x, &m->d_xold, &m->d_rv, &m->d_pinnedx,
+                         &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
+                         &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
+                         &m->d_tmp};
+  for (auto *b : bufs) b->release();
+  if (m->h_ctl) (void)hipHostFree(m->h_ctl);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return MLX_OK;
+}
+
+int mlx_mesh_info(const mlx_mesh *m, int32_t *numColors, int64_t *bucketSizes) {
+  if (!m || !numColors) {
+    set_error("mlx_mesh_info: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  *numColors = m->numColors;
+  if (bucketSizes) {
+    std::vector<int64_t> cnt(m->numColors, 0);
+    for (int64_t e = 0; e < m->nT; ++e) cnt[m->color[e] - 1]++;
+    for (int c = 0; c < m->numColors; ++c) bucketSizes[c] = cnt[c];
+  }
+  return MLX_OK;
+}
+
+int mlx_stvk_voigt_constraints(mlx_mesh *m, const uint8_t *isElastic, double h, double *lambdai,
+                               double *xi, const double *oldp) {
+  if (!m || !isElastic || !lambdai || !xi || !oldp || !(h > 0)) {
+    set_error("mlx_stvk_voigt_constraints: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  MLX_CHECK_HIP(hipSetDevice(m->device));
+  const bool f64 = m->precision == MLX_FP64;
+  std::vector<uint8_t> fl(m->ld, 0);
+  for (int64_t s = 0; s < m->ld; ++s)
+    if (m->perm[s] >= 0) fl[s] = isElastic[m->perm[s]] ? 1 : 0;
+  MLX_TRY(m->d_flags.reserve(m->ld));
+  MLX_CHECK_HIP(hipMemcpy(m->d_flags.ptr, fl.data(), m->ld, hipMemcpyHostToDevice));
+  if (f64) {
+    MLX_TRY(upload_positions<double>(*m, m->d_x, xi));
+    MLX_TRY(upload_positions<double>(*m, m->d_xold, oldp));
+    MLX_TRY(upload_lambda<double>(*m, lambdai));
+    enqueue_sweeps<double>(*m, sweep_params<double>(*m, h, nullptr), m->d_flags.as<uint8_t>());
+  } else {
+    MLX_TRY(upload_positions<float>(*m, m->d_x, xi));
+    MLX_TRY(upload_positions<float>(*m, m->d_xold, oldp));
+    MLX_TRY(upload_lambda<float>(*m, lambdai));
+    enqueue_sweeps<float>(*m, sweep_params<float>(*m, h, nullptr), m->d_flags.as<uint8_t>());
+  }
+  MLX_CHECK_HIP(hipGetLastError());
+  MLX_CHECK_HIP(hipStreamSynchronize(m->stream));
+  if (f64) {
+    MLX_TRY(download_positions<double>(*m, m->d_x, xi));
+    MLX_TRY(download_lambda<double>(*m, lambdai));
+  } else {
+    MLX_TRY(download_positions<float>(*m, m->d_x, xi));
+    MLX_TRY(download_lambda<float>(*m, lambdai));
+  }
+  return MLX_OK;
+}
+
+int mlx_layer_solve(mlx_mesh *m, const mlx_layer_solve_args *a, mlx_layer_solve_out *out) {
+  if (!m || !a || !out || a->nLayers < 1 || a->nLayers > MLX_MAX_LAYERS || a->nSorted < 1 ||
+      !a->layersMap || !a->iterations || !a->isElasticElement || !a->vertexRigidBody ||
+      !a->isBoundaryVertex || !a->numRigids || !a->xi || !a->lambdai || !a->oldp ||
+      !(a->h > 0) || (a->nContacts > 0 && (!a->contactVertexID || !a->contactNormals ||
+                                           !a->pointColliders || !a->contactCompliance)) ||
+      (m->dim == 3 && (!a->residualv || (!m->pinnedDOF.empty() && !a->pinnedx)))) {
+    set_error("mlx_layer_solve: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  MLX_CHECK_HIP(hipSetDevice(m->device));
+  const int64_t nV = m->nV, nT = m->nT;
+  HostMaps maps;
+  maps.nSorted = a->nSorted;
+  maps.elastic.assign(a->isElasticElement, a->isElasticElement + (size_t)nT * a->nSorted);
+  maps.vbody.assign(a->vertexRigidBody, a->vertexRigidBody + (size_t)nV * a->nSorted);
+  maps.boundary.assign(a->isBoundaryVertex, a->isBoundaryVertex + (size_t)nV * a->nSorted);
+  maps.numRigids.assign(a->numRigids, a->numRigids + a->nSorted);
+  for (size_t i = 0; i < maps.vbody.size(); ++i) {
+    const int col = (int)(i / nV);
+    if (maps.vbody[i] < 0 || maps.vbody[i] > maps.numRigids[col]) {
+      set_error("vertexRigidBody out of 0..numRigids");
+      return MLX_ERR_ARG;
+    }
+  }
+  SolveParams sp;
+  sp.dim = m->dim;
+  sp.nLayers = a->nLayers;
+  for (int l = 0; l < a->nLayers; ++l) {
+    sp.layersMap0[l] = a->layersMap[l] - 1;
+    sp.iterations[l] = a->iterations[l];
+    if (sp.layersMap0[l] < 0 || sp.layersMap0[l] >= a->nSorted) {
+      set_error("layersMap(%d) out of range", l + 1);
+      return MLX_ERR_ARG;
+    }
+  }
+  sp.h = a->h;
+  sp.boundaryCompliance = a->boundaryCompliance;
+  sp.contactResistance = a->contactResistance;
+  sp.computeResiduals = a->computeResiduals;
+  sp.hangingStop = a->hangingStop;
+  sp.giveUpEnabled = a->giveUpEnabled;
+  sp.giveUpThreshold = a->giveUpThreshold;
+  sp.lastLayerGiveUpEnabled = a->lastLayerGiveUpEnabled;
+  sp.useRigidContacts = a->useRigidContacts;
+  sp.stopAtImprovementPercent = a->stopAtImprovementPercent;
+  sp.runUntil = a->runUntilResSmallerThan;
+  sp.residualLast = a->residualArrayLast;
+  if (sp.runUntil >= 0 && !sp.computeResiduals) {
+    set_error("runUntilResSmallerThan >= 0 needs computeResiduals (the reference would not terminate)");
+    return MLX_ERR_ARG;
+  }
+  HostContacts hc;
+  MLX_TRY(upload_contacts(*m, m->dim, a->nContacts, a->contactVertexID, a->contactNormals,
+                          a->pointColliders, a->contactCompliance, hc));
+  const bool f64 = m->precision == MLX_FP64;
+  const size_t nPin = m->pinnedDOF.size();
+  std::vector<double> pinv(nPin);
+  for (size_t q = 0; q < nPin; ++q)
+    pinv[q] = m->dim == 3 ? a->pinnedx[q] : a->oldp[m->pinnedDOF[q]];
+  SolveResult res;
+  int r;
+  if (f64) {
+    MLX_TRY(upload_positions<double>(*m, m->d_x, a->xi));
+    MLX_TRY(upload_positions<double>(*m, m->d_xold, a->oldp));
+    MLX_TRY(upload_lambda<double>(*m, a->lambdai));
+    MLX_TRY(upload_scalars<double>(m->d_pinnedx, pinv.data(), nPin));
+    if (m->dim == 3) MLX_TRY(upload_scalars<double>(m->d_rv, a->residualv, 3 * nV));
+    r = layer_solve_dev<double>(*m, maps, sp, hc, res);
+  } else {
+    MLX_TRY(upload_positions<float>(*m, m->d_x, a->xi));
+    MLX_TRY(upload_positions<float>(*m, m->d_xold, a->oldp));
+    MLX_TRY(upload_lambda<float>(*m, a->lambdai));
+    MLX_TRY(upload_scalars<float>(m->d_pinnedx, pinv.data(), nPin));
+    if (m->dim == 3) MLX_TRY(upload_scalars<float>(m->d_rv, a->residualv, 3 * nV));
+    r = layer_solve_dev<float>(*m, maps, sp, hc, res);
+  }
+  if (r != MLX_OK) return r;
+  if (f64) {
+    MLX_TRY(download_positions<double>(*m, m->d_x, a->xi));
+    MLX_TRY(download_lambda<double>(*m, a->lambdai));
+    if (m->dim == 3) MLX_TRY(download_scalars<double>(m->d_rv, a->residualv, 3 * nV));
+  } else {
+    MLX_TRY(download_positions<float>(*m, m->d_x, a->xi));
+    MLX_TRY(download_lambda<float>(*m, a->lambdai));
+    if (m->dim == 3) MLX_TRY(download_scalars<float>(m->d_rv, a->residualv, 3 * nV));
+  }
+  out->runUntilResSmallerThan = res.runUntil;
+  out->residualCount = res.residualCount;
+  if (out->residualArray)
+    for (int i = 0; i < std::min<int>(out->residualCap, (int)res.residuals.size()); ++i)
+      out->residualArray[i] = res.residuals[i];
+  for (int l = 0; l < MLX_MAX_LAYERS; ++l) out->itersPerLayer[l] = res.itersPerLayer[l];
+  out->layersRun = res.layersRun;
+  out->elasticProjections = res.projections;
+  if (out->contactLambdas && a->nContacts > 0) {
+    ContactArrays ca = contact_arrays(*m, a->nContacts);
+    MLX_CHECK_HIP(hipMemcpy(out->contactLambdas, ca.lambda, sizeof(double) * a->nContacts,
+                            hipMemcpyDeviceToHost));
+  }
+  return MLX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// EDiff natives and the device-resident integrate()
+// ---------------------------------------------------------------------------
+
+namespace mlx {
+
+// mexEdiffNorm3D / mexEdiffNorm2D on given F arrays (same expression trees as
+// the fused kernels, no FMA contraction).
+__global__ void k_ediff_from_f3d(int64_t nT, const double *Fa, const double *Fb, double h,
+                                 double *out) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nT) return;
+  const double *a = Fa + 9 * i, *b = Fb + 9 * i;
+  const double a11 = a[0], a21 = a[1], a31 = a[2], a12 = a[3], a22 = a[4], a32 = a[5],
+               a13 = a[6], a23 = a[7], a33 = a[8];
+  const double b11 = b[0], b21 = b[1], b31 = b[2], b12 = b[3], b22 = b[4], b32 = b[5],
+               b13 = b[6], b23 = b[7], b33 = b[8];
+  const double t2 = 1.0 / (h * h);
+  double q1 = (a11 * a12) / 2.0 + (a21 * a22) / 2.0 + (a31 * a32) / 2.0 - (b11 * b12) / 2.0 - (b21 * b22) / 2.0 - (b31 * b32) / 2.0;
+  double q2 = (a11 * a13) / 2.0 + (a21 * a23) / 2.0 + (a31 * a33) / 2.0 - (b11 * b13) / 2.0 - (b21 * b23) / 2.0 - (b31 * b33) / 2.0;
+  double q3 = (a12 * a13) / 2.0 + (a22 * a23) / 2.0 + (a32 * a33) / 2.0 - (b12 * b13) / 2.0 - (b22 * b23) / 2.0 - (b32 * b33) / 2.0;
+  double q4 = (a11 * a11) / 2.0 + (a21 * a21) / 2.0 + (a31 * a31) / 2.0 - (b11 * b11) / 2.0 - (b21 * b21) / 2.0 - (b31 * b31) / 2.0;
+  double q5 = (a12 * a12) / 2.0 + (a22 * a22) / 2.0 + (a32 * a32) / 2.0 - (b12 * b12) / 2.0 - (b22 * b22) / 2.0 - (b32 * b32) / 2.0;
+  double q6 = (a13 * a13) / 2.0 + (a23 * a23) / 2.0 + (a33 * a33) / 2.0 - (b13 * b13) / 2.0 - (b23 * b23) / 2.0 - (b33 * b33) / 2.0;
+  out[i] = t2 * (q1 * q1) * 2.0 + t2 * (q2 * q2) * 2.0 + t2 * (q3 * q3) * 2.0 + t2 * (q4 * q4) +
+           t2 * (q5 * q5) + t2 * (q6 * q6);
+}
+
+__global__ void k_ediff_from_f2d(int64_t nT, const double *Fa, const double *Fb, double h,
+                                 double *out) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nT) return;
+  const double *A = Fa + 4 * i, *B = Fb + 4 * i;
+  const double a = A[0] * A[0] + A[1] * A[1] - B[0] * B[0] - B[1] * B[1];
+  const double b = A[0] * A[2] + A[1] * A[3] - B[0] * B[2] - B[1] * B[3];
+  const double c = A[2] * A[2] + A[3] * A[3] - B[2] * B[2] - B[3] * B[3];
+  out[i] = (0.25 * a * a + 0.5 * b * b + 0.25 * c * c) / h / h;
+}
+
+static int ediff_from_f(int dim, const double *Fa, const double *Fb, int64_t nT, double h,
+                        double *out) {
+  MLX_TRY(check_device(g_device));
+  MLX_CHECK_HIP(hipSetDevice(g_device));
+  const size_t nf = (size_t)(dim == 3 ? 9 : 4) * nT;
+  DevBuf a, b, o;
+  int r = a.reserve(8 * nf);
+  if (r == MLX_OK) r = b.reserve(8 * nf);
+  if (r == MLX_OK) r = o.reserve(8 * (size_t)nT);
+  if (r != MLX_OK) {
+    a.release(); b.release(); o.release();
+    return r;
+  }
+  hipError_t e = hipMemcpy(a.ptr, Fa, 8 * nf, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b.ptr, Fb, 8 * nf, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    if (dim == 3)
+      hipLaunchKernelGGL(k_ediff_from_f3d, dim3(blocks_for(nT)), dim3(kBlock), 0, 0, nT,
+                         a.as<double>(), b.as<double>(), h, o.as<double>());
+    else
+      hipLaunchKernelGGL(k_ediff_from_f2d, dim3(blocks_for(nT)), dim3(kBlock), 0, 0, nT,
+                         a.as<double>(), b.as<double>(), h, o.as<double>());
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, o.ptr, 8 * (size_t)nT, hipMemcpyDeviceToHost);
+  a.release(); b.release(); o.release();
+  if (e != hipSuccess) {
+    set_error("mlx_ediff_norm: %s", hipGetErrorString(e));
+    return MLX_ERR_HIP;
+  }
+  return MLX_OK;
+}
+
+template <typename T, int DIM>
+__global__ void k_gather_pins(int64_t n, const int32_t *dof, const vec4_t<T> *x, T *out) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= n) return;
+  const int d = dof[q];
+  const T *px = reinterpret_cast<const T *>(&x[d / DIM]);
+  out[q] = px[d % DIM];
+}
+
+// cache.oldp = p and pinnedx = p(pinned) when frame == 1 or oldp has a NaN
+// (xpbdLayer3D.m:629-632).
+template <typename T, int DIM>
+__global__ void k_reset_oldp(int64_t nV, const int *flag, int force, const vec4_t<T> *x,
+                             vec4_t<T> *xold, int64_t nPin, const int32_t *dof, T *pinnedx) {
+  if (!force && !*flag) return;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < nV) xold[i] = x[i];
+  if (i < nPin) {
+    const int d = dof[i];
+    const T *px = reinterpret_cast<const T *>(&x[d / DIM]);
+    pinnedx[i] = px[d % DIM];
+  }
+}
+
+// Contact arrays of the plane finder: pointCollider = p - n * distance.
+template <typename T, int DIM>
+__global__ void k_plane_gather(int K, const int32_t *vert, const vec4_t<T> *x, double n0,
+                               double n1, double n2, double D, double compliance,
+                               double *normal, double *point, double *compl_) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= K) return;
+  const vec4_t<T> p = x[vert[i]];
+  const double xs[3] = {(double)p.x, (double)p.y, DIM == 3 ? (double)p.z : 0.0};
+  const double nn[3] = {n0, n1, DIM == 3 ? n2 : 0.0};
+  double dist = xs[0] * n0 + xs[1] * n1;
+  if (DIM == 3) dist = dist + xs[2] * n2;
+  dist = dist - D;
+  for (int d = 0; d < 3; ++d) {
+    normal[3 * i + d] = nn[d];
+    point[3 * i + d] = d < DIM ? xs[d] - nn[d] * dist : 0.0;
+  }
+  compl_[i] = compliance;
+}
+
+// The layer maps of integrate(): strain-rate order (device EDiff, host
+// stable sort), union-find over shared vertices, boundary vertices.
+template <typename T>
+static int build_layer_maps(mlx_state &S, const double *sorted, int ns, double h, HostMaps &maps) {
+  mlx_mesh &m = *S.mesh;
+  const int64_t nT = m.nT, nV = m.nV;
+  maps.nSorted = ns;
+  maps.numRigids.assign(ns, 0);
+  maps.elastic.assign((size_t)ns * nT, 1);
+  maps.vbody.assign((size_t)ns * nV, 0);
+  maps.boundary.assign((size_t)ns * nV, 0);
+  bool any = false;
+  for (int l = 0; l < ns; ++l) any |= std::floor((sorted[l] / 100.0) * (double)nT) > 0;
+  if (!any) return MLX_OK;
+  MLX_TRY(S.d_ediff.reserve(sizeof(double) * nT));
+  if (m.dim == 3)
+    hipLaunchKernelGGL(k_ediff3d<T>, dim3(blocks_for(nT)), dim3(kBlock), 0, m.stream, nT,
+                       m.d_T.as<int32_t>(), m.d_dminv64.as<double>(), m.d_x.as<vec4_t<T>>(),
+                       m.d_xold.as<vec4_t<T>>(), h, S.d_ediff.as<double>());
+  else
+    hipLaunchKernelGGL(k_ediff2d<T>, dim3(blocks_for(nT)), dim3(kBlock), 0, m.stream, nT,
+                       m.d_T.as<int32_t>(), m.d_dminv64.as<double>(), m.d_x.as<vec4_t<T>>(),
+                       m.d_xold.as<vec4_t<T>>(), h, S.d_ediff.as<double>());
+  S.h_ediff.resize(nT);
+  MLX_CHECK_HIP(hipMemcpyAsync(S.h_ediff.data(), S.d_ediff.ptr, sizeof(double) * nT,
+                               hipMemcpyDeviceToHost, m.stream));
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  std::vector<int32_t> order(nT), sets((size_t)nT * ns);
+  stable_order(S.h_ediff.data(), nT, order.data());
+  MLX_TRY(union_find_by_vertice(order.data(), nT, sorted, ns, m.forcedElastic.data(), m.T.data(),
+                                m.vpe, nV, sets.data(), maps.numRigids.data()));
+  std::vector<uint8_t> isRigidVert((size_t)ns * nV);
+  MLX_TRY(find_boundary_vertices(m.T.data(), nT, m.vpe, nV, ns, sets.data(), maps.boundary.data(),
+                                 isRigidVert.data(), maps.vbody.data()));
+  for (size_t i = 0; i < sets.size(); ++i) maps.elastic[i] = sets[i] == 0;
+  return MLX_OK;
+}
+
+template <typename T>
+static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
+                          const mlx_plane *plane, mlx_layer_solve_out *out) {
+  mlx_mesh &m = *S.mesh;
+  const int64_t nV = m.nV, nT = m.nT;
+  const int dim = m.dim;
+  hipStream_t s = m.stream;
+  vec4_t<T> *x = m.d_x.as<vec4_t<T>>();
+  vec4_t<T> *xo = m.d_xold.as<vec4_t<T>>();
+  const int64_t nPin = (int64_t)m.pinnedDOF.size();
+  MLX_TRY(m.d_tmp.reserve(4096));
+  // 1. frame-1 / NaN reset of cache.oldp and pinnedx (3D)
+  if (dim == 3) {
+    int *flag = reinterpret_cast<int *>(m.d_tmp.as<char>() + 4000);
+    MLX_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+    if (frame != 1)
+      hipLaunchKernelGGL(k_any_nan<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, xo, flag);
+    hipLaunchKernelGGL((k_reset_oldp<T, 3>), dim3(blocks_for(std::max(nV, nPin))), dim3(kBlock), 0,
+                       s, nV, flag, frame == 1, x, xo, nPin, m.d_pinned.as<int32_t>(),
+                       m.d_pinnedx.as<T>());
+  }
+  // 2. contacts at the start-of-step positions
+  HostContacts hc;
+  double nrm[3] = {0, 0, 0}, Dp = 0;
+  if (plane && plane->enabled) {
+    double nn = 0;
+    for (int d = 0; d < dim; ++d) nn += plane->normal[d] * plane->normal[d];
+    nn = std::sqrt(nn);
+    for (int d = 0; d < dim; ++d) {
+      nrm[d] = plane->normal[d] / nn;
+      Dp += plane->normal[d] * plane->position[d];
+    }
+    MLX_TRY(S.d_ediff.reserve(sizeof(double) * std::max<int64_t>(nT, nV)));
+    int32_t *fl = S.d_ediff.as<int32_t>();
+    if (dim == 3)
+      hipLaunchKernelGGL((k_plane_flags<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
+                         nrm[0], nrm[1], nrm[2], Dp, 0, fl);
+    else
+      hipLaunchKernelGGL((k_plane_flags<T, 2>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
+                         nrm[0], nrm[1], nrm[2], Dp, 1, fl);
+    std::vector<int32_t> hf(nV);
+    MLX_CHECK_HIP(hipMemcpyAsync(hf.data(), fl, sizeof(int32_t) * nV, hipMemcpyDeviceToHost, s));
+    MLX_CHECK_HIP(hipStreamSynchronize(s));
+    for (int64_t v = 0; v < nV; ++v)
+      if (hf[v]) hc.vert.push_back((int32_t)v);
+    hc.K = (int)hc.vert.size();
+    hc.compliance.assign(hc.K, plane->compliance);
+  }
+  const int K = hc.K;
+  MLX_TRY(m.d_contacts.reserve(8 * (size_t)K * 9 + 4 * (size_t)K + 256));
+  ContactArrays ca = contact_arrays(m, K);
+  if (K > 0) {
+    MLX_CHECK_HIP(hipMemcpyAsync(const_cast<int32_t *>(ca.vert), hc.vert.data(), 4 * (size_t)K,
+                                 hipMemcpyHostToDevice, s));
+    if (dim == 3)
+      hipLaunchKernelGGL((k_plane_gather<T, 3>), dim3(blocks_for(K)), dim3(kBlock), 0, s, K,
+                         ca.vert, x, nrm[0], nrm[1], nrm[2], Dp, plane->compliance,
+                         const_cast<double *>(ca.normal), const_cast<double *>(ca.point),
+                         const_cast<double *>(ca.compliance));
+    else
+      hipLaunchKernelGGL((k_plane_gather<T, 2>), dim3(blocks_for(K)), dim3(kBlock), 0, s, K,
+                         ca.vert, x, nrm[0], nrm[1], nrm[2], Dp, plane->compliance,
+                         const_cast<double *>(ca.normal), const_cast<double *>(ca.point),
+                         const_cast<double *>(ca.compliance));
+  }
+  S.lastContacts = K;
+  // 3. layers (buildLayers + findBoundaryVertices) from p and cache.oldp
+  double sorted[MLX_MAX_LAYERS];
+  int32_t map0[MLX_MAX_LAYERS];
+  int ns = 0;
+  for (int i = 0; i < st->nLayers; ++i) {
+    bool dup = false;
+    for (int j = 0; j < ns; ++j) dup |= sorted[j] == st->layers[i];
+    if (!dup) sorted[ns++] = st->layers[i];
+  }
+  std::sort(sorted, sorted + ns);
+  for (int i = 0; i < st->nLayers; ++i)
+    for (int j = 0; j < ns; ++j)
+      if (sorted[j] == st->layers[i]) map0[i] = j;
+  HostMaps maps;
+  MLX_TRY(build_layer_maps<T>(S, sorted, ns, h, maps));
+  // 4. shift: cache.oldp = p (3D), prevp = p (2D)
+  MLX_CHECK_HIP(hipMemcpyAsync(xo, x, sizeof(vec4_t<T>) * nV, hipMemcpyDeviceToDevice, s));
+  // 5. residual velocity (3D) or explicit predictor (2D)
+  MLX_TRY(m.d_flags.reserve(std::max<size_t>(m.d_flags.bytes, 1)));
+  std::vector<uint8_t> pinMask((size_t)dim * nV, 0);
+  for (auto d : m.pinnedDOF) pinMask[d] = 1;
+  DevBuf &pm = S.d_ediff;  // reuse: layer maps are on the host now
+  MLX_TRY(pm.reserve(std::max(sizeof(double) * nT, pinMask.size())));
+  MLX_CHECK_HIP(hipMemcpyAsync(pm.ptr, pinMask.data(), pinMask.size(), hipMemcpyHostToDevice, s));
+  if (dim == 3) {
+    hipLaunchKernelGGL(k_residual_velocity3d<T>, dim3(blocks_for(3 * nV)), dim3(kBlock), 0, s, nV,
+                       S.d_v.as<T>(), m.d_rv.as<T>(), m.d_mass64.as<double>(), pm.as<uint8_t>(),
+                       h, st->gravity, st->airDrag);
+  } else {
+    hipLaunchKernelGGL(k_predict2d<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, S.d_v.as<T>(),
+                       x, m.d_mass64.as<double>(), pm.as<uint8_t>(), h, st->gravity);
+    // 2D pins snap to prevp
+    if (nPin > 0)
+      hipLaunchKernelGGL((k_gather_pins<T, 2>), dim3(blocks_for(nPin)), dim3(kBlock), 0, s, nPin,
+                         m.d_pinned.as<int32_t>(), xo, m.d_pinnedx.as<T>());
+  }
+  MLX_CHECK_HIP(hipMemsetAsync(m.d_lam.ptr, 0, sizeof(T) * m.nc * m.ld, s));
+  // 6. layerSolve
+  SolveParams sp;
+  sp.dim = dim;
+  sp.nLayers = st->nLayers;
+  for (int l = 0; l < st->nLayers; ++l) {
+    sp.layersMap0[l] = map0[l];
+    sp.iterations[l] = st->iterations[l];
+  }
+  sp.h = h;
+  sp.boundaryCompliance = st->boundaryCompliance;
+  sp.contactResistance = st->contactResistance;
+  sp.computeResiduals = st->computeResiduals;
+  sp.hangingStop = st->hangingStop;
+  sp.giveUpEnabled = st->giveUpEnabled;
+  sp.giveUpThreshold = st->giveUpThreshold;
+  sp.lastLayerGiveUpEnabled = st->lastLayerGiveUpEnabled;
+  sp.useRigidContacts = st->useRigidContacts;
+  sp.stopAtImprovementPercent = st->stopAtImprovementPercent;
+  sp.runUntil = st->runUntilResSmallerThan;
+  sp.residualLast = 0;
+  if (sp.runUntil >= 0 && !sp.computeResiduals) {
+    set_error("runUntilResSmallerThan >= 0 needs computeResiduals");
+    return MLX_ERR_ARG;
+  }
+  SolveResult res;
+  MLX_TRY(layer_solve_dev<T>(m, maps, sp, hc, res));
+  st->runUntilResSmallerThan = res.runUntil;
+  // 7. velocity update and contact velocity update
+  if (dim == 3)
+    hipLaunchKernelGGL((k_velocity<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x, xo,
+                       S.d_v.as<T>(), h);
+  else
+    hipLaunchKernelGGL((k_velocity<T, 2>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x, xo,
+                       S.d_v.as<T>(), h);
+  if (K > 0) {
+    ca = contact_arrays(m, K);
+    if (dim == 3)
+      hipLaunchKernelGGL((k_contact_velocity<T, 3>), dim3(blocks_for(K)), dim3(kBlock), 0, s, ca,
+                         S.d_v.as<T>(), h, st->gravity, st->restitution);
+    else
+      hipLaunchKernelGGL((k_contact_velocity<T, 2>), dim3(blocks_for(K)), dim3(kBlock), 0, s, ca,
+                         S.d_v.as<T>(), h, st->gravity, st->restitution);
+  }
+  MLX_CHECK_HIP(hipGetLastError());
+  MLX_CHECK_HIP(hipStreamSynchronize(s));
+  if (out) {
+    out->runUntilResSmallerThan = res.runUntil;
+    out->residualCount = res.residualCount;
+    if (out->residualArray)
+      for (int i = 0; i < std::min<int>(out->residualCap, (int)res.residuals.size()); ++i)
+        out->residualArray[i] = res.residuals[i];
+    for (int l = 0; l < MLX_MAX_LAYERS; ++l) out->itersPerLayer[l] = res.itersPerLayer[l];
+    out->layersRun = res.layersRun;
+    out->elasticProjections = res.projections;
+    if (out->contactLambdas && K > 0)
+      MLX_CHECK_HIP(hipMemcpy(out->contactLambdas, ca.lambda, sizeof(double) * K,
+                              hipMemcpyDeviceToHost));
+  }
+  return MLX_OK;
+}
+
+}  // namespace mlx
+
+extern "C" {
+
+int mlx_ediff_norm3d(const double *Fa, const double *Fb, int64_t nT, double h, double *out) {
+  if (!Fa || !Fb || !out || nT < 0 || !(h > 0)) {
+    set_error("mlx_ediff_norm3d: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  if (nT == 0) return MLX_OK;
+  return ediff_from_f(3, Fa, Fb, nT, h, out);
+}
+
+int mlx_ediff_norm2d(const double *Fa, const double *Fb, int64_t nT, double h, double *out) {
+  if (!Fa || !Fb || !out || nT < 0 || !(h > 0)) {
+    set_error("mlx_ediff_norm2d: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  if (nT == 0) return MLX_OK;
+  return ediff_from_f(2, Fa, Fb, nT, h, out);
+}
+
+int mlx_state_create(mlx_mesh *m, const double *p, const double *v, mlx_state **out) {
+  if (!m || !p || !v || !out) {
+    set_error("mlx_state_create: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  MLX_CHECK_HIP(hipSetDevice(m->device));
+  mlx_state *S = new mlx_state();
+  S->mesh = m;
+  int r;
+  const size_t n = (size_t)m->dim * m->nV;
+  if (m->precision == MLX_FP64) {
+    r = upload_positions<double>(*m, m->d_x, p);
+    if (r == MLX_OK) r = upload_positions<double>(*m, m->d_xold, p);
+    if (r == MLX_OK) r = upload_scalars<double>(S->d_v, v, n);
+  } else {
+    r = upload_positions<float>(*m, m->d_x, p);
+    if (r == MLX_OK) r = upload_positions<float>(*m, m->d_xold, p);
+    if (r == MLX_OK) r = upload_scalars<float>(S->d_v, v, n);
+  }
+  if (r == MLX_OK) {
+    const size_t tsz = m->precision == MLX_FP64 ? 8 : 4;
+    r = m->d_rv.reserve(tsz * 3 * m->nV);
+    if (r == MLX_OK && hipMemset(m->d_rv.ptr, 0, tsz * 3 * m->nV) != hipSuccess) r = MLX_ERR_HIP;
+  }
+  if (r != MLX_OK) {
+    mlx_state_destroy(S);
+    return r;
+  }
+  *out = S;
+  return MLX_OK;
+}
+
+int mlx_state_destroy(mlx_state *S) {
+  if (!S) return MLX_OK;
+  S->d_v.release();
+  S->d_ediff.release();
+  delete S;
+  return MLX_OK;
+}
+
+int mlx_state_download(mlx_state *S, double *p, double *v, double *lambdai, int32_t *nContacts) {
+  if (!S) {
+    set_error("mlx_state_download: null state");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *S->mesh;
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  const size_t n = (size_t)m.dim * m.nV;
+  if (m.precision == MLX_FP64) {
+    if (p) MLX_TRY(download_positions<double>(m, m.d_x, p));
+    if (v) MLX_TRY(download_scalars<double>(S->d_v, v, n));
+    if (lambdai) MLX_TRY(download_lambda<double>(m, lambdai));
+  } else {
+    if (p) MLX_TRY(download_positions<float>(m, m.d_x, p));
+    if (v) MLX_TRY(download_scalars<float>(S->d_v, v, n));
+    if (lambdai) MLX_TRY(download_lambda<float>(m, lambdai));
+  }
+  if (nContacts) *nContacts = S->lastContacts;
+  return MLX_OK;
+}
+
+int mlx_integrate(mlx_state *S, mlx_settings *st, double h, int32_t frame, const mlx_plane *plane,
+                  mlx_layer_solve_out *out) {
+  if (!S || !st || !(h > 0) || frame < 1 || st->nLayers < 1 || st->nLayers > MLX_MAX_LAYERS) {
+    set_error("mlx_integrate: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  MLX_CHECK_HIP(hipSetDevice(S->mesh->device));
+  if (S->mesh->precision == MLX_FP64) return integrate_impl<double>(*S, st, h, frame, plane, out);
+  return integrate_impl<float>(*S, st, h, frame, plane, out);
+}
+
+}  // extern "C"

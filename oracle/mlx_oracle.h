@@ -1,0 +1,176 @@
+/*
+ * mlx_oracle.h -- CPU FP64 restatement of the multi-layer XPBD substep of
+ * AlexandreMercierAubin/MultiLayerSolver2024.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * product path.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only to check or to time the CPU
+ * baseline.  The product path never links, loads or calls it.
+ *
+ * Conventions: C arrays, 0-based indices, element arrays row-major
+ * (T[e*vpe + k]); small per-element matrices stored like a Matlab page,
+ * i.e. column-major (M[i + n*j]).  All arithmetic is double; the file is
+ * compiled with -ffp-contract=off so that no FMA changes the expression
+ * trees that the reference evaluates.
+ */
+#ifndef MLX_ORACLE_H
+#define MLX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORA_MAX_LAYERS 16
+
+/* Static scene data (the fields xpbdLayer{2,3}D.layerSolve receives). */
+typedef struct {
+  int dim;            /* 2 or 3 */
+  int nV, nT, vpe;    /* vertices, elements, vertices per element (dim+1) */
+  const int32_t *T;   /* nT*vpe, 0-based */
+  const double *DmInv;/* nT*dim*dim column-major pages */
+  const double *alpha;/* nT*nc*nc (nc = 3 in 2D, 6 in 3D) */
+  const double *beta; /* nT*nc*nc */
+  const double *vol;  /* nT (area in 2D) */
+  const double *mass; /* dim*nV, per DOF */
+  const int32_t *color; /* nT, 1-based graph colors */
+  int numColors;
+  const int32_t *pinnedDOFs; int nPinnedDOFs; /* 0-based, ascending */
+  const uint8_t *forcedElastic;                /* nT (pinned elements) */
+} ora_mesh;
+
+typedef struct {
+  int nLayers;
+  double layers[ORA_MAX_LAYERS];   /* rigid percentage per layer, as given */
+  int iterations[ORA_MAX_LAYERS];
+  double boundaryCompliance;
+  double contactResistance;
+  double gravity;
+  double airDrag;
+  int computeResiduals;
+  double runUntil;                 /* runUntilResSmallerThan (in/out) */
+  int hangingStop;
+  int giveUpEnabled;
+  double giveUpThreshold;
+  int lastLayerGiveUpEnabled;
+  double stopAtImprovementPercent;
+  int useRigidContacts;
+  double restitution;              /* contactVelocityUpdate e */
+} ora_settings;
+
+typedef struct {
+  int K;                    /* number of contacts */
+  const int32_t *vertex;    /* K, 0-based vertex ids */
+  const double *normal;     /* K*dim row-major */
+  const double *point;      /* K*dim pointCollider row-major */
+  const double *compliance; /* K */
+} ora_contacts;
+
+/* Per-substep diagnostics. */
+typedef struct {
+  double *lambda;          /* out: nT*nc row-major (nc per element), optional */
+  double *residualArray;   /* out: capacity cap, entry 0 is the initial 0 */
+  int residualCap;
+  int residualCount;       /* out */
+  int itersPerLayer[ORA_MAX_LAYERS]; /* out */
+  int layersRun;           /* out */
+  int64_t elasticProjections; /* out: elastic element updates summed */
+  int32_t *setsPerLayer;   /* out optional: nT*nSorted column-major */
+  int32_t *numRigids;      /* out optional: nSorted */
+} ora_step_out;
+
+/* ---- precompute (Mesh3D / Mesh constructors) ---- */
+int ora_naive_graph_coloring(const int32_t *T, int nT, int vpe, int nV,
+                             int32_t *color);
+void ora_to_lame(double nu, double E, double *mu, double *lambda);
+int ora_precompute3d(int nV, int nT, const double *p, const int32_t *T,
+                     const double *rho, const double *mu, const double *lam,
+                     const double *alpha1, double *DmInv, double *vol,
+                     double *mass, double *alpha, double *beta);
+int ora_precompute2d(int nV, int nT, const double *p, const int32_t *T,
+                     const double *rho, const double *mu, const double *lam,
+                     const double *alpha1, double *DmInv, double *area,
+                     double *mass, double *alpha, double *beta);
+int ora_pinned_elements(const int32_t *T, int nT, int vpe, int nV,
+                        const uint8_t *pinnedVert, uint8_t *forcedElastic);
+
+/* ---- layer building ---- */
+void ora_b_times_p3d(const int32_t *T, int nT, const double *DmInv,
+                     const double *p, double *F);
+void ora_b_times_p2d(const int32_t *T, int nT, const double *DmInv,
+                     const double *p, double *F);
+void ora_ediff_norm3d(const double *Fa, const double *Fb, int nT, double h,
+                      double *out);
+void ora_ediff_norm2d(const double *Fa, const double *Fb, int nT, double h,
+                      double *out);
+void ora_stable_sort_ascend(const double *keys, int n, int32_t *order);
+int ora_union_find_by_vertice(const int32_t *order, int nT,
+                              const double *sortedLayers, int L,
+                              const uint8_t *forcedElastic, const int32_t *T,
+                              int vpe, int nV, int32_t *sets,
+                              int32_t *numRigids);
+void ora_find_boundary_vertices(int nV, int nT, int vpe, const int32_t *T,
+                                int L, const int32_t *sets,
+                                uint8_t *isBoundary, uint8_t *isRigidVert,
+                                int32_t *vertexRigidBody);
+int ora_sorted_layers(const double *layers, int n, double *sorted,
+                      int32_t *map);
+
+/* ---- single-element kernels ---- */
+void ora_stvk_grad3d(const double *DmInv, const double *x, double *gradC,
+                     double *C);
+void ora_stvk_voigt_constraint3d(const double *x, const double *xprev,
+                                 const double *DmInv, const double *alphaT,
+                                 const double *lambda, const double *minv,
+                                 double volume, const double *betaT, double h,
+                                 double *dx, double *dlambda, double *C);
+void ora_stvk_grad2d(const double *DmInv, const double *x, double *gradC,
+                     double *C);
+void ora_stvk_voigt_constraint2d(const double *x, const double *xprev,
+                                 const double *DmInv, const double *alphaT,
+                                 const double *lambda, const double *minv,
+                                 double volume, const double *betaT, double h,
+                                 double *dx, double *dlambda, double *C);
+void ora_exp_rodrigues(const double *omega, double h, double *R);
+
+/* ---- sweeps ---- */
+void ora_stvk_voigt_constraints3d(const ora_mesh *m, const uint8_t *isElastic,
+                                  double h, double *lambda, double *xi,
+                                  const double *oldp);
+void ora_stvk_voigt_constraints2d(const ora_mesh *m, const uint8_t *isElastic,
+                                  double h, double *lambda, double *xi,
+                                  const double *oldp);
+double ora_residual3d(const ora_mesh *m, double h, const double *lambda,
+                      const double *xi, const double *oldp,
+                      const ora_contacts *c, const double *contactLambda,
+                      double contactResistance);
+double ora_residual2d(const ora_mesh *m, double h, const double *lambda,
+                      const double *xi, const double *oldp,
+                      const ora_contacts *c, const double *contactLambda,
+                      double contactResistance);
+
+/* ---- contacts ---- */
+int ora_plane_contacts(int dim, int nV, const double *p, const double *normal,
+                       const double *position, double compliance,
+                       int strict, int32_t *vertex, double *cnormal,
+                       double *cpoint, double *ccompliance);
+void ora_contact_velocity_update(int dim, const ora_contacts *c, double *v,
+                                 double h, double gravity, double e);
+
+/* ---- full substep ---- */
+/* 3D: p,v (3nV) in/out; oldp = cache.oldp in/out; oldOldp out;
+ * pinnedx (nPinnedDOFs) in/out; frame is 1-based. */
+int ora_integrate3d(const ora_mesh *m, ora_settings *s, double h, int frame,
+                    double *p, double *v, double *oldp, double *oldOldp,
+                    double *pinnedx, const ora_contacts *c,
+                    ora_step_out *out);
+/* 2D: prevp/prevPrevp are shifted by the caller (simulate.m:332-334). */
+int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h,
+                    double *p, double *v, const double *prevp,
+                    const double *prevPrevp, const ora_contacts *c,
+                    ora_step_out *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,215 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libmlxoracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product path (multilayersolver2024_amd) never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libmlxoracle.so")
+MAX_LAYERS = 16
+
+_d = C.POINTER(C.c_double)
+_i32 = C.POINTER(C.c_int32)
+_u8 = C.POINTER(C.c_uint8)
+
+
+class OraMesh(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("nV", C.c_int), ("nT", C.c_int), ("vpe", C.c_int),
+        ("T", _i32), ("DmInv", _d), ("alpha", _d), ("beta", _d), ("vol", _d), ("mass", _d),
+        ("color", _i32), ("numColors", C.c_int), ("pinnedDOFs", _i32), ("nPinnedDOFs", C.c_int),
+        ("forcedElastic", _u8),
+    ]
+
+
+class OraSettings(C.Structure):
+    _fields_ = [
+        ("nLayers", C.c_int), ("layers", C.c_double * MAX_LAYERS),
+        ("iterations", C.c_int * MAX_LAYERS), ("boundaryCompliance", C.c_double),
+        ("contactResistance", C.c_double), ("gravity", C.c_double), ("airDrag", C.c_double),
+        ("computeResiduals", C.c_int), ("runUntil", C.c_double), ("hangingStop", C.c_int),
+        ("giveUpEnabled", C.c_int), ("giveUpThreshold", C.c_double),
+        ("lastLayerGiveUpEnabled", C.c_int), ("stopAtImprovementPercent", C.c_double),
+        ("useRigidContacts", C.c_int), ("restitution", C.c_double),
+    ]
+
+
+class OraContacts(C.Structure):
+    _fields_ = [("K", C.c_int), ("vertex", _i32), ("normal", _d), ("point", _d),
+                ("compliance", _d)]
+
+
+class OraStepOut(C.Structure):
+    _fields_ = [
+        ("lambda_", _d), ("residualArray", _d), ("residualCap", C.c_int),
+        ("residualCount", C.c_int), ("itersPerLayer", C.c_int * MAX_LAYERS),
+        ("layersRun", C.c_int), ("elasticProjections", C.c_int64),
+        ("setsPerLayer", _i32), ("numRigids", _i32),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _lib.ora_residual3d.restype = C.c_double
+        _lib.ora_residual2d.restype = C.c_double
+    return _lib
+
+
+def P(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+class OracleMesh:
+    """ora_mesh built from a product Mesh (Matlab layout -> C layout)."""
+
+    def __init__(self, mesh):
+        self.m = mesh
+        dim, nT, nV = mesh.dim, mesh.nT, mesh.nV
+        self.T = np.ascontiguousarray(mesh.T0, dtype=np.int32)
+        self.DmInv = np.ascontiguousarray(mesh.DmInv)
+        self.alpha = np.ascontiguousarray(mesh.alpha)
+        self.beta = np.ascontiguousarray(mesh.beta)
+        self.vol = np.ascontiguousarray(mesh.elV)
+        self.mass = np.ascontiguousarray(mesh.mass)
+        self.color = np.ascontiguousarray(mesh.elementColor, dtype=np.int32)
+        self.pinned = np.ascontiguousarray(mesh.pinnedDOFs - 1, dtype=np.int32)
+        self.forced = np.ascontiguousarray(mesh.forcedElastic, dtype=np.uint8)
+        s = OraMesh()
+        s.dim, s.nV, s.nT, s.vpe = dim, nV, nT, dim + 1
+        s.T, s.DmInv, s.alpha, s.beta = P(self.T, C.c_int32), P(self.DmInv), P(self.alpha), P(self.beta)
+        s.vol, s.mass = P(self.vol), P(self.mass)
+        s.color, s.numColors = P(self.color, C.c_int32), int(mesh.numColors)
+        s.pinnedDOFs, s.nPinnedDOFs = P(self.pinned, C.c_int32), len(self.pinned)
+        s.forcedElastic = P(self.forced, C.c_uint8)
+        self.s = s
+
+
+def settings_from(integ):
+    s = OraSettings()
+    s.nLayers = len(integ.layers)
+    for i, (l, it) in enumerate(zip(integ.layers, integ.iterations)):
+        s.layers[i] = l
+        s.iterations[i] = int(it)
+    s.boundaryCompliance = integ.boundaryCompliance
+    s.contactResistance = integ.contactResistance
+    s.gravity = integ.Gravity
+    s.airDrag = getattr(integ, "airDrag", 0.0)
+    s.computeResiduals = int(bool(integ.computeResiduals))
+    s.runUntil = integ.runUntilResSmallerThan
+    s.hangingStop = int(integ.hangingStop)
+    s.giveUpEnabled = int(bool(integ.giveUpEnabled))
+    s.giveUpThreshold = integ.giveUpThreshold
+    s.lastLayerGiveUpEnabled = int(bool(integ.LastLayerGiveUpEnabled))
+    s.stopAtImprovementPercent = integ.StopAtImprovementPercent
+    s.useRigidContacts = int(bool(getattr(integ, "useRigidContacts", True)))
+    s.restitution = integ.restitution
+    return s
+
+
+def plane_contacts(dim, p, normal, position, compliance, strict):
+    nV = len(p) // dim
+    vert = np.zeros(nV, dtype=np.int32)
+    cn = np.zeros(nV * dim)
+    cp = np.zeros(nV * dim)
+    cc = np.zeros(nV)
+    n = np.asarray(normal, dtype=np.float64)[:dim]
+    n = n / np.linalg.norm(n)
+    pos = np.asarray(position, dtype=np.float64)[:dim]
+    # D uses the normal as given (PlaneContactFinder3D.m:21); callers pass
+    # unit normals, for which the two coincide.
+    K = lib().ora_plane_contacts(dim, nV, P(np.ascontiguousarray(p)), P(n), P(pos),
+                                 C.c_double(compliance), int(strict), P(vert, C.c_int32), P(cn),
+                                 P(cp), P(cc))
+    return dict(vertex=vert[:K].copy(), normal=cn[:K * dim].copy(), point=cp[:K * dim].copy(),
+                compliance=cc[:K].copy(), K=K)
+
+
+class OracleSim:
+    """The oracle's integrate() loop for one mesh (simulate3D / simulate step
+    order, contacts at the start of each step)."""
+
+    def __init__(self, mesh, integ, h, contact=None, v0=None, residual_cap=4096):
+        self.om = OracleMesh(mesh)
+        self.integ = integ
+        self.h = h
+        self.contact = contact
+        self.dim = mesh.dim
+        self.p = mesh.p.copy()
+        self.v = np.zeros_like(self.p) if v0 is None else np.array(v0, dtype=np.float64)
+        self.oldp = self.p.copy()       # cache.oldp (3D) / prevPrevp source (2D)
+        self.oldOldp = self.p.copy()
+        self.pinnedx = np.zeros(len(mesh.pinnedDOFs))
+        self.frame = 0
+        self.s = settings_from(integ)
+        self.resid = np.zeros(residual_cap)
+        self.lam = np.zeros(mesh.nT * mesh.nc)
+        self.last_contacts = 0
+
+    def _contacts(self):
+        if self.contact is None:
+            return None, None
+        c = plane_contacts(self.dim, self.p, self.contact.normal, self.contact.position,
+                           self.contact.compliance, strict=(self.dim == 2))
+        self.last_contacts = c["K"]
+        keep = c
+        oc = OraContacts()
+        oc.K = c["K"]
+        nrm = np.zeros(c["K"] * 3)
+        pts = np.zeros(c["K"] * 3)
+        if self.dim == 3:
+            nrm, pts = c["normal"], c["point"]
+        else:
+            nrm, pts = c["normal"], c["point"]
+        keep["_n"], keep["_p"] = np.ascontiguousarray(nrm), np.ascontiguousarray(pts)
+        oc.vertex = P(c["vertex"], C.c_int32)
+        oc.normal, oc.point = P(keep["_n"]), P(keep["_p"])
+        oc.compliance = P(c["compliance"])
+        return oc, keep
+
+    def step(self):
+        self.frame += 1
+        oc, keep = self._contacts()
+        out = OraStepOut()
+        out.lambda_ = P(self.lam)
+        out.residualArray = P(self.resid)
+        out.residualCap = len(self.resid)
+        L = lib()
+        if self.dim == 3:
+            r = L.ora_integrate3d(C.byref(self.om.s), C.byref(self.s), C.c_double(self.h),
+                                  self.frame, P(self.p), P(self.v), P(self.oldp), P(self.oldOldp),
+                                  P(self.pinnedx), C.byref(oc) if oc else None, C.byref(out))
+        else:
+            prevPrevp = self.oldp.copy()
+            prevp = self.p.copy()
+            r = L.ora_integrate2d(C.byref(self.om.s), C.byref(self.s), C.c_double(self.h),
+                                  P(self.p), P(self.v), P(prevp), P(prevPrevp),
+                                  C.byref(oc) if oc else None, C.byref(out))
+            self.oldp = prevp
+        if r != 0:
+            raise RuntimeError(f"oracle integrate returned {r}")
+        self.integ_runUntil = self.s.runUntil
+        n = min(out.residualCount, len(self.resid))
+        return dict(residuals=self.resid[:n].copy(), iters=list(out.itersPerLayer[:out.layersRun]),
+                    projections=int(out.elasticProjections), runUntil=self.s.runUntil)
+
+    def lambdas(self):
+        m = self.om.m
+        return self.lam.reshape(m.nT, m.nc).copy()
