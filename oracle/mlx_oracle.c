@@ -1571,3 +1571,56 @@ int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h, double *p,
   layers_free(&L);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* test hooks: the rigid-layer steps on explicit body state             */
+/* ------------------------------------------------------------------ */
+
+static void bodies_view(bodies_t *b, int nb, double *M, double *COM, double *COMDOT,
+                        double *R, double *I0, double *I, double *omega) {
+  b->nb = nb; b->M = M; b->COM = COM; b->COMDOT = COMDOT; b->R = R; b->I0 = I0;
+  b->I = I; b->omega = omega;
+}
+
+/* makeRigidsFromConnectivity on explicit arrays (vb: 1-based body per
+ * vertex, 0 = not rigid).  Outputs per body M, COM, COMDOT, I0 (= I),
+ * omega, R (= eye) and ri (3nV). */
+void ora_rigid_properties3d(int nV, int nb, const int32_t *vb, const double *p,
+                            const double *v, const double *mass, double *M,
+                            double *COM, double *COMDOT, double *R, double *I0,
+                            double *I, double *omega, double *ri) {
+  bodies_t b;
+  bodies_view(&b, nb, M, COM, COMDOT, R, I0, I, omega);
+  rigid_properties3d(nV, vb, p, v, mass, &b, ri);
+}
+
+/* rigidSymplecticImperativeStep on explicit arrays (one body). */
+void ora_rigid_step3d(double h, double *COM, const double *COMDOT, double *R,
+                      const double *omega, const double *I0, double *I) {
+  for (int d = 0; d < 3; ++d) COM[d] = COM[d] + h * COMDOT[d];
+  double dR[9], Rn[9], tmp[9], Rt[9];
+  ora_exp_rodrigues(omega, h, dR);
+  matmul_n(3, dR, R, Rn);
+  memcpy(R, Rn, sizeof Rn);
+  matmul_n(3, R, I0, tmp);
+  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) Rt[i + 3 * j] = R[j + 3 * i];
+  matmul_n(3, tmp, Rt, I);
+}
+
+void ora_boundary3d(int nb, double *M, double *COM, double *R, double *I0, double *I,
+                    int nbv, const int32_t *bverts, const int32_t *vb, const double *ri,
+                    const double *mass, double compliance, double *lamB, double *xi,
+                    double h) {
+  bodies_t b;
+  bodies_view(&b, nb, M, COM, NULL, R, I0, I, NULL);
+  boundary3d(&b, nbv, bverts, vb, ri, mass, compliance, lamB, xi, h);
+}
+
+void ora_contacts3d(int nb, double *M, double *COM, double *R, double *I0, double *I,
+                    const ora_contacts *c, const int32_t *vb, const double *ri,
+                    const double *mass, double *cl, double *xi, double h,
+                    double resistance, int useRigid) {
+  bodies_t b;
+  bodies_view(&b, nb, M, COM, NULL, R, I0, I, NULL);
+  contacts3d(&b, c, vb, ri, mass, cl, xi, h, resistance, useRigid);
+}

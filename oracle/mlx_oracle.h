@@ -157,6 +157,22 @@ int ora_plane_contacts(int dim, int nV, const double *p, const double *normal,
 void ora_contact_velocity_update(int dim, const ora_contacts *c, double *v,
                                  double h, double gravity, double e);
 
+/* ---- rigid layer on explicit state (test hooks) ---- */
+void ora_rigid_properties3d(int nV, int nb, const int32_t *vb, const double *p,
+                            const double *v, const double *mass, double *M,
+                            double *COM, double *COMDOT, double *R, double *I0,
+                            double *I, double *omega, double *ri);
+void ora_rigid_step3d(double h, double *COM, const double *COMDOT, double *R,
+                      const double *omega, const double *I0, double *I);
+void ora_boundary3d(int nb, double *M, double *COM, double *R, double *I0, double *I,
+                    int nbv, const int32_t *bverts, const int32_t *vb, const double *ri,
+                    const double *mass, double compliance, double *lamB, double *xi,
+                    double h);
+void ora_contacts3d(int nb, double *M, double *COM, double *R, double *I0, double *I,
+                    const ora_contacts *c, const int32_t *vb, const double *ri,
+                    const double *mass, double *cl, double *xi, double h,
+                    double resistance, int useRigid);
+
 /* ---- full substep ---- */
 /* 3D: p,v (3nV) in/out; oldp = cache.oldp in/out; oldOldp out;
  * pinnedx (nPinnedDOFs) in/out; frame is 1-based. */
