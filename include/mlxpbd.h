@@ -270,6 +270,66 @@ int mlx_integrate(mlx_state *state, mlx_settings *settings, double h,
                   int32_t frame, const mlx_plane *plane,
                   mlx_layer_solve_out *out);
 
+/* ------------------------------------------------------------------ */
+/* Many independent scenes in one launch                                */
+/* ------------------------------------------------------------------ */
+
+typedef struct mlx_batch mlx_batch;
+
+/* merged: the concatenation of nScenes 3D scenes in the layout of
+ * mlx_mesh_desc, as Mesh3D.mergeMesh builds it (3d/@Mesh3D/Mesh3D.m:300-402:
+ * vertex ids offset per scene, each scene keeps its own coloring, pinned DOFs
+ * ascending).  sceneVertOff / sceneElemOff: nScenes+1 offsets (0-based).
+ * Each scene is integrated independently, with its own residual and stop
+ * rule; no extension of the reference (whose merged mesh shares one
+ * residual) beyond that.  Plain method only (every layer 0 %% rigid). */
+int mlx_batch_create(const mlx_mesh_desc *merged, int32_t nScenes,
+                     const int64_t *sceneVertOff, const int64_t *sceneElemOff,
+                     int32_t precision, mlx_batch **out);
+int mlx_batch_destroy(mlx_batch *batch);
+/* Threads per scene workgroup: 256, 512 (default) or 1024. */
+int mlx_batch_set_block(mlx_batch *batch, int32_t threads);
+/* (Re)starts all scenes from p, v (3nV of the merged mesh). */
+int mlx_batch_set_state(mlx_batch *batch, const double *p, const double *v);
+int mlx_batch_download(mlx_batch *batch, double *p, double *v, double *lambdai);
+
+typedef struct {
+  int32_t *iterations;        /* optional nScenes: iterations run */
+  double *lastResidual;       /* optional nScenes */
+  int32_t *contacts;          /* optional nScenes */
+  int64_t elasticProjections; /* all scenes */
+  double kernelMs;            /* duration of the batched solve kernel */
+} mlx_batch_out;
+
+/* One integrate() substep of every scene (xpbdLayer3D.m:621-745 per scene),
+ * the whole projectGS loop of all scenes in one kernel launch. */
+int mlx_batch_integrate(mlx_batch *batch, mlx_settings *settings, double h,
+                        int32_t frame, const mlx_plane *plane,
+                        mlx_batch_out *out);
+
+/* ------------------------------------------------------------------ */
+/* Measurement                                                          */
+/* ------------------------------------------------------------------ */
+
+/* Times every element-sweep launch of the mesh with a pair of HIP events on
+ * the mesh's stream (on = 1), or stops (on = 0).  mlx_profile_read returns
+ * the accumulated kernel time (ms), the number of timed launches that did
+ * work and the elements they projected, then clears the totals. */
+int mlx_profile_enable(mlx_mesh *mesh, int32_t on);
+int mlx_profile_read(mlx_mesh *mesh, double *sweepMs, int64_t *sweepLaunches,
+                     int64_t *sweepElements);
+
+/* Execution options of a mesh.  MLX_OPT_GRAPHS (default 1): iterations run
+ * as replays of a captured hipGraph of MLX_OPT_GRAPH_ITERS iterations
+ * (default 8, 1..256); 0 launches every kernel directly.  Results are the
+ * same either way. */
+#define MLX_OPT_GRAPHS 1
+#define MLX_OPT_GRAPH_ITERS 2
+int mlx_mesh_set_option(mlx_mesh *mesh, int32_t option, int64_t value);
+
+/* hipDeviceSynchronize on the library's device. */
+int mlx_synchronize(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,7 +30,10 @@ EXPORTED = (
     "mlx_mesh_create", "mlx_mesh_destroy", "mlx_mesh_info",
     "mlx_stvk_voigt_constraints", "mlx_layer_solve", "mlx_state_create",
     "mlx_state_destroy", "mlx_state_download", "mlx_integrate",
+    "mlx_profile_enable", "mlx_profile_read", "mlx_synchronize", "mlx_mesh_set_option",
 )
+MLX_OPT_GRAPHS = 1
+MLX_OPT_GRAPH_ITERS = 2
 
 _p = C.c_void_p
 _d = C.POINTER(C.c_double)
@@ -136,6 +139,10 @@ def lib():
         "mlx_state_download": [_p, _d, _d, _d, _i32],
         "mlx_integrate": [_p, C.POINTER(Settings), C.c_double, C.c_int32, C.POINTER(Plane),
                           C.POINTER(LayerSolveOut)],
+        "mlx_profile_enable": [_p, C.c_int32],
+        "mlx_profile_read": [_p, _d, _i64, _i64],
+        "mlx_synchronize": [],
+        "mlx_mesh_set_option": [_p, C.c_int32, C.c_int64],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -95,6 +95,9 @@ struct mlx_mesh {
   std::vector<int64_t> colorOff;    // numColors+1 padded slot offsets
   std::vector<int64_t> colorCnt;    // elements per color
   std::vector<int32_t> perm;        // slot -> element (-1 for padding)
+  int nScenes = 1;                  // > 1: scene-major slot layout (batch)
+  std::vector<int32_t> sceneColor0, sceneNColors;
+  std::vector<int64_t> sceneSlot0;  // nScenes+1
   std::vector<int32_t> slotOf;      // element -> slot
   // per-element material in element order: alpha_d, alpha_o, alpha_s, alpha1
   std::vector<double> mat;
@@ -128,6 +131,28 @@ struct mlx_mesh {
   mlx::DevBuf d_resid;            // double residual array
   mlx::DevBuf d_tmp;              // misc
   mlx::Ctl *h_ctl = nullptr;      // pinned host mirror
+
+  // replayable graphs of iteration chunks (hipGraph), keyed by everything
+  // the captured kernels bake into their parameters
+  struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> sig;
+  };
+  GraphSlot graphs[4];
+  int graphNext = 0;
+  int useGraphs = 1;
+  int graphIters = 8;
+
+  // sweep timing (mlx_profile_enable)
+  struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> ev;          // 2 per recorded launch
+    std::vector<int64_t> elems;          // elements of each recorded launch
+    std::vector<int32_t> iter;           // iteration index of each launch
+    size_t used = 0;
+    double ms = 0;
+    int64_t launches = 0, elements = 0;
+  } prof;
 };
 
 struct mlx_state {

@@ -195,11 +195,7 @@ struct SweepParams {
 // The normal block is solved as a 3x3 system, the three shear rows as
 // scalars with the (4,4) compliance, as decomposeSolve does.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sweep3d(SweepParams<T> P) {
-  if (P.ctl && iteration_stopped(P.ctl)) return;
-  const int64_t s = P.begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (s >= P.end) return;
-  if (P.flags && !P.flags[s]) return;
+__device__ __forceinline__ void project_tet(const SweepParams<T> &P, const int64_t s) {
   const int64_t ld = P.ld;
   const int v0 = P.vid[s], v1 = P.vid[ld + s], v2 = P.vid[2 * ld + s], v3 = P.vid[3 * ld + s];
   const vec4_t<T> X0 = P.x[v0], X1 = P.x[v1], X2 = P.x[v2], X3 = P.x[v3];
@@ -306,14 +302,19 @@ __global__ __launch_bounds__(kBlock) void k_sweep3d(SweepParams<T> P) {
   }
 }
 
-// StVK-Voigt triangle (matStvkVoigtConstraint.m:1-34): same algebra in 2D
-// with constraints (E11, E22, E12) and the full 3x3 solve of justSolve.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sweep2d(SweepParams<T> P) {
+__global__ __launch_bounds__(kBlock) void k_sweep3d(SweepParams<T> P) {
   if (P.ctl && iteration_stopped(P.ctl)) return;
   const int64_t s = P.begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (s >= P.end) return;
   if (P.flags && !P.flags[s]) return;
+  project_tet<T>(P, s);
+}
+
+// StVK-Voigt triangle (matStvkVoigtConstraint.m:1-34): same algebra in 2D
+// with constraints (E11, E22, E12) and the full 3x3 solve of justSolve.
+template <typename T>
+__device__ __forceinline__ void project_tri(const SweepParams<T> &P, const int64_t s) {
   const int64_t ld = P.ld;
   const int v0 = P.vid[s], v1 = P.vid[ld + s], v2 = P.vid[2 * ld + s];
   const vec4_t<T> X0 = P.x[v0], X1 = P.x[v1], X2 = P.x[v2];
@@ -397,6 +398,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep2d(SweepParams<T> P) {
     o.y += sc * (FS[1][0] * G[a][0] + FS[1][1] * G[a][1]);
     P.x[vs[a]] = o;
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep2d(SweepParams<T> P) {
+  if (P.ctl && iteration_stopped(P.ctl)) return;
+  const int64_t s = P.begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (s >= P.end) return;
+  if (P.flags && !P.flags[s]) return;
+  project_tri<T>(P, s);
 }
 
 }  // namespace mlx
@@ -791,6 +801,59 @@ __global__ __launch_bounds__(kBlock) void k_pins(int64_t n, const int32_t *dof, 
 // residual and monitor
 // ---------------------------------------------------------------------------
 
+// Residual terms of one tetrahedron (xpbdLayer3D.m:555-568): sum over its six
+// constraints of (|C_i| - min(|C_i|, |(alpha~ lambda)_i|))^2, in double.
+template <typename T>
+__device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const int64_t s) {
+  const int64_t ld = P.ld;
+  const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
+  const T vol = P.vol[s];
+  const vec4_t<T> X0 = P.x[P.vid[s]], X1 = P.x[P.vid[ld + s]], X2 = P.x[P.vid[2 * ld + s]],
+                  X3 = P.x[P.vid[3 * ld + s]];
+  T D[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) D[k] = P.dminv[k * ld + s];
+  const T e[3][3] = {{X1.x - X0.x, X1.y - X0.y, X1.z - X0.z},
+                     {X2.x - X0.x, X2.y - X0.y, X2.z - X0.z},
+                     {X3.x - X0.x, X3.y - X0.y, X3.z - X0.z}};
+  T F[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      F[r][c] = e[0][r] * D[3 * c] + e[1][r] * D[1 + 3 * c] + e[2][r] * D[2 + 3 * c];
+  T Cg[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Cg[j][k] = F[0][j] * F[0][k] + F[1][j] * F[1][k] + F[2][j] * F[2][k];
+  T lam[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) lam[i] = P.lam[i * ld + s];
+  const T half = T(0.5);
+  T C[6], al[6];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    C[j] = vol * (half * (Cg[j][j] - T(1)));
+    const int k = (j + 1) % 3, l = (j + 2) % 3;
+    al[j] = tad * lam[j] + tao * lam[k] + tao * lam[l];
+  }
+  C[3] = vol * (half * Cg[0][1]);
+  C[4] = vol * (half * Cg[0][2]);
+  C[5] = vol * (half * Cg[1][2]);
+  al[3] = tas * lam[3];
+  al[4] = tas * lam[4];
+  al[5] = tas * lam[5];
+  double acc = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const double a = fabs((double)C[i]), cr = fabs((double)al[i]);
+    const double t = a - (a < cr ? a : cr);
+    acc += t * t;
+  }
+  return acc;
+}
+
 // Element part of simpleEvaluateAllConstraintResiduals (xpbdLayer3D.m:546-568;
 // 2D xpbdLayer2D.m:477-496): per element sum_i (|C_i| - min(|C_i|,
 // |(alpha~ lambda)_i|))^2, reduced per workgroup in double.  lamElem gives,
@@ -808,40 +871,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_elems(SweepParams<T> P, int
     const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
     const T vol = P.vol[s];
     if (DIM == 3) {
-      const vec4_t<T> X0 = P.x[P.vid[s]], X1 = P.x[P.vid[ld + s]], X2 = P.x[P.vid[2 * ld + s]],
-                      X3 = P.x[P.vid[3 * ld + s]];
-      T D[9];
-      for (int k = 0; k < 9; ++k) D[k] = P.dminv[k * ld + s];
-      const T e[3][3] = {{X1.x - X0.x, X1.y - X0.y, X1.z - X0.z},
-                         {X2.x - X0.x, X2.y - X0.y, X2.z - X0.z},
-                         {X3.x - X0.x, X3.y - X0.y, X3.z - X0.z}};
-      T F[3][3];
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c)
-          F[r][c] = e[0][r] * D[3 * c] + e[1][r] * D[1 + 3 * c] + e[2][r] * D[2 + 3 * c];
-      T Cg[3][3];
-      for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k) Cg[j][k] = F[0][j] * F[0][k] + F[1][j] * F[1][k] + F[2][j] * F[2][k];
-      T lam[6];
-      for (int i = 0; i < 6; ++i) lam[i] = P.lam[i * ld + s];
-      const T half = T(0.5);
-      T C[6], al[6];
-      for (int j = 0; j < 3; ++j) {
-        C[j] = vol * (half * (Cg[j][j] - T(1)));
-        const int k = (j + 1) % 3, l = (j + 2) % 3;
-        al[j] = tad * lam[j] + tao * lam[k] + tao * lam[l];
-      }
-      C[3] = vol * (half * Cg[0][1]);
-      C[4] = vol * (half * Cg[0][2]);
-      C[5] = vol * (half * Cg[1][2]);
-      al[3] = tas * lam[3];
-      al[4] = tas * lam[4];
-      al[5] = tas * lam[5];
-      for (int i = 0; i < 6; ++i) {
-        const double a = fabs((double)C[i]), cr = fabs((double)al[i]);
-        const double t = a - (a < cr ? a : cr);
-        acc += t * t;
-      }
+      acc = residual_tet<T>(P, s);
     } else {
       const vec4_t<T> X0 = P.x[P.vid[s]], X1 = P.x[P.vid[ld + s]], X2 = P.x[P.vid[2 * ld + s]];
       const T D[4] = {P.dminv[s], P.dminv[ld + s], P.dminv[2 * ld + s], P.dminv[3 * ld + s]};
@@ -917,6 +947,41 @@ __global__ __launch_bounds__(kBlock) void k_residual_contacts(ContactArrays c, c
   if (threadIdx.x == 0) partials[blockIdx.x] = o[0];
 }
 
+// simpleMonitorError (xpbdLayer3D.m:515-544; 2D xpbdLayer2D.m:520-549) on
+// one scene's control block, given the new residual.
+__device__ __forceinline__ void monitor_update(Ctl &c, const MonitorParams &mp, double newRes,
+                                               double *resid, int residCap) {
+  const int idx = c.residualCount;
+  if (resid && idx < residCap) resid[idx] = newRes;
+  c.residualCount = idx + 1;
+  const double last = fabs(c.lastResidual);
+  c.lastResidual = newRes;
+  const double nr = fabs(newRes);
+  const double change = nr - last;
+  if (c.setPercentStop > 0) {
+    double ru = (1 - c.setPercentStop / 100.0) * nr;
+    if (mp.is3d) ru = ru > 1e-8 ? ru : 1e-8;
+    c.runUntil = ru;
+    c.setPercentStop = 0;
+  }
+  if (change < 0) c.errorChangeDecreased = 1;
+  if (mp.giveUpEnabled && change > mp.giveUpThreshold && c.errorChangeDecreased &&
+      (!c.isLastLayer || mp.lastLayerGiveUpEnabled))
+    c.endLayer = 1;
+  const int stopHanging = c.residualCount > mp.hangingStop && mp.hangingStop >= 0;
+  int stop;
+  if (mp.is3d)
+    stop = stopHanging || (c.runUntil >= 0 && c.runUntil >= nr &&
+                           (c.isLastLayer || c.normresv < c.runUntil));
+  else
+    stop = (c.runUntil >= 0 && c.runUntil >= nr && (c.isLastLayer || c.normresv < mp.h * mp.h)) ||
+           stopHanging;
+  if (stop) {
+    c.stopSolve = 1;
+    c.stopLayer = c.layer;
+  }
+}
+
 // End of one projectGS iteration: when residuals are monitored, sum the
 // partials, append the residual and apply simpleMonitorError
 // (xpbdLayer3D.m:515-544; 2D xpbdLayer2D.m:520-549); count the iteration.
@@ -934,36 +999,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_end(Ctl *ctl, MonitorParams mp,
   if (threadIdx.x != 0) return;
   ctl->iters += 1;
   if (!mp.computeResiduals) return;
-  const double newRes = sqrt(o[0]);
-  const int idx = ctl->residualCount;
-  if (idx < residCap) resid[idx] = newRes;
-  ctl->residualCount = idx + 1;
-  const double last = fabs(ctl->lastResidual);
-  ctl->lastResidual = newRes;
-  const double nr = fabs(newRes);
-  const double change = nr - last;
-  if (ctl->setPercentStop > 0) {
-    double ru = (1 - ctl->setPercentStop / 100.0) * nr;
-    if (mp.is3d) ru = ru > 1e-8 ? ru : 1e-8;
-    ctl->runUntil = ru;
-    ctl->setPercentStop = 0;
-  }
-  if (change < 0) ctl->errorChangeDecreased = 1;
-  if (mp.giveUpEnabled && change > mp.giveUpThreshold && ctl->errorChangeDecreased &&
-      (!ctl->isLastLayer || mp.lastLayerGiveUpEnabled))
-    ctl->endLayer = 1;
-  const int stopHanging = ctl->residualCount > mp.hangingStop && mp.hangingStop >= 0;
-  int stop;
-  if (mp.is3d)
-    stop = stopHanging || (ctl->runUntil >= 0 && ctl->runUntil >= nr &&
-                           (ctl->isLastLayer || ctl->normresv < ctl->runUntil));
-  else
-    stop = (ctl->runUntil >= 0 && ctl->runUntil >= nr &&
-            (ctl->isLastLayer || ctl->normresv < mp.h * mp.h)) || stopHanging;
-  if (stop) {
-    ctl->stopSolve = 1;
-    ctl->stopLayer = ctl->layer;
-  }
+  monitor_update(*ctl, mp, sqrt(o[0]), resid, residCap);
 }
 
 // Sum of squares of n doubles-or-floats into partials (grid-stride, fixed grid).
@@ -1200,6 +1236,152 @@ __global__ void k_plane_flags(int64_t nV, const vec4_t<T> *x, double n0, double 
   if (DIM == 3) dist = dist + (double)p.z * n2;
   dist = dist - D;
   flag[v] = strict ? (dist < 0) : (dist <= 0);
+}
+
+}  // namespace mlx
+
+namespace mlx {
+
+// ---------------------------------------------------------------------------
+// batched scenes: one workgroup runs one scene's whole layerSolve
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct BatchParams {
+  SweepParams<T> P;              // merged element/vertex arrays (ctl unused)
+  const int32_t *color0;         // per scene: first entry in colorOff/colorCnt
+  const int32_t *ncolor;         // per scene: number of colors
+  const int64_t *colorOff;       // per (scene, color): first slot
+  const int64_t *colorCnt;       // per (scene, color): elements
+  const int64_t *slot0;          // nScenes+1 slot range per scene
+  const int64_t *cont0;          // nScenes+1 contact range per scene
+  const int64_t *pin0;           // nScenes+1 pinned-DOF range per scene
+  const int32_t *pinDof;         // 0-based DOF
+  const T *pinVal;
+  ContactArrays c;
+  const double *minv;
+  MonitorParams mp;
+  int iterations;
+  double runUntil0, stopPct, contactResistance, contactCompliance, h;
+  int hardCap;
+  // per-scene outputs
+  int32_t *iters;
+  double *lastResidual;
+  double *runUntilOut;
+  int32_t *residualCount;
+};
+
+// One scene per workgroup (plain method: a single all-elastic layer): the
+// projectGS loop of xpbdLayer3D.m:315-341 with colors separated by workgroup
+// barriers, then contacts, pins and the scene's own residual/stop rule.
+// Every loop is bounded (hardCap) so the kernel always returns.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
+  __shared__ double sh[NT];
+  __shared__ int sStop;
+  __shared__ double sRunUntil;
+  const int sc = blockIdx.x;
+  const int tid = threadIdx.x;
+  Ctl c;
+  c.stopSolve = 0;
+  c.endLayer = 0;
+  c.iters = 0;
+  c.errorChangeDecreased = 0;
+  c.residualCount = 1;
+  c.stopLayer = -1;
+  c.isLastLayer = 1;
+  c.layer = 0;
+  c.runUntil = B.runUntil0;
+  c.setPercentStop = B.stopPct;
+  c.normresv = 0;
+  c.lastResidual = 0;
+  if (tid == 0) {
+    sStop = 0;
+    sRunUntil = B.runUntil0;
+  }
+  __syncthreads();
+  const int nc = B.ncolor[sc], cb = B.color0[sc];
+  const int64_t k0 = B.cont0[sc], k1 = B.cont0[sc + 1];
+  const int64_t q0 = B.pin0[sc], q1 = B.pin0[sc + 1];
+  const int64_t s0 = B.slot0[sc], s1 = B.slot0[sc + 1];
+  const double h2 = B.h * B.h;
+  int it = 0;
+  while (it < B.hardCap) {
+    for (int k = 0; k < nc; ++k) {
+      const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
+      for (int64_t s = a + tid; s < e; s += NT) project_tet<T>(B.P, s);
+      __syncthreads();
+    }
+    for (int64_t i = k0 + tid; i < k1; i += NT) {
+      const int v = B.c.vert[i];
+      const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
+      vec4_t<T> p = B.P.x[v];
+      const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
+      double phi = n[0] * (xv[0] - pc[0]) + n[1] * (xv[1] - pc[1]) + n[2] * (xv[2] - pc[2]);
+      if (phi > 0) phi = 0;
+      const double C = B.contactResistance * phi;
+      const double aT = B.c.compliance[i] / h2;
+      const double mi = B.minv[v];
+      const double dl = (-C - aT * B.c.lambda[i]) / (2 * mi + aT);
+      p.x = (T)(xv[0] + mi * (dl * n[0]));
+      p.y = (T)(xv[1] + mi * (dl * n[1]));
+      p.z = (T)(xv[2] + mi * (dl * n[2]));
+      B.c.lambda[i] += dl;
+      B.P.x[v] = p;
+    }
+    __syncthreads();
+    for (int64_t q = q0 + tid; q < q1; q += NT) {
+      const int d = B.pinDof[q];
+      T *px = reinterpret_cast<T *>(&B.P.x[d / 3]);
+      px[d % 3] = B.pinVal[q];
+    }
+    __syncthreads();
+    ++it;
+    if (B.mp.computeResiduals) {
+      double acc = 0;
+      for (int64_t s = s0 + tid; s < s1; s += NT)
+        if (B.P.vid[s] >= 0) acc += residual_tet<T>(B.P, s);
+      const double Ks = (double)(k1 - k0);
+      for (int64_t i = k0 + tid; i < k1; i += NT) {
+        const int v = B.c.vert[i];
+        const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
+        const vec4_t<T> p = B.P.x[v];
+        double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]) +
+                     n[2] * ((double)p.z - pc[2]);
+        if (phi > 0) phi = 0;
+        const double a = fabs(B.contactResistance * phi);
+        const double cr = fabs((B.contactCompliance / h2) * B.c.lambda[i]);
+        const double t = a - (a < cr ? a : cr);
+        acc += Ks * (t * t);
+      }
+      double v1[1] = {acc}, o[1];
+      // block_sum with NT threads
+      sh[tid] = v1[0];
+      __syncthreads();
+      for (int st = NT / 2; st > 0; st >>= 1) {
+        if (tid < st) sh[tid] += sh[tid + st];
+        __syncthreads();
+      }
+      o[0] = sh[0];
+      if (tid == 0) {
+        monitor_update(c, B.mp, sqrt(o[0]), nullptr, 0);
+        sStop = c.stopSolve | c.endLayer;
+        sRunUntil = c.runUntil;
+      }
+      __syncthreads();
+    }
+    const int stop = sStop;
+    const double ru = sRunUntil;
+    __syncthreads();
+    if (stop) break;
+    if (!(it < B.iterations || ru >= 0)) break;
+  }
+  if (tid == 0) {
+    B.iters[sc] = it;
+    B.lastResidual[sc] = c.lastResidual;
+    B.runUntilOut[sc] = c.runUntil;
+    B.residualCount[sc] = c.residualCount;
+  }
 }
 
 }  // namespace mlx

@@ -186,20 +186,63 @@ static SweepParams<T> sweep_params(mlx_mesh &m, double h, const Ctl *ctl) {
   return P;
 }
 
+// Records an event pair around a launch when profiling is on and the pool
+// has room (events are created on demand, kept for the mesh's lifetime).
+static hipEvent_t prof_event(mlx_mesh &m, size_t i) {
+  while (m.prof.ev.size() <= i) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    m.prof.ev.push_back(e);
+  }
+  return m.prof.ev[i];
+}
+
+static constexpr size_t kProfPairs = 8192;
+
 template <typename T>
-static void enqueue_sweeps(mlx_mesh &m, SweepParams<T> P, const uint8_t *flags) {
+static void enqueue_sweeps(mlx_mesh &m, SweepParams<T> P, const uint8_t *flags,
+                           const int64_t *elasticPerColor = nullptr, int iter = 0) {
   P.flags = flags;
   for (int c = 0; c < m.numColors; ++c) {
     P.begin = m.colorOff[c];
     P.end = m.colorOff[c] + m.colorCnt[c];
     if (P.end <= P.begin) continue;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (m.prof.on && m.prof.used < kProfPairs) {
+      e0 = prof_event(m, 2 * m.prof.used);
+      e1 = prof_event(m, 2 * m.prof.used + 1);
+      if (e0 && e1) {
+        m.prof.elems.resize(m.prof.used + 1);
+        m.prof.iter.resize(m.prof.used + 1);
+        m.prof.elems[m.prof.used] = elasticPerColor ? elasticPerColor[c] : m.colorCnt[c];
+        m.prof.iter[m.prof.used] = iter;
+        m.prof.used++;
+        (void)hipEventRecord(e0, m.stream);
+      }
+    }
     if (m.dim == 3)
       hipLaunchKernelGGL(k_sweep3d<T>, dim3(blocks_for(P.end - P.begin)), dim3(kBlock), 0,
                          m.stream, P);
     else
       hipLaunchKernelGGL(k_sweep2d<T>, dim3(blocks_for(P.end - P.begin)), dim3(kBlock), 0,
                          m.stream, P);
+    if (e0 && e1) (void)hipEventRecord(e1, m.stream);
   }
+}
+
+// After a stream sync: fold the recorded launches whose iteration index is
+// below itersDone (the rest returned at once: the layer had stopped).
+static void prof_harvest(mlx_mesh &m, int itersDone) {
+  for (size_t i = 0; i < m.prof.used; ++i) {
+    if (m.prof.iter[i] >= itersDone || m.prof.elems[i] == 0) continue;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, m.prof.ev[2 * i], m.prof.ev[2 * i + 1]) == hipSuccess) {
+      m.prof.ms += ms;
+      m.prof.launches += 1;
+      m.prof.elements += m.prof.elems[i];
+    }
+  }
+  m.prof.used = 0;
 }
 
 // Per-layer CSR lists, built on the host from the layer maps.
@@ -294,15 +337,16 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   hipStream_t st = m.stream;
   // elastic flags in slot order, one column per sorted layer
   std::vector<uint8_t> flags((size_t)ns * ld, 0);
-  std::vector<int64_t> nElastic(ns, 0);
+  std::vector<int64_t> nElastic(ns, 0), perColor((size_t)ns * m.numColors, 0);
   for (int l = 0; l < ns; ++l)
-    for (int64_t s = 0; s < ld; ++s) {
-      const int32_t e = m.perm[s];
-      if (e < 0) continue;
-      const uint8_t f = maps.elastic[(size_t)nT * l + e] ? 1 : 0;
-      flags[(size_t)ld * l + s] = f;
-      nElastic[l] += f;
-    }
+    for (int c = 0; c < m.numColors; ++c)
+      for (int64_t s = m.colorOff[c]; s < m.colorOff[c] + m.colorCnt[c]; ++s) {
+        const int32_t e = m.perm[s];
+        const uint8_t f = maps.elastic[(size_t)nT * l + e] ? 1 : 0;
+        flags[(size_t)ld * l + s] = f;
+        nElastic[l] += f;
+        perColor[(size_t)m.numColors * l + c] += f;
+      }
   MLX_TRY(m.d_flags.reserve(flags.size()));
   MLX_TRY(m.d_vbody.reserve(sizeof(int32_t) * maps.vbody.size()));
   MLX_CHECK_HIP(hipMemcpyAsync(m.d_flags.ptr, flags.data(), flags.size(), hipMemcpyHostToDevice, st));
@@ -447,8 +491,9 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
                                    &inf, sizeof(double), hipMemcpyHostToDevice, st));
     }
 
+    int iterEnqueued = 0;
     auto enqueue_iteration = [&]() {
-      enqueue_sweeps<T>(m, P, fl);
+      enqueue_sweeps<T>(m, P, fl, perColor.data() + (size_t)m.numColors * col, iterEnqueued++);
       if (m.dim == 3 && L.nbnd > 0)
         hipLaunchKernelGGL(k_boundary<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, minv, bodies,
                            ri, m.d_lamB.as<double>(), sp.boundaryCompliance, sp.h, ctl);
@@ -496,17 +541,71 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
                          m.d_resid.as<double>(), residCap);
     };
 
-    for (int it = 0; it < sp.iterations[layer]; ++it) enqueue_iteration();
+    // Iterations run as replays of a captured chunk of G iterations when
+    // graphs are on (not while profiling: the timing events are per launch).
+    auto graph_for = [&](int G) -> hipGraphExec_t {
+      auto bits = [](double d) {
+        uint64_t u;
+        std::memcpy(&u, &d, 8);
+        return u;
+      };
+      std::vector<uint64_t> sig = {
+          (uint64_t)col, (uint64_t)G, (uint64_t)L.nb, (uint64_t)L.nbnd, (uint64_t)L.nec,
+          (uint64_t)L.nrc, (uint64_t)K, (uint64_t)nUniq, (uint64_t)sp.computeResiduals,
+          (uint64_t)sp.useRigidContacts, (uint64_t)(uintptr_t)fl, (uint64_t)(uintptr_t)m.d_csr.ptr,
+          (uint64_t)(uintptr_t)m.d_contacts.ptr, (uint64_t)(uintptr_t)m.d_tmp.ptr,
+          (uint64_t)(uintptr_t)m.d_lamB.ptr, (uint64_t)(uintptr_t)m.d_bodies.ptr,
+          (uint64_t)(uintptr_t)m.d_ri.ptr, (uint64_t)(uintptr_t)m.d_x.ptr,
+          (uint64_t)(uintptr_t)m.d_xold.ptr, (uint64_t)(uintptr_t)m.d_lam.ptr,
+          (uint64_t)(uintptr_t)m.d_partials.ptr, (uint64_t)(uintptr_t)m.d_resid.ptr,
+          (uint64_t)(uintptr_t)m.d_pinnedx.ptr, (uint64_t)(uintptr_t)m.d_ctl.ptr,
+          (uint64_t)(uintptr_t)m.d_vbody.ptr, bits(sp.h), bits(sp.boundaryCompliance),
+          bits(sp.contactResistance), bits(sp.giveUpThreshold), (uint64_t)sp.hangingStop,
+          (uint64_t)sp.giveUpEnabled, (uint64_t)sp.lastLayerGiveUpEnabled};
+      for (auto &g : m.graphs)
+        if (g.exec && g.sig == sig) return g.exec;
+      hipGraph_t graph = nullptr;
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+      for (int i = 0; i < G; ++i) enqueue_iteration();
+      if (hipStreamEndCapture(st, &graph) != hipSuccess || !graph) return nullptr;
+      hipGraphExec_t ex = nullptr;
+      const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) return nullptr;
+      auto &slot = m.graphs[m.graphNext++ % 4];
+      if (slot.exec) (void)hipGraphExecDestroy(slot.exec);
+      slot.exec = ex;
+      slot.sig = sig;
+      return ex;
+    };
+    auto run_iterations = [&](int n) -> int {
+      const int G = m.graphIters;
+      if (!m.useGraphs || m.prof.on || n < G) {
+        for (int it = 0; it < n; ++it) enqueue_iteration();
+        return MLX_OK;
+      }
+      hipGraphExec_t ex = graph_for(G);
+      if (!ex) {
+        set_error("hipGraph capture/instantiate of the iteration chunk failed");
+        return MLX_ERR_HIP;
+      }
+      for (int k = 0; k < n / G; ++k) MLX_CHECK_HIP(hipGraphLaunch(ex, st));
+      for (int it = 0; it < n % G; ++it) enqueue_iteration();
+      return MLX_OK;
+    };
+    MLX_TRY(run_iterations(sp.iterations[layer]));
     MLX_TRY(read_ctl(m, hc_ctl));
+    if (m.prof.on) prof_harvest(m, hc_ctl.iters);
     if (isLast) {
       // while it <= iterations || (~stopSolve && runUntil >= 0 && isLastLayer)
       int64_t guard = 0;
       int chunk = 2;
       while (!hc_ctl.stopSolve && !hc_ctl.endLayer && hc_ctl.runUntil >= 0) {
-        for (int it = 0; it < chunk; ++it) enqueue_iteration();
+        MLX_TRY(run_iterations(chunk));
         MLX_TRY(read_ctl(m, hc_ctl));
+        if (m.prof.on) prof_harvest(m, hc_ctl.iters);
         guard += chunk;
-        chunk = std::min(chunk * 2, 16);
+        chunk = std::min(chunk * 2, 4 * std::max(m.graphIters, 4));
         if (guard > 10000000) {
           set_error("runUntil loop exceeded 1e7 iterations without converging (hangingStop < 0?)");
           return MLX_ERR_STATE;
@@ -652,7 +751,12 @@ int mlx_init(int32_t device) {
   return MLX_OK;
 }
 
-int mlx_mesh_create(const mlx_mesh_desc *d, int32_t precision, mlx_mesh **out) {
+}  // extern "C"
+
+// Scene upload.  nScenes > 1 lays slots out scene by scene (then color by
+// color, buckets padded to multiples of `pad`), for the batched kernel.
+static int mesh_create_impl(const mlx_mesh_desc *d, int32_t precision, int nScenes,
+                            const int64_t *sceneElemOff, int pad, mlx_mesh **out) {
   if (!d || !out || (d->dim != 2 && d->dim != 3) || d->nV <= 0 || d->nT <= 0 || !d->t ||
       !d->DmInv || !d->alpha || !d->beta || !d->volume || !d->mass || !d->elementColor ||
       !d->forcedElastic || (d->nPinnedDOFs > 0 && !d->pinnedDOFs) ||
@@ -767,28 +871,72 @@ int mlx_mesh_create(const mlx_mesh_desc *d, int32_t precision, mlx_mesh **out) {
           mark[v] = c;
         }
       }
-    m->numColors = nColors;
-    m->colorOff.assign(nColors + 1, 0);
-    m->colorCnt.assign(nColors, 0);
-    int64_t off = 0;
-    for (int c = 1; c <= nColors; ++c) {
-      m->colorOff[c - 1] = off;
-      m->colorCnt[c - 1] = byColor[c];
-      off += (byColor[c] + 63) / 64 * 64;
-    }
-    m->colorOff[nColors] = off;
-    // colorOff marks padded starts; the sweep range ends at start + count
-    m->ld = std::max<int64_t>(off, 64);
-    m->perm.assign(m->ld, -1);
-    m->slotOf.assign(nT, -1);
-    std::vector<int64_t> cur(m->colorOff.begin(), m->colorOff.end());
-    for (int c = 1; c <= nColors; ++c)
-      for (int64_t q = start[c]; q < start[c + 1]; ++q) {
-        const int32_t e = order[q];
-        const int64_t s = cur[c - 1]++;
-        m->perm[s] = e;
-        m->slotOf[e] = (int32_t)s;
+    if (nScenes <= 1) {
+      m->numColors = nColors;
+      m->colorOff.assign(nColors + 1, 0);
+      m->colorCnt.assign(nColors, 0);
+      int64_t off = 0;
+      for (int c = 1; c <= nColors; ++c) {
+        m->colorOff[c - 1] = off;
+        m->colorCnt[c - 1] = byColor[c];
+        off += (byColor[c] + pad - 1) / pad * pad;
       }
+      m->colorOff[nColors] = off;
+      // colorOff marks padded starts; the sweep range ends at start + count
+      m->ld = std::max<int64_t>(off, 64);
+      m->perm.assign(m->ld, -1);
+      m->slotOf.assign(nT, -1);
+      std::vector<int64_t> cur(m->colorOff.begin(), m->colorOff.end());
+      for (int c = 1; c <= nColors; ++c)
+        for (int64_t q = start[c]; q < start[c + 1]; ++q) {
+          const int32_t e = order[q];
+          const int64_t s = cur[c - 1]++;
+          m->perm[s] = e;
+          m->slotOf[e] = (int32_t)s;
+        }
+      m->nScenes = 1;
+      m->sceneColor0.assign(1, 0);
+      m->sceneNColors.assign(1, nColors);
+      m->sceneSlot0 = {0, m->ld};
+    } else {
+      // scene-major: per scene, its colors in ascending order
+      m->nScenes = nScenes;
+      m->sceneColor0.assign(nScenes, 0);
+      m->sceneNColors.assign(nScenes, 0);
+      m->sceneSlot0.assign(nScenes + 1, 0);
+      m->colorOff.clear();
+      m->colorCnt.clear();
+      int64_t off = 0;
+      std::vector<std::pair<int64_t, int64_t>> placed;  // (slot start, element)
+      m->slotOf.assign(nT, -1);
+      std::vector<int32_t> permv;
+      for (int sc = 0; sc < nScenes; ++sc) {
+        const int64_t e0 = sceneElemOff[sc], e1 = sceneElemOff[sc + 1];
+        int mc = 0;
+        for (int64_t e = e0; e < e1; ++e) mc = std::max(mc, m->color[e]);
+        std::vector<std::vector<int32_t>> buck(mc + 1);
+        for (int64_t e = e0; e < e1; ++e) buck[m->color[e]].push_back((int32_t)e);
+        m->sceneColor0[sc] = (int32_t)m->colorCnt.size();
+        m->sceneSlot0[sc] = off;
+        for (int c = 1; c <= mc; ++c) {
+          m->colorOff.push_back(off);
+          m->colorCnt.push_back((int64_t)buck[c].size());
+          for (size_t q = 0; q < buck[c].size(); ++q) {
+            permv.resize(off + q + 1, -1);
+            permv[off + q] = buck[c][q];
+            m->slotOf[buck[c][q]] = (int32_t)(off + q);
+          }
+          off += ((int64_t)buck[c].size() + pad - 1) / pad * pad;
+        }
+        m->sceneNColors[sc] = mc;
+      }
+      m->sceneSlot0[nScenes] = off;
+      m->numColors = (int)m->colorCnt.size();
+      m->colorOff.push_back(off);
+      m->ld = std::max<int64_t>(off, 64);
+      permv.resize(m->ld, -1);
+      m->perm = permv;
+    }
   }
   m->pinnedDOF.resize(d->nPinnedDOFs);
   for (int64_t q = 0; q < d->nPinnedDOFs; ++q) {
@@ -841,6 +989,12 @@ int mlx_mesh_create(const mlx_mesh_desc *d, int32_t precision, mlx_mesh **out) {
   return MLX_OK;
 }
 
+extern "C" {
+
+int mlx_mesh_create(const mlx_mesh_desc *d, int32_t precision, mlx_mesh **out) {
+  return mesh_create_impl(d, precision, 1, nullptr, 64, out);
+}
+
 int mlx_mesh_destroy(mlx_mesh *m) {
   if (!m) return MLX_OK;
   (void)hipSetDevice(m->device);
@@ -852,6 +1006,9 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
                          &m->d_tmp};
   for (auto *b : bufs) b->release();
+  for (auto e : m->prof.ev) (void)hipEventDestroy(e);
+  for (auto &g : m->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (m->h_ctl) (void)hipHostFree(m->h_ctl);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
@@ -1426,6 +1583,416 @@ int mlx_integrate(mlx_state *S, mlx_settings *st, double h, int32_t frame, const
   MLX_CHECK_HIP(hipSetDevice(S->mesh->device));
   if (S->mesh->precision == MLX_FP64) return integrate_impl<double>(*S, st, h, frame, plane, out);
   return integrate_impl<float>(*S, st, h, frame, plane, out);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int mlx_profile_enable(mlx_mesh *m, int32_t on) {
+  if (!m) {
+    set_error("mlx_profile_enable: null mesh");
+    return MLX_ERR_ARG;
+  }
+  m->prof.on = on != 0;
+  m->prof.used = 0;
+  return MLX_OK;
+}
+
+int mlx_profile_read(mlx_mesh *m, double *ms, int64_t *launches, int64_t *elements) {
+  if (!m) {
+    set_error("mlx_profile_read: null mesh");
+    return MLX_ERR_ARG;
+  }
+  if (ms) *ms = m->prof.ms;
+  if (launches) *launches = m->prof.launches;
+  if (elements) *elements = m->prof.elements;
+  m->prof.ms = 0;
+  m->prof.launches = 0;
+  m->prof.elements = 0;
+  return MLX_OK;
+}
+
+int mlx_mesh_set_option(mlx_mesh *m, int32_t option, int64_t value) {
+  if (!m) {
+    set_error("mlx_mesh_set_option: null mesh");
+    return MLX_ERR_ARG;
+  }
+  switch (option) {
+    case MLX_OPT_GRAPHS:
+      m->useGraphs = value != 0;
+      return MLX_OK;
+    case MLX_OPT_GRAPH_ITERS:
+      if (value < 1 || value > 256) break;
+      m->graphIters = (int)value;
+      return MLX_OK;
+    default:
+      break;
+  }
+  set_error("mlx_mesh_set_option: bad option %d or value %lld", option, (long long)value);
+  return MLX_ERR_ARG;
+}
+
+int mlx_synchronize(void) {
+  MLX_CHECK_HIP(hipSetDevice(g_device));
+  MLX_CHECK_HIP(hipDeviceSynchronize());
+  return MLX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched independent scenes (one persistent launch per substep)
+// ---------------------------------------------------------------------------
+
+struct mlx_batch {
+  mlx_mesh *m = nullptr;
+  int nScenes = 0;
+  int block = 512;
+  std::vector<int64_t> vertOff, elemOff, cont0, pin0;
+  DevBuf d_tables;   // color0 | ncolor | colorOff | colorCnt | slot0 | cont0 | pin0
+  DevBuf d_v;        // velocity T 3nV
+  DevBuf d_out;      // per-scene outputs
+  DevBuf d_pinMask;  // uint8 3nV
+  DevBuf d_flags;    // int32 nV (plane test)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace mlx {
+
+// residual velocity and release of every (elastic) vertex for the plain
+// method (xpbdLayer3D.m:662-664 then :138-139).
+template <typename T>
+__global__ void k_batch_prologue(int64_t nV, const T *v, vec4_t<T> *x, const double *mass,
+                                 const uint8_t *pinnedDof, double h, double g, double airDrag) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nV) return;
+  vec4_t<T> p = x[i];
+  T *px = reinterpret_cast<T *>(&p);
+  const double m = mass[i];
+  for (int d = 0; d < 3; ++d) {
+    double r = (double)v[3 * i + d];
+    if (!pinnedDof[3 * i + d]) {
+      const double f = d == 2 ? 0.0 + m * g : 0.0;
+      r = r + h * f / m;
+    }
+    r = (1 - airDrag) * r;
+    px[d] = (T)((double)px[d] + h * (double)(T)r);
+  }
+  x[i] = p;
+}
+
+template <typename T, int NT>
+static void launch_batch(const BatchParams<T> &B, int n, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch_plain<T, NT>), dim3(n), dim3(NT), 0, s, B);
+}
+
+template <typename T>
+static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int frame,
+                                const mlx_plane *plane, mlx_batch_out *out) {
+  mlx_mesh &m = *b.m;
+  const int64_t nV = m.nV;
+  hipStream_t s = m.stream;
+  vec4_t<T> *x = m.d_x.as<vec4_t<T>>();
+  vec4_t<T> *xo = m.d_xold.as<vec4_t<T>>();
+  const int64_t nPin = (int64_t)m.pinnedDOF.size();
+  for (int i = 0; i < st->nLayers; ++i)
+    if (st->layers[i] != 0) {
+      set_error("the batched path runs the plain method: every layer must be 0 %% rigid");
+      return MLX_ERR_UNSUPPORTED;
+    }
+  if (st->runUntilResSmallerThan >= 0 || st->stopAtImprovementPercent > 0) {
+    if (!st->computeResiduals || st->hangingStop < 0) {
+      set_error("a residual stop in the batched path needs computeResiduals and hangingStop >= 0");
+      return MLX_ERR_UNSUPPORTED;
+    }
+  }
+  MLX_TRY(m.d_tmp.reserve(4096));
+  int *flag = reinterpret_cast<int *>(m.d_tmp.as<char>() + 4000);
+  MLX_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+  if (frame != 1)
+    hipLaunchKernelGGL(k_any_nan<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, xo, flag);
+  hipLaunchKernelGGL((k_reset_oldp<T, 3>), dim3(blocks_for(std::max(nV, nPin))), dim3(kBlock), 0, s,
+                     nV, flag, frame == 1, x, xo, nPin, m.d_pinned.as<int32_t>(), m.d_pinnedx.as<T>());
+  // contacts
+  std::vector<int32_t> cv;
+  double nrm[3] = {0, 0, 0}, Dp = 0;
+  if (plane && plane->enabled) {
+    double nn = 0;
+    for (int d = 0; d < 3; ++d) nn += plane->normal[d] * plane->normal[d];
+    nn = std::sqrt(nn);
+    for (int d = 0; d < 3; ++d) {
+      nrm[d] = plane->normal[d] / nn;
+      Dp += plane->normal[d] * plane->position[d];
+    }
+    MLX_TRY(b.d_flags.reserve(sizeof(int32_t) * nV));
+    hipLaunchKernelGGL((k_plane_flags<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
+                       nrm[0], nrm[1], nrm[2], Dp, 0, b.d_flags.as<int32_t>());
+    std::vector<int32_t> hf(nV);
+    MLX_CHECK_HIP(hipMemcpyAsync(hf.data(), b.d_flags.ptr, sizeof(int32_t) * nV,
+                                 hipMemcpyDeviceToHost, s));
+    MLX_CHECK_HIP(hipStreamSynchronize(s));
+    for (int64_t v = 0; v < nV; ++v)
+      if (hf[v]) cv.push_back((int32_t)v);
+  }
+  const int K = (int)cv.size();
+  b.cont0.assign(b.nScenes + 1, 0);
+  {
+    size_t q = 0;
+    for (int sc = 0; sc < b.nScenes; ++sc) {
+      b.cont0[sc] = (int64_t)q;
+      while (q < cv.size() && cv[q] < b.vertOff[sc + 1]) ++q;
+    }
+    b.cont0[b.nScenes] = K;
+  }
+  MLX_TRY(m.d_contacts.reserve(8 * (size_t)K * 9 + 4 * (size_t)K + 256));
+  MLX_CHECK_HIP(hipStreamSynchronize(s));
+  ContactArrays ca = contact_arrays(m, K);
+  if (K > 0) {
+    MLX_CHECK_HIP(hipMemcpy(const_cast<int32_t *>(ca.vert), cv.data(), 4 * (size_t)K,
+                            hipMemcpyHostToDevice));
+    hipLaunchKernelGGL((k_plane_gather<T, 3>), dim3(blocks_for(K)), dim3(kBlock), 0, s, K, ca.vert,
+                       x, nrm[0], nrm[1], nrm[2], Dp, plane->compliance,
+                       const_cast<double *>(ca.normal), const_cast<double *>(ca.point),
+                       const_cast<double *>(ca.compliance));
+    MLX_CHECK_HIP(hipMemsetAsync(ca.lambda, 0, sizeof(double) * K, s));
+  }
+  // per-scene tables: color0 | ncolor | colorOff | colorCnt | slot0 | cont0 | pin0
+  const int ns = b.nScenes, ncT = (int)m.colorCnt.size();
+  const size_t o_color0 = 0, o_ncolor = o_color0 + 4 * (size_t)ns,
+               o_coff = (o_ncolor + 4 * (size_t)ns + 7) / 8 * 8, o_ccnt = o_coff + 8 * (size_t)ncT,
+               o_slot = o_ccnt + 8 * (size_t)ncT, o_cont = o_slot + 8 * (size_t)(ns + 1),
+               o_pin = o_cont + 8 * (size_t)(ns + 1), o_end = o_pin + 8 * (size_t)(ns + 1);
+  std::vector<char> tab(o_end);
+  std::memcpy(tab.data() + o_color0, m.sceneColor0.data(), 4 * (size_t)ns);
+  std::memcpy(tab.data() + o_ncolor, m.sceneNColors.data(), 4 * (size_t)ns);
+  std::memcpy(tab.data() + o_coff, m.colorOff.data(), 8 * (size_t)ncT);
+  std::memcpy(tab.data() + o_ccnt, m.colorCnt.data(), 8 * (size_t)ncT);
+  std::memcpy(tab.data() + o_slot, m.sceneSlot0.data(), 8 * (size_t)(ns + 1));
+  std::memcpy(tab.data() + o_cont, b.cont0.data(), 8 * (size_t)(ns + 1));
+  std::memcpy(tab.data() + o_pin, b.pin0.data(), 8 * (size_t)(ns + 1));
+  MLX_TRY(b.d_tables.reserve(o_end));
+  MLX_CHECK_HIP(hipMemcpy(b.d_tables.ptr, tab.data(), o_end, hipMemcpyHostToDevice));
+  // shift, prologue, lambda = 0
+  MLX_CHECK_HIP(hipMemcpyAsync(xo, x, sizeof(vec4_t<T>) * nV, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_batch_prologue<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV,
+                     b.d_v.as<T>(), x, m.d_mass64.as<double>(), b.d_pinMask.as<uint8_t>(), h,
+                     st->gravity, st->airDrag);
+  MLX_CHECK_HIP(hipMemsetAsync(m.d_lam.ptr, 0, sizeof(T) * m.nc * m.ld, s));
+  // the persistent solve
+  MLX_TRY(b.d_out.reserve((4 + 8 + 8 + 4) * (size_t)ns + 64));
+  char *ob = b.d_out.as<char>();
+  BatchParams<T> B;
+  B.P = sweep_params<T>(m, h, nullptr);
+  const char *tb = b.d_tables.as<char>();
+  B.color0 = reinterpret_cast<const int32_t *>(tb + o_color0);
+  B.ncolor = reinterpret_cast<const int32_t *>(tb + o_ncolor);
+  B.colorOff = reinterpret_cast<const int64_t *>(tb + o_coff);
+  B.colorCnt = reinterpret_cast<const int64_t *>(tb + o_ccnt);
+  B.slot0 = reinterpret_cast<const int64_t *>(tb + o_slot);
+  B.cont0 = reinterpret_cast<const int64_t *>(tb + o_cont);
+  B.pin0 = reinterpret_cast<const int64_t *>(tb + o_pin);
+  B.pinDof = m.d_pinned.as<int32_t>();
+  B.pinVal = m.d_pinnedx.as<T>();
+  B.c = ca;
+  B.minv = m.d_minv64.as<double>();
+  B.mp.is3d = 1;
+  B.mp.computeResiduals = st->computeResiduals;
+  B.mp.hangingStop = st->hangingStop;
+  B.mp.giveUpEnabled = st->giveUpEnabled;
+  B.mp.lastLayerGiveUpEnabled = st->lastLayerGiveUpEnabled;
+  B.mp.giveUpThreshold = st->giveUpThreshold;
+  B.mp.h = h;
+  int iters = 0;
+  for (int i = 0; i < st->nLayers; ++i) iters += st->iterations[i];
+  B.iterations = iters;
+  B.runUntil0 = st->runUntilResSmallerThan;
+  B.stopPct = st->stopAtImprovementPercent;
+  B.contactResistance = st->contactResistance;
+  B.contactCompliance = plane && plane->enabled ? plane->compliance : 0.0;
+  B.h = h;
+  B.hardCap = std::max(iters, st->hangingStop >= 0 ? st->hangingStop + 1 : iters);
+  B.iters = reinterpret_cast<int32_t *>(ob);
+  B.lastResidual = reinterpret_cast<double *>(ob + (4 * (size_t)ns + 7) / 8 * 8);
+  B.runUntilOut = B.lastResidual + ns;
+  B.residualCount = reinterpret_cast<int32_t *>(B.runUntilOut + ns);
+  if (!b.ev0) {
+    MLX_CHECK_HIP(hipEventCreate(&b.ev0));
+    MLX_CHECK_HIP(hipEventCreate(&b.ev1));
+  }
+  MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
+  if (b.block == 256) launch_batch<T, 256>(B, ns, s);
+  else if (b.block == 1024) launch_batch<T, 1024>(B, ns, s);
+  else launch_batch<T, 512>(B, ns, s);
+  MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
+  hipLaunchKernelGGL((k_velocity<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x, xo,
+                     b.d_v.as<T>(), h);
+  if (K > 0)
+    hipLaunchKernelGGL((k_contact_velocity<T, 3>), dim3(blocks_for(K)), dim3(kBlock), 0, s, ca,
+                       b.d_v.as<T>(), h, st->gravity, st->restitution);
+  MLX_CHECK_HIP(hipGetLastError());
+  MLX_CHECK_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> it(ns), rc(ns);
+  std::vector<double> lr(ns);
+  MLX_CHECK_HIP(hipMemcpy(it.data(), B.iters, 4 * (size_t)ns, hipMemcpyDeviceToHost));
+  MLX_CHECK_HIP(hipMemcpy(lr.data(), B.lastResidual, 8 * (size_t)ns, hipMemcpyDeviceToHost));
+  float kms = 0;
+  MLX_CHECK_HIP(hipEventElapsedTime(&kms, b.ev0, b.ev1));
+  if (out) {
+    int64_t proj = 0;
+    for (int sc = 0; sc < ns; ++sc) {
+      proj += (int64_t)it[sc] * (b.elemOff[sc + 1] - b.elemOff[sc]);
+      if (out->iterations) out->iterations[sc] = it[sc];
+      if (out->lastResidual) out->lastResidual[sc] = lr[sc];
+      if (out->contacts) out->contacts[sc] = (int32_t)(b.cont0[sc + 1] - b.cont0[sc]);
+    }
+    out->elasticProjections = proj;
+    out->kernelMs = kms;
+  }
+  return MLX_OK;
+}
+
+}  // namespace mlx
+
+extern "C" {
+
+int mlx_batch_create(const mlx_mesh_desc *d, int32_t nScenes, const int64_t *vertOff,
+                     const int64_t *elemOff, int32_t precision, mlx_batch **out) {
+  if (!d || !out || nScenes < 1 || !vertOff || !elemOff || d->dim != 3 ||
+      vertOff[0] != 0 || elemOff[0] != 0 || vertOff[nScenes] != d->nV || elemOff[nScenes] != d->nT) {
+    set_error("mlx_batch_create: bad arguments (3D scenes, offsets must span the merged mesh)");
+    return MLX_ERR_ARG;
+  }
+  for (int sc = 0; sc < nScenes; ++sc)
+    if (vertOff[sc + 1] < vertOff[sc] || elemOff[sc + 1] <= elemOff[sc]) {
+      set_error("mlx_batch_create: scene %d has a bad range", sc);
+      return MLX_ERR_ARG;
+    }
+  // every element's vertices must lie in its scene
+  for (int sc = 0; sc < nScenes; ++sc)
+    for (int64_t e = elemOff[sc]; e < elemOff[sc + 1]; ++e)
+      for (int k = 0; k < 4; ++k) {
+        const int64_t v = (int64_t)d->t[e + d->nT * k] - 1;
+        if (v < vertOff[sc] || v >= vertOff[sc + 1]) {
+          set_error("element %lld uses a vertex outside scene %d", (long long)e + 1, sc);
+          return MLX_ERR_ARG;
+        }
+      }
+  mlx_batch *b = new mlx_batch();
+  int r = mesh_create_impl(d, precision, nScenes, elemOff, 16, &b->m);
+  if (r != MLX_OK) {
+    delete b;
+    return r;
+  }
+  b->nScenes = nScenes;
+  b->vertOff.assign(vertOff, vertOff + nScenes + 1);
+  b->elemOff.assign(elemOff, elemOff + nScenes + 1);
+  // pinned DOFs are ascending, so each scene's are contiguous
+  b->pin0.assign(nScenes + 1, 0);
+  {
+    const auto &pd = b->m->pinnedDOF;
+    for (size_t q = 1; q < pd.size(); ++q)
+      if (pd[q] <= pd[q - 1]) {
+        mlx_batch_destroy(b);
+        set_error("pinnedDOFs must be strictly ascending for a batch");
+        return MLX_ERR_ARG;
+      }
+    size_t q = 0;
+    for (int sc = 0; sc < nScenes; ++sc) {
+      b->pin0[sc] = (int64_t)q;
+      while (q < pd.size() && pd[q] < 3 * vertOff[sc + 1]) ++q;
+    }
+    b->pin0[nScenes] = (int64_t)pd.size();
+  }
+  std::vector<uint8_t> mask(3 * (size_t)d->nV, 0);
+  for (auto q : b->m->pinnedDOF) mask[q] = 1;
+  if ((r = b->d_pinMask.reserve(mask.size())) != MLX_OK ||
+      hipMemcpy(b->d_pinMask.ptr, mask.data(), mask.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    mlx_batch_destroy(b);
+    return r != MLX_OK ? r : MLX_ERR_HIP;
+  }
+  *out = b;
+  return MLX_OK;
+}
+
+int mlx_batch_destroy(mlx_batch *b) {
+  if (!b) return MLX_OK;
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  b->d_tables.release();
+  b->d_v.release();
+  b->d_out.release();
+  b->d_pinMask.release();
+  b->d_flags.release();
+  mlx_mesh_destroy(b->m);
+  delete b;
+  return MLX_OK;
+}
+
+int mlx_batch_set_block(mlx_batch *b, int32_t threads) {
+  if (!b || (threads != 256 && threads != 512 && threads != 1024)) {
+    set_error("mlx_batch_set_block: threads must be 256, 512 or 1024");
+    return MLX_ERR_ARG;
+  }
+  b->block = threads;
+  return MLX_OK;
+}
+
+int mlx_batch_set_state(mlx_batch *b, const double *p, const double *v) {
+  if (!b || !p || !v) {
+    set_error("mlx_batch_set_state: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *b->m;
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  const size_t n = 3 * (size_t)m.nV;
+  if (m.precision == MLX_FP64) {
+    MLX_TRY(upload_positions<double>(m, m.d_x, p));
+    MLX_TRY(upload_positions<double>(m, m.d_xold, p));
+    MLX_TRY(upload_scalars<double>(b->d_v, v, n));
+  } else {
+    MLX_TRY(upload_positions<float>(m, m.d_x, p));
+    MLX_TRY(upload_positions<float>(m, m.d_xold, p));
+    MLX_TRY(upload_scalars<float>(b->d_v, v, n));
+  }
+  return MLX_OK;
+}
+
+int mlx_batch_download(mlx_batch *b, double *p, double *v, double *lambdai) {
+  if (!b) {
+    set_error("mlx_batch_download: null batch");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *b->m;
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  const size_t n = 3 * (size_t)m.nV;
+  if (m.precision == MLX_FP64) {
+    if (p) MLX_TRY(download_positions<double>(m, m.d_x, p));
+    if (v) MLX_TRY(download_scalars<double>(b->d_v, v, n));
+    if (lambdai) MLX_TRY(download_lambda<double>(m, lambdai));
+  } else {
+    if (p) MLX_TRY(download_positions<float>(m, m.d_x, p));
+    if (v) MLX_TRY(download_scalars<float>(b->d_v, v, n));
+    if (lambdai) MLX_TRY(download_lambda<float>(m, lambdai));
+  }
+  return MLX_OK;
+}
+
+int mlx_batch_integrate(mlx_batch *b, mlx_settings *st, double h, int32_t frame,
+                        const mlx_plane *plane, mlx_batch_out *out) {
+  if (!b || !st || !(h > 0) || frame < 1 || st->nLayers < 1 || st->nLayers > MLX_MAX_LAYERS) {
+    set_error("mlx_batch_integrate: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  MLX_CHECK_HIP(hipSetDevice(b->m->device));
+  if (!b->d_v.ptr) {
+    set_error("mlx_batch_integrate: call mlx_batch_set_state first");
+    return MLX_ERR_STATE;
+  }
+  if (b->m->precision == MLX_FP64) return batch_integrate_impl<double>(*b, st, h, frame, plane, out);
+  return batch_integrate_impl<float>(*b, st, h, frame, plane, out);
 }
 
 }  // extern "C"

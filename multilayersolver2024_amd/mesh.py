@@ -99,6 +99,9 @@ class DeviceMesh:
         L.check("mlx_mesh_create", lib.mlx_mesh_create(C.byref(self._desc), precision, C.byref(h)))
         self.handle = h
 
+    def set_option(self, option, value):
+        L.check("mlx_mesh_set_option", L.lib().mlx_mesh_set_option(self.handle, option, int(value)))
+
     def info(self):
         n = np.zeros(1, dtype=np.int32)
         L.check("mlx_mesh_info", L.lib().mlx_mesh_info(self.handle, L.ptr(n, C.c_int32), None))

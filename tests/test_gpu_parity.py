@@ -73,3 +73,93 @@ def test_cantilever2d_fp64_200_steps(oracle_lib):
         assert rel(lam, osim.lambdas()) < 1e-9
     finally:
         sim.close()
+
+
+def test_graph_replay_matches_eager(oracle_lib):
+    """Iteration chunks replayed as a captured hipGraph give the same bits as
+    direct launches, substep after substep (eager work and syncs between)."""
+    m, integ, plane = beam_case()
+    a = Simulation(m, integ, 1 / 60, plane, precision=L.MLX_FP64)
+    import copy
+    b = Simulation(m, copy.deepcopy(integ), 1 / 60, plane, precision=L.MLX_FP64)
+    a.dmesh.set_option(L.MLX_OPT_GRAPHS, 0)
+    b.dmesh.set_option(L.MLX_OPT_GRAPHS, 1)
+    b.dmesh.set_option(L.MLX_OPT_GRAPH_ITERS, 2)
+    try:
+        for k in range(6):
+            ra, rb = a.step(), b.step()
+            pa, va, la, ka = a.download()
+            pb, vb, lb, kb = b.download()
+            _say("graph", k, ra.itersPerLayer, rb.itersPerLayer)
+            assert ra.itersPerLayer == rb.itersPerLayer and ka == kb
+            assert np.array_equal(pa, pb) and np.array_equal(la, lb) and np.array_equal(va, vb)
+    finally:
+        a.close()
+        b.close()
+
+
+def test_fp32_beam_within_1e4_of_fp64_oracle(oracle_lib):
+    m, integ, plane = beam_case(nx=12, ny=4, nz=4)
+    import copy
+    sim = Simulation(m, integ, 1 / 60, plane, precision=L.MLX_FP32)
+    osim = O.OracleSim(m, copy.deepcopy(integ), 1 / 60, plane)
+    try:
+        for k in range(8):
+            r = sim.step()
+            o = osim.step()
+            p, v, lam, K = sim.download()
+            e = rel(p, osim.p)
+            _say("fp32", k, r.itersPerLayer, e, rel(lam, osim.lambdas()))
+            assert e < 1e-4
+    finally:
+        sim.close()
+
+
+def test_layer_solve_seam_matches_oracle_substep(oracle_lib):
+    """mlx_layer_solve fed the arrays integrate() builds at frame 1 returns
+    the oracle's end-of-substep xi and lambda."""
+    import copy
+    import ctypes as C
+    from multilayersolver2024_amd.integrator import layer_solve
+    from multilayersolver2024_amd.mesh import DeviceMesh
+    m, integ, plane = beam_case(z0=-0.01)
+    h = 1 / 60
+    osim = O.OracleSim(m, copy.deepcopy(integ), h, plane)
+    c = O.plane_contacts(3, m.p, [0, 0, 1], [0, 0, 0], 1e-4, strict=False)
+    osim.step()
+    # maps at frame 1: identity strain order
+    ns = len(integ.sortedLayers)
+    lay = np.array(integ.sortedLayers, dtype=np.float64)
+    order = L.i32(np.arange(1, m.nT + 1))
+    sets = np.zeros(m.nT * ns, np.int32)
+    nr = np.zeros(ns, np.int32)
+    L.check("uf", L.lib().mlx_union_find_by_vertice(L.ptr(order, C.c_int32), m.nT, L.ptr(lay), ns,
+                                                    L.ptr(m.forcedElastic, C.c_uint8),
+                                                    L.ptr(m.t, C.c_int32), 4, m.nV,
+                                                    L.ptr(sets, C.c_int32), L.ptr(nr, C.c_int32)))
+    isB, isR, vb = np.zeros(m.nV * ns, np.uint8), np.zeros(m.nV * ns, np.uint8), np.zeros(m.nV * ns, np.int32)
+    L.check("fbv", L.lib().mlx_find_boundary_vertices(L.ptr(m.t, C.c_int32), m.nT, 4, m.nV, ns,
+                                                      L.ptr(sets, C.c_int32), L.ptr(isB, C.c_uint8),
+                                                      L.ptr(isR, C.c_uint8), L.ptr(vb, C.c_int32)))
+    maps = dict(isElasticElement=(sets == 0).astype(np.uint8).reshape(ns, m.nT).T,
+                vertexRigidBody=vb.reshape(ns, m.nV).T, isBoundaryVertex=isB.reshape(ns, m.nV).T,
+                isRigidVertex=isR.reshape(ns, m.nV).T, numRigids=nr)
+    f = np.zeros(3 * m.nV)
+    f[2::3] = m.mass[2::3] * integ.Gravity
+    unp = np.ones(3 * m.nV, bool)
+    unp[m.pinnedDOFs - 1] = False
+    rv = np.where(unp, h * f / m.mass, 0.0)
+    K = c["K"]
+    contacts = dict(vertexID=c["vertex"] + 1, normals=c["normal"].reshape(K, 3),
+                    points=c["point"].reshape(K, 3), compliance=c["compliance"])
+    dm = DeviceMesh(m, L.MLX_FP64)
+    try:
+        xi, lam, rv2, out, res, cl = layer_solve(dm, integ, h, maps, m.p, np.zeros((m.nT, 6)), m.p,
+                                                 residualv=rv, pinnedx=m.p[m.pinnedDOFs - 1],
+                                                 contacts=contacts)
+        _say("seam", K, list(out.itersPerLayer[:out.layersRun]))
+        assert K > 0
+        assert rel(xi, osim.p) < 1e-9
+        assert rel(lam, osim.lambdas()) < 1e-9
+    finally:
+        dm.close()
