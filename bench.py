@@ -14,8 +14,10 @@ reductions in FP64.
 A step is one integrate() substep: layer maps, layerSolve to the stop rule,
 velocity update.  value = tet projections / s over the timed steps (every
 elastic element update in every iteration); ms_per_step = wall ms/substep.
-The roofline entry is for the element-sweep kernel, timed with HIP events
-on its stream during the timed steps.  cpu_baseline is the FP64 CPU oracle
+The roofline entry is for the element-sweep kernel, timed with a HIP event
+pair per launch on its stream over --profile-steps substeps that follow the
+timed ones (the timed substeps replay captured hipGraphs, which cannot carry
+per-launch events).  cpu_baseline is the FP64 CPU oracle
 (a restatement of the reference path, one core) on a bounded sample.
 """
 import argparse
@@ -43,7 +45,11 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_TET = {"fp32": 280, "fp64": 544}
 
 
-def make_integrator(stop_rule=True, iterations=(2, 4, 14)):
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_integrator(stop_rule=True, iterations=(2, 4, 14), hanging_stop=1000):
     it = XpbdLayer3D()
     it.Gravity = -9.81
     it.setLayers([98.4375, 87.5, 0])
@@ -51,8 +57,64 @@ def make_integrator(stop_rule=True, iterations=(2, 4, 14)):
     it.computeResiduals = True
     if stop_rule:
         it.StopAtImprovementPercent = 99.99
-    it.hangingStop = 1000
+    it.hangingStop = hanging_stop
     return it
+
+
+def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
+    """Batched form: nscenes beams (tet counts log-uniform in [2000, 60000],
+    random aspect ratios, per-scene E log-uniform in [1e4, 1e7]) integrated
+    in one persistent launch per substep, plain method (one all-elastic
+    layer, 20 iterations, the same 1e-4 relative-residual stop per scene).
+    The block size is chosen by timing each candidate on one substep."""
+    from multilayersolver2024_amd.batch import BatchSimulation, merge_meshes, random_beams
+
+    merged = merge_meshes([Mesh(s) for s in random_beams(nscenes, seed=0)])
+    it = XpbdLayer3D()
+    it.Gravity = -9.81
+    it.iterations = [20]
+    it.computeResiduals = True
+    it.StopAtImprovementPercent = 99.99
+    it.hangingStop = 1000
+    prec = L.MLX_FP32 if precision == "fp32" else L.MLX_FP64
+    tune = {}
+    for blk in (256, 512, 1024):
+        b = BatchSimulation(merged, it, h, plane, precision=prec, block=blk)
+        b.step()
+        proj, kms = b.step()
+        tune[blk] = proj / (kms / 1e3)
+        log(f"batched block {blk}: {kms:.2f} ms kernel, {tune[blk]:.3e} tet-proj/s")
+        b.close()
+    best = max(tune, key=tune.get)
+    b = BatchSimulation(merged, it, h, plane, precision=prec, block=best)
+    for _ in range(warmup):
+        b.step()
+    L.check("mlx_synchronize", L.lib().mlx_synchronize())
+    t0 = time.perf_counter()
+    proj, kms, iters = 0, 0.0, []
+    for _ in range(steps):
+        pr, k = b.step()
+        proj += pr
+        kms += k
+        iters.append(float(np.mean(b.iters)))
+    L.check("mlx_synchronize", L.lib().mlx_synchronize())
+    wall = time.perf_counter() - t0
+    b.close()
+    bpt = BYTES_PER_TET[precision]
+    achieved = proj * bpt / (kms / 1e3) / 1e9
+    return {
+        "value": proj / wall, "unit": "tet-proj/s", "ms_per_step": 1e3 * wall / steps,
+        "steps": steps, "warmup": warmup,
+        "workload": f"{nscenes} independent beams, {merged.nT} tets total "
+                    f"({min(m.nT for m in merged.meshes)}..{max(m.nT for m in merged.meshes)} per "
+                    f"scene), plain method, 20 iterations + rel. residual 1e-4 stop per scene, "
+                    f"ground contact, h=1/60, one launch per substep",
+        "block_size": best, "autotune_tet_proj_per_s": {str(k): v for k, v in tune.items()},
+        "mean_iterations_per_scene_substep": float(np.mean(iters)),
+        "roofline": {"bound": "hbm", "kernel": "k_batch_plain", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "bytes_per_unit": bpt, "avg_launch_ms": kms / steps},
+    }
 
 
 def cpu_baseline(mesh, h, plane, budget_s=12.0, max_steps=4):
@@ -86,6 +148,12 @@ def main():
     ap.add_argument("--ny", type=int, default=16)
     ap.add_argument("--nz", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--hanging-stop", type=int, default=1000,
+                    help="iteration cap of the stop rule (reference default 1000)")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-warmup", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus must be 1 (multi-GPU sharding of the "
@@ -97,12 +165,14 @@ def main():
     mesh = Mesh(sc)
     plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
     prec = L.MLX_FP32 if args.precision == "fp32" else L.MLX_FP64
-    sim = Simulation(mesh, make_integrator(), h, plane, precision=prec)
+    log(f"plain: {mesh.nT} tets, {mesh.nV} vertices, {mesh.numColors} colors")
+    sim = Simulation(mesh, make_integrator(hanging_stop=args.hanging_stop), h, plane,
+                     precision=prec)
     lib = L.lib()
     ncol, _ = sim.dmesh.info()
     for _ in range(args.warmup):
         sim.step()
-    L.check("mlx_profile_enable", lib.mlx_profile_enable(sim.dmesh.handle, 1))
+    log(f"warmup done ({args.warmup} substeps)")
     L.check("mlx_synchronize", lib.mlx_synchronize())
     t0 = time.perf_counter()
     proj, iters, contacts = 0, [], 0
@@ -112,6 +182,13 @@ def main():
         iters.append(sum(r.itersPerLayer))
     L.check("mlx_synchronize", lib.mlx_synchronize())
     t1 = time.perf_counter()
+    # Kernel timing window: the next substeps with a HIP event pair around
+    # every sweep launch on the mesh's stream (direct launches, since the
+    # events are per launch; the kernels are the ones the graphs replay).
+    log(f"timed: {args.steps} substeps in {t1 - t0:.3f} s")
+    L.check("mlx_profile_enable", lib.mlx_profile_enable(sim.dmesh.handle, 1))
+    for _ in range(args.profile_steps):
+        sim.step()
     ms = C.c_double()
     launches, elems = C.c_int64(), C.c_int64()
     L.check("mlx_profile_read", lib.mlx_profile_read(sim.dmesh.handle, C.byref(ms),
@@ -161,7 +238,12 @@ def main():
             "elements_per_launch": elems.value / max(launches.value, 1),
         },
     }
+    if not args.no_batched:
+        log("batched leg")
+        line["batched"] = bench_batched(h, plane, args.batch_steps, args.batch_warmup,
+                                        args.precision)
     if not args.no_cpu_baseline:
+        log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(mesh, h, plane)
     print(json.dumps(line), flush=True)
 

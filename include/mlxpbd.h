@@ -160,6 +160,20 @@ int mlx_stvk_voigt_constraints(mlx_mesh *mesh, const uint8_t *isElasticElement,
                                double h, double *lambdai, double *xi,
                                const double *oldp);
 
+/* Replaces xpbdLayer2D.neoHookeanConstraints(mesh2d, lambdai, xi,
+ * elasticElements, h) (2d/integrator/xpbdLayer2D.m:260-294) with its per-element
+ * MEX pair mexNeoHookeanDConstraint / mexNeoHookeanHConstraint
+ * (2d/mex/mexNeoHookeanDConstraint.cpp:46-138, mexNeoHookeanHConstraint.cpp:47-138):
+ * one sweep over all colors of the deviatoric (||F||, compliance 1/(A mu h^2))
+ * and volumetric (det F - (1 + mu/lambda), compliance 1/(A lambda h^2))
+ * constraints, both evaluated at the positions each color starts from.
+ * 2D mesh only.  elMu, elLambda: nT Lame parameters (mesh2d.elMu, elLambda).
+ * lambdai: nT x 2 column-major (lambda_D, lambda_H), in/out.  xi: 2nV in/out. */
+int mlx_neohookean_constraints2d(mlx_mesh *mesh, const uint8_t *isElasticElement,
+                                 double h, const double *elMu,
+                                 const double *elLambda, double *lambdai,
+                                 double *xi);
+
 /* Arguments of layerSolve (3d/integrator/xpbdLayer3D.m:44-93; 2D
  * 2d/integrator/xpbdLayer2D.m:606-641), minus those already in the mesh. */
 typedef struct {

@@ -31,6 +31,8 @@ EXPORTED = (
     "mlx_stvk_voigt_constraints", "mlx_layer_solve", "mlx_state_create",
     "mlx_state_destroy", "mlx_state_download", "mlx_integrate",
     "mlx_profile_enable", "mlx_profile_read", "mlx_synchronize", "mlx_mesh_set_option",
+    "mlx_batch_create", "mlx_batch_destroy", "mlx_batch_set_block", "mlx_batch_set_state",
+    "mlx_batch_download", "mlx_batch_integrate", "mlx_neohookean_constraints2d",
 )
 MLX_OPT_GRAPHS = 1
 MLX_OPT_GRAPH_ITERS = 2
@@ -97,6 +99,13 @@ class Plane(C.Structure):
     ]
 
 
+class BatchOut(C.Structure):
+    _fields_ = [
+        ("iterations", _i32), ("lastResidual", _d), ("contacts", _i32),
+        ("elasticProjections", C.c_int64), ("kernelMs", C.c_double),
+    ]
+
+
 class MlxError(RuntimeError):
     def __init__(self, fn, code, msg):
         super().__init__(f"{fn} failed ({code}): {msg}")
@@ -143,6 +152,14 @@ def lib():
         "mlx_profile_read": [_p, _d, _i64, _i64],
         "mlx_synchronize": [],
         "mlx_mesh_set_option": [_p, C.c_int32, C.c_int64],
+        "mlx_neohookean_constraints2d": [_p, _u8, C.c_double, _d, _d, _d, _d],
+        "mlx_batch_create": [C.POINTER(MeshDesc), C.c_int32, _i64, _i64, C.c_int32, C.POINTER(_p)],
+        "mlx_batch_destroy": [_p],
+        "mlx_batch_set_block": [_p, C.c_int32],
+        "mlx_batch_set_state": [_p, _d, _d],
+        "mlx_batch_download": [_p, _d, _d, _d],
+        "mlx_batch_integrate": [_p, C.POINTER(Settings), C.c_double, C.c_int32, C.POINTER(Plane),
+                                C.POINTER(BatchOut)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -1,0 +1,129 @@
+"""Many independent scenes integrated in one persistent launch.
+
+``merge_meshes`` concatenates precomputed scenes the way Mesh3D.mergeMesh
+does (3d/@Mesh3D/Mesh3D.m:300-402): vertex ids offset per scene and each
+scene keeping its own graph coloring.  ``BatchSimulation`` drives
+``mlx_batch_integrate``: every scene runs its own plain-method layerSolve
+(own residual and stop rule) inside one workgroup of a single kernel launch.
+``random_beams`` builds the batched workload: beams with log-uniform tet
+counts, random aspect ratios and per-scene materials from PCG32.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from . import scenes
+from .mesh import Mesh
+
+
+class MergedMesh:
+    """Concatenation of 3D Mesh objects (duck-types Mesh.desc())."""
+
+    def __init__(self, meshes):
+        self.meshes = meshes
+        self.dim = 3
+        self.nc = 6
+        self.vertOff = np.zeros(len(meshes) + 1, dtype=np.int64)
+        self.elemOff = np.zeros(len(meshes) + 1, dtype=np.int64)
+        for i, m in enumerate(meshes):
+            self.vertOff[i + 1] = self.vertOff[i] + m.nV
+            self.elemOff[i + 1] = self.elemOff[i] + m.nT
+        self.nV, self.nT = int(self.vertOff[-1]), int(self.elemOff[-1])
+        self.T0 = np.concatenate([m.T0 + int(self.vertOff[i]) for i, m in enumerate(meshes)])
+        self.t = L.fortran(self.T0 + 1, np.int32)
+        self.p = np.concatenate([m.p for m in meshes])
+        self.DmInv = np.concatenate([m.DmInv for m in meshes])
+        self.alpha = np.concatenate([m.alpha for m in meshes])
+        self.beta = np.concatenate([m.beta for m in meshes])
+        self.elV = np.concatenate([m.elV for m in meshes])
+        self.mass = np.concatenate([m.mass for m in meshes])
+        self.elementColor = L.i32(np.concatenate([m.elementColor for m in meshes]))
+        self.numColors = max(m.numColors for m in meshes)
+        self.pinnedDOFs = L.i32(np.concatenate(
+            [m.pinnedDOFs + 3 * int(self.vertOff[i]) for i, m in enumerate(meshes)]))
+        self.forcedElastic = L.u8(np.concatenate([m.forcedElastic for m in meshes]))
+
+    def desc(self):
+        return Mesh.desc(self)
+
+
+def merge_meshes(meshes):
+    return MergedMesh(meshes)
+
+
+class BatchSimulation:
+    def __init__(self, merged, integrator, h, contact=None, precision=L.MLX_FP32, block=512,
+                 device=0):
+        lib = L.lib()
+        L.check("mlx_init", lib.mlx_init(device))
+        self.merged = merged
+        self.h = float(h)
+        self._desc = merged.desc()
+        b = C.c_void_p()
+        L.check("mlx_batch_create", lib.mlx_batch_create(
+            C.byref(self._desc), len(merged.meshes), L.ptr(merged.vertOff, C.c_int64),
+            L.ptr(merged.elemOff, C.c_int64), precision, C.byref(b)))
+        self.handle = b
+        self.set_block(block)
+        v0 = np.zeros_like(merged.p)
+        L.check("mlx_batch_set_state", lib.mlx_batch_set_state(b, L.ptr(merged.p), L.ptr(v0)))
+        self.frame = 0
+        self._settings = integrator.settings()
+        self._plane = contact.plane() if contact is not None else None
+        ns = len(merged.meshes)
+        self.iters = np.zeros(ns, dtype=np.int32)
+        self.lastres = np.zeros(ns)
+        self.contacts = np.zeros(ns, dtype=np.int32)
+
+    def set_block(self, threads):
+        L.check("mlx_batch_set_block", L.lib().mlx_batch_set_block(self.handle, int(threads)))
+
+    def step(self):
+        self.frame += 1
+        out = L.BatchOut()
+        out.iterations = L.ptr(self.iters, C.c_int32)
+        out.lastResidual = L.ptr(self.lastres)
+        out.contacts = L.ptr(self.contacts, C.c_int32)
+        pl = C.byref(self._plane) if self._plane is not None else None
+        L.check("mlx_batch_integrate", L.lib().mlx_batch_integrate(
+            self.handle, C.byref(self._settings), self.h, self.frame, pl, C.byref(out)))
+        return int(out.elasticProjections), float(out.kernelMs)
+
+    def download(self):
+        m = self.merged
+        p = np.zeros(3 * m.nV)
+        v = np.zeros(3 * m.nV)
+        lam = np.zeros(6 * m.nT)
+        L.check("mlx_batch_download", L.lib().mlx_batch_download(
+            self.handle, L.ptr(p), L.ptr(v), L.ptr(lam)))
+        return p, v, lam.reshape(6, m.nT).T.copy()
+
+    def close(self):
+        if self.handle:
+            L.lib().mlx_batch_destroy(self.handle)
+            self.handle = None
+
+
+def random_beams(n=256, seed=0, tets=(2000, 60000), cell=0.02, z0=0.05,
+                 E_range=(1e4, 1e7), nu_range=(0.30, 0.45)):
+    """n beam scenes: tet count log-uniform in `tets` (snapped to 5 a b c
+    cubes with aspect ratios drawn from the seed), per-scene E log-uniform,
+    nu uniform; each pinned at x = 0 over the ground plane z = 0."""
+    rng = scenes.Pcg32(seed, stream=77)
+    out = []
+    lo, hi = math.log10(tets[0]), math.log10(tets[1])
+    for i in range(n):
+        target = 10 ** (lo + (hi - lo) * rng.next_double())
+        r1 = 1.0 + 3.0 * rng.next_double()
+        r2 = 1.0 + rng.next_double()
+        cubes = target / 5.0
+        c = max(1, int(round((cubes / (r1 * r2 * r2)) ** (1.0 / 3.0))))
+        b = max(1, int(round(c * r2)))
+        a = max(1, int(round(cubes / (b * c))))
+        E = 10 ** (math.log10(E_range[0]) + (math.log10(E_range[1]) - math.log10(E_range[0]))
+                   * rng.next_double())
+        nu = nu_range[0] + (nu_range[1] - nu_range[0]) * rng.next_double()
+        out.append(scenes.beam3d(a, b, c, cell=cell, z0=z0, E=E, nu=nu))
+    return out

@@ -130,6 +130,7 @@ struct mlx_mesh {
   mlx::DevBuf d_ctl;              // Ctl
   mlx::DevBuf d_resid;            // double residual array
   mlx::DevBuf d_tmp;              // misc
+  mlx::DevBuf d_nh;               // T 4*ld: 2D NeoHookean mu | lambda | lambda_D | lambda_H
   mlx::Ctl *h_ctl = nullptr;      // pinned host mirror
 
   // replayable graphs of iteration chunks (hipGraph), keyed by everything

@@ -1385,3 +1385,79 @@ __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
 }
 
 }  // namespace mlx
+
+namespace mlx {
+
+// ---------------------------------------------------------------------------
+// 2D NeoHookean constraint pair (xpbdLayer2D.neoHookeanConstraints)
+// ---------------------------------------------------------------------------
+
+// One color: per triangle the deviatoric constraint C_D = ||F||_F
+// (compliance 1/(area h^2 mu)) and the volumetric C_H = det F - gamma
+// (compliance 1/(area h^2 lambda), gamma = 1 + mu/lambda), both evaluated at
+// the positions the color starts from, their displacements summed
+// (mexNeoHookeanDConstraint.cpp / mexNeoHookeanHConstraint.cpp).  With
+// F = sum_a x_a g_a^T: grad_a ||F|| = F g_a / ||F||, grad_a det F = cof(F) g_a.
+// nh: mu | lambda_Lame | lambda_D | lambda_H, each ld long (slot order).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_neohookean2d(SweepParams<T> P, T *nh) {
+  const int64_t s = P.begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (s >= P.end) return;
+  if (P.flags && !P.flags[s]) return;
+  const int64_t ld = P.ld;
+  const int v0 = P.vid[s], v1 = P.vid[ld + s], v2 = P.vid[2 * ld + s];
+  const vec4_t<T> X0 = P.x[v0], X1 = P.x[v1], X2 = P.x[v2];
+  const T D[4] = {P.dminv[s], P.dminv[ld + s], P.dminv[2 * ld + s], P.dminv[3 * ld + s]};
+  const T area = P.vol[s];
+  const T mu = nh[s], lamL = nh[ld + s];
+  const T e[2][2] = {{X0.x - X2.x, X0.y - X2.y}, {X1.x - X2.x, X1.y - X2.y}};
+  T F[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) F[r][c] = e[0][r] * D[2 * c] + e[1][r] * D[1 + 2 * c];
+  const T G[3][2] = {{D[0], D[2]}, {D[1], D[3]}, {-(D[0] + D[1]), -(D[2] + D[3])}};
+  const T w[3] = {X0.w, X1.w, X2.w};
+  const T ih2 = P.ih2;
+  // deviatoric
+  const T n2 = F[0][0] * F[0][0] + F[0][1] * F[0][1] + F[1][0] * F[1][0] + F[1][1] * F[1][1];
+  const T CD = sqrt(n2);
+  const T atD = ((T(1) / area) * ih2) * (T(1) / mu);
+  T gD[3][2], sD = atD;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) gD[a][r] = (F[r][0] * G[a][0] + F[r][1] * G[a][1]) / CD;
+    sD += w[a] * (gD[a][0] * gD[a][0] + gD[a][1] * gD[a][1]);
+  }
+  const T lD = nh[2 * ld + s];
+  const T dlD = -(CD + atD * lD) / sD;
+  // volumetric
+  const T detF = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+  const T gamma = T(1) + mu / lamL;
+  const T CH = detF - gamma;
+  const T atH = ((T(1) / area) * ih2) * (T(1) / lamL);
+  const T cof[2][2] = {{F[1][1], -F[1][0]}, {-F[0][1], F[0][0]}};
+  T gH[3][2], sH = atH;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) gH[a][r] = cof[r][0] * G[a][0] + cof[r][1] * G[a][1];
+    sH += w[a] * (gH[a][0] * gH[a][0] + gH[a][1] * gH[a][1]);
+  }
+  const T lH = nh[3 * ld + s];
+  const T dlH = -(CH + atH * lH) / sH;
+  nh[2 * ld + s] = lD + dlD;
+  nh[3 * ld + s] = lH + dlH;
+  const vec4_t<T> Xs[3] = {X0, X1, X2};
+  const int vs[3] = {v0, v1, v2};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    vec4_t<T> o = Xs[a];
+    o.x = (o.x + w[a] * gD[a][0] * dlD) + w[a] * gH[a][0] * dlH;
+    o.y = (o.y + w[a] * gD[a][1] * dlD) + w[a] * gH[a][1] * dlH;
+    P.x[vs[a]] = o;
+  }
+}
+
+}  // namespace mlx

@@ -1004,7 +1004,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_slotOf, &m->d_x, &m->d_xold, &m->d_rv, &m->d_pinnedx,
                          &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
-                         &m->d_tmp};
+                         &m->d_tmp, &m->d_nh};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto &g : m->graphs)
@@ -1996,3 +1996,65 @@ int mlx_batch_integrate(mlx_batch *b, mlx_settings *st, double h, int32_t frame,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// 2D NeoHookean constraint sweep (row a7)
+// ---------------------------------------------------------------------------
+
+namespace mlx {
+
+template <typename T>
+static int neohookean2d_impl(mlx_mesh &m, const uint8_t *isElastic, double h, const double *mu,
+                             const double *lamL, double *lambdai, double *xi) {
+  const int64_t ld = m.ld, nT = m.nT;
+  std::vector<T> nh(4 * (size_t)ld, T(0));
+  std::vector<uint8_t> fl(ld, 0);
+  for (int64_t s = 0; s < ld; ++s) {
+    const int32_t e = m.perm[s];
+    if (e < 0) continue;
+    nh[s] = (T)mu[e];
+    nh[ld + s] = (T)lamL[e];
+    nh[2 * ld + s] = (T)lambdai[e];
+    nh[3 * ld + s] = (T)lambdai[nT + e];
+    fl[s] = isElastic[e] ? 1 : 0;
+  }
+  MLX_TRY(m.d_nh.reserve(sizeof(T) * nh.size()));
+  MLX_TRY(m.d_flags.reserve(ld));
+  MLX_CHECK_HIP(hipMemcpy(m.d_nh.ptr, nh.data(), sizeof(T) * nh.size(), hipMemcpyHostToDevice));
+  MLX_CHECK_HIP(hipMemcpy(m.d_flags.ptr, fl.data(), ld, hipMemcpyHostToDevice));
+  MLX_TRY(upload_positions<T>(m, m.d_x, xi));
+  SweepParams<T> P = sweep_params<T>(m, h, nullptr);
+  P.flags = m.d_flags.as<uint8_t>();
+  for (int c = 0; c < m.numColors; ++c) {
+    P.begin = m.colorOff[c];
+    P.end = m.colorOff[c] + m.colorCnt[c];
+    if (P.end <= P.begin) continue;
+    hipLaunchKernelGGL(k_neohookean2d<T>, dim3(blocks_for(P.end - P.begin)), dim3(kBlock), 0,
+                       m.stream, P, m.d_nh.as<T>());
+  }
+  MLX_CHECK_HIP(hipGetLastError());
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  MLX_CHECK_HIP(hipMemcpy(nh.data(), m.d_nh.ptr, sizeof(T) * nh.size(), hipMemcpyDeviceToHost));
+  for (int64_t s = 0; s < ld; ++s) {
+    const int32_t e = m.perm[s];
+    if (e < 0) continue;
+    lambdai[e] = (double)nh[2 * ld + s];
+    lambdai[nT + e] = (double)nh[3 * ld + s];
+  }
+  return download_positions<T>(m, m.d_x, xi);
+}
+
+}  // namespace mlx
+
+extern "C" int mlx_neohookean_constraints2d(mlx_mesh *m, const uint8_t *isElastic, double h,
+                                            const double *elMu, const double *elLambda,
+                                            double *lambdai, double *xi) {
+  if (!m || m->dim != 2 || !isElastic || !elMu || !elLambda || !lambdai || !xi || !(h > 0)) {
+    set_error("mlx_neohookean_constraints2d: bad arguments (2D mesh required)");
+    return MLX_ERR_ARG;
+  }
+  MLX_CHECK_HIP(hipSetDevice(m->device));
+  if (m->precision == MLX_FP64)
+    return neohookean2d_impl<double>(*m, isElastic, h, elMu, elLambda, lambdai, xi);
+  return neohookean2d_impl<float>(*m, isElastic, h, elMu, elLambda, lambdai, xi);
+}

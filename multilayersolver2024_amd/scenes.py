@@ -95,27 +95,25 @@ def beam3d(nx=64, ny=16, nz=16, cell=0.02, z0=0.05, seed=0, E=None, nu=None,
     vid = lambda i, j, k: (i * NY + j) * NZ + k
     ii, jj, kk = np.meshgrid(np.arange(nx + 1), np.arange(NY), np.arange(NZ), indexing="ij")
     P = np.stack([ii.ravel() * cell, jj.ravel() * cell, kk.ravel() * cell + z0], axis=1)
-    tets = []
-    for i in range(nx):
-        for j in range(ny):
-            for k in range(nz):
-                c = {(a, b, d): vid(i + a, j + b, k + d) for a in (0, 1) for b in (0, 1) for d in (0, 1)}
-                if (i + j + k) % 2 == 0:
-                    cen = [(0, 0, 0), (1, 1, 0), (1, 0, 1), (0, 1, 1)]
-                    corners = [((1, 0, 0), [(0, 0, 0), (1, 1, 0), (1, 0, 1)]),
-                               ((0, 1, 0), [(0, 0, 0), (1, 1, 0), (0, 1, 1)]),
-                               ((0, 0, 1), [(0, 0, 0), (1, 0, 1), (0, 1, 1)]),
-                               ((1, 1, 1), [(1, 1, 0), (1, 0, 1), (0, 1, 1)])]
-                else:
-                    cen = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1)]
-                    corners = [((0, 0, 0), [(1, 0, 0), (0, 1, 0), (0, 0, 1)]),
-                               ((1, 1, 0), [(1, 0, 0), (0, 1, 0), (1, 1, 1)]),
-                               ((1, 0, 1), [(1, 0, 0), (0, 0, 1), (1, 1, 1)]),
-                               ((0, 1, 1), [(0, 1, 0), (0, 0, 1), (1, 1, 1)])]
-                for apex, base in corners:
-                    tets.append([c[apex]] + [c[q] for q in base])
-                tets.append([c[q] for q in cen])
-    T = np.array(tets, dtype=np.int32)
+    ci, cj, ck = (a.ravel() for a in np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz),
+                                                 indexing="ij"))
+    corner = {(a, b, d): vid(ci + a, cj + b, ck + d) for a in (0, 1) for b in (0, 1) for d in (0, 1)}
+    even = [[(1, 0, 0), (0, 0, 0), (1, 1, 0), (1, 0, 1)],
+            [(0, 1, 0), (0, 0, 0), (1, 1, 0), (0, 1, 1)],
+            [(0, 0, 1), (0, 0, 0), (1, 0, 1), (0, 1, 1)],
+            [(1, 1, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1)],
+            [(0, 0, 0), (1, 1, 0), (1, 0, 1), (0, 1, 1)]]
+    odd = [[(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)],
+           [(1, 1, 0), (1, 0, 0), (0, 1, 0), (1, 1, 1)],
+           [(1, 0, 1), (1, 0, 0), (0, 0, 1), (1, 1, 1)],
+           [(0, 1, 1), (0, 1, 0), (0, 0, 1), (1, 1, 1)],
+           [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1)]]
+    par = ((ci + cj + ck) % 2)[:, None]
+    T = np.empty((len(ci), 5, 4), dtype=np.int64)
+    for q in range(5):
+        for r in range(4):
+            T[:, q, r] = np.where(par[:, 0] == 0, corner[even[q][r]], corner[odd[q][r]])
+    T = T.reshape(-1, 4).astype(np.int32)
     # orient positively: swap the last two vertices where det(Dm) < 0
     a, b, cc, d = (P[T[:, q]] for q in range(4))
     det = np.einsum("ij,ij->i", b - a, np.cross(cc - a, d - a))

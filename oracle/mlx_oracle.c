@@ -1624,3 +1624,125 @@ void ora_contacts3d(int nb, double *M, double *COM, double *R, double *I0, doubl
   bodies_view(&b, nb, M, COM, NULL, R, I0, I, NULL);
   contacts3d(&b, c, vb, ri, mass, cl, xi, h, resistance, useRigid);
 }
+
+/* ------------------------------------------------------------------ */
+/* 2D NeoHookean constraint pair (row a7)                               */
+/* ------------------------------------------------------------------ */
+
+/* mexNeoHookeanDConstraint.cpp:46-138: C_D = ||F||_F with
+ * F = [p0-p2, p1-p2] DmInv, alpha~ = 1/(area h^2 mu); one XPBD scalar step
+ * dlambda = -(C + alpha~ lambda) / (sum_k m_k^-1 gradC_k^2 + alpha~),
+ * dx_k = m_k^-1 gradC_k dlambda.  Expression tree of the MEX kept.
+ * Pins: C against ||F||_F, gradient against finite differences, dlambda
+ * against the scalar XPBD closed form (tests). */
+void ora_neohookean_d2d(double h, const double *x, const double *D, double mu,
+                        double area, double lambda, const double *mass,
+                        double *dx, double *dlambda, double *C) {
+  double t2 = D[0] + D[1], t3 = D[2] + D[3];
+  double t4 = 1.0 / area, t5 = 1.0 / (h * h);
+  double t6 = 1.0 / mass[0], t7 = 1.0 / mass[1], t8 = 1.0 / mass[2], t9 = 1.0 / mass[3],
+         t10 = 1.0 / mass[4], t11 = 1.0 / mass[5];
+  double t12 = 1.0 / mu;
+  double t13 = -x[4], t14 = -x[5];
+  double t15 = t13 + x[0], t16 = t14 + x[1], t17 = t13 + x[2], t18 = t14 + x[3];
+  double t27 = t4 * t5 * t12;
+  double t19 = D[0] * t15, t20 = D[2] * t15, t21 = D[0] * t16, t22 = D[2] * t16;
+  double t23 = D[1] * t17, t24 = D[3] * t17, t25 = D[1] * t18, t26 = D[3] * t18;
+  double t28 = lambda * t27;
+  double t29 = t19 + t23, t30 = t20 + t24, t31 = t21 + t25, t32 = t22 + t26;
+  double t33 = t29 * t29, t34 = t30 * t30, t35 = t31 * t31, t36 = t32 * t32;
+  double t37 = D[0] * t29 * 2.0, t38 = D[2] * t30 * 2.0, t39 = D[0] * t31 * 2.0,
+         t40 = D[2] * t32 * 2.0, t41 = D[1] * t29 * 2.0, t42 = D[3] * t30 * 2.0,
+         t43 = D[1] * t31 * 2.0, t44 = D[3] * t32 * 2.0;
+  double t45 = t2 * t29 * 2.0, t46 = t3 * t30 * 2.0, t47 = t2 * t31 * 2.0, t48 = t3 * t32 * 2.0;
+  double t49 = t37 + t38, t50 = t39 + t40, t51 = t41 + t42, t52 = t43 + t44;
+  double t57 = t45 + t46, t58 = t47 + t48;
+  double t61 = t33 + t34 + t35 + t36;
+  double t53 = t49 * t49, t54 = t50 * t50, t55 = t51 * t51, t56 = t52 * t52;
+  double t59 = t57 * t57, t60 = t58 * t58;
+  double t62 = 1.0 / t61, t63 = sqrt(t61), t64 = 1.0 / t63, t65 = t28 + t63;
+  double t66 = (t6 * t53 * t62) / 4.0, t67 = (t7 * t54 * t62) / 4.0,
+         t68 = (t8 * t55 * t62) / 4.0, t69 = (t9 * t56 * t62) / 4.0,
+         t70 = (t10 * t59 * t62) / 4.0, t71 = (t11 * t60 * t62) / 4.0;
+  double t72 = t27 + t66 + t67 + t68 + t69 + t70 + t71;
+  double t73 = 1.0 / t72;
+  dx[0] = t6 * t49 * t64 * t65 * t73 * (-1.0 / 2.0);
+  dx[1] = t7 * t50 * t64 * t65 * t73 * (-1.0 / 2.0);
+  dx[2] = t8 * t51 * t64 * t65 * t73 * (-1.0 / 2.0);
+  dx[3] = t9 * t52 * t64 * t65 * t73 * (-1.0 / 2.0);
+  dx[4] = (t10 * t57 * t64 * t65 * t73) / 2.0;
+  dx[5] = (t11 * t58 * t64 * t65 * t73) / 2.0;
+  *dlambda = -t65 * t73;
+  if (C) *C = t63;
+}
+
+/* mexNeoHookeanHConstraint.cpp:47-138: C_H = det F - gamma,
+ * gamma = 1 + mu/lambda (Mesh.m:244), alpha~ = 1/(area h^2 lambda). */
+void ora_neohookean_h2d(double h, const double *x, const double *D, double lamL,
+                        double area, double lambda, const double *mass, double gamma,
+                        double *dx, double *dlambda, double *C) {
+  double t2 = D[0] * D[3] * x[0], t3 = D[2] * D[1] * x[0], t4 = D[0] * D[3] * x[1],
+         t5 = D[2] * D[1] * x[1], t6 = D[0] * D[3] * x[2], t7 = D[2] * D[1] * x[2],
+         t8 = D[0] * D[3] * x[3], t9 = D[2] * D[1] * x[3], t10 = D[0] * D[3] * x[4],
+         t11 = D[2] * D[1] * x[4], t12 = D[0] * D[3] * x[5], t13 = D[2] * D[1] * x[5];
+  double t14 = 1.0 / area, t15 = 1.0 / (h * h);
+  double t16 = 1.0 / mass[0], t17 = 1.0 / mass[1], t18 = 1.0 / mass[2], t19 = 1.0 / mass[3],
+         t20 = 1.0 / mass[4], t21 = 1.0 / mass[5];
+  double t22 = 1.0 / lamL;
+  double t23 = t2 * x[3], t24 = t4 * x[2], t25 = t3 * x[3], t26 = t5 * x[2], t27 = t2 * x[5],
+         t28 = t4 * x[4], t29 = t3 * x[5], t30 = t5 * x[4], t31 = t6 * x[5], t32 = t8 * x[4],
+         t33 = t7 * x[5], t34 = t9 * x[4];
+  double t35 = -t3, t36 = -t5, t37 = -t6, t38 = -t7, t39 = -t8, t40 = -t9, t41 = -t10, t42 = -t12;
+  double t49 = t14 * t15 * t22;
+  double t43 = -t23, t44 = -t26, t45 = -t28, t46 = -t29, t47 = -t31, t48 = -t34;
+  double t50 = lambda * t49;
+  double t52 = t2 + t7 + t35 + t37, t53 = t2 + t11 + t35 + t41, t54 = t4 + t9 + t36 + t39,
+         t55 = t4 + t13 + t36 + t42, t56 = t6 + t11 + t38 + t41, t57 = t8 + t13 + t40 + t42;
+  double t51 = -t50;
+  double t58 = t52 * t52, t59 = t53 * t53, t60 = t54 * t54, t61 = t55 * t55, t62 = t56 * t56,
+         t63 = t57 * t57;
+  double t64 = t21 * t58, t65 = t19 * t59, t66 = t20 * t60, t67 = t17 * t62, t68 = t18 * t61,
+         t69 = t16 * t63;
+  double t70 = gamma + t24 + t25 + t27 + t30 + t32 + t33 + t43 + t44 + t45 + t46 + t47 + t48 + t51;
+  double t71 = t49 + t64 + t65 + t66 + t67 + t68 + t69;
+  double t72 = 1.0 / t71;
+  dx[0] = t16 * t57 * t70 * t72;
+  dx[1] = -t17 * t56 * t70 * t72;
+  dx[2] = -t18 * t55 * t70 * t72;
+  dx[3] = t19 * t53 * t70 * t72;
+  dx[4] = t20 * t54 * t70 * t72;
+  dx[5] = -t21 * t52 * t70 * t72;
+  *dlambda = t70 * t72;
+  if (C) *C = -gamma + t23 - t24 - t25 + t26 - t27 + t28 + t29 - t30 + t31 - t32 - t33 + t34;
+}
+
+/* xpbdLayer2D.neoHookeanConstraints (xpbdLayer2D.m:260-294): per color, per
+ * element, the D and H steps are both evaluated at the positions the color
+ * started from (xi is written back after the color), their dx summed.
+ * lambda: nT x 2 row-major (lambda_D, lambda_H). */
+void ora_neohookean_constraints2d(const ora_mesh *m, const uint8_t *isElastic, double h,
+                                  const double *mu, const double *lamL, double *lambda,
+                                  double *xi) {
+  for (int c = 1; c <= m->numColors; ++c)
+    for (int e = 0; e < m->nT; ++e) {
+      if (m->color[e] != c || !isElastic[e]) continue;
+      double x[6], ms[6], dxD[6], dxH[6], dlD, dlH;
+      for (int a = 0; a < 3; ++a)
+        for (int d = 0; d < 2; ++d) {
+          int dof = 2 * m->T[3 * e + a] + d;
+          x[2 * a + d] = xi[dof];
+          ms[2 * a + d] = m->mass[dof];
+        }
+      double gamma = 1 + mu[e] / lamL[e];
+      ora_neohookean_d2d(h, x, m->DmInv + 4 * e, mu[e], m->vol[e], lambda[2 * e], ms, dxD, &dlD, NULL);
+      ora_neohookean_h2d(h, x, m->DmInv + 4 * e, lamL[e], m->vol[e], lambda[2 * e + 1], ms, gamma,
+                         dxH, &dlH, NULL);
+      lambda[2 * e] = lambda[2 * e] + dlD;
+      lambda[2 * e + 1] = lambda[2 * e + 1] + dlH;
+      for (int a = 0; a < 3; ++a)
+        for (int d = 0; d < 2; ++d) {
+          int dof = 2 * m->T[3 * e + a] + d;
+          xi[dof] = (x[2 * a + d] + dxD[2 * a + d]) + dxH[2 * a + d];
+        }
+    }
+}

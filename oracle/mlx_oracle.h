@@ -173,6 +173,17 @@ void ora_contacts3d(int nb, double *M, double *COM, double *R, double *I0, doubl
                     const double *mass, double *cl, double *xi, double h,
                     double resistance, int useRigid);
 
+/* ---- 2D NeoHookean D/H pair ---- */
+void ora_neohookean_d2d(double h, const double *x, const double *DmInv, double mu,
+                        double area, double lambda, const double *mass,
+                        double *dx, double *dlambda, double *C);
+void ora_neohookean_h2d(double h, const double *x, const double *DmInv, double lamL,
+                        double area, double lambda, const double *mass, double gamma,
+                        double *dx, double *dlambda, double *C);
+void ora_neohookean_constraints2d(const ora_mesh *m, const uint8_t *isElastic, double h,
+                                  const double *mu, const double *lamL, double *lambda,
+                                  double *xi);
+
 /* ---- full substep ---- */
 /* 3D: p,v (3nV) in/out; oldp = cache.oldp in/out; oldOldp out;
  * pinnedx (nPinnedDOFs) in/out; frame is 1-based. */

@@ -163,3 +163,32 @@ def test_layer_solve_seam_matches_oracle_substep(oracle_lib):
         assert rel(lam, osim.lambdas()) < 1e-9
     finally:
         dm.close()
+
+
+def test_neohookean2d_sweep_matches_oracle(oracle_lib):
+    import ctypes as C
+    from multilayersolver2024_amd import scenes
+    from multilayersolver2024_amd.mesh import DeviceMesh, Mesh
+    sc = scenes.cantilever2d(20, 6)
+    m = Mesh(sc)
+    om = O.OracleMesh(m)
+    rng = np.random.default_rng(42)
+    xi = m.p + 0.01 * rng.normal(size=m.p.shape)
+    isEl = L.u8(rng.random(m.nT) < 0.8)
+    mu, lamL = L.f64(sc["mu"]), L.f64(sc["lam"])
+    xo = xi.copy()
+    lo = np.zeros(2 * m.nT)
+    dm = DeviceMesh(m, L.MLX_FP64)
+    xg = xi.copy()
+    lg = np.zeros(2 * m.nT)  # column-major nT x 2
+    try:
+        for sweep in range(5):
+            O.lib().ora_neohookean_constraints2d(C.byref(om.s), O.P(isEl, C.c_uint8), C.c_double(0.01),
+                                                 O.P(mu), O.P(lamL), O.P(lo), O.P(xo))
+            L.check("nh", L.lib().mlx_neohookean_constraints2d(dm.handle, L.ptr(isEl, C.c_uint8), 0.01,
+                                                               L.ptr(mu), L.ptr(lamL), L.ptr(lg), L.ptr(xg)))
+        lg_rm = lg.reshape(2, m.nT).T.reshape(-1)
+        _say("neohookean", rel(xg, xo), rel(lg_rm, lo))
+        assert rel(xg, xo) < 1e-9 and rel(lg_rm, lo) < 1e-9
+    finally:
+        dm.close()
