@@ -285,6 +285,22 @@ int mlx_integrate(mlx_state *state, mlx_settings *settings, double h,
                   mlx_layer_solve_out *out);
 
 /* ------------------------------------------------------------------ */
+/* Coarse-layer linear solve                                            */
+/* ------------------------------------------------------------------ */
+
+/* Preconditioned conjugate gradient of solvePCG(iterations, A, b, x0,
+ * preconditioner) (2d/tools/solver/solvePCG.m:1-57) with the diagonal
+ * preconditioner z = pinv .* r, on the GPU in FP64 (one workgroup: sized for
+ * coarse-layer systems).  A: n x n symmetric, given by the compressed arrays
+ * of a Matlab sparse (mxGetJc colptr n+1, mxGetIr rowind, 0-based as Matlab
+ * stores them, values).  x: x0 in, solution out.  resvec (optional,
+ * iterations): ||A x - b|| after each iteration.  Not on the reference's
+ * XPBD path, which solves its coarse layer by per-body accumulation. */
+int mlx_pcg_solve(int64_t n, const int32_t *colptr, const int32_t *rowind,
+                  const double *val, const double *b, const double *pinv,
+                  int32_t iterations, double *x, double *resvec);
+
+/* ------------------------------------------------------------------ */
 /* Many independent scenes in one launch                                */
 /* ------------------------------------------------------------------ */
 

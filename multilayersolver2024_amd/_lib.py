@@ -33,6 +33,7 @@ EXPORTED = (
     "mlx_profile_enable", "mlx_profile_read", "mlx_synchronize", "mlx_mesh_set_option",
     "mlx_batch_create", "mlx_batch_destroy", "mlx_batch_set_block", "mlx_batch_set_state",
     "mlx_batch_download", "mlx_batch_integrate", "mlx_neohookean_constraints2d",
+    "mlx_pcg_solve",
 )
 MLX_OPT_GRAPHS = 1
 MLX_OPT_GRAPH_ITERS = 2
@@ -153,6 +154,7 @@ def lib():
         "mlx_synchronize": [],
         "mlx_mesh_set_option": [_p, C.c_int32, C.c_int64],
         "mlx_neohookean_constraints2d": [_p, _u8, C.c_double, _d, _d, _d, _d],
+        "mlx_pcg_solve": [C.c_int64, _i32, _i32, _d, _d, _d, C.c_int32, _d, _d],
         "mlx_batch_create": [C.POINTER(MeshDesc), C.c_int32, _i64, _i64, C.c_int32, C.POINTER(_p)],
         "mlx_batch_destroy": [_p],
         "mlx_batch_set_block": [_p, C.c_int32],

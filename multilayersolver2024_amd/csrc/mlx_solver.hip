@@ -2058,3 +2058,131 @@ extern "C" int mlx_neohookean_constraints2d(mlx_mesh *m, const uint8_t *isElasti
     return neohookean2d_impl<double>(*m, isElastic, h, elMu, elLambda, lambdai, xi);
   return neohookean2d_impl<float>(*m, isElastic, h, elMu, elLambda, lambdai, xi);
 }
+
+// ---------------------------------------------------------------------------
+// preconditioned conjugate gradient for the coarse layer (row a13)
+// ---------------------------------------------------------------------------
+
+namespace mlx {
+
+__device__ __forceinline__ double wg_sum1024(double v, double *sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// solvePCG.m:1-57 in one workgroup (coarse systems are small): SpMV,
+// FP64 dot products with a fixed reduction tree, diagonal preconditioner.
+__global__ __launch_bounds__(1024) void k_pcg(int n, const int32_t *rp, const int32_t *ci,
+                                              const double *v, const double *b,
+                                              const double *pinv, int iters, double *x,
+                                              double *r, double *z, double *d, double *Ap,
+                                              double *resvec) {
+  __shared__ double sh[1024];
+  const int t = threadIdx.x;
+  double acc = 0;
+  for (int i = t; i < n; i += 1024) {
+    double s = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) s = s + v[k] * x[ci[k]];
+    r[i] = b[i] - s;
+    z[i] = pinv[i] * r[i];
+    d[i] = z[i];
+    acc += r[i] * z[i];
+  }
+  double rTz = wg_sum1024(acc, sh);
+  for (int it = 0; it < iters; ++it) {
+    acc = 0;
+    for (int i = t; i < n; i += 1024) {
+      double s = 0.0;
+      for (int k = rp[i]; k < rp[i + 1]; ++k) s = s + v[k] * d[ci[k]];
+      Ap[i] = s;
+      acc += d[i] * s;
+    }
+    const double qAp = wg_sum1024(acc, sh);
+    const double alpha = (qAp == 0) ? 0.0 : rTz / qAp;
+    acc = 0;
+    for (int i = t; i < n; i += 1024) {
+      x[i] = x[i] + alpha * d[i];
+      r[i] = r[i] - alpha * Ap[i];
+      z[i] = pinv[i] * r[i];
+      acc += r[i] * z[i];
+    }
+    const double rTznew = wg_sum1024(acc, sh);
+    const double beta = rTznew / rTz;
+    rTz = rTznew;
+    for (int i = t; i < n; i += 1024) d[i] = z[i] + beta * d[i];
+    __syncthreads();
+    if (resvec) {
+      acc = 0;
+      for (int i = t; i < n; i += 1024) {
+        double s = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) s = s + v[k] * x[ci[k]];
+        acc += (s - b[i]) * (s - b[i]);
+      }
+      const double ss = wg_sum1024(acc, sh);
+      if (t == 0) resvec[it] = sqrt(ss);
+    }
+  }
+}
+
+}  // namespace mlx
+
+extern "C" int mlx_pcg_solve(int64_t n, const int32_t *colptr, const int32_t *rowind,
+                             const double *val, const double *b, const double *pinv,
+                             int32_t iterations, double *x, double *resvec) {
+  if (n <= 0 || n > (1 << 26) || !colptr || !rowind || !val || !b || !pinv || !x || iterations < 0) {
+    set_error("mlx_pcg_solve: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  const int64_t nnz = colptr[n];
+  for (int64_t i = 0; i < n; ++i)
+    if (colptr[i + 1] < colptr[i]) {
+      set_error("mlx_pcg_solve: column pointers must be non-decreasing");
+      return MLX_ERR_ARG;
+    }
+  for (int64_t k = 0; k < nnz; ++k)
+    if (rowind[k] < 0 || rowind[k] >= n) {
+      set_error("mlx_pcg_solve: row index out of range");
+      return MLX_ERR_ARG;
+    }
+  MLX_TRY(check_device(g_device));
+  MLX_CHECK_HIP(hipSetDevice(g_device));
+  DevBuf buf;
+  const size_t bytes = 4 * (n + 1) + 4 * nnz + 8 * nnz + 8 * (7 * n + iterations + 1) + 64;
+  MLX_TRY(buf.reserve(bytes));
+  char *p = buf.as<char>();
+  int32_t *d_cp = reinterpret_cast<int32_t *>(p);
+  int32_t *d_ri = d_cp + (n + 1);
+  double *d_v = reinterpret_cast<double *>(p + ((4 * (n + 1) + 4 * nnz + 7) / 8) * 8);
+  double *d_b = d_v + nnz, *d_pinv = d_b + n, *d_x = d_pinv + n, *d_r = d_x + n, *d_z = d_r + n,
+         *d_d = d_z + n, *d_Ap = d_d + n, *d_res = d_Ap + n;
+  hipError_t e = hipMemcpy(d_cp, colptr, 4 * (n + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_ri, rowind, 4 * nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_v, val, 8 * nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_b, b, 8 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_pinv, pinv, 8 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_x, x, 8 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    // A is symmetric: the column-compressed arrays of a Matlab sparse are its
+    // row-compressed arrays too.
+    hipLaunchKernelGGL(k_pcg, dim3(1), dim3(1024), 0, 0, (int)n, d_cp, d_ri, d_v, d_b, d_pinv,
+                       iterations, d_x, d_r, d_z, d_d, d_Ap, resvec ? d_res : nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(x, d_x, 8 * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && resvec && iterations > 0)
+    e = hipMemcpy(resvec, d_res, 8 * (size_t)iterations, hipMemcpyDeviceToHost);
+  buf.release();
+  if (e != hipSuccess) {
+    set_error("mlx_pcg_solve: %s", hipGetErrorString(e));
+    return MLX_ERR_HIP;
+  }
+  return MLX_OK;
+}

@@ -1746,3 +1746,58 @@ void ora_neohookean_constraints2d(const ora_mesh *m, const uint8_t *isElastic, d
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* preconditioned conjugate gradient (row a13)                          */
+/* ------------------------------------------------------------------ */
+
+static void csr_mv(int n, const int32_t *rp, const int32_t *ci, const double *v,
+                   const double *x, double *y) {
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) s = s + v[k] * x[ci[k]];
+    y[i] = s;
+  }
+}
+
+static double dotn(int n, const double *a, const double *b) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s = s + a[i] * b[i];
+  return s;
+}
+
+/* solvePCG.m:1-57 with a diagonal preconditioner z = pinv .* r: fixed
+ * iteration count, alpha = 0 when d'Ad == 0, resvec(i) = ||A x - b||.
+ * A is CSR (0-based).  x: in x0, out solution.  resvec may be NULL.
+ * Pins: equals numpy.linalg.solve after n iterations on small SPD systems;
+ * hand-computed first iteration; the d'Ad == 0 guard (tests). */
+void ora_solve_pcg(int n, const int32_t *rp, const int32_t *ci, const double *v,
+                   const double *b, const double *pinv, int iterations, double *x,
+                   double *resvec) {
+  double *r = (double *)malloc(sizeof(double) * n), *z = (double *)malloc(sizeof(double) * n);
+  double *d = (double *)malloc(sizeof(double) * n), *Ap = (double *)malloc(sizeof(double) * n);
+  csr_mv(n, rp, ci, v, x, Ap);
+  for (int i = 0; i < n; ++i) r[i] = b[i] - Ap[i];
+  for (int i = 0; i < n; ++i) z[i] = pinv[i] * r[i];
+  for (int i = 0; i < n; ++i) d[i] = z[i];
+  double rTz = dotn(n, r, z);
+  for (int it = 0; it < iterations; ++it) {
+    csr_mv(n, rp, ci, v, d, Ap);
+    double qAp = dotn(n, d, Ap);
+    double alpha = (qAp == 0) ? 0.0 : rTz / qAp;
+    for (int i = 0; i < n; ++i) x[i] = x[i] + alpha * d[i];
+    for (int i = 0; i < n; ++i) r[i] = r[i] - alpha * Ap[i];
+    for (int i = 0; i < n; ++i) z[i] = pinv[i] * r[i];
+    double rTznew = dotn(n, r, z);
+    double beta = rTznew / rTz;
+    rTz = rTznew;
+    for (int i = 0; i < n; ++i) d[i] = z[i] + beta * d[i];
+    if (resvec) {
+      csr_mv(n, rp, ci, v, x, Ap);
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) s = s + (Ap[i] - b[i]) * (Ap[i] - b[i]);
+      resvec[it] = sqrt(s);
+    }
+  }
+  free(r); free(z); free(d); free(Ap);
+}

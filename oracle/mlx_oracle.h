@@ -184,6 +184,11 @@ void ora_neohookean_constraints2d(const ora_mesh *m, const uint8_t *isElastic, d
                                   const double *mu, const double *lamL, double *lambda,
                                   double *xi);
 
+/* ---- PCG (solvePCG.m) ---- */
+void ora_solve_pcg(int n, const int32_t *rowptr, const int32_t *col, const double *val,
+                   const double *b, const double *pinv, int iterations, double *x,
+                   double *resvec);
+
 /* ---- full substep ---- */
 /* 3D: p,v (3nV) in/out; oldp = cache.oldp in/out; oldOldp out;
  * pinnedx (nPinnedDOFs) in/out; frame is 1-based. */
