@@ -191,6 +191,11 @@ class OracleSim:
         out.lambda_ = P(self.lam)
         out.residualArray = P(self.resid)
         out.residualCap = len(self.resid)
+        ns = len(np.unique(np.asarray(self.integ.layers, dtype=np.float64)))
+        sets = np.zeros(self.om.m.nT * ns, dtype=np.int32)
+        nrig = np.zeros(ns, dtype=np.int32)
+        out.setsPerLayer = P(sets, C.c_int32)
+        out.numRigids = P(nrig, C.c_int32)
         L = lib()
         if self.dim == 3:
             r = L.ora_integrate3d(C.byref(self.om.s), C.byref(self.s), C.c_double(self.h),
@@ -208,7 +213,8 @@ class OracleSim:
         self.integ_runUntil = self.s.runUntil
         n = min(out.residualCount, len(self.resid))
         return dict(residuals=self.resid[:n].copy(), iters=list(out.itersPerLayer[:out.layersRun]),
-                    projections=int(out.elasticProjections), runUntil=self.s.runUntil)
+                    projections=int(out.elasticProjections), runUntil=self.s.runUntil,
+                    sets=sets.reshape(ns, self.om.m.nT).T.copy(), numRigids=nrig)
 
     def lambdas(self):
         m = self.om.m

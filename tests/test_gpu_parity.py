@@ -120,9 +120,11 @@ def test_fp32_plain_within_1e4_of_fp64_oracle(oracle_lib):
 def test_fp32_multilayer_error_is_bounded(oracle_lib):
     """With strain-rate layers the rigid sets are a discontinuous function of
     the positions: once FP32 rounding flips a near-tie in the strain-rate
-    order, FP32 and FP64 pick different rigid elements.  The FP32 path is
-    then a valid trajectory of its own; its distance to the FP64 oracle is
-    stated (printed) and bounded loosely."""
+    order, FP32 and FP64 pick different rigid elements and the trajectories
+    part (measured: 1e-7 for three substeps, then growing to ~6e-3 by the
+    eighth).  The FP32 path is then a valid trajectory of its own, so the
+    distance is bounded loosely here; the precision of the FP32 solve on
+    identical layer maps is checked tightly by the FP32 seam test."""
     m, integ, plane = beam_case(nx=12, ny=4, nz=4)
     import copy
     sim = Simulation(m, integ, 1 / 60, plane, precision=L.MLX_FP32)
@@ -134,40 +136,40 @@ def test_fp32_multilayer_error_is_bounded(oracle_lib):
             p, v, lam, K = sim.download()
             e = rel(p, osim.p)
             _say("fp32 layers", k, r.itersPerLayer, e, rel(lam, osim.lambdas()))
-            assert e < 5e-3
+            assert e < (1e-6 if k < 3 else 2e-2)
     finally:
         sim.close()
 
 
-def test_layer_solve_seam_matches_oracle_substep(oracle_lib):
-    """mlx_layer_solve fed the arrays integrate() builds at frame 1 returns
-    the oracle's end-of-substep xi and lambda."""
-    import copy
+def _maps_from_sets(m, sets, nr):
+    """findBoundaryVertices through the C ABI for per-layer body ids."""
     import ctypes as C
-    from multilayersolver2024_amd.integrator import layer_solve
-    from multilayersolver2024_amd.mesh import DeviceMesh
+    ns = sets.shape[1]
+    flat = L.i32(sets.T.reshape(-1))
+    isB = np.zeros(m.nV * ns, np.uint8)
+    isR = np.zeros(m.nV * ns, np.uint8)
+    vb = np.zeros(m.nV * ns, np.int32)
+    L.check("fbv", L.lib().mlx_find_boundary_vertices(
+        L.ptr(m.t, C.c_int32), m.nT, m.vpe, m.nV, ns, L.ptr(flat, C.c_int32),
+        L.ptr(isB, C.c_uint8), L.ptr(isR, C.c_uint8), L.ptr(vb, C.c_int32)))
+    return dict(isElasticElement=(sets == 0).astype(np.uint8),
+                vertexRigidBody=vb.reshape(ns, m.nV).T, isBoundaryVertex=isB.reshape(ns, m.nV).T,
+                isRigidVertex=isR.reshape(ns, m.nV).T, numRigids=L.i32(nr))
+
+
+def _seam_case():
+    """Frame 1 of the beam sunk 1 cm into the ground plane (45 contacts on
+    rigid, boundary, pinned and elastic vertices), with the layer sets the
+    oracle built for that substep and the inputs integrate() hands to
+    layerSolve at frame 1 (xi = oldp = p, residual velocity h f / m)."""
+    import copy
     m, integ, plane = beam_case(z0=-0.01)
     h = 1 / 60
     osim = O.OracleSim(m, copy.deepcopy(integ), h, plane)
     c = O.plane_contacts(3, m.p, [0, 0, 1], [0, 0, 0], 1e-4, strict=False)
-    osim.step()
-    # maps at frame 1: identity strain order
-    ns = len(integ.sortedLayers)
-    lay = np.array(integ.sortedLayers, dtype=np.float64)
-    order = L.i32(np.arange(1, m.nT + 1))
-    sets = np.zeros(m.nT * ns, np.int32)
-    nr = np.zeros(ns, np.int32)
-    L.check("uf", L.lib().mlx_union_find_by_vertice(L.ptr(order, C.c_int32), m.nT, L.ptr(lay), ns,
-                                                    L.ptr(m.forcedElastic, C.c_uint8),
-                                                    L.ptr(m.t, C.c_int32), 4, m.nV,
-                                                    L.ptr(sets, C.c_int32), L.ptr(nr, C.c_int32)))
-    isB, isR, vb = np.zeros(m.nV * ns, np.uint8), np.zeros(m.nV * ns, np.uint8), np.zeros(m.nV * ns, np.int32)
-    L.check("fbv", L.lib().mlx_find_boundary_vertices(L.ptr(m.t, C.c_int32), m.nT, 4, m.nV, ns,
-                                                      L.ptr(sets, C.c_int32), L.ptr(isB, C.c_uint8),
-                                                      L.ptr(isR, C.c_uint8), L.ptr(vb, C.c_int32)))
-    maps = dict(isElasticElement=(sets == 0).astype(np.uint8).reshape(ns, m.nT).T,
-                vertexRigidBody=vb.reshape(ns, m.nV).T, isBoundaryVertex=isB.reshape(ns, m.nV).T,
-                isRigidVertex=isR.reshape(ns, m.nV).T, numRigids=nr)
+    o = osim.step()
+    assert int(np.count_nonzero(o["numRigids"])) > 0
+    maps = _maps_from_sets(m, o["sets"], o["numRigids"])
     f = np.zeros(3 * m.nV)
     f[2::3] = m.mass[2::3] * integ.Gravity
     unp = np.ones(3 * m.nV, bool)
@@ -176,15 +178,28 @@ def test_layer_solve_seam_matches_oracle_substep(oracle_lib):
     K = c["K"]
     contacts = dict(vertexID=c["vertex"] + 1, normals=c["normal"].reshape(K, 3),
                     points=c["point"].reshape(K, 3), compliance=c["compliance"])
-    dm = DeviceMesh(m, L.MLX_FP64)
+    return m, integ, h, osim, maps, rv, contacts
+
+
+@pytest.mark.parametrize("precision,tol", [(L.MLX_FP64, 1e-9), (L.MLX_FP32, 1e-5)])
+def test_layer_solve_seam_matches_oracle_substep(precision, tol, oracle_lib):
+    """mlx_layer_solve fed the arrays integrate() builds at frame 1 returns
+    the oracle's end-of-substep xi and lambda (FP64 to 1e-9; FP32 bulk
+    projection to 1e-5 on the same layer maps)."""
+    from multilayersolver2024_amd.integrator import layer_solve
+    from multilayersolver2024_amd.mesh import DeviceMesh
+    m, integ, h, osim, maps, rv, contacts = _seam_case()
+    K = len(contacts["vertexID"])
+    dm = DeviceMesh(m, precision)
     try:
         xi, lam, rv2, out, res, cl = layer_solve(dm, integ, h, maps, m.p, np.zeros((m.nT, 6)), m.p,
                                                  residualv=rv, pinnedx=m.p[m.pinnedDOFs - 1],
                                                  contacts=contacts)
-        _say("seam", K, list(out.itersPerLayer[:out.layersRun]))
+        e, el = rel(xi, osim.p), rel(lam, osim.lambdas())
+        _say("seam", precision, K, list(out.itersPerLayer[:out.layersRun]), e, el)
         assert K > 0
-        assert rel(xi, osim.p) < 1e-9
-        assert rel(lam, osim.lambdas()) < 1e-9
+        assert e < tol
+        assert el < (tol if precision == L.MLX_FP64 else 1e-3)
     finally:
         dm.close()
 
