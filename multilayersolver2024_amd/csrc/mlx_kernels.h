@@ -21,6 +21,7 @@ struct Body {
   double M, invM;
   double COM[3], COMDOT[3], omega[3];
   double R[9], I0[9], I[9], Rstep[9];  // column-major
+  double angle, Iz;                    // 2D pose angle and scalar inertia
 };
 
 __device__ __forceinline__ bool iteration_stopped(const Ctl *ctl) {
@@ -676,6 +677,146 @@ __global__ __launch_bounds__(kBlock) void k_boundary(LayerLists L, vec4_t<T> *x,
 }
 
 // ---------------------------------------------------------------------------
+// rigid layer, 2D
+// ---------------------------------------------------------------------------
+
+// RigidBody.pageRelativeVertexPosition with angle2rotm2D (RigidBody.m:166-169,
+// 2d/tools/math/angle2rotm2D.m:1-5): pos = [c -s; s c] r_i + COM.
+__device__ __forceinline__ void rigid_pos2d(double angle, const double *ri, const double *COM,
+                                            double *out) {
+  const double c = cos(angle), s = sin(angle);
+  out[0] = (c * ri[0] + (-s) * ri[1]) + COM[0];
+  out[1] = (s * ri[0] + c * ri[1]) + COM[1];
+}
+
+// makeRigidsFromConnectivity / matComputeRigidBodyProperties2D
+// (AdaptiveMesh.m:27-37, RigidBody.m:171-215): mass, COM = sum(m x)/M,
+// scalar inertia sum(m |x - COM|^2), angle = 0, r_i = x - COM (stride 3).
+// The 2D layerSolve does not step bodies, so velocities are not formed.
+// One workgroup per body, fixed reduction tree.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict2d(LayerLists L, const vec4_t<T> *x,
+                                                       const double *mass, Body *bodies,
+                                                       double *ri) {
+  __shared__ double sh[3 * kBlock];
+  const int k = blockIdx.x;
+  const int off = L.rig_off[k], n = L.rig_off[k + 1] - off;
+  double a[3] = {0, 0, 0};
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = L.rig_vert[off + q];
+    const double m = mass[v];
+    const vec4_t<T> p = x[v];
+    a[0] += m;
+    a[1] += (double)p.x * m;
+    a[2] += (double)p.y * m;
+  }
+  double s3[3];
+  block_sum<3>(a, sh, s3);
+  const double M = s3[0];
+  const double com[2] = {s3[1] / M, s3[2] / M};
+  double b[1] = {0};
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = L.rig_vert[off + q];
+    const vec4_t<T> p = x[v];
+    const double r0 = (double)p.x - com[0], r1 = (double)p.y - com[1];
+    b[0] += mass[v] * (r0 * r0 + r1 * r1);
+    ri[3 * v] = r0;
+    ri[3 * v + 1] = r1;
+    ri[3 * v + 2] = 0;
+  }
+  double s1[1];
+  block_sum<1>(b, sh, s1);
+  if (threadIdx.x == 0) {
+    Body &B = bodies[k];
+    B.M = M;
+    B.invM = 1.0 / M;
+    B.COM[0] = com[0];
+    B.COM[1] = com[1];
+    B.COM[2] = 0;
+    B.angle = 0;
+    B.Iz = s1[0];
+  }
+}
+
+// rigidElasticBoundaryAccumulationConstraints (xpbdLayer2D.m:296-330) with
+// rigidBoundaryConstraint2D.projectVectorized (rigidBoundaryConstraint2D.m:4-34).
+// One workgroup per body: each boundary vertex against the body's pose at
+// the start, w = (1/M)/nb + Iinv (r x n)^2 with nb the body's boundary-vertex
+// count, alpha~ = alpha/h^2; sums of p/M and (Iinv/2) r x p are applied and
+// the boundary vertices snap to the rigid positions.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_boundary2d(LayerLists L, vec4_t<T> *x,
+                                                       const double *minv, Body *bodies,
+                                                       const double *ri, double *lamB,
+                                                       double aT, const Ctl *ctl) {
+  if (iteration_stopped(ctl)) return;
+  __shared__ double sh[3 * kBlock];
+  const int k = blockIdx.x;
+  Body &B = bodies[k];
+  const int off = L.bnd_off[k], n = L.bnd_off[k + 1] - off;
+  const double Iinv = 1.0 / B.Iz, invMs = B.invM / n;
+  double acc[3] = {0, 0, 0};
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = L.bnd_vert[off + q];
+    double rx[2];
+    rigid_pos2d(B.angle, ri + 3 * v, B.COM, rx);
+    const vec4_t<T> p = x[v];
+    const double r[2] = {rx[0] - B.COM[0], rx[1] - B.COM[1]};
+    const double dir[2] = {rx[0] - (double)p.x, rx[1] - (double)p.y};
+    const double C = sqrt(dir[0] * dir[0] + dir[1] * dir[1]);
+    double nn[2] = {dir[0] / C, dir[1] / C};
+    if (C == 0) nn[0] = nn[1] = 0;
+    const double crn = r[0] * nn[1] - r[1] * nn[0];
+    const double w = invMs + Iinv * (crn * crn);
+    const double dl = (-C - aT * lamB[off + q]) / ((minv[v] + w) + aT);
+    lamB[off + q] += dl;
+    const double pp[2] = {dl * nn[0], dl * nn[1]};
+    acc[0] += pp[0] * B.invM;
+    acc[1] += pp[1] * B.invM;
+    acc[2] += (0.5 * Iinv) * (r[0] * pp[1] - r[1] * pp[0]);
+  }
+  double s3[3];
+  block_sum<3>(acc, sh, s3);
+  __shared__ double sA, sC[3];
+  if (threadIdx.x == 0) {
+    B.COM[0] = B.COM[0] + s3[0];
+    B.COM[1] = B.COM[1] + s3[1];
+    B.angle = B.angle + s3[2];
+    sA = B.angle;
+    sC[0] = B.COM[0];
+    sC[1] = B.COM[1];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = L.bnd_vert[off + q];
+    double xn[2];
+    rigid_pos2d(sA, ri + 3 * v, sC, xn);
+    vec4_t<T> p = x[v];
+    p.x = (T)xn[0];
+    p.y = (T)xn[1];
+    x[v] = p;
+  }
+}
+
+// rotateAllVerts2D.m:1-7: rigid vertices to R(angle) r_i + COM.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rotate_all2d(int64_t nV, const int32_t *vbody,
+                                                         vec4_t<T> *x, const Body *bodies,
+                                                         const double *ri) {
+  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nV) return;
+  const int b = vbody[v];
+  if (b == 0) return;
+  const Body &B = bodies[b - 1];
+  double xn[2];
+  rigid_pos2d(B.angle, ri + 3 * v, B.COM, xn);
+  vec4_t<T> p = x[v];
+  p.x = (T)xn[0];
+  p.y = (T)xn[1];
+  x[v] = p;
+}
+
+// ---------------------------------------------------------------------------
 // contacts
 // ---------------------------------------------------------------------------
 
@@ -780,6 +921,67 @@ __global__ __launch_bounds__(kBlock) void k_contacts_rigid(LayerLists L, Contact
     p.x = (T)xn[0];
     p.y = (T)xn[1];
     p.z = (T)xn[2];
+    x[v] = p;
+  }
+}
+
+// penaltyConstraint2D.projectRigid (penaltyConstraint2D.m:28-58) with the
+// per-body accumulation of the 2D contactConstraints (xpbdLayer2D.m:418-444):
+// nb = the body's rigid-contact count, w = (1/M)/nb + Iinv (r x n)^2,
+// dlambda = (-C - alpha~ lambda) / (2 w + alpha~); sums of p/M and
+// (Iinv/2) r x p applied; the contact vertices snap to the rigid positions.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_contacts_rigid2d(LayerLists L, ContactArrays c,
+                                                             vec4_t<T> *x, Body *bodies,
+                                                             const double *ri, double resistance,
+                                                             double h, const Ctl *ctl) {
+  if (iteration_stopped(ctl)) return;
+  __shared__ double sh[3 * kBlock];
+  const int k = blockIdx.x;
+  Body &B = bodies[k];
+  const int off = L.rc_off[k], n = L.rc_off[k + 1] - off;
+  const double Iinv = 1.0 / B.Iz, invMs = B.invM / n;
+  double acc[3] = {0, 0, 0};
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int i = L.rc_idx[off + q];
+    const int v = c.vert[i];
+    const double *nn = c.normal + 3 * i, *pc = c.point + 3 * i;
+    double rx[2];
+    rigid_pos2d(B.angle, ri + 3 * v, B.COM, rx);
+    const double r[2] = {rx[0] - B.COM[0], rx[1] - B.COM[1]};
+    const vec4_t<T> p = x[v];
+    double phi = ((double)p.x - pc[0]) * nn[0] + ((double)p.y - pc[1]) * nn[1];
+    if (phi > 0) phi = 0;
+    const double C = resistance * phi;
+    const double crn = r[0] * nn[1] - r[1] * nn[0];
+    const double w = invMs + Iinv * (crn * crn);
+    const double aT = c.compliance[i] / (h * h);
+    const double dl = (-C - aT * c.lambda[i]) / (2 * w + aT);
+    c.lambda[i] += dl;
+    const double pp[2] = {dl * nn[0], dl * nn[1]};
+    acc[0] += pp[0] * B.invM;
+    acc[1] += pp[1] * B.invM;
+    acc[2] += (0.5 * Iinv) * (r[0] * pp[1] - r[1] * pp[0]);
+  }
+  double s3[3];
+  block_sum<3>(acc, sh, s3);
+  __shared__ double sA, sC[3];
+  if (threadIdx.x == 0) {
+    B.COM[0] = B.COM[0] + s3[0];
+    B.COM[1] = B.COM[1] + s3[1];
+    B.angle = B.angle + s3[2];
+    sA = B.angle;
+    sC[0] = B.COM[0];
+    sC[1] = B.COM[1];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < n; q += kBlock) {
+    const int v = c.vert[L.rc_idx[off + q]];
+    double xn[2];
+    rigid_pos2d(sA, ri + 3 * v, sC, xn);
+    vec4_t<T> p = x[v];
+    p.x = (T)xn[0];
+    p.y = (T)xn[1];
     x[v] = p;
   }
 }

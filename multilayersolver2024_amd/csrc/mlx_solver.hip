@@ -444,10 +444,6 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     LayerCSR L;
     build_csr(m, maps, col, hc, L);
     const bool hasRigidEl = nElastic[col] < nT;
-    if (m.dim == 2 && hasRigidEl) {
-      set_error("2D layers with rigid elements are not implemented on the GPU path");
-      return MLX_ERR_UNSUPPORTED;
-    }
     MLX_TRY(m.d_csr.reserve(sizeof(int32_t) * L.buf.size()));
     MLX_CHECK_HIP(hipMemcpyAsync(m.d_csr.ptr, L.buf.data(), sizeof(int32_t) * L.buf.size(),
                                  hipMemcpyHostToDevice, st));
@@ -483,6 +479,10 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       hipLaunchKernelGGL(k_layer_begin, dim3(1), dim3(kBlock), 0, st, ctl, partials, 256, layer,
                          (int)isLast, layer == 0 ? sp.stopAtImprovementPercent : 0.0);
     } else {
+      // bodies from the current positions, not stepped (xpbdLayer2D.m:676-681)
+      if (L.nb > 0)
+        hipLaunchKernelGGL(k_restrict2d<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, mass, bodies,
+                           ri);
       hipLaunchKernelGGL(k_layer_begin, dim3(1), dim3(kBlock), 0, st, ctl, partials, 0, layer,
                          (int)isLast, layer == 0 ? sp.stopAtImprovementPercent : 0.0);
       // 2D layerSolve passes resvnorm = inf to projectGS (xpbdLayer2D.m:735)
@@ -497,6 +497,10 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       if (m.dim == 3 && L.nbnd > 0)
         hipLaunchKernelGGL(k_boundary<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, minv, bodies,
                            ri, m.d_lamB.as<double>(), sp.boundaryCompliance, sp.h, ctl);
+      if (m.dim == 2 && L.nbnd > 0)
+        hipLaunchKernelGGL(k_boundary2d<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, minv, bodies,
+                           ri, m.d_lamB.as<double>(), sp.boundaryCompliance / (sp.h * sp.h),
+                           ctl);
       if (L.nec > 0) {
         if (m.dim == 3)
           hipLaunchKernelGGL((k_contacts_elastic<T, 3>), dim3(blocks_for(L.nec)), dim3(kBlock), 0,
@@ -508,6 +512,9 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       if (m.dim == 3 && L.nrc > 0 && sp.useRigidContacts)
         hipLaunchKernelGGL(k_contacts_rigid<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, ca, x, bodies,
                            ri, sp.contactResistance, sp.h, ctl);
+      if (m.dim == 2 && L.nrc > 0 && sp.useRigidContacts)
+        hipLaunchKernelGGL(k_contacts_rigid2d<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, ca, x,
+                           bodies, ri, sp.contactResistance, sp.h, ctl);
       if (nPin > 0) {
         if (m.dim == 3)
           hipLaunchKernelGGL((k_pins<T, 3>), dim3(blocks_for(nPin)), dim3(kBlock), 0, st, nPin,
@@ -612,6 +619,9 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
         }
       }
     }
+    if (m.dim == 2 && L.nb > 0)
+      hipLaunchKernelGGL(k_rotate_all2d<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV, vb, x,
+                         bodies, ri);
     if (m.dim == 3 && L.nb > 0) {
       hipLaunchKernelGGL(k_rotate_all<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV, vb, x,
                          bodies, ri);

@@ -24,6 +24,11 @@ class _XpbdLayerBase:
         self.sortedLayers = sorted(set(self.layers))
         self.layersMap = [self.sortedLayers.index(x) + 1 for x in self.layers]
 
+    def rigid_contacts_enabled(self):
+        """useRigidContacts (xpbdLayer3D.m:37) / useRigidConstacts, the 2D
+        spelling (xpbdLayer2D.m:37)."""
+        return bool(getattr(self, "useRigidContacts", getattr(self, "useRigidConstacts", True)))
+
     def settings(self):
         s = L.Settings()
         if len(self.layers) != len(self.iterations):
@@ -42,7 +47,7 @@ class _XpbdLayerBase:
         s.giveUpEnabled = int(bool(self.giveUpEnabled))
         s.giveUpThreshold = self.giveUpThreshold
         s.lastLayerGiveUpEnabled = int(bool(self.LastLayerGiveUpEnabled))
-        s.useRigidContacts = int(bool(getattr(self, "useRigidContacts", True)))
+        s.useRigidContacts = int(self.rigid_contacts_enabled())
         s.stopAtImprovementPercent = self.StopAtImprovementPercent
         s.runUntilResSmallerThan = self.runUntilResSmallerThan
         return s
@@ -212,7 +217,7 @@ def layer_solve(dmesh, integrator, h, maps, xi, lambdai, oldp, residualv=None, p
     a.giveUpEnabled = int(bool(integrator.giveUpEnabled))
     a.giveUpThreshold = integrator.giveUpThreshold
     a.lastLayerGiveUpEnabled = int(bool(integrator.LastLayerGiveUpEnabled))
-    a.useRigidContacts = int(bool(getattr(integrator, "useRigidContacts", True)))
+    a.useRigidContacts = int(integrator.rigid_contacts_enabled())
     a.stopAtImprovementPercent = integrator.StopAtImprovementPercent
     a.runUntilResSmallerThan = integrator.runUntilResSmallerThan
     a.residualArrayLast = 0.0

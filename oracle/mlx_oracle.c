@@ -1463,10 +1463,236 @@ int ora_integrate3d(const ora_mesh *m, ora_settings *s, double h, int frame,
 /* 2D substep                                                           */
 /* ------------------------------------------------------------------ */
 
+/* ------------------------------------------------------------------ */
+/* rigid layer (2D)                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int nb;
+  double *M, *COM, *angle, *I; /* per body: mass, COM (2), angle, scalar inertia */
+} bodies2d_t;
+
+static void bodies2d_alloc(bodies2d_t *b, int nb) {
+  int n = nb > 0 ? nb : 1;
+  b->nb = nb;
+  b->M = (double *)calloc(n, sizeof(double));
+  b->COM = (double *)calloc(2 * n, sizeof(double));
+  b->angle = (double *)calloc(n, sizeof(double));
+  b->I = (double *)calloc(n, sizeof(double));
+}
+static void bodies2d_free(bodies2d_t *b) { free(b->M); free(b->COM); free(b->angle); free(b->I); }
+
+/* AdaptiveMesh.makeRigidsFromConnectivity (AdaptiveMesh.m:27-37) with
+ * RigidBody.matComputeRigidBodyProperties2D (RigidBody.m:171-215): per body
+ * mass (x-DOF masses), COM = sum(m p)/M, inertia sum(m |p - COM|^2),
+ * angle = 0, R = I; ri = p - COM for rigid vertices.  The COM velocity and
+ * angular velocity the routine also returns are unused by the 2D layerSolve
+ * (xpbdLayer2D.m:681) and are not formed.  Sums in ascending vertex order. */
+static void rigid_properties2d(int nV, const int32_t *vb, const double *p,
+                               const double *mass, bodies2d_t *b, double *ri) {
+  int nb = b->nb;
+  double *mx = (double *)calloc(2 * (nb > 0 ? nb : 1), sizeof(double));
+  for (int i = 0; i < nV; ++i) {
+    int k = vb[i] - 1;
+    if (k < 0) continue;
+    double m = mass[2 * i];
+    b->M[k] = b->M[k] + m;
+    mx[2 * k] = mx[2 * k] + p[2 * i] * m;
+    mx[2 * k + 1] = mx[2 * k + 1] + p[2 * i + 1] * m;
+  }
+  for (int k = 0; k < nb; ++k) {
+    b->COM[2 * k] = mx[2 * k] / b->M[k];
+    b->COM[2 * k + 1] = mx[2 * k + 1] / b->M[k];
+    b->angle[k] = 0;
+    b->I[k] = 0;
+  }
+  for (int i = 0; i < nV; ++i) {
+    int k = vb[i] - 1;
+    if (k < 0) continue;
+    double rx = p[2 * i] - b->COM[2 * k], ry = p[2 * i + 1] - b->COM[2 * k + 1];
+    ri[2 * i] = rx;
+    ri[2 * i + 1] = ry;
+    b->I[k] = b->I[k] + mass[2 * i] * (rx * rx + ry * ry);
+  }
+  free(mx);
+}
+
+/* RigidBody.pageRelativeVertexPosition with angle2rotm2D (RigidBody.m:166-169,
+ * 2d/tools/math/angle2rotm2D.m:1-5): pos = [c -s; s c] ri + COM. */
+static void rigid_pos2d(double angle, const double *ri, const double *COM, double *out) {
+  double c = cos(angle), s = sin(angle);
+  out[0] = (c * ri[0] + (-s) * ri[1]) + COM[0];
+  out[1] = (s * ri[0] + c * ri[1]) + COM[1];
+}
+
+/* cross2D.m:1-7: v1x v2y - v1y v2x. */
+static double cross2d(const double *a, const double *b) { return a[0] * b[1] - a[1] * b[0]; }
+
+/* rigidElasticBoundaryAccumulationConstraints (xpbdLayer2D.m:296-330) with
+ * rigidBoundaryConstraint2D.projectVectorized (rigidBoundaryConstraint2D.m:4-34):
+ * every boundary vertex is projected against its body's pose at the start of
+ * the call; the per-body sums of dCOM = p/M and dAngle = (Iinv/2) r x p are
+ * applied; boundary vertices are then set to the rigid positions (the elastic
+ * correction is discarded).  w splits the body mass over the body's
+ * boundary-vertex count nb: w = (1/M)/nb + Iinv (r x n)^2. */
+static void boundary2d(bodies2d_t *b, int nbv, const int32_t *bverts, const int32_t *vb,
+                       const double *ri, const double *mass, double alpha,
+                       double *lamB, double *xi, double h) {
+  if (nbv == 0) return;
+  int nb = b->nb;
+  int n1 = nb > 0 ? nb : 1;
+  int *cnt = (int *)calloc(n1, sizeof(int));
+  double *dC = (double *)calloc(2 * n1, sizeof(double));
+  double *dA = (double *)calloc(n1, sizeof(double));
+  for (int q = 0; q < nbv; ++q) cnt[vb[bverts[q]] - 1]++;
+  double h2 = h * h;
+  double aT = alpha / h2;
+  for (int q = 0; q < nbv; ++q) {
+    int v = bverts[q], k = vb[v] - 1;
+    double rx[2], r[2], dir[2], n[2];
+    rigid_pos2d(b->angle[k], ri + 2 * v, b->COM + 2 * k, rx);
+    r[0] = rx[0] - b->COM[2 * k];
+    r[1] = rx[1] - b->COM[2 * k + 1];
+    dir[0] = rx[0] - xi[2 * v];
+    dir[1] = rx[1] - xi[2 * v + 1];
+    double C = sqrt(dir[0] * dir[0] + dir[1] * dir[1]);
+    n[0] = dir[0] / C;
+    n[1] = dir[1] / C;
+    if (C == 0) { n[0] = 0; n[1] = 0; }
+    double Iinv = 1.0 / b->I[k];
+    double invM = 1.0 / b->M[k];
+    double invMs = invM / cnt[k];
+    double crn = cross2d(r, n);
+    double w = invMs + Iinv * (crn * crn);
+    double minvSum = 1.0 / mass[2 * v + 1] + w;
+    double dl = (-C - aT * lamB[q]) / (minvSum + aT);
+    double pp[2] = {dl * n[0], dl * n[1]};
+    lamB[q] = lamB[q] + dl;
+    dC[2 * k] = dC[2 * k] + pp[0] * invM;
+    dC[2 * k + 1] = dC[2 * k + 1] + pp[1] * invM;
+    dA[k] = dA[k] + (0.5 * Iinv) * cross2d(r, pp);
+  }
+  for (int k = 0; k < nb; ++k) {
+    b->COM[2 * k] = b->COM[2 * k] + dC[2 * k];
+    b->COM[2 * k + 1] = b->COM[2 * k + 1] + dC[2 * k + 1];
+    b->angle[k] = b->angle[k] + dA[k];
+  }
+  for (int q = 0; q < nbv; ++q) {
+    int v = bverts[q], k = vb[v] - 1;
+    rigid_pos2d(b->angle[k], ri + 2 * v, b->COM + 2 * k, xi + 2 * v);
+  }
+  free(cnt); free(dC); free(dA);
+}
+
+/* contactConstraints (xpbdLayer2D.m:388-444): elastic contacts by
+ * penaltyConstraint2D.projectElastic (penaltyConstraint2D.m:4-26), computed
+ * from the incoming xi and written in contact order; then, when rigid
+ * contacts exist and useRigidConstacts is set, penaltyConstraint2D.projectRigid
+ * (:28-58) for every rigid contact against its body's pose at that point,
+ * the per-body sums applied (COM += sum p/M, angle += sum (Iinv/2) r x p),
+ * and the rigid contact vertices set to their rigid positions.  nb is the
+ * body's rigid-contact count; w = (1/M)/nb + Iinv (r x n)^2 and the
+ * denominator is 2 w + alpha~. */
+static void contacts2d(bodies2d_t *b, const ora_contacts *c, const int32_t *vb,
+                       const double *ri, const double *mass, double *cl, double *xi,
+                       double h, double resistance, int useRigid) {
+  if (!c || c->K == 0) return;
+  double h2 = h * h;
+  double *newx = (double *)malloc(sizeof(double) * 2 * c->K);
+  for (int i = 0; i < c->K; ++i) {
+    int vv = c->vertex[i];
+    if (vb[vv] != 0) continue;
+    const double *n = c->normal + 2 * i, *pc = c->point + 2 * i;
+    double x[2] = {xi[2 * vv], xi[2 * vv + 1]};
+    double phi = n[0] * (x[0] - pc[0]) + n[1] * (x[1] - pc[1]);
+    if (phi > 0) phi = 0;
+    /* gradCminv = n*2*minv, den = gradCminv.n + alpha/h^2, dx = gradCminv * dlambda */
+    double C = resistance * phi;
+    double aT = c->compliance[i] / h2;
+    double num = -(C + aT * cl[i]);
+    double gm0 = n[0] * 2 * (1.0 / mass[2 * vv]);
+    double gm1 = n[1] * 2 * (1.0 / mass[2 * vv + 1]);
+    double den = (gm0 * n[0] + gm1 * n[1]) + aT;
+    double dl = num / den;
+    newx[2 * i] = x[0] + gm0 * dl;
+    newx[2 * i + 1] = x[1] + gm1 * dl;
+    cl[i] = cl[i] + dl;
+  }
+  for (int i = 0; i < c->K; ++i) {
+    int vv = c->vertex[i];
+    if (vb[vv] != 0) continue;
+    xi[2 * vv] = newx[2 * i];
+    xi[2 * vv + 1] = newx[2 * i + 1];
+  }
+  free(newx);
+  int nr = 0;
+  for (int i = 0; i < c->K; ++i) nr += vb[c->vertex[i]] != 0;
+  if (nr < 1 || !useRigid) return;
+  int nb = b->nb, n1 = nb > 0 ? nb : 1;
+  int *cnt = (int *)calloc(n1, sizeof(int));
+  double *dC = (double *)calloc(2 * n1, sizeof(double));
+  double *dA = (double *)calloc(n1, sizeof(double));
+  for (int i = 0; i < c->K; ++i) {
+    int k = vb[c->vertex[i]] - 1;
+    if (k >= 0) cnt[k]++;
+  }
+  for (int i = 0; i < c->K; ++i) {
+    int v = c->vertex[i], k = vb[v] - 1;
+    if (k < 0) continue;
+    const double *n = c->normal + 2 * i, *pc = c->point + 2 * i;
+    double rx[2], r[2];
+    rigid_pos2d(b->angle[k], ri + 2 * v, b->COM + 2 * k, rx);
+    r[0] = rx[0] - b->COM[2 * k];
+    r[1] = rx[1] - b->COM[2 * k + 1];
+    double phi = (xi[2 * v] - pc[0]) * n[0] + (xi[2 * v + 1] - pc[1]) * n[1];
+    if (phi > 0) phi = 0;
+    double C = resistance * phi;
+    double Iinv = 1.0 / b->I[k];
+    double invM = 1.0 / b->M[k];
+    double invMs = invM / cnt[k];
+    double crn = cross2d(r, n);
+    double w = invMs + Iinv * (crn * crn);
+    double aT = c->compliance[i] / h2;
+    double dl = (-C - aT * cl[i]) / (2 * w + aT);
+    double pp[2] = {dl * n[0], dl * n[1]};
+    cl[i] = cl[i] + dl;
+    dC[2 * k] = dC[2 * k] + pp[0] * invM;
+    dC[2 * k + 1] = dC[2 * k + 1] + pp[1] * invM;
+    dA[k] = dA[k] + (0.5 * Iinv) * cross2d(r, pp);
+  }
+  for (int k = 0; k < nb; ++k) {
+    b->COM[2 * k] = b->COM[2 * k] + dC[2 * k];
+    b->COM[2 * k + 1] = b->COM[2 * k + 1] + dC[2 * k + 1];
+    b->angle[k] = b->angle[k] + dA[k];
+  }
+  for (int i = 0; i < c->K; ++i) {
+    int v = c->vertex[i], k = vb[v] - 1;
+    if (k < 0) continue;
+    rigid_pos2d(b->angle[k], ri + 2 * v, b->COM + 2 * k, xi + 2 * v);
+  }
+  free(cnt); free(dC); free(dA);
+}
+
+/* rotateAllVerts2D.m:1-7: rigid vertices to R(angle) ri + COM. */
+static void rotate_all_verts2d(int nV, const int32_t *vb, const double *ri,
+                               const bodies2d_t *b, double *xi) {
+  for (int i = 0; i < nV; ++i) {
+    int k = vb[i] - 1;
+    if (k < 0) continue;
+    rigid_pos2d(b->angle[k], ri + 2 * i, b->COM + 2 * k, xi + 2 * i);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* 2D substep                                                           */
+/* ------------------------------------------------------------------ */
+
 /* xpbdLayer2D.integrate (xpbdLayer2D.m:793-917) with layerSolve (:606-738)
- * and projectGS (:105-201), for layer sets without rigid elements (the
- * plain method).  Returns -2 if a layer holds rigid elements: the 2D rigid
- * layer is not restated here. */
+ * and projectGS (:105-201).  Per layer the rigid bodies are rebuilt from the
+ * current positions (:676-681); unlike 3D they are not stepped.  Contact
+ * lambdas start at zero in every layer (projectGS does not return them).
+ * Pins: the numpy restatement of the 2D layer (tests), plus the pieces'
+ * closed forms. */
 int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h, double *p,
                     double *v, const double *prevp, const double *prevPrevp,
                     const ora_contacts *c, ora_step_out *out) {
@@ -1476,8 +1702,6 @@ int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h, double *p,
   for (int i = 0; i < nV; ++i) f[2 * i + 1] = f[2 * i + 1] + m->mass[2 * i + 1] * s->gravity;
   layers_t L;
   build_layers(m, s, p, prevPrevp, h, &L);
-  for (size_t k = 0; k < (size_t)nT * L.nSorted; ++k)
-    if (L.sets[k] != 0) { layers_free(&L); free(f); return -2; }
   if (out && out->setsPerLayer)
     memcpy(out->setsPerLayer, L.sets, sizeof(int32_t) * (size_t)nT * L.nSorted);
   if (out && out->numRigids) memcpy(out->numRigids, L.numRigids, sizeof(int32_t) * L.nSorted);
@@ -1492,7 +1716,7 @@ int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h, double *p,
   }
   double *lambda = (double *)calloc(3 * (size_t)(nT > 0 ? nT : 1), sizeof(double));
   int K = c ? c->K : 0;
-  double *cl = (double *)calloc(K > 0 ? K : 1, sizeof(double));
+  double *clL = (double *)calloc(K > 0 ? K : 1, sizeof(double));
   monitor_t mo;
   double dummy[1];
   mo.arr = out && out->residualArray ? out->residualArray : dummy;
@@ -1501,47 +1725,33 @@ int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h, double *p,
   mo.errorChangeDecreased = 0;
   monitor_push(&mo, 0.0);
   uint8_t *isEl = (uint8_t *)malloc(nT > 0 ? nT : 1);
-  for (int e = 0; e < nT; ++e) isEl[e] = 1;
+  double *ri = (double *)calloc(N2 > 0 ? N2 : 1, sizeof(double));
+  int32_t *bverts = (int32_t *)malloc(sizeof(int32_t) * (nV > 0 ? nV : 1));
   int64_t projections = 0;
   int layersRun = 0;
   for (int layer = 0; layer < s->nLayers; ++layer) {
     double setPercentStop = (layer == 0) ? s->stopAtImprovementPercent : 0;
     int isLast = layer == s->nLayers - 1;
-    double *clL = (double *)malloc(sizeof(double) * (K > 0 ? K : 1));
-    memcpy(clL, cl, sizeof(double) * (K > 0 ? K : 1));
+    int col = L.map[layer];
+    const int32_t *sets = L.sets + (size_t)nT * col;
+    const int32_t *vb = L.vertexRigidBody + (size_t)nV * col;
+    const uint8_t *isB = L.isBoundary + (size_t)nV * col;
+    int nElastic = 0;
+    for (int e = 0; e < nT; ++e) { isEl[e] = sets[e] == 0; nElastic += isEl[e]; }
+    bodies2d_t b;
+    bodies2d_alloc(&b, L.numRigids[col]);
+    rigid_properties2d(nV, vb, xi, m->mass, &b, ri);
+    int nbv = 0;
+    for (int i = 0; i < nV; ++i) if (isB[i]) bverts[nbv++] = i;
+    double *lamB = (double *)calloc(nbv > 0 ? nbv : 1, sizeof(double));
+    for (int i = 0; i < K; ++i) clL[i] = 0;
     int it = 1, stopSolve = 0, iters = 0;
     while (it <= s->iterations[layer] || (!stopSolve && s->runUntil >= 0 && isLast)) {
       it = it + 1;
       ora_stvk_voigt_constraints2d(m, isEl, h, lambda, xi, prevp);
-      projections += nT;
-      /* elastic contacts (xpbdLayer2D.m:388-403, penaltyConstraint2D) */
-      if (c && c->K > 0) {
-        double h2 = h * h;
-        double *newx = (double *)malloc(sizeof(double) * 2 * c->K);
-        for (int i = 0; i < c->K; ++i) {
-          int vv = c->vertex[i];
-          const double *n = c->normal + 2 * i, *pc = c->point + 2 * i;
-          double x[2] = {xi[2 * vv], xi[2 * vv + 1]};
-          double phi = n[0] * (x[0] - pc[0]) + n[1] * (x[1] - pc[1]);
-          if (phi > 0) phi = 0;
-          /* penaltyConstraint2D.projectElastic (penaltyConstraint2D.m:4-26):
-           * gradCminv = n*2*minv, den = gradCminv.n + alpha/h^2,
-           * dx = gradCminv * dlambda. */
-          double C = s->contactResistance * phi;
-          double aT = c->compliance[i] / h2;
-          double num = -(C + aT * clL[i]);
-          double gm0 = n[0] * 2 * (1.0 / m->mass[2 * vv]);
-          double gm1 = n[1] * 2 * (1.0 / m->mass[2 * vv + 1]);
-          double den = (gm0 * n[0] + gm1 * n[1]) + aT;
-          double dl = num / den;
-          newx[2 * i] = x[0] + gm0 * dl;
-          newx[2 * i + 1] = x[1] + gm1 * dl;
-          clL[i] = clL[i] + dl;
-        }
-        for (int i = 0; i < c->K; ++i)
-          for (int d = 0; d < 2; ++d) xi[2 * c->vertex[i] + d] = newx[2 * i + d];
-        free(newx);
-      }
+      projections += nElastic;
+      boundary2d(&b, nbv, bverts, vb, ri, m->mass, s->boundaryCompliance, lamB, xi, h);
+      contacts2d(&b, c, vb, ri, m->mass, clL, xi, h, s->contactResistance, s->useRigidContacts);
       for (int q = 0; q < m->nPinnedDOFs; ++q) xi[m->pinnedDOFs[q]] = prevp[m->pinnedDOFs[q]];
       ++iters;
       int endLayer = 0;
@@ -1551,7 +1761,9 @@ int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h, double *p,
       }
       if (endLayer || stopSolve) break;
     }
-    free(clL);
+    rotate_all_verts2d(nV, vb, ri, &b, xi);
+    free(lamB);
+    bodies2d_free(&b);
     if (out) out->itersPerLayer[layer] = iters;
     layersRun = layer + 1;
     if (stopSolve) break;
@@ -1567,7 +1779,7 @@ int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h, double *p,
     out->layersRun = layersRun;
     out->elasticProjections = projections;
   }
-  free(f); free(pinnedDof); free(xi); free(lambda); free(cl); free(isEl);
+  free(f); free(pinnedDof); free(xi); free(lambda); free(clL); free(isEl); free(ri); free(bverts);
   layers_free(&L);
   return 0;
 }
@@ -1801,3 +2013,45 @@ void ora_solve_pcg(int n, const int32_t *rp, const int32_t *ci, const double *v,
   }
   free(r); free(z); free(d); free(Ap);
 }
+
+/* ------------------------------------------------------------------ */
+/* test hooks: the 2D rigid-layer steps on explicit body state          */
+/* ------------------------------------------------------------------ */
+
+void ora_rigid_properties2d(int nV, int nb, const int32_t *vb, const double *p,
+                            const double *mass, double *M, double *COM, double *I,
+                            double *ri) {
+  bodies2d_t b;
+  bodies2d_alloc(&b, nb);
+  rigid_properties2d(nV, vb, p, mass, &b, ri);
+  for (int k = 0; k < nb; ++k) {
+    M[k] = b.M[k];
+    COM[2 * k] = b.COM[2 * k];
+    COM[2 * k + 1] = b.COM[2 * k + 1];
+    I[k] = b.I[k];
+  }
+  bodies2d_free(&b);
+}
+
+static void bodies2d_wrap(bodies2d_t *b, int nb, double *M, double *COM, double *angle,
+                          double *I) {
+  b->nb = nb; b->M = M; b->COM = COM; b->angle = angle; b->I = I;
+}
+
+void ora_boundary2d(int nb, double *M, double *COM, double *angle, double *I, int nbv,
+                    const int32_t *bverts, const int32_t *vb, const double *ri,
+                    const double *mass, double alpha, double *lamB, double *xi, double h) {
+  bodies2d_t b;
+  bodies2d_wrap(&b, nb, M, COM, angle, I);
+  boundary2d(&b, nbv, bverts, vb, ri, mass, alpha, lamB, xi, h);
+}
+
+void ora_contacts2d(int nb, double *M, double *COM, double *angle, double *I,
+                    const ora_contacts *c, const int32_t *vb, const double *ri,
+                    const double *mass, double *cl, double *xi, double h, double resistance,
+                    int useRigid) {
+  bodies2d_t b;
+  bodies2d_wrap(&b, nb, M, COM, angle, I);
+  contacts2d(&b, c, vb, ri, mass, cl, xi, h, resistance, useRigid);
+}
+

@@ -196,6 +196,18 @@ int ora_integrate3d(const ora_mesh *m, ora_settings *s, double h, int frame,
                     double *p, double *v, double *oldp, double *oldOldp,
                     double *pinnedx, const ora_contacts *c,
                     ora_step_out *out);
+/* 2D rigid-layer steps on explicit body state (test hooks): bodies from
+ * connectivity, boundary accumulation and contacts (xpbdLayer2D.m:296-444). */
+void ora_rigid_properties2d(int nV, int nb, const int32_t *vb, const double *p,
+                            const double *mass, double *M, double *COM, double *I,
+                            double *ri);
+void ora_boundary2d(int nb, double *M, double *COM, double *angle, double *I, int nbv,
+                    const int32_t *bverts, const int32_t *vb, const double *ri,
+                    const double *mass, double alpha, double *lamB, double *xi, double h);
+void ora_contacts2d(int nb, double *M, double *COM, double *angle, double *I,
+                    const ora_contacts *c, const int32_t *vb, const double *ri,
+                    const double *mass, double *cl, double *xi, double h, double resistance,
+                    int useRigid);
 /* 2D: prevp/prevPrevp are shifted by the caller (simulate.m:332-334). */
 int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h,
                     double *p, double *v, const double *prevp,

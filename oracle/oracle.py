@@ -119,7 +119,7 @@ def settings_from(integ):
     s.giveUpThreshold = integ.giveUpThreshold
     s.lastLayerGiveUpEnabled = int(bool(integ.LastLayerGiveUpEnabled))
     s.stopAtImprovementPercent = integ.StopAtImprovementPercent
-    s.useRigidContacts = int(bool(getattr(integ, "useRigidContacts", True)))
+    s.useRigidContacts = int(integ.rigid_contacts_enabled())
     s.restitution = integ.restitution
     return s
 

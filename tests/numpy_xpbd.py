@@ -61,14 +61,16 @@ class NumpyXpbd:
             Ds = (X[:, :2] - X[:, 2:]).transpose(0, 2, 1)
         return Ds @ self.DmInv
 
-    def sweep(self, x, xo, lam):
+    def sweep(self, x, xo, lam, elastic=None):
         d, h, pairs = self.d, self.h, (PAIRS3 if self.d == 3 else PAIRS2)
         aT = self.A / h ** 2
         bT = self.B * h ** 2
         gam = aT @ bT / h
         color = self.m.elementColor
         for c in range(1, self.m.numColors + 1):
-            es = np.nonzero(color == c)[0]
+            es = np.nonzero((color == c) & (True if elastic is None else elastic))[0]
+            if len(es) == 0:
+                continue
             F = self.F(x)[es]
             E = 0.5 * (np.transpose(F, (0, 2, 1)) @ F - np.eye(d))
             vol = self.m.elV[es]
@@ -158,6 +160,141 @@ class NumpyXpbd:
                 vn = self.v[vid].copy()
                 n = c["normal"][i]
                 self.v[vid] = vn + n * (-vn + min(-e * (n @ vn), 0.0))
+        self.p = x
+        self.lam = lam
+        return x
+
+
+def _cross2(a, b):
+    return a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]
+
+
+class NumpyXpbd2DLayers(NumpyXpbd):
+    """2D multi-layer substep with rigid layers, vectorised: per layer the
+    bodies are the given element sets; each body is a rigid pose (COM, angle)
+    with mass sum(m), inertia sum(m |r|^2) built from the current positions;
+    per iteration the elastic elements are swept, the boundary vertices pull
+    their bodies (mass split over the body's boundary vertices, half the
+    angular impulse), contacts act on elastic vertices directly and on rigid
+    vertices through their body, pins snap; at the end of the layer every
+    rigid vertex takes its body's pose.  The layer sets come from outside
+    (they are checked bit-exactly elsewhere); no residual monitor."""
+
+    def __init__(self, mesh, gravity, h, iterations, layers_map, contact=None, restitution=1e-5,
+                 contact_resistance=1.0, boundary_compliance=1e-8, use_rigid_contacts=True):
+        super().__init__(mesh, gravity, h, 0, contact=contact, restitution=restitution,
+                         contact_resistance=contact_resistance)
+        self.layer_iters = list(iterations)
+        self.layers_map = [int(k) - 1 for k in layers_map]
+        self.alpha_b = boundary_compliance
+        self.use_rigid_contacts = use_rigid_contacts
+
+    @staticmethod
+    def _rot(ang):
+        c, s = np.cos(ang), np.sin(ang)
+        return np.stack([np.stack([c, -s], -1), np.stack([s, c], -1)], -2)
+
+    def _pose(self, verts, body, COM, ang, ri):
+        return np.einsum("kij,kj->ki", self._rot(ang[body]), ri[verts]) + COM[body]
+
+    def _apply(self, body, p, r, M, I, COM, ang, nb):
+        for d in range(2):
+            COM[:, d] += np.bincount(body, weights=p[:, d] / M[body], minlength=nb)
+        ang += np.bincount(body, weights=0.5 / I[body] * _cross2(r, p), minlength=nb)
+
+    def step_layers(self, sets):
+        """One substep with per-layer body ids sets[:, col] (nT x nSorted)."""
+        h, m = self.h, self.mass[:, 0]
+        c = self.find_contacts()
+        unp = np.ones(self.p.size, bool)
+        unp[self.pinned] = False
+        unp = unp.reshape(-1, 2)
+        f = np.zeros_like(self.p)
+        f[:, 1] = self.mass[:, 1] * self.g
+        xo = self.p.copy()
+        v = self.v + np.where(unp, h * f / self.mass, 0.0)
+        x = self.p + np.where(unp, h * v, 0.0)
+        lam = np.zeros((self.m.nT, 3))
+        for layer, col in enumerate(self.layers_map):
+            bid = sets[:, col]
+            rigid_el = bid > 0
+            nb = int(bid.max()) if rigid_el.any() else 0
+            vb = np.zeros(self.m.nV, np.int64)
+            vb[self.T[rigid_el].reshape(-1)] = np.repeat(bid[rigid_el], 3)
+            on_elastic = np.zeros(self.m.nV, bool)
+            on_elastic[self.T[~rigid_el].reshape(-1)] = True
+            bverts = np.nonzero((vb > 0) & on_elastic)[0]
+            rig = np.nonzero(vb > 0)[0]
+            kb = vb[rig] - 1
+            M = np.bincount(kb, weights=m[rig], minlength=nb)
+            COM = np.stack([np.bincount(kb, weights=x[rig, d] * m[rig], minlength=nb)
+                            for d in range(2)], 1) / np.maximum(M, 1e-300)[:, None]
+            ri = np.zeros_like(x)
+            ri[rig] = x[rig] - COM[kb]
+            I = np.bincount(kb, weights=m[rig] * (ri[rig] ** 2).sum(1), minlength=nb)
+            ang = np.zeros(nb)
+            lamB = np.zeros(len(bverts))
+            cl = np.zeros(len(c["vertex"])) if c is not None else None
+            for it in range(self.layer_iters[layer]):
+                x, lam = self.sweep(x, xo, lam, elastic=~rigid_el)
+                if len(bverts):
+                    k = vb[bverts] - 1
+                    xr = self._pose(bverts, k, COM, ang, ri)
+                    r = xr - COM[k]
+                    dvec = xr - x[bverts]
+                    C = np.sqrt((dvec ** 2).sum(1))
+                    with np.errstate(invalid="ignore", divide="ignore"):
+                        n = dvec / C[:, None]
+                    n[C == 0] = 0
+                    nbc = np.bincount(k, minlength=nb)[k]
+                    w = (1.0 / M[k]) / nbc + _cross2(r, n) ** 2 / I[k]
+                    aT = self.alpha_b / h ** 2
+                    dl = (-C - aT * lamB) / (1.0 / m[bverts] + w + aT)
+                    lamB += dl
+                    self._apply(k, dl[:, None] * n, r, M, I, COM, ang, nb)
+                    x[bverts] = self._pose(bverts, k, COM, ang, ri)
+                if c is not None and len(c["vertex"]):
+                    cv = c["vertex"]
+                    el = vb[cv] == 0
+                    for i in np.nonzero(el)[0]:
+                        vv, nrm = cv[i], c["normal"][i]
+                        phi = min(nrm @ (x[vv] - c["point"][i]), 0.0)
+                        aT = c["compliance"][i] / h ** 2
+                        gm = nrm * 2 / self.mass[vv]
+                        dl = -(self.res * phi + aT * cl[i]) / (gm @ nrm + aT)
+                        x[vv] = x[vv] + gm * dl
+                        cl[i] += dl
+                    ri_idx = np.nonzero(~el)[0]
+                    if len(ri_idx) and self.use_rigid_contacts:
+                        verts = cv[ri_idx]
+                        k = vb[verts] - 1
+                        xr = self._pose(verts, k, COM, ang, ri)
+                        r = xr - COM[k]
+                        nrm = c["normal"][ri_idx]
+                        phi = np.minimum(((x[verts] - c["point"][ri_idx]) * nrm).sum(1), 0.0)
+                        nbc = np.bincount(k, minlength=nb)[k]
+                        w = (1.0 / M[k]) / nbc + _cross2(r, nrm) ** 2 / I[k]
+                        aT = c["compliance"][ri_idx] / h ** 2
+                        dl = (-self.res * phi - aT * cl[ri_idx]) / (2 * w + aT)
+                        cl[ri_idx] += dl
+                        self._apply(k, dl[:, None] * nrm, r, M, I, COM, ang, nb)
+                        x[verts] = self._pose(verts, k, COM, ang, ri)
+                xf = x.reshape(-1)
+                xf[self.pinned] = xo.reshape(-1)[self.pinned]
+                x = xf.reshape(-1, 2)
+            if len(rig):
+                x[rig] = self._pose(rig, kb, COM, ang, ri)
+        self.v = (x - xo) / h
+        if c is not None:
+            e = self.e
+            for i, vid in enumerate(c["vertex"]):
+                vn = self.v[vid].copy()
+                if np.linalg.norm(vn) <= 2 * self.g * h:
+                    ee = 0.0
+                else:
+                    ee = e
+                n = c["normal"][i]
+                self.v[vid] = vn + n * (-vn + min(-ee * (n @ vn), 0.0))
         self.p = x
         self.lam = lam
         return x

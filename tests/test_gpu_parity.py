@@ -75,6 +75,44 @@ def test_cantilever2d_fp64_200_steps(oracle_lib):
         sim.close()
 
 
+@pytest.mark.parametrize("layers,iters,pct", [((90, 50, 0), (3, 3, 6), 0.0),
+                                              ((75, 0), (4, 8), 99.0)])
+def test_cantilever2d_rigid_layers_fp64_match_oracle(layers, iters, pct, oracle_lib):
+    """2D multi-layer substeps with rigid bodies (boundary coupling, rigid and
+    elastic ground contacts, rotateAllVerts2D) against the oracle, with the
+    residual monitor on (and the relative stop rule in the second case)."""
+    import copy
+    from multilayersolver2024_amd import scenes
+    from multilayersolver2024_amd.integrator import PlaneContactFinder, XpbdLayer2D
+    from multilayersolver2024_amd.mesh import Mesh
+    m = Mesh(scenes.cantilever2d(20, 6, cell=0.05))
+    integ = XpbdLayer2D()
+    integ.Gravity = -9.81
+    integ.setLayers(list(layers))
+    integ.iterations = list(iters)
+    integ.computeResiduals = True
+    integ.StopAtImprovementPercent = pct
+    plane = PlaneContactFinder([0, 1], [0, -0.004])
+    sim = Simulation(m, integ, 0.01, plane, precision=L.MLX_FP64)
+    osim = O.OracleSim(m, copy.deepcopy(integ), 0.01, plane)
+    try:
+        nb = 0
+        for k in range(30):
+            r = sim.step()
+            o = osim.step()
+            p, v, lam, K = sim.download()
+            nb = max(nb, int(o["numRigids"].max()))
+            _say("2d layers", k, r.itersPerLayer, o["iters"], K, rel(p, osim.p))
+            assert r.itersPerLayer == o["iters"], k
+            assert K == osim.last_contacts, k
+            assert rel(p, osim.p) < 1e-9, k
+            assert rel(v, osim.v) < 1e-7, k
+            assert rel(lam, osim.lambdas()) < 1e-9, k
+        assert nb > 0
+    finally:
+        sim.close()
+
+
 def test_graph_replay_matches_eager(oracle_lib):
     """Iteration chunks replayed as a captured hipGraph give the same bits as
     direct launches, substep after substep (eager work and syncs between)."""
