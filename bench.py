@@ -14,10 +14,13 @@ reductions in FP64.
 A step is one integrate() substep: layer maps, layerSolve to the stop rule,
 velocity update.  value = tet projections / s over the timed steps (every
 elastic element update in every iteration); ms_per_step = wall ms/substep.
-The roofline entry is for the element-sweep kernel, timed with a HIP event
-pair per launch on its stream over --profile-steps substeps that follow the
-timed ones (the timed substeps replay captured hipGraphs, which cannot carry
-per-launch events).  cpu_baseline is the FP64 CPU oracle
+The roofline entry is for the element-sweep kernel: over --profile-steps
+substeps that follow the timed ones, one HIP event pair (no system fence)
+brackets the back-to-back color launches of every iteration on the mesh's
+stream (direct launches; the timed substeps replay the same kernels from
+captured hipGraphs), and the average launch duration is that time over the
+number of launches.  `phases` splits the timed substeps' host wall time into
+contacts + layer maps and layerSolve.  cpu_baseline is the FP64 CPU oracle
 (a restatement of the reference path, one core) on a bounded sample.
 """
 import argparse
@@ -43,6 +46,10 @@ HBM_PEAK_GBS = 8000.0
 # + x write 48 + x_old 48 + 1/m 16 + DmInv 36 + vol 4 + material
 # (alpha_d, alpha_o, alpha_s, alpha1) 16 + lambda read 24 + lambda write 24.
 BYTES_PER_TET = {"fp32": 280, "fp64": 544}
+# Algorithmic bytes of one element's residual evaluation in FP32: 4 vertex
+# ids 16 + x read 48 + DmInv 36 + vol 4 + (alpha_d, alpha_o, alpha_s) 12 +
+# lambda read 24 (FP64: 16 + 96 + 72 + 8 + 24 + 48).
+RESIDUAL_BYTES_PER_TET = {"fp32": 140, "fp64": 264}
 
 
 def log(*a):
@@ -152,6 +159,12 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--hanging-stop", type=int, default=1000,
                     help="iteration cap of the stop rule (reference default 1000)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="replay iteration chunks as hipGraphs (1) or launch directly (0)")
+    ap.add_argument("--persistent", type=int, default=0,
+                    help="run layers without rigid elements in the persistent cooperative loop")
+    ap.add_argument("--persist-wg", type=int, default=0,
+                    help="workgroups of the persistent iteration loop (0 = from the largest color)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-warmup", type=int, default=2)
     args = ap.parse_args()
@@ -169,11 +182,18 @@ def main():
     sim = Simulation(mesh, make_integrator(hanging_stop=args.hanging_stop), h, plane,
                      precision=prec)
     lib = L.lib()
+    sim.dmesh.set_option(L.MLX_OPT_GRAPHS, args.graphs)
+    sim.dmesh.set_option(L.MLX_OPT_PERSISTENT, args.persistent)
+    sim.dmesh.set_option(L.MLX_OPT_PERSIST_WG, args.persist_wg)
     ncol, _ = sim.dmesh.info()
     for _ in range(args.warmup):
         sim.step()
     log(f"warmup done ({args.warmup} substeps)")
     L.check("mlx_synchronize", lib.mlx_synchronize())
+    ph = [C.c_double() for _ in range(3)]
+    nsub = C.c_int64()
+    L.check("mlx_profile_phases", lib.mlx_profile_phases(sim.dmesh.handle, *[C.byref(x) for x in ph],
+                                                         C.byref(nsub)))
     t0 = time.perf_counter()
     proj, iters, contacts = 0, [], 0
     for _ in range(args.steps):
@@ -182,24 +202,53 @@ def main():
         iters.append(sum(r.itersPerLayer))
     L.check("mlx_synchronize", lib.mlx_synchronize())
     t1 = time.perf_counter()
-    # Kernel timing window: the next substeps with a HIP event pair around
-    # every sweep launch on the mesh's stream (direct launches, since the
-    # events are per launch; the kernels are the ones the graphs replay).
+    L.check("mlx_profile_phases", lib.mlx_profile_phases(sim.dmesh.handle, *[C.byref(x) for x in ph],
+                                                         C.byref(nsub)))
+    phases = {"layer_maps_ms_per_step": ph[0].value / max(nsub.value, 1),
+              "layer_solve_ms_per_step": ph[1].value / max(nsub.value, 1),
+              "integrate_ms_per_step": ph[2].value / max(nsub.value, 1)}
+    # Kernel timing window: the next substeps.  The persistent loop (layers
+    # without rigid elements) is timed with one HIP event pair per launch;
+    # the per-color sweeps of the rigid layers with one pair around each
+    # iteration's back-to-back color launches.
     log(f"timed: {args.steps} substeps in {t1 - t0:.3f} s")
     L.check("mlx_profile_enable", lib.mlx_profile_enable(sim.dmesh.handle, 1))
     for _ in range(args.profile_steps):
         sim.step()
-    ms = C.c_double()
+    ms, pms = C.c_double(), C.c_double()
     launches, elems = C.c_int64(), C.c_int64()
+    plaunches, pelems, piters = C.c_int64(), C.c_int64(), C.c_int64()
     L.check("mlx_profile_read", lib.mlx_profile_read(sim.dmesh.handle, C.byref(ms),
                                                      C.byref(launches), C.byref(elems)))
+    L.check("mlx_profile_read_persistent", lib.mlx_profile_read_persistent(
+        sim.dmesh.handle, C.byref(pms), C.byref(plaunches), C.byref(pelems), C.byref(piters)))
     L.check("mlx_profile_enable", lib.mlx_profile_enable(sim.dmesh.handle, 0))
     _, _, _, contacts = sim.download()
     sim.close()
     wall = t1 - t0
     bpt = BYTES_PER_TET[args.precision]
+    ftype = "float" if args.precision == "fp32" else "double"
     sweep_s = ms.value / 1e3
-    achieved = (elems.value * bpt) / sweep_s / 1e9 if sweep_s > 0 else 0.0
+    sweeps = {"kernel": f"k_sweep3d<{ftype}>", "ms": ms.value, "launches": launches.value,
+              "elements": elems.value, "bytes_per_unit": bpt,
+              "avg_launch_us": 1e3 * ms.value / max(launches.value, 1),
+              "achieved_GBs": (elems.value * bpt) / sweep_s / 1e9 if sweep_s > 0 else 0.0}
+    if plaunches.value > 0:
+        # one unit = one element in one iteration: its projection plus its
+        # residual evaluation (the monitor runs every iteration)
+        bpu = bpt + RESIDUAL_BYTES_PER_TET[args.precision]
+        pk_s = pms.value / 1e3
+        achieved = pelems.value * bpu / pk_s / 1e9
+        roof = {"kernel": f"k_persist<{ftype},3>", "bytes_per_unit": bpu,
+                "avg_launch_us": 1e3 * pms.value / plaunches.value, "launches": plaunches.value,
+                "elements_per_launch": pelems.value / plaunches.value,
+                "iterations_per_launch": piters.value / plaunches.value,
+                "per_color_sweeps": sweeps}
+    else:
+        achieved = sweeps["achieved_GBs"]
+        roof = {"kernel": sweeps["kernel"], "bytes_per_unit": bpt,
+                "avg_launch_us": sweeps["avg_launch_us"], "launches": launches.value,
+                "elements_per_launch": elems.value / max(launches.value, 1)}
     line = {
         "metric": "ms/substep & tet-proj/s to rel. residual 1e-4: plain vs learned+batched, 1 GPU",
         "value": proj / wall,
@@ -219,24 +268,22 @@ def main():
                         f"iterations [2,4,14], rel. residual 1e-4 stop, ground contact, h=1/60",
             "form": "plain",
             "precision": args.precision,
+            "graphs": bool(args.graphs),
+            "persistent": bool(args.persistent),
             "colors": ncol,
             "mean_iterations_per_substep": float(np.mean(iters)),
             "tet_projections": int(proj),
             "contacts_last_substep": int(contacts),
+            "phases": phases,
         },
-        "roofline": {
+        "roofline": dict({
             "bound": "hbm",
-            "kernel": f"k_sweep3d<{'float' if args.precision == 'fp32' else 'double'}>",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "bytes_per_unit": bpt,
-            "avg_launch_us": 1e3 * ms.value / max(launches.value, 1),
-            "launches": launches.value,
-            "elements_per_launch": elems.value / max(launches.value, 1),
-        },
+        }, **roof),
     }
     if not args.no_batched:
         log("batched leg")

@@ -341,20 +341,46 @@ int mlx_batch_integrate(mlx_batch *batch, mlx_settings *settings, double h,
 /* Measurement                                                          */
 /* ------------------------------------------------------------------ */
 
-/* Times every element-sweep launch of the mesh with a pair of HIP events on
- * the mesh's stream (on = 1), or stops (on = 0).  mlx_profile_read returns
- * the accumulated kernel time (ms), the number of timed launches that did
- * work and the elements they projected, then clears the totals. */
+/* Times the element-sweep launches of the mesh (on = 1) with one pair of
+ * HIP events (no system-scope fence) around the back-to-back color launches
+ * of each iteration on the mesh's stream, or stops (on = 0).  Iterations
+ * run as direct launches while it is on.  mlx_profile_read returns the
+ * accumulated time (ms), the number of sweep launches covered and the
+ * elastic elements they projected, then clears the totals. */
 int mlx_profile_enable(mlx_mesh *mesh, int32_t on);
 int mlx_profile_read(mlx_mesh *mesh, double *sweepMs, int64_t *sweepLaunches,
                      int64_t *sweepElements);
 
+/* Host wall time of the mlx_integrate calls on states of this mesh since the
+ * last read: contact detection plus layer maps (buildLayers and
+ * findBoundaryVertices), the layerSolve proper, and the whole call
+ * (including the velocity updates and the final stream synchronization);
+ * substeps counts the calls.  Clears the totals. */
+int mlx_profile_phases(mlx_mesh *mesh, double *layersMs, double *solveMs, double *totalMs,
+                       int64_t *substeps);
+
+/* Time of the persistent-loop launches (MLX_OPT_PERSISTENT) while profiling
+ * is on: one HIP event pair around each launch on the mesh's stream.
+ * Returns the accumulated ms, launches, elastic elements projected and
+ * iterations run, then clears the totals. */
+int mlx_profile_read_persistent(mlx_mesh *mesh, double *ms, int64_t *launches,
+                                int64_t *elements, int64_t *iterations);
+
 /* Execution options of a mesh.  MLX_OPT_GRAPHS (default 1): iterations run
  * as replays of a captured hipGraph of MLX_OPT_GRAPH_ITERS iterations
  * (default 8, 1..256); 0 launches every kernel directly.  Results are the
- * same either way. */
+ * same either way.  MLX_OPT_PERSISTENT (default 0): layers without rigid
+ * elements run their iterations inside one cooperative launch per chunk of
+ * iterations (colors separated by grid barriers); positions and lambdas
+ * follow the same updates as with per-color launches, the residual is
+ * summed in another order (last-bit differences).  On the 81,920-tet beam
+ * it is slower than graph replays (every barrier's acquire drops the L2),
+ * hence off by default.  MLX_OPT_PERSIST_WG: workgroups of that launch
+ * (0 = from the largest color, 1..256). */
 #define MLX_OPT_GRAPHS 1
 #define MLX_OPT_GRAPH_ITERS 2
+#define MLX_OPT_PERSISTENT 3
+#define MLX_OPT_PERSIST_WG 4
 int mlx_mesh_set_option(mlx_mesh *mesh, int32_t option, int64_t value);
 
 /* hipDeviceSynchronize on the library's device. */

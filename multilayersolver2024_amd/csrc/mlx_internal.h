@@ -143,17 +143,32 @@ struct mlx_mesh {
   int graphNext = 0;
   int useGraphs = 1;
   int graphIters = 8;
+  // persistent iteration loop (layers without rigid elements)
+  int usePersistent = 0;
+  int persistWG = 0;               // 0 = from the largest color
+  mlx::DevBuf d_colors;            // int32 colorOff[numColors] | colorCnt[numColors]
+  mlx::DevBuf d_sync;              // uint32 barrier counter | int32 error flag
 
   // sweep timing (mlx_profile_enable)
   struct Prof {
     bool on = false;
-    std::vector<hipEvent_t> ev;          // 2 per recorded launch
-    std::vector<int64_t> elems;          // elements of each recorded launch
-    std::vector<int32_t> iter;           // iteration index of each launch
+    std::vector<hipEvent_t> ev;          // 2 per recorded iteration of sweeps
+    std::vector<int64_t> elems;          // elastic elements projected in it
+    std::vector<int64_t> nlaunch;        // sweep launches in it
+    std::vector<int32_t> iter;           // its iteration index
     size_t used = 0;
     double ms = 0;
     int64_t launches = 0, elements = 0;
+    hipEvent_t pev[2] = {nullptr, nullptr};  // persistent launch pair
+    double pms = 0;
+    int64_t plaunches = 0, pelements = 0, piters = 0;
   } prof;
+
+  // host wall time of mlx_integrate phases (mlx_profile_phases)
+  struct Phase {
+    double layersMs = 0, solveMs = 0, totalMs = 0;
+    int64_t substeps = 0;
+  } phase;
 };
 
 struct mlx_state {

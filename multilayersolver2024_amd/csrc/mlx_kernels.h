@@ -830,16 +830,12 @@ struct ContactArrays {
 };
 
 // penaltyConstraint.projectElastic (penaltyConstraint.m:4-28), 3D;
-// penaltyConstraint2D.projectElastic (penaltyConstraint2D.m:4-26) in 2D.
+// penaltyConstraint2D.projectElastic (penaltyConstraint2D.m:4-26) in 2D:
+// contact i on an elastic vertex.
 template <typename T, int DIM>
-__global__ __launch_bounds__(kBlock) void k_contacts_elastic(LayerLists L, ContactArrays c,
-                                                             vec4_t<T> *x, const double *minv,
-                                                             double resistance, double h,
-                                                             const Ctl *ctl) {
-  if (iteration_stopped(ctl)) return;
-  const int q = blockIdx.x * kBlock + threadIdx.x;
-  if (q >= L.nec) return;
-  const int i = L.ec_idx[q];
+__device__ __forceinline__ void contact_elastic_one(const ContactArrays &c, int i, vec4_t<T> *x,
+                                                    const double *minv, double resistance,
+                                                    double h) {
   const int v = c.vert[i];
   const double *n = c.normal + 3 * i, *pc = c.point + 3 * i;
   vec4_t<T> p = x[v];
@@ -864,6 +860,17 @@ __global__ __launch_bounds__(kBlock) void k_contacts_elastic(LayerLists L, Conta
   }
   c.lambda[i] += dl;
   x[v] = p;
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void k_contacts_elastic(LayerLists L, ContactArrays c,
+                                                             vec4_t<T> *x, const double *minv,
+                                                             double resistance, double h,
+                                                             const Ctl *ctl) {
+  if (iteration_stopped(ctl)) return;
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= L.nec) return;
+  contact_elastic_one<T, DIM>(c, L.ec_idx[q], x, minv, resistance, h);
 }
 
 // penaltyConstraint.projectRigid (penaltyConstraint.m:30-63) with the
@@ -988,15 +995,21 @@ __global__ __launch_bounds__(kBlock) void k_contacts_rigid2d(LayerLists L, Conta
 
 // xi(pinnedDOFs) = pinnedx (xpbdLayer3D.m:333; 2D pins to oldp, :187).
 template <typename T, int DIM>
+__device__ __forceinline__ void pin_one(const int32_t *dof, const T *val, vec4_t<T> *x,
+                                        int64_t q) {
+  const int d = dof[q];
+  const int v = d / DIM, c = d % DIM;
+  T *px = reinterpret_cast<T *>(&x[v]);
+  px[c] = val[q];
+}
+
+template <typename T, int DIM>
 __global__ __launch_bounds__(kBlock) void k_pins(int64_t n, const int32_t *dof, const T *val,
                                                  vec4_t<T> *x, const Ctl *ctl) {
   if (iteration_stopped(ctl)) return;
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= n) return;
-  const int d = dof[q];
-  const int v = d / DIM, c = d % DIM;
-  T *px = reinterpret_cast<T *>(&x[v]);
-  px[c] = val[q];
+  pin_one<T, DIM>(dof, val, x, q);
 }
 
 // ---------------------------------------------------------------------------
@@ -1058,8 +1071,43 @@ __device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const in
 
 // Element part of simpleEvaluateAllConstraintResiduals (xpbdLayer3D.m:546-568;
 // 2D xpbdLayer2D.m:477-496): per element sum_i (|C_i| - min(|C_i|,
-// |(alpha~ lambda)_i|))^2, reduced per workgroup in double.  lamElem gives,
-// for the 2D quirk, the slot whose lambda column feeds each page entry.
+// |(alpha~ lambda)_i|))^2 in double.  lamSlot2d gives, for the 2D quirk, the
+// slot whose lambda column feeds each page entry.
+template <typename T, int DIM>
+__device__ __forceinline__ double residual_elem(const SweepParams<T> &P, int64_t s,
+                                                const int32_t *lamSlot2d) {
+  if (DIM == 3) return residual_tet<T>(P, s);
+  const int64_t ld = P.ld;
+  const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
+  const T vol = P.vol[s];
+  const vec4_t<T> X0 = P.x[P.vid[s]], X1 = P.x[P.vid[ld + s]], X2 = P.x[P.vid[2 * ld + s]];
+  const T D[4] = {P.dminv[s], P.dminv[ld + s], P.dminv[2 * ld + s], P.dminv[3 * ld + s]};
+  const T e[2][2] = {{X0.x - X2.x, X0.y - X2.y}, {X1.x - X2.x, X1.y - X2.y}};
+  T F[2][2];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) F[r][c] = e[0][r] * D[2 * c] + e[1][r] * D[1 + 2 * c];
+  const T c00 = F[0][0] * F[0][0] + F[1][0] * F[1][0];
+  const T c11 = F[0][1] * F[0][1] + F[1][1] * F[1][1];
+  const T c01 = F[0][0] * F[0][1] + F[1][0] * F[1][1];
+  const T half = T(0.5);
+  const T C[3] = {vol * (half * (c00 - T(1))), vol * (half * (c11 - T(1))), vol * (half * c01)};
+  // lambda page e mixes elements: entry r is lambdai(L mod nT, L div nT)
+  // with L = 3e + r (reshape of the nT x 3 array, xpbdLayer2D.m:489).
+  T lp[3];
+  for (int r = 0; r < 3; ++r) {
+    const int32_t code = lamSlot2d[3 * s + r];  // slot*3 + component
+    lp[r] = P.lam[(code % 3) * ld + code / 3];
+  }
+  const T al[3] = {tad * lp[0] + tao * lp[1], tao * lp[0] + tad * lp[1], tas * lp[2]};
+  double acc = 0;
+  for (int i = 0; i < 3; ++i) {
+    const double a = fabs((double)C[i]), cr = fabs((double)al[i]);
+    const double t = a - (a < cr ? a : cr);
+    acc += t * t;
+  }
+  return acc;
+}
+
 template <typename T, int DIM>
 __global__ __launch_bounds__(kBlock) void k_residual_elems(SweepParams<T> P, int64_t nSlots,
                                                            const int32_t *lamSlot2d,
@@ -1068,39 +1116,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_elems(SweepParams<T> P, int
   __shared__ double sh[kBlock];
   const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double acc = 0;
-  if (s < nSlots && P.vid[s] >= 0) {
-    const int64_t ld = P.ld;
-    const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
-    const T vol = P.vol[s];
-    if (DIM == 3) {
-      acc = residual_tet<T>(P, s);
-    } else {
-      const vec4_t<T> X0 = P.x[P.vid[s]], X1 = P.x[P.vid[ld + s]], X2 = P.x[P.vid[2 * ld + s]];
-      const T D[4] = {P.dminv[s], P.dminv[ld + s], P.dminv[2 * ld + s], P.dminv[3 * ld + s]};
-      const T e[2][2] = {{X0.x - X2.x, X0.y - X2.y}, {X1.x - X2.x, X1.y - X2.y}};
-      T F[2][2];
-      for (int r = 0; r < 2; ++r)
-        for (int c = 0; c < 2; ++c) F[r][c] = e[0][r] * D[2 * c] + e[1][r] * D[1 + 2 * c];
-      const T c00 = F[0][0] * F[0][0] + F[1][0] * F[1][0];
-      const T c11 = F[0][1] * F[0][1] + F[1][1] * F[1][1];
-      const T c01 = F[0][0] * F[0][1] + F[1][0] * F[1][1];
-      const T half = T(0.5);
-      const T C[3] = {vol * (half * (c00 - T(1))), vol * (half * (c11 - T(1))), vol * (half * c01)};
-      // lambda page e mixes elements: entry r is lambdai(L mod nT, L div nT)
-      // with L = 3e + r (reshape of the nT x 3 array, xpbdLayer2D.m:489).
-      T lp[3];
-      for (int r = 0; r < 3; ++r) {
-        const int32_t code = lamSlot2d[3 * s + r];  // slot*3 + component
-        lp[r] = P.lam[(code % 3) * ld + code / 3];
-      }
-      const T al[3] = {tad * lp[0] + tao * lp[1], tao * lp[0] + tad * lp[1], tas * lp[2]};
-      for (int i = 0; i < 3; ++i) {
-        const double a = fabs((double)C[i]), cr = fabs((double)al[i]);
-        const double t = a - (a < cr ? a : cr);
-        acc += t * t;
-      }
-    }
-  }
+  if (s < nSlots && P.vid[s] >= 0) acc = residual_elem<T, DIM>(P, s, lamSlot2d);
   double o[1];
   double v1[1] = {acc};
   block_sum<1>(v1, sh, o);
@@ -1113,6 +1129,34 @@ __global__ __launch_bounds__(kBlock) void k_residual_elems(SweepParams<T> P, int
 // value j.  uniqAlpha/uniqCount list the distinct compliance values with
 // multiplicities (3D); in 2D each contact counts once with its own value.
 template <typename T, int DIM>
+__device__ __forceinline__ double residual_contact_one(const ContactArrays &c, int i,
+                                                       const vec4_t<T> *x, double resistance,
+                                                       double h, const double *uniqAlpha,
+                                                       const double *uniqCount, int nUniq) {
+  const int v = c.vert[i];
+  const double *n = c.normal + 3 * i, *pc = c.point + 3 * i;
+  const vec4_t<T> p = x[v];
+  double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]);
+  if (DIM == 3) phi = phi + n[2] * ((double)p.z - pc[2]);
+  if (phi > 0) phi = 0;
+  const double a = fabs(resistance * phi);
+  const double h2 = h * h;
+  double acc = 0;
+  if (DIM == 3) {
+    for (int u = 0; u < nUniq; ++u) {
+      const double cr = fabs((uniqAlpha[u] / h2) * c.lambda[i]);
+      const double t = a - (a < cr ? a : cr);
+      acc += uniqCount[u] * (t * t);
+    }
+  } else {
+    const double cr = fabs((c.compliance[i] / h2) * c.lambda[i]);
+    const double t = a - (a < cr ? a : cr);
+    acc += t * t;
+  }
+  return acc;
+}
+
+template <typename T, int DIM>
 __global__ __launch_bounds__(kBlock) void k_residual_contacts(ContactArrays c, const vec4_t<T> *x,
                                                               double resistance, double h,
                                                               const double *uniqAlpha,
@@ -1122,27 +1166,8 @@ __global__ __launch_bounds__(kBlock) void k_residual_contacts(ContactArrays c, c
   __shared__ double sh[kBlock];
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double acc = 0;
-  if (i < c.K) {
-    const int v = c.vert[i];
-    const double *n = c.normal + 3 * i, *pc = c.point + 3 * i;
-    const vec4_t<T> p = x[v];
-    double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]);
-    if (DIM == 3) phi = phi + n[2] * ((double)p.z - pc[2]);
-    if (phi > 0) phi = 0;
-    const double a = fabs(resistance * phi);
-    const double h2 = h * h;
-    if (DIM == 3) {
-      for (int u = 0; u < nUniq; ++u) {
-        const double cr = fabs((uniqAlpha[u] / h2) * c.lambda[i]);
-        const double t = a - (a < cr ? a : cr);
-        acc += uniqCount[u] * (t * t);
-      }
-    } else {
-      const double cr = fabs((c.compliance[i] / h2) * c.lambda[i]);
-      const double t = a - (a < cr ? a : cr);
-      acc += t * t;
-    }
-  }
+  if (i < c.K)
+    acc = residual_contact_one<T, DIM>(c, i, x, resistance, h, uniqAlpha, uniqCount, nUniq);
   double o[1];
   double v1[1] = {acc};
   block_sum<1>(v1, sh, o);
@@ -1659,6 +1684,134 @@ __global__ __launch_bounds__(kBlock) void k_neohookean2d(SweepParams<T> P, T *nh
     o.x = (o.x + w[a] * gD[a][0] * dlD) + w[a] * gH[a][0] * dlH;
     o.y = (o.y + w[a] * gD[a][1] * dlD) + w[a] * gH[a][1] * dlH;
     P.x[vs[a]] = o;
+  }
+}
+
+}  // namespace mlx
+
+namespace mlx {
+
+// ---------------------------------------------------------------------------
+// persistent iteration loop for layers without rigid bodies
+// ---------------------------------------------------------------------------
+
+// Grid-wide barrier of a cooperative launch: one arrival counter in device
+// memory.  Every thread issues an agent-scope release fence (its own stores
+// are written back past its XCD's L2), the workgroup meets, one thread
+// arrives and waits for `target` arrivals, the workgroup meets again and
+// every thread issues an agent-scope acquire fence (stale cache lines are
+// dropped).  The k-th barrier of a launch waits for k * gridDim.x arrivals.
+// The wait is bounded: a workgroup that never arrives sets *err and the
+// barrier returns false instead of spinning forever.
+constexpr long kBarrierSpinLimit = 1L << 26;
+
+__device__ __forceinline__ bool grid_barrier(unsigned *count, unsigned target, int *err) {
+  __shared__ int s_ok;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    long spins = 0;
+    while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kBarrierSpinLimit) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+    }
+    s_ok = ok;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return s_ok != 0;
+}
+
+template <typename T>
+struct PersistParams {
+  SweepParams<T> P;              // P.ctl is the layer's control block
+  const int32_t *colorOff;       // first slot of each color
+  const int32_t *colorCnt;       // elements of each color
+  int numColors;
+  const int32_t *lamSlot2d;      // 2D residual lambda pages (DIM == 2)
+  ContactArrays ca;
+  const int32_t *ecIdx;          // elastic contacts of the layer
+  int nec;
+  const double *minv;
+  double resistance, h;
+  const double *uniq, *ucnt;     // 3D contact-residual compliance values
+  int nUniq;
+  const int32_t *pinDof;
+  const T *pinVal;
+  int64_t nPin;
+  MonitorParams mp;
+  double *partials;              // gridDim.x entries
+  double *resid;
+  int residCap;
+  unsigned *bar;                 // arrival counter, zero at launch
+  int *err;                      // barrier failure flag
+  int maxIters;
+};
+
+// Up to maxIters projectGS iterations (xpbdLayer3D.m:306-344; 2D
+// xpbdLayer2D.m:174-198) of a layer whose elements are all elastic, in one
+// cooperative launch: the colors in order, each split over the workgroups
+// and closed by a grid barrier; the elastic contacts then the pins (one
+// workgroup, contact order then pin order); the residual (each workgroup a
+// fixed share of the elements, workgroup 0 the contacts) and the monitor.
+// Every workgroup re-reads the control block after the last barrier of an
+// iteration, so all stop together when the monitor sets stopSolve/endLayer.
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void k_persist(PersistParams<T> Q) {
+  __shared__ double sh[kBlock];
+  const int G = gridDim.x, w = blockIdx.x, t = threadIdx.x;
+  const int64_t stride = (int64_t)G * kBlock;
+  Ctl *ctl = const_cast<Ctl *>(Q.P.ctl);
+  unsigned nbar = 0;
+  for (int it = 0; it < Q.maxIters; ++it) {
+    if (iteration_stopped(ctl)) break;
+    for (int c = 0; c < Q.numColors; ++c) {
+      const int cnt = Q.colorCnt[c];
+      if (cnt == 0) continue;
+      const int64_t off = Q.colorOff[c], end = off + cnt;
+      for (int64_t s = off + (int64_t)w * kBlock + t; s < end; s += stride) {
+        if (DIM == 3)
+          project_tet<T>(Q.P, s);
+        else
+          project_tri<T>(Q.P, s);
+      }
+      if (!grid_barrier(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
+    }
+    if (w == 0 && (Q.nec > 0 || Q.nPin > 0)) {
+      for (int q = t; q < Q.nec; q += kBlock)
+        contact_elastic_one<T, DIM>(Q.ca, Q.ecIdx[q], Q.P.x, Q.minv, Q.resistance, Q.h);
+      __syncthreads();
+      for (int64_t q = t; q < Q.nPin; q += kBlock) pin_one<T, DIM>(Q.pinDof, Q.pinVal, Q.P.x, q);
+    }
+    if (!grid_barrier(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
+    if (Q.mp.computeResiduals) {
+      double acc = 0;
+      for (int64_t s = (int64_t)w * kBlock + t; s < Q.P.ld; s += stride)
+        if (Q.P.vid[s] >= 0) acc += residual_elem<T, DIM>(Q.P, s, Q.lamSlot2d);
+      if (w == 0)
+        for (int i = t; i < Q.ca.K; i += kBlock)
+          acc += residual_contact_one<T, DIM>(Q.ca, i, Q.P.x, Q.resistance, Q.h, Q.uniq, Q.ucnt,
+                                              Q.nUniq);
+      double v1[1] = {acc}, o[1];
+      block_sum<1>(v1, sh, o);
+      if (t == 0) Q.partials[w] = o[0];
+      if (!grid_barrier(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
+    }
+    if (w == 0 && t == 0) {
+      ctl->iters += 1;
+      if (Q.mp.computeResiduals) {
+        double sum = 0;
+        for (int q = 0; q < G; ++q) sum += Q.partials[q];
+        monitor_update(*ctl, Q.mp, sqrt(sum), Q.resid, Q.residCap);
+      }
+    }
+    if (!grid_barrier(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
   }
 }
 

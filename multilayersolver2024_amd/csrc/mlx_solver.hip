@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -186,12 +187,15 @@ static SweepParams<T> sweep_params(mlx_mesh &m, double h, const Ctl *ctl) {
   return P;
 }
 
-// Records an event pair around a launch when profiling is on and the pool
-// has room (events are created on demand, kept for the mesh's lifetime).
+// Profiling records one event pair around the color sweeps of an iteration
+// (all launches back to back on the mesh's stream), so the measured time is
+// kernel execution plus the dispatch gaps between consecutive launches, not
+// the cost of an event pair per launch.  Events are created on demand
+// without the system-scope fence and kept for the mesh's lifetime.
 static hipEvent_t prof_event(mlx_mesh &m, size_t i) {
   while (m.prof.ev.size() <= i) {
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     m.prof.ev.push_back(e);
   }
   return m.prof.ev[i];
@@ -203,42 +207,50 @@ template <typename T>
 static void enqueue_sweeps(mlx_mesh &m, SweepParams<T> P, const uint8_t *flags,
                            const int64_t *elasticPerColor = nullptr, int iter = 0) {
   P.flags = flags;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (m.prof.on && m.prof.used < kProfPairs) {
+    e0 = prof_event(m, 2 * m.prof.used);
+    e1 = prof_event(m, 2 * m.prof.used + 1);
+    if (e0 && e1) {
+      int64_t elems = 0, launches = 0;
+      for (int c = 0; c < m.numColors; ++c) {
+        if (m.colorCnt[c] == 0) continue;
+        ++launches;
+        elems += elasticPerColor ? elasticPerColor[c] : m.colorCnt[c];
+      }
+      m.prof.elems.resize(m.prof.used + 1);
+      m.prof.nlaunch.resize(m.prof.used + 1);
+      m.prof.iter.resize(m.prof.used + 1);
+      m.prof.elems[m.prof.used] = elems;
+      m.prof.nlaunch[m.prof.used] = launches;
+      m.prof.iter[m.prof.used] = iter;
+      m.prof.used++;
+      (void)hipEventRecord(e0, m.stream);
+    }
+  }
   for (int c = 0; c < m.numColors; ++c) {
     P.begin = m.colorOff[c];
     P.end = m.colorOff[c] + m.colorCnt[c];
     if (P.end <= P.begin) continue;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (m.prof.on && m.prof.used < kProfPairs) {
-      e0 = prof_event(m, 2 * m.prof.used);
-      e1 = prof_event(m, 2 * m.prof.used + 1);
-      if (e0 && e1) {
-        m.prof.elems.resize(m.prof.used + 1);
-        m.prof.iter.resize(m.prof.used + 1);
-        m.prof.elems[m.prof.used] = elasticPerColor ? elasticPerColor[c] : m.colorCnt[c];
-        m.prof.iter[m.prof.used] = iter;
-        m.prof.used++;
-        (void)hipEventRecord(e0, m.stream);
-      }
-    }
     if (m.dim == 3)
       hipLaunchKernelGGL(k_sweep3d<T>, dim3(blocks_for(P.end - P.begin)), dim3(kBlock), 0,
                          m.stream, P);
     else
       hipLaunchKernelGGL(k_sweep2d<T>, dim3(blocks_for(P.end - P.begin)), dim3(kBlock), 0,
                          m.stream, P);
-    if (e0 && e1) (void)hipEventRecord(e1, m.stream);
   }
+  if (e0 && e1) (void)hipEventRecord(e1, m.stream);
 }
 
-// After a stream sync: fold the recorded launches whose iteration index is
-// below itersDone (the rest returned at once: the layer had stopped).
+// After a stream sync: fold the recorded iterations whose index is below
+// itersDone (the rest returned at once: the layer had stopped).
 static void prof_harvest(mlx_mesh &m, int itersDone) {
   for (size_t i = 0; i < m.prof.used; ++i) {
     if (m.prof.iter[i] >= itersDone || m.prof.elems[i] == 0) continue;
     float ms = 0;
     if (hipEventElapsedTime(&ms, m.prof.ev[2 * i], m.prof.ev[2 * i + 1]) == hipSuccess) {
       m.prof.ms += ms;
-      m.prof.launches += 1;
+      m.prof.launches += m.prof.nlaunch[i];
       m.prof.elements += m.prof.elems[i];
     }
   }
@@ -433,9 +445,42 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   }
 
   const int64_t nPin = (int64_t)m.pinnedDOF.size();
+  // persistent loop: color table on the device, barrier counter, grid size
+  int persistG = 0;
+  if (m.usePersistent) {
+    std::vector<int32_t> ct(2 * (size_t)m.numColors);
+    int64_t maxCnt = 1;
+    for (int c = 0; c < m.numColors; ++c) {
+      ct[c] = (int32_t)m.colorOff[c];
+      ct[m.numColors + c] = (int32_t)m.colorCnt[c];
+      maxCnt = std::max<int64_t>(maxCnt, m.colorCnt[c]);
+    }
+    MLX_TRY(m.d_colors.reserve(sizeof(int32_t) * ct.size() + 4));
+    MLX_CHECK_HIP(hipMemcpyAsync(m.d_colors.ptr, ct.data(), sizeof(int32_t) * ct.size(),
+                                 hipMemcpyHostToDevice, st));
+    MLX_TRY(m.d_sync.reserve(256));
+    persistG = m.persistWG > 0 ? m.persistWG
+                               : (int)std::min<int64_t>(32, std::max<int64_t>(4, (maxCnt + kBlock - 1) / kBlock));
+    int perCU = 0, dev = 0;
+    const void *fn = m.dim == 3 ? reinterpret_cast<const void *>(&k_persist<T, 3>)
+                                : reinterpret_cast<const void *>(&k_persist<T, 2>);
+    MLX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, kBlock, 0));
+    MLX_CHECK_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    MLX_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
+    persistG = std::min(persistG, perCU * prop.multiProcessorCount);
+    if (persistG > kMaxPartials || persistG < 1) {
+      set_error("persistent loop: no valid grid size");
+      return MLX_ERR_STATE;
+    }
+    if (m.prof.on)
+      for (auto &e : m.prof.pev)
+        if (!e) MLX_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+  }
   res.projections = 0;
   res.layersRun = 0;
   Ctl hc_ctl;
+  std::memset(&hc_ctl, 0, sizeof hc_ctl);
   for (int layer = 0; layer < sp.nLayers; ++layer) {
     const int col = sp.layersMap0[layer];
     const bool isLast = layer == sp.nLayers - 1;
@@ -491,6 +536,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
                                    &inf, sizeof(double), hipMemcpyHostToDevice, st));
     }
 
+    hc_ctl.iters = 0;  // k_layer_begin cleared the device count
     int iterEnqueued = 0;
     auto enqueue_iteration = [&]() {
       enqueue_sweeps<T>(m, P, fl, perColor.data() + (size_t)m.numColors * col, iterEnqueued++);
@@ -585,7 +631,65 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       slot.sig = sig;
       return ex;
     };
+    // Layers without rigid elements: the iterations run inside one
+    // cooperative launch (k_persist) per call.
+    const bool persist = m.usePersistent && !hasRigidEl && L.nb == 0;
+    auto run_persistent = [&](int n) -> int {
+      if (n <= 0) return MLX_OK;
+      MLX_CHECK_HIP(hipMemsetAsync(m.d_sync.ptr, 0, 8, st));
+      PersistParams<T> Q;
+      Q.P = P;
+      Q.P.flags = nullptr;
+      Q.colorOff = m.d_colors.as<int32_t>();
+      Q.colorCnt = m.d_colors.as<int32_t>() + m.numColors;
+      Q.numColors = m.numColors;
+      Q.lamSlot2d = d_lamSlot2d;
+      Q.ca = ca;
+      Q.ecIdx = LL.ec_idx;
+      Q.nec = L.nec;
+      Q.minv = minv;
+      Q.resistance = sp.contactResistance;
+      Q.h = sp.h;
+      Q.uniq = d_uniq;
+      Q.ucnt = d_cnt;
+      Q.nUniq = nUniq;
+      Q.pinDof = m.d_pinned.as<int32_t>();
+      Q.pinVal = m.d_pinnedx.as<T>();
+      Q.nPin = nPin;
+      Q.mp = mp;
+      Q.partials = partials;
+      Q.resid = m.d_resid.as<double>();
+      Q.residCap = residCap;
+      Q.bar = m.d_sync.as<unsigned>();
+      Q.err = reinterpret_cast<int *>(m.d_sync.as<unsigned>() + 1);
+      Q.maxIters = n;
+      void *args[] = {&Q};
+      const void *fn = m.dim == 3 ? reinterpret_cast<const void *>(&k_persist<T, 3>)
+                                  : reinterpret_cast<const void *>(&k_persist<T, 2>);
+      const bool timed = m.prof.on && m.prof.pev[0] && m.prof.pev[1];
+      if (timed) MLX_CHECK_HIP(hipEventRecord(m.prof.pev[0], st));
+      MLX_CHECK_HIP(hipLaunchCooperativeKernel(fn, dim3(persistG), dim3(kBlock), args, 0, st));
+      if (timed) MLX_CHECK_HIP(hipEventRecord(m.prof.pev[1], st));
+      const int before = hc_ctl.iters;
+      int32_t sync[2];
+      MLX_CHECK_HIP(hipMemcpyAsync(sync, m.d_sync.ptr, 8, hipMemcpyDeviceToHost, st));
+      MLX_TRY(read_ctl(m, hc_ctl));
+      if (sync[1] != 0) {
+        set_error("persistent iteration loop: a workgroup did not reach the grid barrier");
+        return MLX_ERR_STATE;
+      }
+      if (timed) {
+        float ms = 0;
+        MLX_CHECK_HIP(hipEventElapsedTime(&ms, m.prof.pev[0], m.prof.pev[1]));
+        m.prof.pms += ms;
+        m.prof.plaunches += 1;
+        m.prof.piters += hc_ctl.iters - before;
+        m.prof.pelements += (int64_t)(hc_ctl.iters - before) * nElastic[col];
+      }
+      return MLX_OK;
+    };
     auto run_iterations = [&](int n) -> int {
+      if (persist) return run_persistent(n);
       const int G = m.graphIters;
       if (!m.useGraphs || m.prof.on || n < G) {
         for (int it = 0; it < n; ++it) enqueue_iteration();
@@ -612,7 +716,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
         MLX_TRY(read_ctl(m, hc_ctl));
         if (m.prof.on) prof_harvest(m, hc_ctl.iters);
         guard += chunk;
-        chunk = std::min(chunk * 2, 4 * std::max(m.graphIters, 4));
+        chunk = persist ? 512 : std::min(chunk * 2, 4 * std::max(m.graphIters, 4));
         if (guard > 10000000) {
           set_error("runUntil loop exceeded 1e7 iterations without converging (hangingStop < 0?)");
           return MLX_ERR_STATE;
@@ -1014,9 +1118,11 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_slotOf, &m->d_x, &m->d_xold, &m->d_rv, &m->d_pinnedx,
                          &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
-                         &m->d_tmp, &m->d_nh};
+                         &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
+  for (auto e : m->prof.pev)
+    if (e) (void)hipEventDestroy(e);
   for (auto &g : m->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (m->h_ctl) (void)hipHostFree(m->h_ctl);
@@ -1346,6 +1452,8 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
   vec4_t<T> *x = m.d_x.as<vec4_t<T>>();
   vec4_t<T> *xo = m.d_xold.as<vec4_t<T>>();
   const int64_t nPin = (int64_t)m.pinnedDOF.size();
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
   MLX_TRY(m.d_tmp.reserve(4096));
   // 1. frame-1 / NaN reset of cache.oldp and pinnedx (3D)
   if (dim == 3) {
@@ -1417,6 +1525,7 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
       if (sorted[j] == st->layers[i]) map0[i] = j;
   HostMaps maps;
   MLX_TRY(build_layer_maps<T>(S, sorted, ns, h, maps));
+  const auto t1 = clk::now();
   // 4. shift: cache.oldp = p (3D), prevp = p (2D)
   MLX_CHECK_HIP(hipMemcpyAsync(xo, x, sizeof(vec4_t<T>) * nV, hipMemcpyDeviceToDevice, s));
   // 5. residual velocity (3D) or explicit predictor (2D)
@@ -1464,7 +1573,9 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
     return MLX_ERR_ARG;
   }
   SolveResult res;
+  const auto t2 = clk::now();
   MLX_TRY(layer_solve_dev<T>(m, maps, sp, hc, res));
+  const auto t3 = clk::now();
   st->runUntilResSmallerThan = res.runUntil;
   // 7. velocity update and contact velocity update
   if (dim == 3)
@@ -1484,6 +1595,15 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
   }
   MLX_CHECK_HIP(hipGetLastError());
   MLX_CHECK_HIP(hipStreamSynchronize(s));
+  {
+    auto ms = [](clk::time_point a, clk::time_point b) {
+      return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    m.phase.layersMs += ms(t0, t1);
+    m.phase.solveMs += ms(t2, t3);
+    m.phase.totalMs += ms(t0, clk::now());
+    m.phase.substeps += 1;
+  }
   if (out) {
     out->runUntilResSmallerThan = res.runUntil;
     out->residualCount = res.residualCount;
@@ -1609,6 +1729,35 @@ int mlx_profile_enable(mlx_mesh *m, int32_t on) {
   return MLX_OK;
 }
 
+int mlx_profile_phases(mlx_mesh *m, double *layersMs, double *solveMs, double *totalMs,
+                       int64_t *substeps) {
+  if (!m) {
+    set_error("mlx_profile_phases: null mesh");
+    return MLX_ERR_ARG;
+  }
+  if (layersMs) *layersMs = m->phase.layersMs;
+  if (solveMs) *solveMs = m->phase.solveMs;
+  if (totalMs) *totalMs = m->phase.totalMs;
+  if (substeps) *substeps = m->phase.substeps;
+  m->phase = mlx_mesh::Phase();
+  return MLX_OK;
+}
+
+int mlx_profile_read_persistent(mlx_mesh *m, double *ms, int64_t *launches, int64_t *elements,
+                                int64_t *iterations) {
+  if (!m) {
+    set_error("mlx_profile_read_persistent: null mesh");
+    return MLX_ERR_ARG;
+  }
+  if (ms) *ms = m->prof.pms;
+  if (launches) *launches = m->prof.plaunches;
+  if (elements) *elements = m->prof.pelements;
+  if (iterations) *iterations = m->prof.piters;
+  m->prof.pms = 0;
+  m->prof.plaunches = m->prof.pelements = m->prof.piters = 0;
+  return MLX_OK;
+}
+
 int mlx_profile_read(mlx_mesh *m, double *ms, int64_t *launches, int64_t *elements) {
   if (!m) {
     set_error("mlx_profile_read: null mesh");
@@ -1635,6 +1784,13 @@ int mlx_mesh_set_option(mlx_mesh *m, int32_t option, int64_t value) {
     case MLX_OPT_GRAPH_ITERS:
       if (value < 1 || value > 256) break;
       m->graphIters = (int)value;
+      return MLX_OK;
+    case MLX_OPT_PERSISTENT:
+      m->usePersistent = value != 0;
+      return MLX_OK;
+    case MLX_OPT_PERSIST_WG:
+      if (value < 0 || value > 256) break;
+      m->persistWG = (int)value;
       return MLX_OK;
     default:
       break;

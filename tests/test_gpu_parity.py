@@ -123,6 +123,8 @@ def test_graph_replay_matches_eager(oracle_lib):
     a.dmesh.set_option(L.MLX_OPT_GRAPHS, 0)
     b.dmesh.set_option(L.MLX_OPT_GRAPHS, 1)
     b.dmesh.set_option(L.MLX_OPT_GRAPH_ITERS, 2)
+    for sim in (a, b):
+        sim.dmesh.set_option(L.MLX_OPT_PERSISTENT, 0)
     try:
         for k in range(6):
             ra, rb = a.step(), b.step()
@@ -131,6 +133,43 @@ def test_graph_replay_matches_eager(oracle_lib):
             _say("graph", k, ra.itersPerLayer, rb.itersPerLayer)
             assert ra.itersPerLayer == rb.itersPerLayer and ka == kb
             assert np.array_equal(pa, pb) and np.array_equal(la, lb) and np.array_equal(va, vb)
+    finally:
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("precision,wg", [(L.MLX_FP64, 0), (L.MLX_FP64, 7), (L.MLX_FP32, 0)])
+def test_persistent_loop_matches_per_color_launches(precision, wg, oracle_lib):
+    """The cooperative persistent loop (colors split over workgroups, grid
+    barriers) runs the same element, contact and pin updates as per-color
+    launches; only the residual is summed in another order.  In FP64 the
+    iteration counts match and the states agree to 1e-12 (the two kernels
+    may fuse multiply-adds differently); in FP32 the states agree to 1e-5.
+    Runs the stop rule (relative 1e-4) with ground contacts."""
+    import copy
+    m, integ, plane = beam_case(nx=10, ny=4, nz=4, z0=0.004, pct=99.99)
+    a = Simulation(m, integ, 1 / 60, plane, precision=precision)
+    b = Simulation(m, copy.deepcopy(integ), 1 / 60, plane, precision=precision)
+    a.dmesh.set_option(L.MLX_OPT_PERSISTENT, 0)
+    b.dmesh.set_option(L.MLX_OPT_PERSISTENT, 1)
+    b.dmesh.set_option(L.MLX_OPT_PERSIST_WG, wg)
+    try:
+        kmax = 0
+        for k in range(8):
+            ra, rb = a.step(), b.step()
+            pa, va, la, ka = a.download()
+            pb, vb, lb, kb = b.download()
+            kmax = max(kmax, ka)
+            _say("persist", k, ra.itersPerLayer, rb.itersPerLayer, ka, rel(pb, pa), rel(lb, la))
+            assert ka == kb
+            if precision == L.MLX_FP64:
+                assert ra.itersPerLayer == rb.itersPerLayer
+                assert rel(pb, pa) < 1e-12 and rel(lb, la) < 1e-12 and rel(vb, va) < 1e-10
+                np.testing.assert_allclose(rb.residualArray, ra.residualArray, rtol=1e-9,
+                                           atol=1e-300)
+            else:
+                assert rel(pb, pa) < 1e-5
+        assert kmax > 0
     finally:
         a.close()
         b.close()
