@@ -337,6 +337,18 @@ int mlx_batch_integrate(mlx_batch *batch, mlx_settings *settings, double h,
                         int32_t frame, const mlx_plane *plane,
                         mlx_batch_out *out);
 
+/* buildLayers + findBoundaryVertices on the device (xpbdLayer3D.m:601-619,
+ * xpbdLayer2D.m:551-569), from the strain-rate keys that mexEdiffNorm3D/2D
+ * return (nT doubles, element order): stable ascending sort with NaN last,
+ * the union-find bodies of mexUnionFindByVertice for each sorted layer
+ * (pinned elements of the mesh stay elastic), and the vertex maps.  Outputs
+ * have the layouts of mlx_union_find_by_vertice (sets nT x nSorted,
+ * numRigids) and mlx_find_boundary_vertices (isBoundary, vertexRigidBody
+ * nV x nSorted), column-major; the values are identical to theirs. */
+int mlx_layer_maps(mlx_mesh *mesh, const double *keys, const double *sortedLayers,
+                   int32_t nSorted, int32_t *sets, int32_t *numRigids, uint8_t *isBoundary,
+                   int32_t *vertexRigidBody);
+
 /* ------------------------------------------------------------------ */
 /* Measurement                                                          */
 /* ------------------------------------------------------------------ */
@@ -381,6 +393,11 @@ int mlx_profile_read_persistent(mlx_mesh *mesh, double *ms, int64_t *launches,
 #define MLX_OPT_GRAPH_ITERS 2
 #define MLX_OPT_PERSISTENT 3
 #define MLX_OPT_PERSIST_WG 4
+/* MLX_OPT_DEVICE_MAPS (default 1): mlx_integrate builds the layer maps on
+ * the device (mlx_layer_maps); 0 downloads the strain-rate keys and builds
+ * them on the host (mlx_union_find_by_vertice, mlx_find_boundary_vertices).
+ * The maps are identical either way. */
+#define MLX_OPT_DEVICE_MAPS 5
 int mlx_mesh_set_option(mlx_mesh *mesh, int32_t option, int64_t value);
 
 /* hipDeviceSynchronize on the library's device. */

@@ -77,6 +77,12 @@ class BatchSimulation:
         self.lastres = np.zeros(ns)
         self.contacts = np.zeros(ns, dtype=np.int32)
 
+    def set_state(self, p, v):
+        """Positions and velocities of every scene (merged vertex order)."""
+        p = L.f64(p)
+        v = L.f64(v)
+        L.check("mlx_batch_set_state", L.lib().mlx_batch_set_state(self.handle, L.ptr(p), L.ptr(v)))
+
     def set_block(self, threads):
         L.check("mlx_batch_set_block", L.lib().mlx_batch_set_block(self.handle, int(threads)))
 

@@ -147,6 +147,11 @@ struct mlx_mesh {
   int usePersistent = 0;
   int persistWG = 0;               // 0 = from the largest color
   mlx::DevBuf d_colors;            // int32 colorOff[numColors] | colorCnt[numColors]
+  // layer maps on the device (mlx_maps.hip)
+  int deviceMaps = 1;              // MLX_OPT_DEVICE_MAPS
+  mlx::DevBuf d_forced;            // uint8 nT forcedElastic
+  mlx::DevBuf d_maps;              // sort keys/values, ranks, parents, labels, outputs
+  mlx::DevBuf d_scan;              // rocPRIM temporary storage
   mlx::DevBuf d_sync;              // uint32 barrier counter | int32 error flag
 
   // sweep timing (mlx_profile_enable)
@@ -180,3 +185,25 @@ struct mlx_state {
   int32_t lastContacts = 0;
 };
 
+namespace mlx {
+
+// Layer maps of one substep in the reference's layouts, column l for sorted
+// layer l: elastic flags (element order), vertex body ids, boundary flags,
+// body counts.
+struct HostMaps {
+  int nSorted = 0;
+  std::vector<uint8_t> elastic;   // nSorted x nT (element order)
+  std::vector<int32_t> vbody;     // nSorted x nV
+  std::vector<uint8_t> boundary;  // nSorted x nV
+  std::vector<int32_t> numRigids; // nSorted
+};
+
+// buildLayers + findBoundaryVertices on the device from per-element
+// strain-rate keys already in device memory (mlx_maps.hip): stable radix
+// sort, connected components per sorted layer with bodies numbered by
+// ascending minimum element, vertex maps.  Results are copied to `maps`
+// (and the per-layer body ids to *setsOut, nT x ns, when given).
+int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int ns,
+                   HostMaps &maps, std::vector<int32_t> *setsOut = nullptr);
+
+}  // namespace mlx

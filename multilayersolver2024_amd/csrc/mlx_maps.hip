@@ -1,0 +1,286 @@
+// Layer maps on the device: buildLayers (xpbdLayer3D.m:601-619; 2D
+// xpbdLayer2D.m:740-790, ordering type 0) and findBoundaryVertices
+// (xpbdLayer2D.m:551-569).
+//
+// 1. Strain-rate keys (EDiff per element, already on the device) are sorted
+//    ascending and stably, NaN last, like Matlab's sort(...,"ascend"): the
+//    keys are mapped to order-preserving 64-bit integers and radix-sorted
+//    with the element indices as values (LSD radix sort is stable).
+// 2. For each sorted layer l the rigid candidates are the first
+//    floor(layers(l)/100 * nT) elements of that order, minus the pinned
+//    (forced elastic) ones.  mexUnionFindByVertice merges each inserted
+//    element with the elements sharing its vertices and numbers the bodies
+//    by ascending smallest element (copyLayer).  Here: every element is
+//    hooked to the smallest inserted element of each of its vertices with a
+//    lock-free union-find that always hooks the larger root under the
+//    smaller one, so each tree's root is its component's smallest element;
+//    an exclusive scan of the root flags numbers the bodies in that order.
+// 3. Vertex maps: rigid vertices carry their body id, boundary vertices are
+//    rigid vertices that also belong to an elastic element.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "mlx_internal.h"
+
+namespace mlx {
+namespace {
+
+constexpr int kMapBlock = 256;
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kMapBlock - 1) / kMapBlock); }
+
+// Order-preserving key: -0 and +0 compare equal in Matlab's sort and every
+// NaN goes last in input order, so -0 becomes +0 and any NaN one quiet NaN
+// before the usual sign-flip transform of IEEE doubles.
+__global__ void k_sort_keys(int64_t n, const double *keys, uint64_t *out, int32_t *idx) {
+  const int64_t i = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (i >= n) return;
+  double k = keys[i];
+  if (k != k) k = __longlong_as_double(0x7ff8000000000000LL);
+  if (k == 0.0) k = 0.0;
+  uint64_t b = (uint64_t)__double_as_longlong(k);
+  b = (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+  out[i] = b;
+  idx[i] = (int32_t)i;
+}
+
+__global__ void k_rank(int64_t n, const int32_t *order, int32_t *rank) {
+  const int64_t i = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (i >= n) return;
+  rank[order[i]] = (int32_t)i;
+}
+
+// parent[e] = e for inserted elements, -1 otherwise; vmin cleared.
+__global__ void k_cc_init(int64_t nT, int64_t nV, const int32_t *rank, const uint8_t *forced,
+                          int32_t nInsert, int32_t *parent, int32_t *vmin) {
+  const int64_t i = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (i < nT) parent[i] = (rank[i] < nInsert && !forced[i]) ? (int32_t)i : -1;
+  if (i < nV) vmin[i] = INT_MAX;
+}
+
+// vmin[v] = smallest inserted element containing v.
+__global__ void k_cc_vmin(int64_t nT, int vpe, const int32_t *T, const int32_t *parent,
+                          int32_t *vmin) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT || parent[e] < 0) return;
+  for (int k = 0; k < vpe; ++k) atomicMin(&vmin[T[e * vpe + k]], (int32_t)e);
+}
+
+__device__ __forceinline__ int32_t cc_load(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int32_t cc_find(const int32_t *parent, int32_t x) {
+  int32_t p = cc_load(parent + x);
+  while (p != x) {
+    x = p;
+    p = cc_load(parent + x);
+  }
+  return x;
+}
+
+// Union of e with vmin[v] for each of its vertices.  Roots only ever get a
+// smaller parent, so every retry either finds the two in one tree or hooks
+// one root; the loop ends.
+__global__ void k_cc_union(int64_t nT, int vpe, const int32_t *T, const int32_t *vmin,
+                           int32_t *parent) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT || cc_load(parent + e) < 0) return;
+  for (int k = 0; k < vpe; ++k) {
+    int32_t a = (int32_t)e, b = vmin[T[e * vpe + k]];
+    while (true) {
+      a = cc_find(parent, a);
+      b = cc_find(parent, b);
+      if (a == b) break;
+      if (a < b) {
+        const int32_t t = a;
+        a = b;
+        b = t;
+      }
+      // hook root a (larger) under b (smaller)
+      int32_t expected = a;
+      if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        break;
+      a = expected;
+    }
+  }
+}
+
+// After all unions: each inserted element points at its root, and roots
+// are flagged for the numbering scan.
+__global__ void k_cc_flatten(int64_t nT, int32_t *parent, int32_t *isRoot) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int32_t p = parent[e];
+  if (p < 0) {
+    isRoot[e] = 0;
+    return;
+  }
+  const int32_t r = cc_find(parent, (int32_t)e);
+  isRoot[e] = r == e ? 1 : 0;
+}
+
+__global__ void k_cc_label(int64_t nT, const int32_t *parent, const int32_t *scan,
+                           int32_t *sets) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int32_t p = parent[e];
+  if (p < 0) {
+    sets[e] = 0;
+    return;
+  }
+  sets[e] = scan[cc_find(parent, (int32_t)e)] + 1;
+}
+
+__global__ void k_vmaps_clear(int64_t nV, int32_t *vb, uint8_t *rigid, uint8_t *ofElastic) {
+  const int64_t v = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (v >= nV) return;
+  vb[v] = 0;
+  rigid[v] = 0;
+  ofElastic[v] = 0;
+}
+
+// Every rigid element writes its body id to its vertices (all rigid
+// elements at a vertex share one body, so the writes agree).
+__global__ void k_vmaps(int64_t nT, int vpe, const int32_t *T, const int32_t *sets, int32_t *vb,
+                        uint8_t *rigid, uint8_t *ofElastic) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int32_t b = sets[e];
+  for (int k = 0; k < vpe; ++k) {
+    const int32_t v = T[e * vpe + k];
+    if (b != 0) {
+      rigid[v] = 1;
+      vb[v] = b;
+    } else {
+      ofElastic[v] = 1;
+    }
+  }
+}
+
+__global__ void k_boundary_flags(int64_t nV, const uint8_t *rigid, const uint8_t *ofElastic,
+                                 uint8_t *boundary) {
+  const int64_t v = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (v >= nV) return;
+  boundary[v] = rigid[v] && ofElastic[v];
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int ns,
+                   HostMaps &maps, std::vector<int32_t> *setsOut) {
+  const int64_t nT = m.nT, nV = m.nV;
+  const int vpe = m.vpe;
+  hipStream_t st = m.stream;
+  maps.nSorted = ns;
+  maps.numRigids.assign(ns, 0);
+  maps.elastic.assign((size_t)ns * nT, 1);
+  maps.vbody.assign((size_t)ns * nV, 0);
+  maps.boundary.assign((size_t)ns * nV, 0);
+  if (nT == 0) return MLX_OK;
+  if (nT > INT_MAX - 1) {
+    set_error("layer maps: more than 2^31 elements");
+    return MLX_ERR_UNSUPPORTED;
+  }
+  if (m.d_forced.bytes < (size_t)nT) {
+    MLX_TRY(m.d_forced.reserve((size_t)nT));
+    MLX_CHECK_HIP(hipMemcpyAsync(m.d_forced.ptr, m.forcedElastic.data(), (size_t)nT,
+                                 hipMemcpyHostToDevice, st));
+  }
+  // scratch layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += align256(bytes);
+    return o;
+  };
+  const size_t oKeyA = take(8 * nT), oKeyB = take(8 * nT), oIdxA = take(4 * nT),
+               oOrder = take(4 * nT), oRank = take(4 * nT), oParent = take(4 * nT),
+               oVmin = take(4 * nV), oRoot = take(4 * nT), oScan = take(4 * nT),
+               oSets = take(4 * nT * (size_t)ns), oVb = take(4 * nV * (size_t)ns),
+               oRig = take(nV), oOfEl = take(nV), oBnd = take(nV * (size_t)ns);
+  MLX_TRY(m.d_maps.reserve(off));
+  char *base = m.d_maps.as<char>();
+  auto at32 = [&](size_t o) { return reinterpret_cast<int32_t *>(base + o); };
+  uint64_t *keyA = reinterpret_cast<uint64_t *>(base + oKeyA);
+  uint64_t *keyB = reinterpret_cast<uint64_t *>(base + oKeyB);
+  int32_t *idxA = at32(oIdxA), *order = at32(oOrder), *rank = at32(oRank),
+          *parent = at32(oParent), *vmin = at32(oVmin), *isRoot = at32(oRoot),
+          *scan = at32(oScan), *sets = at32(oSets), *vb = at32(oVb);
+  uint8_t *rig = reinterpret_cast<uint8_t *>(base + oRig);
+  uint8_t *ofEl = reinterpret_cast<uint8_t *>(base + oOfEl);
+  uint8_t *bnd = reinterpret_cast<uint8_t *>(base + oBnd);
+  const int32_t *T = m.d_T.as<int32_t>();
+  const uint8_t *forced = m.d_forced.as<uint8_t>();
+
+  // 1. stable ascending sort of the keys
+  hipLaunchKernelGGL(k_sort_keys, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, d_keys, keyA,
+                     idxA);
+  size_t tmpSort = 0, tmpScan = 0;
+  MLX_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tmpSort, keyA, keyB, idxA, order,
+                                          (unsigned int)nT, 0, 64, st));
+  MLX_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmpScan, isRoot, scan, 0, (size_t)nT,
+                                        rocprim::plus<int32_t>(), st));
+  MLX_TRY(m.d_scan.reserve(std::max<size_t>(std::max(tmpSort, tmpScan), 256)));
+  MLX_CHECK_HIP(rocprim::radix_sort_pairs(m.d_scan.ptr, tmpSort, keyA, keyB, idxA, order,
+                                          (unsigned int)nT, 0, 64, st));
+  hipLaunchKernelGGL(k_rank, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, order, rank);
+
+  // 2. bodies per sorted layer
+  const int64_t nMax = std::max(nT, nV);
+  for (int l = 0; l < ns; ++l) {
+    const double end = std::floor((sorted[l] / 100.0) * (double)nT);
+    const int32_t nInsert = end <= 0 ? 0 : (end >= (double)nT ? (int32_t)nT : (int32_t)end);
+    int32_t *setsL = sets + (size_t)nT * l;
+    hipLaunchKernelGGL(k_cc_init, dim3(grid_for(nMax)), dim3(kMapBlock), 0, st, nT, nV, rank,
+                       forced, nInsert, parent, vmin);
+    hipLaunchKernelGGL(k_cc_vmin, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T, parent,
+                       vmin);
+    hipLaunchKernelGGL(k_cc_union, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T, vmin,
+                       parent);
+    hipLaunchKernelGGL(k_cc_flatten, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, parent,
+                       isRoot);
+    size_t tb = tmpScan;
+    MLX_CHECK_HIP(rocprim::exclusive_scan(m.d_scan.ptr, tb, isRoot, scan, 0, (size_t)nT,
+                                          rocprim::plus<int32_t>(), st));
+    hipLaunchKernelGGL(k_cc_label, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, parent, scan,
+                       setsL);
+    // 3. vertex maps of this layer
+    int32_t *vbL = vb + (size_t)nV * l;
+    uint8_t *bndL = bnd + (size_t)nV * l;
+    hipLaunchKernelGGL(k_vmaps_clear, dim3(grid_for(nV)), dim3(kMapBlock), 0, st, nV, vbL, rig,
+                       ofEl);
+    hipLaunchKernelGGL(k_vmaps, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T, setsL,
+                       vbL, rig, ofEl);
+    hipLaunchKernelGGL(k_boundary_flags, dim3(grid_for(nV)), dim3(kMapBlock), 0, st, nV, rig, ofEl,
+                       bndL);
+    // body count = sum of root flags = last scan entry + last flag
+    int32_t tail[2];
+    MLX_CHECK_HIP(hipMemcpyAsync(&tail[0], scan + (nT - 1), 4, hipMemcpyDeviceToHost, st));
+    MLX_CHECK_HIP(hipMemcpyAsync(&tail[1], isRoot + (nT - 1), 4, hipMemcpyDeviceToHost, st));
+    MLX_CHECK_HIP(hipStreamSynchronize(st));
+    maps.numRigids[l] = tail[0] + tail[1];
+  }
+  MLX_CHECK_HIP(hipGetLastError());
+  std::vector<int32_t> hs((size_t)nT * ns);
+  MLX_CHECK_HIP(hipMemcpyAsync(hs.data(), sets, 4 * hs.size(), hipMemcpyDeviceToHost, st));
+  MLX_CHECK_HIP(hipMemcpyAsync(maps.vbody.data(), vb, 4 * maps.vbody.size(), hipMemcpyDeviceToHost,
+                               st));
+  MLX_CHECK_HIP(hipMemcpyAsync(maps.boundary.data(), bnd, maps.boundary.size(),
+                               hipMemcpyDeviceToHost, st));
+  MLX_CHECK_HIP(hipStreamSynchronize(st));
+  for (size_t i = 0; i < hs.size(); ++i) maps.elastic[i] = hs[i] == 0;
+  if (setsOut) setsOut->swap(hs);
+  return MLX_OK;
+}
+
+}  // namespace mlx

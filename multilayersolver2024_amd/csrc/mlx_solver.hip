@@ -73,14 +73,6 @@ static inline unsigned blocks_for(int64_t n) {
 // host-side layer data
 // ---------------------------------------------------------------------------
 
-struct HostMaps {
-  int nSorted = 0;
-  std::vector<uint8_t> elastic;   // nSorted x nT (element order)
-  std::vector<int32_t> vbody;     // nSorted x nV
-  std::vector<uint8_t> boundary;  // nSorted x nV
-  std::vector<int32_t> numRigids; // nSorted
-};
-
 struct HostContacts {
   int K = 0;
   std::vector<int32_t> vert;        // 0-based
@@ -1118,7 +1110,8 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_slotOf, &m->d_x, &m->d_xold, &m->d_rv, &m->d_pinnedx,
                          &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
-                         &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync};
+                         &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced,
+                         &m->d_maps, &m->d_scan};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
@@ -1427,6 +1420,7 @@ static int build_layer_maps(mlx_state &S, const double *sorted, int ns, double h
     hipLaunchKernelGGL(k_ediff2d<T>, dim3(blocks_for(nT)), dim3(kBlock), 0, m.stream, nT,
                        m.d_T.as<int32_t>(), m.d_dminv64.as<double>(), m.d_x.as<vec4_t<T>>(),
                        m.d_xold.as<vec4_t<T>>(), h, S.d_ediff.as<double>());
+  if (m.deviceMaps) return layer_maps_dev(m, S.d_ediff.as<double>(), sorted, ns, maps);
   S.h_ediff.resize(nT);
   MLX_CHECK_HIP(hipMemcpyAsync(S.h_ediff.data(), S.d_ediff.ptr, sizeof(double) * nT,
                                hipMemcpyDeviceToHost, m.stream));
@@ -1743,6 +1737,35 @@ int mlx_profile_phases(mlx_mesh *m, double *layersMs, double *solveMs, double *t
   return MLX_OK;
 }
 
+int mlx_layer_maps(mlx_mesh *m, const double *keys, const double *sortedLayers, int32_t nSorted,
+                   int32_t *sets, int32_t *numRigids, uint8_t *isBoundary,
+                   int32_t *vertexRigidBody) {
+  if (!m || (!keys && m->nT > 0) || !sortedLayers || nSorted < 1 || nSorted > MLX_MAX_LAYERS ||
+      !sets || !numRigids || !isBoundary || !vertexRigidBody) {
+    set_error("mlx_layer_maps: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  for (int l = 0; l < nSorted; ++l)
+    if (!(sortedLayers[l] >= 0 && sortedLayers[l] <= 100) || (l > 0 && !(sortedLayers[l] > sortedLayers[l - 1]))) {
+      set_error("mlx_layer_maps: sortedLayers must be ascending percentages");
+      return MLX_ERR_ARG;
+    }
+  MLX_CHECK_HIP(hipSetDevice(m->device));
+  DevBuf dk;
+  MLX_TRY(dk.reserve(sizeof(double) * std::max<int64_t>(m->nT, 1)));
+  MLX_CHECK_HIP(hipMemcpy(dk.ptr, keys, sizeof(double) * m->nT, hipMemcpyHostToDevice));
+  HostMaps maps;
+  std::vector<int32_t> hs;
+  const int r = layer_maps_dev(*m, dk.as<double>(), sortedLayers, nSorted, maps, &hs);
+  dk.release();
+  if (r != MLX_OK) return r;
+  std::copy(hs.begin(), hs.end(), sets);
+  std::copy(maps.numRigids.begin(), maps.numRigids.end(), numRigids);
+  std::copy(maps.boundary.begin(), maps.boundary.end(), isBoundary);
+  std::copy(maps.vbody.begin(), maps.vbody.end(), vertexRigidBody);
+  return MLX_OK;
+}
+
 int mlx_profile_read_persistent(mlx_mesh *m, double *ms, int64_t *launches, int64_t *elements,
                                 int64_t *iterations) {
   if (!m) {
@@ -1791,6 +1814,9 @@ int mlx_mesh_set_option(mlx_mesh *m, int32_t option, int64_t value) {
     case MLX_OPT_PERSIST_WG:
       if (value < 0 || value > 256) break;
       m->persistWG = (int)value;
+      return MLX_OK;
+    case MLX_OPT_DEVICE_MAPS:
+      m->deviceMaps = value != 0;
       return MLX_OK;
     default:
       break;
