@@ -8,8 +8,10 @@ h = 1/60 s, gravity -9.81 on z, ground plane z = 0, layers
 [98.4375, 87.5, 0] (two coarse layers with 1/64 and 1/8 of the elements
 elastic) with iterations [2, 4, 14], and the relative-residual stop
 (StopAtImprovementPercent = 99.99, i.e. residual <= 1e-4 x the first one,
-hangingStop 1000).  Bulk projection in FP32, rigid layer and residual
-reductions in FP64.
+hangingStop 1000), useRigidContacts off (see make_integrator).  Bulk
+projection in FP32, rigid layer and residual reductions in FP64.  The five
+warmup substeps are the free fall (one iteration each: the first residual
+is under the 1e-8 floor); the timed ones start at touchdown.
 
 A step is one integrate() substep: layer maps, layerSolve to the stop rule,
 velocity update.  value = tet projections / s over the timed steps (every

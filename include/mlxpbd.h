@@ -378,9 +378,10 @@ int mlx_profile_phases(mlx_mesh *mesh, double *layersMs, double *solveMs, double
 int mlx_profile_read_persistent(mlx_mesh *mesh, double *ms, int64_t *launches,
                                 int64_t *elements, int64_t *iterations);
 
-/* Execution options of a mesh.  MLX_OPT_GRAPHS (default 1): iterations run
- * as replays of a captured hipGraph of MLX_OPT_GRAPH_ITERS iterations
- * (default 8, 1..256); 0 launches every kernel directly.  Results are the
+/* Execution options of a mesh.  MLX_OPT_GRAPHS (default 1): iterations of
+ * layers without rigid bodies run as replays of captured hipGraphs of 16,
+ * 8, 4 and 2 iterations, the largest at most MLX_OPT_GRAPH_ITERS (default
+ * 16, 1..256); 0 launches every kernel directly.  Results are the
  * same either way.  MLX_OPT_PERSISTENT (default 0): layers without rigid
  * elements run their iterations inside one cooperative launch per chunk of
  * iterations (colors separated by grid barriers); positions and lambdas

@@ -132,6 +132,7 @@ struct mlx_mesh {
   mlx::DevBuf d_tmp;              // misc
   mlx::DevBuf d_nh;               // T 4*ld: 2D NeoHookean mu | lambda | lambda_D | lambda_H
   mlx::Ctl *h_ctl = nullptr;      // pinned host mirror
+  int32_t *h_counts = nullptr;    // pinned {K, nec, nrc, nUniq} staging for the device counts
 
   // replayable graphs of iteration chunks (hipGraph), keyed by everything
   // the captured kernels bake into their parameters
@@ -139,10 +140,11 @@ struct mlx_mesh {
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> sig;
   };
-  GraphSlot graphs[4];
+  GraphSlot graphs[8];
   int graphNext = 0;
   int useGraphs = 1;
-  int graphIters = 8;
+  int graphIters = 16;             // largest captured chunk (MLX_OPT_GRAPH_ITERS)
+  int prevExtraIters = 0;          // last layer: iterations past the fixed count, previous call
   // persistent iteration loop (layers without rigid elements)
   int usePersistent = 0;
   int persistWG = 0;               // 0 = from the largest color

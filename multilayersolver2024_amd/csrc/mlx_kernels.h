@@ -820,13 +820,18 @@ __global__ __launch_bounds__(kBlock) void k_rotate_all2d(int64_t nV, const int32
 // contacts
 // ---------------------------------------------------------------------------
 
+// Contact arrays at fixed offsets (capacity = vertex count: one contact per
+// vertex at most), so kernels captured in a graph stay valid when the
+// number of contacts changes.  counts = {K, elastic contacts of the layer,
+// rigid contacts of the layer, distinct compliance values}, on the device.
 struct ContactArrays {
   const int32_t *vert;      // K, 0-based
   const double *normal;     // 3K (xyz per contact)
   const double *point;      // 3K
   const double *compliance; // K
   double *lambda;           // K
-  int K;
+  const int32_t *counts;    // device {K, nec, nrc, nUniq}
+  int K;                    // host copy of counts[0] (kernels outside graphs)
 };
 
 // penaltyConstraint.projectElastic (penaltyConstraint.m:4-28), 3D;
@@ -869,7 +874,7 @@ __global__ __launch_bounds__(kBlock) void k_contacts_elastic(LayerLists L, Conta
                                                              const Ctl *ctl) {
   if (iteration_stopped(ctl)) return;
   const int q = blockIdx.x * kBlock + threadIdx.x;
-  if (q >= L.nec) return;
+  if (q >= c.counts[1]) return;
   contact_elastic_one<T, DIM>(c, L.ec_idx[q], x, minv, resistance, h);
 }
 
@@ -1160,14 +1165,14 @@ template <typename T, int DIM>
 __global__ __launch_bounds__(kBlock) void k_residual_contacts(ContactArrays c, const vec4_t<T> *x,
                                                               double resistance, double h,
                                                               const double *uniqAlpha,
-                                                              const double *uniqCount, int nUniq,
+                                                              const double *uniqCount,
                                                               double *partials, const Ctl *ctl) {
   if (iteration_stopped(ctl)) return;
   __shared__ double sh[kBlock];
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double acc = 0;
-  if (i < c.K)
-    acc = residual_contact_one<T, DIM>(c, i, x, resistance, h, uniqAlpha, uniqCount, nUniq);
+  if (i < c.counts[0])
+    acc = residual_contact_one<T, DIM>(c, i, x, resistance, h, uniqAlpha, uniqCount, c.counts[3]);
   double o[1];
   double v1[1] = {acc};
   block_sum<1>(v1, sh, o);

@@ -105,6 +105,29 @@ struct SolveResult {
 static constexpr int kMaxPartials = 65536;
 
 // Upload the 1-based column-major contact arrays of a layerSolve call.
+// Contact arrays live at offsets fixed by the capacity (one contact per
+// vertex at most), so captured graphs survive a change of the contact count.
+static size_t contact_cap(const mlx_mesh &m) { return (size_t)std::max<int64_t>(m.nV, 1); }
+
+static int reserve_contacts(mlx_mesh &m) {
+  const size_t cap = contact_cap(m);
+  return m.d_contacts.reserve(68 * cap + 64 + 256);
+}
+
+static ContactArrays contact_arrays(mlx_mesh &m, int K) {
+  const size_t cap = contact_cap(m);
+  char *base = m.d_contacts.as<char>();
+  ContactArrays c;
+  c.normal = reinterpret_cast<const double *>(base);
+  c.point = reinterpret_cast<const double *>(base + 24 * cap);
+  c.compliance = reinterpret_cast<const double *>(base + 48 * cap);
+  c.lambda = reinterpret_cast<double *>(base + 56 * cap);
+  c.vert = reinterpret_cast<const int32_t *>(base + 64 * cap);
+  c.counts = reinterpret_cast<const int32_t *>(base + ((68 * cap + 15) & ~(size_t)15));
+  c.K = K;
+  return c;
+}
+
 static int upload_contacts(mlx_mesh &m, int dim, int K, const int32_t *vid1,
                            const double *normals, const double *points,
                            const double *compl_, HostContacts &hc) {
@@ -132,31 +155,20 @@ static int upload_contacts(mlx_mesh &m, int dim, int K, const int32_t *vid1,
       pts[3 * i + d] = points[i + (size_t)K * d];
     }
   }
-  size_t bytes = 8 * (size_t)K * 9 + 4 * (size_t)K + 256;
-  MLX_TRY(m.d_contacts.reserve(bytes));
-  char *base = m.d_contacts.as<char>();
+  MLX_TRY(reserve_contacts(m));
+  ContactArrays c = contact_arrays(m, K);
   if (K > 0) {
     MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
-    MLX_CHECK_HIP(hipMemcpy(base, nrm.data(), 24 * (size_t)K, hipMemcpyHostToDevice));
-    MLX_CHECK_HIP(hipMemcpy(base + 24 * (size_t)K, pts.data(), 24 * (size_t)K, hipMemcpyHostToDevice));
-    MLX_CHECK_HIP(hipMemcpy(base + 48 * (size_t)K, hc.compliance.data(), 8 * (size_t)K,
+    MLX_CHECK_HIP(hipMemcpy(const_cast<double *>(c.normal), nrm.data(), 24 * (size_t)K,
                             hipMemcpyHostToDevice));
-    MLX_CHECK_HIP(hipMemcpy(base + 64 * (size_t)K, hc.vert.data(), 4 * (size_t)K,
+    MLX_CHECK_HIP(hipMemcpy(const_cast<double *>(c.point), pts.data(), 24 * (size_t)K,
+                            hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemcpy(const_cast<double *>(c.compliance), hc.compliance.data(),
+                            8 * (size_t)K, hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemcpy(const_cast<int32_t *>(c.vert), hc.vert.data(), 4 * (size_t)K,
                             hipMemcpyHostToDevice));
   }
   return MLX_OK;
-}
-
-static ContactArrays contact_arrays(mlx_mesh &m, int K) {
-  char *base = m.d_contacts.as<char>();
-  ContactArrays c;
-  c.normal = reinterpret_cast<const double *>(base);
-  c.point = reinterpret_cast<const double *>(base + 24 * (size_t)K);
-  c.compliance = reinterpret_cast<const double *>(base + 48 * (size_t)K);
-  c.lambda = reinterpret_cast<double *>(base + 56 * (size_t)K);
-  c.vert = reinterpret_cast<const int32_t *>(base + 64 * (size_t)K);
-  c.K = K;
-  return c;
 }
 
 template <typename T>
@@ -379,10 +391,12 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     }
   }
   const int nUniq = (int)uniq.size();
-  MLX_TRY(m.d_tmp.reserve(sizeof(double) * (2 * nUniq + 2) + sizeof(int32_t) * 3 * (m.dim == 2 ? ld : 1)));
+  // fixed layout (capacity, not count) so captured graphs stay valid
+  const int uniqCap = std::max(64, nUniq);
+  MLX_TRY(m.d_tmp.reserve(sizeof(double) * 2 * uniqCap + sizeof(int32_t) * 3 * (m.dim == 2 ? ld : 1)));
   double *d_uniq = m.d_tmp.as<double>();
-  double *d_cnt = d_uniq + nUniq + 1;
-  int32_t *d_lamSlot2d = reinterpret_cast<int32_t *>(d_cnt + nUniq + 1);
+  double *d_cnt = d_uniq + uniqCap;
+  int32_t *d_lamSlot2d = reinterpret_cast<int32_t *>(d_cnt + uniqCap);
   if (nUniq > 0) {
     MLX_CHECK_HIP(hipMemcpyAsync(d_uniq, uniq.data(), 8 * nUniq, hipMemcpyHostToDevice, st));
     MLX_CHECK_HIP(hipMemcpyAsync(d_cnt, cnt.data(), 8 * nUniq, hipMemcpyHostToDevice, st));
@@ -430,7 +444,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   mp.giveUpThreshold = sp.giveUpThreshold;
   mp.h = sp.h;
   const int nResElemBlocks = (int)blocks_for(ld);
-  const int nResContactBlocks = K > 0 ? (int)blocks_for(K) : 0;
+  const int nResContactBlocks = K > 0 ? (int)blocks_for((int64_t)contact_cap(m)) : 0;
   if (nResElemBlocks + nResContactBlocks > kMaxPartials) {
     set_error("scene too large for the residual partial buffer");
     return MLX_ERR_UNSUPPORTED;
@@ -501,6 +515,13 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     if (L.nbnd > 0)
       MLX_CHECK_HIP(hipMemsetAsync(m.d_lamB.ptr, 0, sizeof(double) * L.nbnd, st));
     if (K > 0) MLX_CHECK_HIP(hipMemsetAsync(ca.lambda, 0, sizeof(double) * K, st));
+    // contact counts of this layer, read on the device by the iteration kernels
+    m.h_counts[0] = K;
+    m.h_counts[1] = L.nec;
+    m.h_counts[2] = L.nrc;
+    m.h_counts[3] = nUniq;
+    MLX_CHECK_HIP(hipMemcpyAsync(const_cast<int32_t *>(ca.counts), m.h_counts, 4 * sizeof(int32_t),
+                                 hipMemcpyHostToDevice, st));
 
     if (m.dim == 3) {
       if (L.nb > 0)
@@ -541,10 +562,10 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
                            ctl);
       if (L.nec > 0) {
         if (m.dim == 3)
-          hipLaunchKernelGGL((k_contacts_elastic<T, 3>), dim3(blocks_for(L.nec)), dim3(kBlock), 0,
+          hipLaunchKernelGGL((k_contacts_elastic<T, 3>), dim3(blocks_for((int64_t)contact_cap(m))), dim3(kBlock), 0,
                              st, LL, ca, x, minv, sp.contactResistance, sp.h, ctl);
         else
-          hipLaunchKernelGGL((k_contacts_elastic<T, 2>), dim3(blocks_for(L.nec)), dim3(kBlock), 0,
+          hipLaunchKernelGGL((k_contacts_elastic<T, 2>), dim3(blocks_for((int64_t)contact_cap(m))), dim3(kBlock), 0,
                              st, LL, ca, x, minv, sp.contactResistance, sp.h, ctl);
       }
       if (m.dim == 3 && L.nrc > 0 && sp.useRigidContacts)
@@ -573,11 +594,11 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
         if (K > 0) {
           if (m.dim == 3)
             hipLaunchKernelGGL((k_residual_contacts<T, 3>), dim3(nResContactBlocks), dim3(kBlock),
-                               0, st, ca, x, sp.contactResistance, sp.h, d_uniq, d_cnt, nUniq,
+                               0, st, ca, x, sp.contactResistance, sp.h, d_uniq, d_cnt,
                                partials + nResElemBlocks, ctl);
           else
             hipLaunchKernelGGL((k_residual_contacts<T, 2>), dim3(nResContactBlocks), dim3(kBlock),
-                               0, st, ca, x, sp.contactResistance, sp.h, d_uniq, d_cnt, nUniq,
+                               0, st, ca, x, sp.contactResistance, sp.h, d_uniq, d_cnt,
                                partials + nResElemBlocks, ctl);
           nPart += nResContactBlocks;
         }
@@ -588,15 +609,17 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
 
     // Iterations run as replays of a captured chunk of G iterations when
     // graphs are on (not while profiling: the timing events are per launch).
-    auto graph_for = [&](int G) -> hipGraphExec_t {
+    auto graph_for = [&](int G, bool capture) -> hipGraphExec_t {
       auto bits = [](double d) {
         uint64_t u;
         std::memcpy(&u, &d, 8);
         return u;
       };
       std::vector<uint64_t> sig = {
-          (uint64_t)col, (uint64_t)G, (uint64_t)L.nb, (uint64_t)L.nbnd, (uint64_t)L.nec,
-          (uint64_t)L.nrc, (uint64_t)K, (uint64_t)nUniq, (uint64_t)sp.computeResiduals,
+          (uint64_t)col, (uint64_t)G, (uint64_t)L.nb, (uint64_t)L.nbnd, (uint64_t)(L.nec > 0),
+          (uint64_t)L.nrc, (uint64_t)(K > 0), (uint64_t)uniqCap, (uint64_t)L.o_rig_vert,
+          (uint64_t)L.o_bnd_off, (uint64_t)L.o_bnd_vert, (uint64_t)L.o_rc_off,
+          (uint64_t)L.o_rc_idx, (uint64_t)L.o_ec, (uint64_t)sp.computeResiduals,
           (uint64_t)sp.useRigidContacts, (uint64_t)(uintptr_t)fl, (uint64_t)(uintptr_t)m.d_csr.ptr,
           (uint64_t)(uintptr_t)m.d_contacts.ptr, (uint64_t)(uintptr_t)m.d_tmp.ptr,
           (uint64_t)(uintptr_t)m.d_lamB.ptr, (uint64_t)(uintptr_t)m.d_bodies.ptr,
@@ -609,6 +632,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
           (uint64_t)sp.giveUpEnabled, (uint64_t)sp.lastLayerGiveUpEnabled};
       for (auto &g : m.graphs)
         if (g.exec && g.sig == sig) return g.exec;
+      if (!capture) return nullptr;
       hipGraph_t graph = nullptr;
       if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
       for (int i = 0; i < G; ++i) enqueue_iteration();
@@ -617,7 +641,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (ie != hipSuccess) return nullptr;
-      auto &slot = m.graphs[m.graphNext++ % 4];
+      auto &slot = m.graphs[m.graphNext++ % 8];
       if (slot.exec) (void)hipGraphExecDestroy(slot.exec);
       slot.exec = ex;
       slot.sig = sig;
@@ -680,35 +704,50 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       }
       return MLX_OK;
     };
+    // n iterations as replays of captured chunks of 16, 8, 4 and 2
+    // iterations (at most MLX_OPT_GRAPH_ITERS), the rest launched directly.
+    // Graphs are captured only for layers without bodies: their launch
+    // parameters do not change from substep to substep (contact counts are
+    // read on the device), so one capture serves the whole run; layers with
+    // bodies reuse a graph only if one matches.
     auto run_iterations = [&](int n) -> int {
       if (persist) return run_persistent(n);
-      const int G = m.graphIters;
-      if (!m.useGraphs || m.prof.on || n < G) {
-        for (int it = 0; it < n; ++it) enqueue_iteration();
-        return MLX_OK;
+      int left = n;
+      if (m.useGraphs && !m.prof.on) {
+        for (int G = 16; G >= 2; G /= 2) {
+          if (G > m.graphIters) continue;
+          while (left >= G) {
+            hipGraphExec_t ex = graph_for(G, L.nb == 0);
+            if (!ex) break;
+            MLX_CHECK_HIP(hipGraphLaunch(ex, st));
+            left -= G;
+          }
+        }
       }
-      hipGraphExec_t ex = graph_for(G);
-      if (!ex) {
-        set_error("hipGraph capture/instantiate of the iteration chunk failed");
-        return MLX_ERR_HIP;
-      }
-      for (int k = 0; k < n / G; ++k) MLX_CHECK_HIP(hipGraphLaunch(ex, st));
-      for (int it = 0; it < n % G; ++it) enqueue_iteration();
+      for (int it = 0; it < left; ++it) enqueue_iteration();
       return MLX_OK;
     };
-    MLX_TRY(run_iterations(sp.iterations[layer]));
+    // The last layer keeps iterating while the relative stop is not met.
+    // When runUntil is already known (set in an earlier layer), the
+    // iterations the previous substep needed past the fixed count are
+    // enqueued with the fixed ones: kernels return at once after the
+    // monitor stops, so a guess that is too long costs only empty launches.
+    int first = sp.iterations[layer];
+    const bool predict = isLast && layer > 0 && hc_ctl.runUntil >= 0 && !persist;
+    if (predict) first += std::max(0, m.prevExtraIters - 1);
+    MLX_TRY(run_iterations(first));
     MLX_TRY(read_ctl(m, hc_ctl));
     if (m.prof.on) prof_harvest(m, hc_ctl.iters);
     if (isLast) {
       // while it <= iterations || (~stopSolve && runUntil >= 0 && isLastLayer)
       int64_t guard = 0;
-      int chunk = 2;
+      int chunk = predict ? 4 : 2;
       while (!hc_ctl.stopSolve && !hc_ctl.endLayer && hc_ctl.runUntil >= 0) {
         MLX_TRY(run_iterations(chunk));
         MLX_TRY(read_ctl(m, hc_ctl));
         if (m.prof.on) prof_harvest(m, hc_ctl.iters);
         guard += chunk;
-        chunk = persist ? 512 : std::min(chunk * 2, 4 * std::max(m.graphIters, 4));
+        chunk = persist ? 512 : std::min(chunk * 2, 32);
         if (guard > 10000000) {
           set_error("runUntil loop exceeded 1e7 iterations without converging (hangingStop < 0?)");
           return MLX_ERR_STATE;
@@ -727,6 +766,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     }
     MLX_CHECK_HIP(hipGetLastError());
     res.itersPerLayer[layer] = hc_ctl.iters;
+    if (isLast) m.prevExtraIters = std::max(0, hc_ctl.iters - sp.iterations[layer]);
     res.projections += (int64_t)hc_ctl.iters * nElastic[col];
     res.layersRun = layer + 1;
     if (hc_ctl.stopSolve) break;
@@ -1064,6 +1104,11 @@ static int mesh_create_impl(const mlx_mesh_desc *d, int32_t precision, int nScen
     set_error("hipHostMalloc failed");
     return fail(MLX_ERR_HIP);
   }
+  if (hipHostMalloc((void **)&m->h_counts, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+    m->h_counts = nullptr;
+    set_error("hipHostMalloc failed");
+    return fail(MLX_ERR_HIP);
+  }
   int r = precision == MLX_FP64 ? upload_mesh_arrays<double>(*m, *d) : upload_mesh_arrays<float>(*m, *d);
   if (r != MLX_OK) return fail(r);
   // element-order arrays for layer building, per-vertex masses, pins
@@ -1119,6 +1164,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
   for (auto &g : m->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (m->h_ctl) (void)hipHostFree(m->h_ctl);
+  if (m->h_counts) (void)hipHostFree(m->h_counts);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return MLX_OK;
@@ -1487,7 +1533,7 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
     hc.compliance.assign(hc.K, plane->compliance);
   }
   const int K = hc.K;
-  MLX_TRY(m.d_contacts.reserve(8 * (size_t)K * 9 + 4 * (size_t)K + 256));
+  MLX_TRY(reserve_contacts(m));
   ContactArrays ca = contact_arrays(m, K);
   if (K > 0) {
     MLX_CHECK_HIP(hipMemcpyAsync(const_cast<int32_t *>(ca.vert), hc.vert.data(), 4 * (size_t)K,
@@ -1937,7 +1983,7 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     }
     b.cont0[b.nScenes] = K;
   }
-  MLX_TRY(m.d_contacts.reserve(8 * (size_t)K * 9 + 4 * (size_t)K + 256));
+  MLX_TRY(reserve_contacts(m));
   MLX_CHECK_HIP(hipStreamSynchronize(s));
   ContactArrays ca = contact_arrays(m, K);
   if (K > 0) {
