@@ -52,6 +52,12 @@ BYTES_PER_TET = {"fp32": 280, "fp64": 544}
 # ids 16 + x read 48 + DmInv 36 + vol 4 + (alpha_d, alpha_o, alpha_s) 12 +
 # lambda read 24 (FP64: 16 + 96 + 72 + 8 + 24 + 48).
 RESIDUAL_BYTES_PER_TET = {"fp32": 140, "fp64": 264}
+# HBM-side bytes per k_sweep3d<float> dispatch on this workload, from the
+# rocprofv3 FETCH_SIZE / WRITE_SIZE runs (KiB per dispatch, averaged over
+# 8,200 dispatches), scaled by what the same counters read for a 1 GiB
+# device copy (FETCH_SIZE reads half the bytes fetched, WRITE_SIZE all).
+TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 167.92, "fetch_scale": 2.0,
+                                    "write": 94.74, "write_scale": 1.0}}
 
 
 def log(*a):
@@ -273,6 +279,9 @@ def main():
         roof = {"kernel": sweeps["kernel"], "bytes_per_unit": bpt,
                 "avg_launch_us": sweeps["avg_launch_us"], "launches": launches.value,
                 "elements_per_launch": elems.value / max(launches.value, 1)}
+    tk = TRAFFIC_KIB.get(roof["kernel"]) if (args.nx, args.ny, args.nz) == (64, 16, 16) else None
+    traffic = (1024 * (tk["fetch"] * tk["fetch_scale"] + tk["write"] * tk["write_scale"])
+               if tk else None)
     line = {
         "metric": "ms/substep & tet-proj/s to rel. residual 1e-4: plain vs learned+batched, 1 GPU",
         "value": proj / wall,
@@ -308,7 +317,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
         }, **roof),
     }
     if not args.no_batched:
