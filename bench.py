@@ -91,7 +91,9 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
     random aspect ratios, per-scene E log-uniform in [1e4, 1e7]) integrated
     in one persistent launch per substep, plain method (one all-elastic
     layer, 20 iterations, the same 1e-4 relative-residual stop per scene).
-    The block size is chosen by timing each candidate on one substep."""
+    The block size and the scene split (target tets per workgroup; 0 = one
+    workgroup per scene) are chosen by timing each candidate pair on the
+    same substep."""
     from multilayersolver2024_amd.batch import BatchSimulation, merge_meshes, random_beams
 
     merged = merge_meshes([Mesh(s) for s in random_beams(nscenes, seed=0)])
@@ -111,14 +113,18 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
     frame0 = b.frame
     tune = {}
     for blk in (256, 512, 1024):
-        b.set_block(blk)
-        b.set_state(p0, v0)
-        b.frame = frame0
-        proj, kms = b.step()
-        tune[blk] = proj / (kms / 1e3)
-        log(f"batched block {blk}: {kms:.2f} ms kernel, {tune[blk]:.3e} tet-proj/s")
+        for split in (0, 1500, 3000, 6000):
+            b.set_block(blk)
+            b.set_split(split)
+            b.set_state(p0, v0)
+            b.frame = frame0
+            proj, kms = b.step()
+            tune[(blk, split)] = proj / (kms / 1e3)
+            log(f"batched block {blk} split {split}: {kms:.2f} ms kernel, "
+                f"{tune[(blk, split)]:.3e} tet-proj/s")
     best = max(tune, key=tune.get)
-    b.set_block(best)
+    b.set_block(best[0])
+    b.set_split(best[1])
     b.set_state(p0, v0)
     b.frame = frame0
     L.check("mlx_synchronize", L.lib().mlx_synchronize())
@@ -143,7 +149,8 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
                     f"({min(m.nT for m in merged.meshes)}..{max(m.nT for m in merged.meshes)} per "
                     f"scene), plain method, 20 iterations + rel. residual 1e-4 stop per scene, "
                     f"ground contact, h=1/60, one launch per substep",
-        "block_size": best, "autotune_tet_proj_per_s": {str(k): v for k, v in tune.items()},
+        "block_size": best[0], "split_tets_per_workgroup": best[1],
+        "autotune_tet_proj_per_s": {f"{k[0]}/{k[1]}": v for k, v in tune.items()},
         "mean_iterations_per_scene_substep": float(np.mean(iters)),
         "max_iterations_per_scene_substep": int(np.max(iters_max)),
         "finite": bool(np.isfinite(p_end).all() and np.isfinite(v_end).all()),

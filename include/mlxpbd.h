@@ -333,6 +333,15 @@ typedef struct {
 
 /* One integrate() substep of every scene (xpbdLayer3D.m:621-745 per scene),
  * the whole projectGS loop of all scenes in one kernel launch. */
+/* Splits large scenes over several workgroups (tetsPerWorkgroup > 0): a
+ * scene of n tets gets round(n / tetsPerWorkgroup) workgroups (at least
+ * one), which meet at a per-scene grid barrier after every color, and the
+ * whole grid is one cooperative launch (the target grows until every
+ * workgroup is resident).  0 (default): one workgroup per scene.  Results
+ * follow the same per-element updates either way; the residual is summed in
+ * another order (last-bit differences). */
+int mlx_batch_set_split(mlx_batch *batch, int32_t tetsPerWorkgroup);
+
 int mlx_batch_integrate(mlx_batch *batch, mlx_settings *settings, double h,
                         int32_t frame, const mlx_plane *plane,
                         mlx_batch_out *out);

@@ -55,7 +55,7 @@ def merge_meshes(meshes):
 
 class BatchSimulation:
     def __init__(self, merged, integrator, h, contact=None, precision=L.MLX_FP32, block=512,
-                 device=0):
+                 device=0, split=0):
         lib = L.lib()
         L.check("mlx_init", lib.mlx_init(device))
         self.merged = merged
@@ -67,6 +67,7 @@ class BatchSimulation:
             L.ptr(merged.elemOff, C.c_int64), precision, C.byref(b)))
         self.handle = b
         self.set_block(block)
+        self.set_split(split)
         v0 = np.zeros_like(merged.p)
         L.check("mlx_batch_set_state", lib.mlx_batch_set_state(b, L.ptr(merged.p), L.ptr(v0)))
         self.frame = 0
@@ -82,6 +83,11 @@ class BatchSimulation:
         p = L.f64(p)
         v = L.f64(v)
         L.check("mlx_batch_set_state", L.lib().mlx_batch_set_state(self.handle, L.ptr(p), L.ptr(v)))
+
+    def set_split(self, tets_per_workgroup):
+        """Workgroups per scene = round(tets / tets_per_workgroup); 0: one."""
+        L.check("mlx_batch_set_split", L.lib().mlx_batch_set_split(self.handle,
+                                                                   int(tets_per_workgroup)))
 
     def set_block(self, threads):
         L.check("mlx_batch_set_block", L.lib().mlx_batch_set_block(self.handle, int(threads)))

@@ -1821,3 +1821,158 @@ __global__ __launch_bounds__(kBlock) void k_persist(PersistParams<T> Q) {
 }
 
 }  // namespace mlx
+
+namespace mlx {
+
+// ---------------------------------------------------------------------------
+// batched scenes split over several workgroups
+// ---------------------------------------------------------------------------
+
+// Work split of k_batch_split: workgroup w serves scene wgScene[w] as rank
+// wgRank[w] of sceneG[scene] workgroups (a scene's workgroups are
+// consecutive).  Scenes with more than one workgroup meet at a per-scene
+// grid barrier (grid_barrier on bar[scene]) after every color, after the
+// contacts and pins, after the residual partials and after the monitor.
+struct SplitParams {
+  const int32_t *wgScene, *wgRank, *sceneG;
+  unsigned *bar;          // per scene arrival counter, zero at launch
+  int32_t *sceneStop;     // per scene stop flag published by rank 0
+  double *sceneRunUntil;  // per scene runUntil published by rank 0
+  double *partials;       // per workgroup residual partial
+  int *err;               // barrier failure flag
+};
+
+// The plain-method projectGS loop of k_batch_plain with a scene's elements
+// of each color, its residual and its pins/contacts split over the scene's
+// workgroups.  Rank 0 owns the scene's control block (monitor and stop
+// rule) and runs the contacts then the pins; all ranks stop together on the
+// flag it publishes.  One workgroup per scene reduces to workgroup barriers.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_batch_split(BatchParams<T> B, SplitParams S) {
+  __shared__ double sh[NT];
+  __shared__ int sStop;
+  __shared__ double sRunUntil;
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int sc = S.wgScene[w], r = S.wgRank[w], G = S.sceneG[sc];
+  unsigned nbar = 0;
+  Ctl c;
+  c.stopSolve = 0;
+  c.endLayer = 0;
+  c.iters = 0;
+  c.errorChangeDecreased = 0;
+  c.residualCount = 1;
+  c.stopLayer = -1;
+  c.isLastLayer = 1;
+  c.layer = 0;
+  c.runUntil = B.runUntil0;
+  c.setPercentStop = B.stopPct;
+  c.normresv = 0;
+  c.lastResidual = 0;
+  const int nc = B.ncolor[sc], cb = B.color0[sc];
+  const int64_t k0 = B.cont0[sc], k1 = B.cont0[sc + 1];
+  const int64_t q0 = B.pin0[sc], q1 = B.pin0[sc + 1];
+  const int64_t s0 = B.slot0[sc], s1 = B.slot0[sc + 1];
+  const int64_t stride = (int64_t)G * NT;
+  const double h2 = B.h * B.h;
+#define MLX_SCENE_SYNC()                                                  \
+  do {                                                                    \
+    if (G == 1) {                                                         \
+      __syncthreads();                                                    \
+    } else if (!grid_barrier(S.bar + sc, ++nbar * (unsigned)G, S.err)) {  \
+      return;                                                             \
+    }                                                                     \
+  } while (0)
+  int it = 0;
+  double ru = B.runUntil0;
+  while (it < B.hardCap) {
+    for (int k = 0; k < nc; ++k) {
+      const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
+      for (int64_t s = a + (int64_t)r * NT + tid; s < e; s += stride) project_tet<T>(B.P, s);
+      MLX_SCENE_SYNC();
+    }
+    if (r == 0) {
+      for (int64_t i = k0 + tid; i < k1; i += NT) {
+        const int v = B.c.vert[i];
+        const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
+        vec4_t<T> p = B.P.x[v];
+        const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
+        double phi = n[0] * (xv[0] - pc[0]) + n[1] * (xv[1] - pc[1]) + n[2] * (xv[2] - pc[2]);
+        if (phi > 0) phi = 0;
+        const double C = B.contactResistance * phi;
+        const double aT = B.c.compliance[i] / h2;
+        const double mi = B.minv[v];
+        const double dl = (-C - aT * B.c.lambda[i]) / (2 * mi + aT);
+        p.x = (T)(xv[0] + mi * (dl * n[0]));
+        p.y = (T)(xv[1] + mi * (dl * n[1]));
+        p.z = (T)(xv[2] + mi * (dl * n[2]));
+        B.c.lambda[i] += dl;
+        B.P.x[v] = p;
+      }
+      __syncthreads();
+      for (int64_t q = q0 + tid; q < q1; q += NT) {
+        const int d = B.pinDof[q];
+        T *px = reinterpret_cast<T *>(&B.P.x[d / 3]);
+        px[d % 3] = B.pinVal[q];
+      }
+    }
+    MLX_SCENE_SYNC();
+    ++it;
+    int stop = 0;
+    if (B.mp.computeResiduals) {
+      double acc = 0;
+      for (int64_t s = s0 + (int64_t)r * NT + tid; s < s1; s += stride)
+        if (B.P.vid[s] >= 0) acc += residual_tet<T>(B.P, s);
+      if (r == 0) {
+        const double Ks = (double)(k1 - k0);
+        for (int64_t i = k0 + tid; i < k1; i += NT) {
+          const int v = B.c.vert[i];
+          const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
+          const vec4_t<T> p = B.P.x[v];
+          double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]) +
+                       n[2] * ((double)p.z - pc[2]);
+          if (phi > 0) phi = 0;
+          const double a = fabs(B.contactResistance * phi);
+          const double cr = fabs((B.contactCompliance / h2) * B.c.lambda[i]);
+          const double t = a - (a < cr ? a : cr);
+          acc += Ks * (t * t);
+        }
+      }
+      sh[tid] = acc;
+      __syncthreads();
+      for (int st = NT / 2; st > 0; st >>= 1) {
+        if (tid < st) sh[tid] += sh[tid + st];
+        __syncthreads();
+      }
+      if (tid == 0) S.partials[w] = sh[0];
+      MLX_SCENE_SYNC();
+      if (r == 0 && tid == 0) {
+        double sum = 0;
+        for (int q = 0; q < G; ++q) sum += S.partials[w + q];
+        monitor_update(c, B.mp, sqrt(sum), nullptr, 0);
+        S.sceneStop[sc] = c.stopSolve | c.endLayer;
+        S.sceneRunUntil[sc] = c.runUntil;
+      }
+      MLX_SCENE_SYNC();
+      if (tid == 0) {
+        sStop = S.sceneStop[sc];
+        sRunUntil = S.sceneRunUntil[sc];
+      }
+      __syncthreads();
+      stop = sStop;
+      ru = sRunUntil;
+      __syncthreads();
+    }
+    if (stop) break;
+    if (!(it < B.iterations || ru >= 0)) break;
+  }
+#undef MLX_SCENE_SYNC
+  if (r == 0 && tid == 0) {
+    B.iters[sc] = it;
+    B.lastResidual[sc] = c.lastResidual;
+    B.runUntilOut[sc] = c.runUntil;
+    B.residualCount[sc] = c.residualCount;
+  }
+}
+
+}  // namespace mlx
+

@@ -1893,6 +1893,9 @@ struct mlx_batch {
   DevBuf d_out;      // per-scene outputs
   DevBuf d_pinMask;  // uint8 3nV
   DevBuf d_flags;    // int32 nV (plane test)
+  DevBuf d_split;    // split schedule, per-scene barriers and flags, partials
+  int split = 0;     // target tets per workgroup (0: one workgroup per scene)
+  int splitWGs = 0;  // workgroups of the last split launch
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -2058,11 +2061,79 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     MLX_CHECK_HIP(hipEventCreate(&b.ev0));
     MLX_CHECK_HIP(hipEventCreate(&b.ev1));
   }
-  MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
-  if (b.block == 256) launch_batch<T, 256>(B, ns, s);
-  else if (b.block == 1024) launch_batch<T, 1024>(B, ns, s);
-  else launch_batch<T, 512>(B, ns, s);
-  MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
+  int *splitErr = nullptr;
+  if (b.split > 0) {
+    // split schedule: scene sc gets G = round(nT_sc / split) workgroups (at
+    // least 1), scaled down until every workgroup of the grid is resident
+    const void *fn = b.block == 256 ? reinterpret_cast<const void *>(&k_batch_split<T, 256>)
+                     : b.block == 1024 ? reinterpret_cast<const void *>(&k_batch_split<T, 1024>)
+                                       : reinterpret_cast<const void *>(&k_batch_split<T, 512>);
+    int perCU = 0, dev = 0;
+    MLX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, b.block, 0));
+    MLX_CHECK_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    MLX_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
+    const int64_t cap = (int64_t)perCU * prop.multiProcessorCount;
+    if (cap < ns) {
+      set_error("batched split: %d scenes do not fit %lld resident workgroups", ns, (long long)cap);
+      return MLX_ERR_UNSUPPORTED;
+    }
+    double per = (double)b.split;
+    std::vector<int32_t> G(ns);
+    int64_t total = 0;
+    for (int guard = 0; guard < 64; ++guard) {
+      total = 0;
+      for (int sc = 0; sc < ns; ++sc) {
+        const double nt = (double)(b.elemOff[sc + 1] - b.elemOff[sc]);
+        G[sc] = (int32_t)std::max(1.0, std::min(64.0, std::floor(nt / per + 0.5)));
+        total += G[sc];
+      }
+      if (total <= cap) break;
+      per *= 1.25;
+    }
+    if (total > cap) {
+      set_error("batched split: no schedule fits %lld resident workgroups", (long long)cap);
+      return MLX_ERR_UNSUPPORTED;
+    }
+    std::vector<int32_t> wgScene(total), wgRank(total);
+    int64_t w = 0;
+    for (int sc = 0; sc < ns; ++sc)
+      for (int q = 0; q < G[sc]; ++q, ++w) {
+        wgScene[w] = sc;
+        wgRank[w] = q;
+      }
+    const size_t o_scene = 0, o_rank = o_scene + 4 * total, o_G = o_rank + 4 * total,
+                 o_bar = (o_G + 4 * (size_t)ns + 255) / 256 * 256, o_stop = o_bar + 4 * (size_t)ns,
+                 o_err = o_stop + 4 * (size_t)ns, o_ru = (o_err + 4 + 7) / 8 * 8,
+                 o_part = o_ru + 8 * (size_t)ns, o_end = o_part + 8 * total;
+    MLX_TRY(b.d_split.reserve(o_end));
+    char *sb = b.d_split.as<char>();
+    MLX_CHECK_HIP(hipMemcpy(sb + o_scene, wgScene.data(), 4 * total, hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemcpy(sb + o_rank, wgRank.data(), 4 * total, hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemcpy(sb + o_G, G.data(), 4 * (size_t)ns, hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemsetAsync(sb + o_bar, 0, o_ru - o_bar, s));
+    SplitParams SP;
+    SP.wgScene = reinterpret_cast<const int32_t *>(sb + o_scene);
+    SP.wgRank = reinterpret_cast<const int32_t *>(sb + o_rank);
+    SP.sceneG = reinterpret_cast<const int32_t *>(sb + o_G);
+    SP.bar = reinterpret_cast<unsigned *>(sb + o_bar);
+    SP.sceneStop = reinterpret_cast<int32_t *>(sb + o_stop);
+    SP.err = reinterpret_cast<int *>(sb + o_err);
+    SP.sceneRunUntil = reinterpret_cast<double *>(sb + o_ru);
+    SP.partials = reinterpret_cast<double *>(sb + o_part);
+    splitErr = SP.err;
+    b.splitWGs = (int)total;
+    void *args[] = {&B, &SP};
+    MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
+    MLX_CHECK_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)total), dim3(b.block), args, 0, s));
+    MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
+  } else {
+    MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
+    if (b.block == 256) launch_batch<T, 256>(B, ns, s);
+    else if (b.block == 1024) launch_batch<T, 1024>(B, ns, s);
+    else launch_batch<T, 512>(B, ns, s);
+    MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
+  }
   hipLaunchKernelGGL((k_velocity<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x, xo,
                      b.d_v.as<T>(), h);
   if (K > 0)
@@ -2070,6 +2141,14 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
                        b.d_v.as<T>(), h, st->gravity, st->restitution);
   MLX_CHECK_HIP(hipGetLastError());
   MLX_CHECK_HIP(hipStreamSynchronize(s));
+  if (splitErr) {
+    int e = 0;
+    MLX_CHECK_HIP(hipMemcpy(&e, splitErr, 4, hipMemcpyDeviceToHost));
+    if (e) {
+      set_error("batched split: a workgroup did not reach its scene barrier");
+      return MLX_ERR_STATE;
+    }
+  }
   std::vector<int32_t> it(ns), rc(ns);
   std::vector<double> lr(ns);
   MLX_CHECK_HIP(hipMemcpy(it.data(), B.iters, 4 * (size_t)ns, hipMemcpyDeviceToHost));
@@ -2162,8 +2241,18 @@ int mlx_batch_destroy(mlx_batch *b) {
   b->d_out.release();
   b->d_pinMask.release();
   b->d_flags.release();
+  b->d_split.release();
   mlx_mesh_destroy(b->m);
   delete b;
+  return MLX_OK;
+}
+
+int mlx_batch_set_split(mlx_batch *b, int32_t tetsPerWorkgroup) {
+  if (!b || tetsPerWorkgroup < 0) {
+    set_error("mlx_batch_set_split: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  b->split = tetsPerWorkgroup;
   return MLX_OK;
 }
 

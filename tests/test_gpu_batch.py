@@ -27,15 +27,19 @@ def _integ(iters=12, pct=0.0):
     return it
 
 
-@pytest.mark.parametrize("pct,block", [(0.0, 256), (99.99, 1024)])
-def test_batch_fp64_matches_oracle_per_scene(pct, block, oracle_lib):
+@pytest.mark.parametrize("pct,block,split", [(0.0, 256, 0), (99.99, 1024, 0), (0.0, 256, 40),
+                                             (99.99, 512, 30)])
+def test_batch_fp64_matches_oracle_per_scene(pct, block, split, oracle_lib):
+    """Each scene of the batch equals the oracle run alone, with one
+    workgroup per scene and with scenes split over several workgroups
+    (split = target tets per workgroup: the 80..240-tet scenes get 2..8)."""
     scs = [scenes.beam3d(4, 2, 2, seed=1, z0=0.01), scenes.beam3d(6, 3, 2, seed=2, z0=0.02),
            scenes.beam3d(3, 3, 3, seed=3, z0=-0.01), scenes.beam3d(8, 2, 3, seed=4, z0=0.03)]
     meshes = [Mesh(s) for s in scs]
     merged = merge_meshes(meshes)
     plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
     it = _integ(pct=pct)
-    b = BatchSimulation(merged, it, 1 / 60, plane, precision=L.MLX_FP64, block=block)
+    b = BatchSimulation(merged, it, 1 / 60, plane, precision=L.MLX_FP64, block=block, split=split)
     osims = [O.OracleSim(m, copy.deepcopy(it), 1 / 60, plane) for m in meshes]
     try:
         for k in range(5):
@@ -58,3 +62,37 @@ def test_batch_workload_builds():
     scs = random_beams(8, seed=0, tets=(2000, 6000))
     n = [len(s["t"]) for s in scs]
     assert all(1500 <= x <= 8000 for x in n)
+
+
+@pytest.mark.parametrize("precision", [L.MLX_FP64, L.MLX_FP32])
+def test_batch_split_matches_one_workgroup_per_scene(precision):
+    """The split schedule (several workgroups per large scene, per-scene
+    grid barriers, one cooperative launch) follows the same per-element
+    updates as one workgroup per scene on uneven scenes with contacts and
+    the relative stop rule: equal iteration counts and contact counts,
+    positions within 1e-12 (FP64) / 1e-5 (FP32)."""
+    scs = random_beams(12, seed=3, tets=(600, 9000))
+    merged = merge_meshes([Mesh(sc) for sc in scs])
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    it = _integ(iters=20, pct=99.99)
+    it.hangingStop = 200
+    a = BatchSimulation(merged, it, 1 / 60, plane, precision=precision, block=512, split=0)
+    b = BatchSimulation(merged, copy.deepcopy(it), 1 / 60, plane, precision=precision, block=512,
+                        split=1500)
+    try:
+        for k in range(8):
+            a.step()
+            b.step()
+            pa, va, la = a.download()
+            pb, vb, lb = b.download()
+            print("split", k, list(a.iters), list(b.iters), rel(pb, pa), flush=True)
+            assert np.array_equal(a.contacts, b.contacts)
+            if precision == L.MLX_FP64:
+                assert np.array_equal(a.iters, b.iters)
+                assert rel(pb, pa) < 1e-12 and rel(lb, la) < 1e-10
+            else:
+                assert rel(pb, pa) < 1e-5
+    finally:
+        a.close()
+        b.close()
+
