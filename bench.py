@@ -200,7 +200,9 @@ def main():
     ap.add_argument("--persistent", type=int, default=0,
                     help="run layers without rigid elements in the persistent cooperative loop")
     ap.add_argument("--persist-wg", type=int, default=0,
-                    help="workgroups of the persistent iteration loop (0 = from the largest color)")
+                    help="workgroups of the persistent iteration loop (0 = from the mesh size)")
+    ap.add_argument("--persist-local", type=int, default=1,
+                    help="persistent loop on one XCD's workgroups with L2-keeping barriers")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-warmup", type=int, default=6)
     args = ap.parse_args()
@@ -221,6 +223,7 @@ def main():
     sim.dmesh.set_option(L.MLX_OPT_GRAPHS, args.graphs)
     sim.dmesh.set_option(L.MLX_OPT_PERSISTENT, args.persistent)
     sim.dmesh.set_option(L.MLX_OPT_PERSIST_WG, args.persist_wg)
+    sim.dmesh.set_option(L.MLX_OPT_PERSIST_LOCAL, args.persist_local)
     ncol, _ = sim.dmesh.info()
     for _ in range(args.warmup):
         sim.step()
@@ -310,6 +313,7 @@ def main():
             "precision": args.precision,
             "graphs": bool(args.graphs),
             "persistent": bool(args.persistent),
+            "persist_local": bool(args.persist_local) if args.persistent else None,
             "colors": ncol,
             "mean_iterations_per_substep": float(np.mean(iters)),
             "tet_projections": int(proj),

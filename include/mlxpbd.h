@@ -408,6 +408,14 @@ int mlx_profile_read_persistent(mlx_mesh *mesh, double *ms, int64_t *launches,
  * them on the host (mlx_union_find_by_vertice, mlx_find_boundary_vertices).
  * The maps are identical either way. */
 #define MLX_OPT_DEVICE_MAPS 5
+/* MLX_OPT_PERSIST_LOCAL (default 1): the persistent loop runs on the
+ * workgroups of one XCD (MLX_OPT_PERSIST_WG is then the size of that group;
+ * 8 times as many are launched, one group per XCD, and all but one group
+ * return at once), which meet at a barrier without the L2 write-back of
+ * an agent-scope release (stores acknowledged, one cache invalidate per
+ * workgroup).  0: all launched workgroups take part, with agent-scope
+ * fences. */
+#define MLX_OPT_PERSIST_LOCAL 6
 int mlx_mesh_set_option(mlx_mesh *mesh, int32_t option, int64_t value);
 
 /* hipDeviceSynchronize on the library's device. */

@@ -147,7 +147,8 @@ struct mlx_mesh {
   int prevExtraIters = 0;          // last layer: iterations past the fixed count, previous call
   // persistent iteration loop (layers without rigid elements)
   int usePersistent = 0;
-  int persistWG = 0;               // 0 = from the largest color
+  int persistWG = 0;               // 0 = from the largest color (local: from the slot count)
+  int persistLocal = 1;            // MLX_OPT_PERSIST_LOCAL: one XCD's workgroups, xcd_barrier
   mlx::DevBuf d_colors;            // int32 colorOff[numColors] | colorCnt[numColors]
   // layer maps on the device (mlx_maps.hip)
   int deviceMaps = 1;              // MLX_OPT_DEVICE_MAPS

@@ -28,6 +28,32 @@ __device__ __forceinline__ bool iteration_stopped(const Ctl *ctl) {
   return (ctl->stopSolve | ctl->endLayer) != 0;
 }
 
+// Load of data that other workgroups of the same launch write (COH): device-
+// scope relaxed atomic loads in 8- or 4-byte pieces, which bypass the CU's
+// vector L1 and read the L2, so no L1 invalidation is needed after a
+// barrier among workgroups of one XCD.  Otherwise an ordinary load.
+template <bool COH, typename V>
+__device__ __forceinline__ V ldc(const V *p) {
+  if constexpr (!COH) {
+    return *p;
+  } else if constexpr (sizeof(V) % 8 == 0) {
+    uint64_t w[sizeof(V) / 8];
+    const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(V) / 8); ++k)
+      w[k] = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(V, w);
+  } else {
+    static_assert(sizeof(V) % 4 == 0, "ldc: size must be a multiple of 4");
+    uint32_t w[sizeof(V) / 4];
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(p);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(V) / 4); ++k)
+      w[k] = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(V, w);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // small dense helpers (double)
 // ---------------------------------------------------------------------------
@@ -195,18 +221,54 @@ struct SweepParams {
 // gamma = alpha~ beta~ / h is alpha1/h times I for the isotropic material.
 // The normal block is solved as a 3x3 system, the three shear rows as
 // scalars with the (4,4) compliance, as decomposeSolve does.
+// What a tet projection reads besides the current positions: slot data and
+// the positions at the start of the substep (neither changes during a
+// layer solve), so it can be fetched ahead of the barrier that publishes
+// the positions.  lam is the slot's own lambda, written only by the thread
+// that projects the slot.
 template <typename T>
-__device__ __forceinline__ void project_tet(const SweepParams<T> &P, const int64_t s) {
+struct TetIn {
+  int v[4];
+  T D[9];
+  T vol, tad, tao, tas, g;
+  T lam[6];
+  vec4_t<T> O[4];
+};
+
+template <typename T>
+__device__ __forceinline__ void tet_fetch(const SweepParams<T> &P, const int64_t s, TetIn<T> &in) {
   const int64_t ld = P.ld;
-  const int v0 = P.vid[s], v1 = P.vid[ld + s], v2 = P.vid[2 * ld + s], v3 = P.vid[3 * ld + s];
-  const vec4_t<T> X0 = P.x[v0], X1 = P.x[v1], X2 = P.x[v2], X3 = P.x[v3];
-  const vec4_t<T> O0 = P.xold[v0], O1 = P.xold[v1], O2 = P.xold[v2], O3 = P.xold[v3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) in.v[a] = P.vid[a * ld + s];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) in.O[a] = P.xold[in.v[a]];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) in.D[k] = P.dminv[k * ld + s];
+  in.vol = P.vol[s];
+  in.tad = P.mat[s] * P.ih2;
+  in.tao = P.mat[ld + s] * P.ih2;
+  in.tas = P.mat[2 * ld + s] * P.ih2;
+  in.g = P.mat[3 * ld + s] * P.ih;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) in.lam[i] = P.lam[i * ld + s];
+}
+
+// The projection proper, on fetched slot data and the current positions
+// (loaded with ldc<COH>).
+template <typename T, bool COH>
+__device__ __forceinline__ void tet_solve(const SweepParams<T> &P, const int64_t s,
+                                          const TetIn<T> &in) {
+  const int64_t ld = P.ld;
+  const int v0 = in.v[0], v1 = in.v[1], v2 = in.v[2], v3 = in.v[3];
+  const vec4_t<T> X0 = ldc<COH>(P.x + v0), X1 = ldc<COH>(P.x + v1), X2 = ldc<COH>(P.x + v2),
+                  X3 = ldc<COH>(P.x + v3);
+  const vec4_t<T> O0 = in.O[0], O1 = in.O[1], O2 = in.O[2], O3 = in.O[3];
   T D[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) D[k] = P.dminv[k * ld + s];
-  const T vol = P.vol[s];
-  const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
-  const T g = P.mat[3 * ld + s] * P.ih;
+  for (int k = 0; k < 9; ++k) D[k] = in.D[k];
+  const T vol = in.vol;
+  const T tad = in.tad, tao = in.tao, tas = in.tas;
+  const T g = in.g;
 
   // edge vectors (current and displacement since x_old)
   const T e[3][3] = {{X1.x - X0.x, X1.y - X0.y, X1.z - X0.z},
@@ -247,7 +309,7 @@ __device__ __forceinline__ void project_tet(const SweepParams<T> &P, const int64
   const T half = T(0.5), vol2 = vol * vol;
   T lam[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) lam[i] = P.lam[i * ld + s];
+  for (int i = 0; i < 6; ++i) lam[i] = in.lam[i];
   // normal block
   T C[3], num[3], M[3][3];
 #pragma unroll
@@ -301,6 +363,13 @@ __device__ __forceinline__ void project_tet(const SweepParams<T> &P, const int64
     o.z += sc * (FS[2][0] * G[a][0] + FS[2][1] * G[a][1] + FS[2][2] * G[a][2]);
     P.x[vs[a]] = o;
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void project_tet(const SweepParams<T> &P, const int64_t s) {
+  TetIn<T> in;
+  tet_fetch<T>(P, s, in);
+  tet_solve<T, false>(P, s, in);
 }
 
 template <typename T>
@@ -837,13 +906,14 @@ struct ContactArrays {
 // penaltyConstraint.projectElastic (penaltyConstraint.m:4-28), 3D;
 // penaltyConstraint2D.projectElastic (penaltyConstraint2D.m:4-26) in 2D:
 // contact i on an elastic vertex.
-template <typename T, int DIM>
+template <typename T, int DIM, bool COH = false>
 __device__ __forceinline__ void contact_elastic_one(const ContactArrays &c, int i, vec4_t<T> *x,
                                                     const double *minv, double resistance,
                                                     double h) {
   const int v = c.vert[i];
   const double *n = c.normal + 3 * i, *pc = c.point + 3 * i;
-  vec4_t<T> p = x[v];
+  vec4_t<T> p = ldc<COH>(x + v);
+  const double lam_i = ldc<COH>(c.lambda + i);
   const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
   double phi = n[0] * (xv[0] - pc[0]) + n[1] * (xv[1] - pc[1]);
   if (DIM == 3) phi = phi + n[2] * (xv[2] - pc[2]);
@@ -853,17 +923,17 @@ __device__ __forceinline__ void contact_elastic_one(const ContactArrays &c, int 
   const double mi = minv[v];
   double dl;
   if (DIM == 3) {
-    dl = (-C - aT * c.lambda[i]) / (2 * mi + aT);
+    dl = (-C - aT * lam_i) / (2 * mi + aT);
     p.x = (T)(xv[0] + mi * (dl * n[0]));
     p.y = (T)(xv[1] + mi * (dl * n[1]));
     p.z = (T)(xv[2] + mi * (dl * n[2]));
   } else {
     const double g0 = n[0] * 2 * mi, g1 = n[1] * 2 * mi;
-    dl = -(C + aT * c.lambda[i]) / ((g0 * n[0] + g1 * n[1]) + aT);
+    dl = -(C + aT * lam_i) / ((g0 * n[0] + g1 * n[1]) + aT);
     p.x = (T)(xv[0] + g0 * dl);
     p.y = (T)(xv[1] + g1 * dl);
   }
-  c.lambda[i] += dl;
+  c.lambda[i] = lam_i + dl;
   x[v] = p;
 }
 
@@ -1023,13 +1093,13 @@ __global__ __launch_bounds__(kBlock) void k_pins(int64_t n, const int32_t *dof, 
 
 // Residual terms of one tetrahedron (xpbdLayer3D.m:555-568): sum over its six
 // constraints of (|C_i| - min(|C_i|, |(alpha~ lambda)_i|))^2, in double.
-template <typename T>
+template <typename T, bool COH = false>
 __device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const int64_t s) {
   const int64_t ld = P.ld;
   const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
   const T vol = P.vol[s];
-  const vec4_t<T> X0 = P.x[P.vid[s]], X1 = P.x[P.vid[ld + s]], X2 = P.x[P.vid[2 * ld + s]],
-                  X3 = P.x[P.vid[3 * ld + s]];
+  const vec4_t<T> X0 = ldc<COH>(P.x + P.vid[s]), X1 = ldc<COH>(P.x + P.vid[ld + s]),
+                  X2 = ldc<COH>(P.x + P.vid[2 * ld + s]), X3 = ldc<COH>(P.x + P.vid[3 * ld + s]);
   T D[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) D[k] = P.dminv[k * ld + s];
@@ -1049,7 +1119,7 @@ __device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const in
     for (int k = 0; k < 3; ++k) Cg[j][k] = F[0][j] * F[0][k] + F[1][j] * F[1][k] + F[2][j] * F[2][k];
   T lam[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) lam[i] = P.lam[i * ld + s];
+  for (int i = 0; i < 6; ++i) lam[i] = ldc<COH>(P.lam + i * ld + s);
   const T half = T(0.5);
   T C[6], al[6];
 #pragma unroll
@@ -1078,10 +1148,11 @@ __device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const in
 // 2D xpbdLayer2D.m:477-496): per element sum_i (|C_i| - min(|C_i|,
 // |(alpha~ lambda)_i|))^2 in double.  lamSlot2d gives, for the 2D quirk, the
 // slot whose lambda column feeds each page entry.
-template <typename T, int DIM>
+template <typename T, int DIM, bool COH = false>
 __device__ __forceinline__ double residual_elem(const SweepParams<T> &P, int64_t s,
                                                 const int32_t *lamSlot2d) {
-  if (DIM == 3) return residual_tet<T>(P, s);
+  if (DIM == 3) return residual_tet<T, COH>(P, s);
+  static_assert(DIM == 3 || !COH, "coherent loads are for 3D only");
   const int64_t ld = P.ld;
   const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
   const T vol = P.vol[s];
@@ -1133,14 +1204,15 @@ __global__ __launch_bounds__(kBlock) void k_residual_elems(SweepParams<T> P, int
 // expands to K x 1 x K: every contact i contributes once per compliance
 // value j.  uniqAlpha/uniqCount list the distinct compliance values with
 // multiplicities (3D); in 2D each contact counts once with its own value.
-template <typename T, int DIM>
+template <typename T, int DIM, bool COH = false>
 __device__ __forceinline__ double residual_contact_one(const ContactArrays &c, int i,
                                                        const vec4_t<T> *x, double resistance,
                                                        double h, const double *uniqAlpha,
                                                        const double *uniqCount, int nUniq) {
   const int v = c.vert[i];
   const double *n = c.normal + 3 * i, *pc = c.point + 3 * i;
-  const vec4_t<T> p = x[v];
+  const vec4_t<T> p = ldc<COH>(x + v);
+  const double lam_i = ldc<COH>(c.lambda + i);
   double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]);
   if (DIM == 3) phi = phi + n[2] * ((double)p.z - pc[2]);
   if (phi > 0) phi = 0;
@@ -1149,12 +1221,12 @@ __device__ __forceinline__ double residual_contact_one(const ContactArrays &c, i
   double acc = 0;
   if (DIM == 3) {
     for (int u = 0; u < nUniq; ++u) {
-      const double cr = fabs((uniqAlpha[u] / h2) * c.lambda[i]);
+      const double cr = fabs((uniqAlpha[u] / h2) * lam_i);
       const double t = a - (a < cr ? a : cr);
       acc += uniqCount[u] * (t * t);
     }
   } else {
-    const double cr = fabs((c.compliance[i] / h2) * c.lambda[i]);
+    const double cr = fabs((c.compliance[i] / h2) * lam_i);
     const double t = a - (a < cr ? a : cr);
     acc += t * t;
   }
@@ -1733,6 +1805,83 @@ __device__ __forceinline__ bool grid_barrier(unsigned *count, unsigned target, i
   return s_ok != 0;
 }
 
+// XCD (accelerator complex die) that runs the calling wave: hardware
+// register XCC_ID (gfx940+), low 4 bits.  Workgroups with the same id share
+// one L2.
+__device__ __forceinline__ unsigned xcd_id() {
+  return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (15 << 11)) & 15u;
+}
+
+// Barrier among workgroups that all run on one XCD, hence share its L2,
+// and read each other's data only with ldc<true> (past the vector L1), in
+// two halves so that loads of unchanging data can be issued in between.
+// xcd_arrive: every thread waits until its own memory operations are
+// acknowledged (stores have reached the L2, the vector L1 being
+// write-through), the workgroup meets, one thread adds its arrival.
+// xcd_wait: that thread waits for `target` arrivals, the workgroup meets.
+// Nothing is written back from or dropped in the L2 (a device-scope
+// acquire would invalidate it; a workgroup-scope buffer_inv does not drop
+// the CU's L1 lines for ordinary workgroups, measured: every read of
+// another workgroup's data through the L1 was stale with it, none with
+// L1-bypassing loads).  The wait is bounded like grid_barrier's.
+__device__ __forceinline__ void xcd_arrive(unsigned *count) {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool xcd_wait(unsigned *count, unsigned target, int *err) {
+  __shared__ int s_ok;
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    long spins = 0;
+    while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kBarrierSpinLimit) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+    }
+    s_ok = ok;
+  }
+  __syncthreads();
+  asm volatile("" ::: "memory");
+  return s_ok != 0;
+}
+
+// Workgroup groups by XCD for a cooperative launch: each workgroup takes a
+// rank on its XCD's counter reg[xcd], all workgroups meet once at a grid
+// barrier (counter reg[16]), then the XCD with the most workgroups (lowest
+// id on ties) is chosen.  Returns this workgroup's rank in the chosen group
+// and sets *n to its size, or returns -1 for workgroups of other XCDs (they
+// take no further part).  reg: 17 zeroed counters.
+__device__ __forceinline__ int xcd_group(unsigned *reg, int *err, int *n) {
+  __shared__ int s_rank, s_n;
+  if (threadIdx.x == 0) {
+    const unsigned x = xcd_id();
+    s_rank = (int)__hip_atomic_fetch_add(&reg[x], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_n = (int)x;
+  }
+  if (!grid_barrier(reg + 16, gridDim.x, err)) return -1;
+  if (threadIdx.x == 0) {
+    unsigned best = 0, bn = 0;
+    for (unsigned k = 0; k < 16; ++k) {
+      const unsigned c = __hip_atomic_load(&reg[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (c > bn) {
+        bn = c;
+        best = k;
+      }
+    }
+    s_n = (unsigned)s_n == best ? (int)bn : 0;
+  }
+  __syncthreads();
+  *n = s_n;
+  return s_n > 0 ? s_rank : -1;
+}
+
 template <typename T>
 struct PersistParams {
   SweepParams<T> P;              // P.ctl is the layer's control block
@@ -1756,67 +1905,133 @@ struct PersistParams {
   int residCap;
   unsigned *bar;                 // arrival counter, zero at launch
   int *err;                      // barrier failure flag
+  unsigned *reg;                 // LOCAL: 17 XCD group counters, zero at launch
   int maxIters;
+  int maxColorCnt;               // elements of the largest color
 };
 
 // Up to maxIters projectGS iterations (xpbdLayer3D.m:306-344; 2D
 // xpbdLayer2D.m:174-198) of a layer whose elements are all elastic, in one
 // cooperative launch: the colors in order, each split over the workgroups
-// and closed by a grid barrier; the elastic contacts then the pins (one
-// workgroup, contact order then pin order); the residual (each workgroup a
-// fixed share of the elements, workgroup 0 the contacts) and the monitor.
-// Every workgroup re-reads the control block after the last barrier of an
+// and closed by a barrier; the elastic contacts (one per vertex, all
+// workgroups), a barrier, then the pins; the residual (each workgroup a
+// fixed share of the elements and contacts) and the monitor.  Every
+// workgroup re-reads the control block after the last barrier of an
 // iteration, so all stop together when the monitor sets stopSolve/endLayer.
-template <typename T, int DIM>
+// LOCAL (3D): only the workgroups of one XCD take part (xcd_group), data
+// written in the launch is read with ldc<true>, and they meet at
+// xcd_arrive/xcd_wait; when every color fits one pass of the group, each
+// thread fetches its slot of the next color (tet_fetch) between the two
+// halves of the barrier, so only the position gathers follow it.
+// Otherwise all workgroups take part and meet at grid_barrier.
+template <typename T, int DIM, bool LOCAL>
 __global__ __launch_bounds__(kBlock) void k_persist(PersistParams<T> Q) {
+  static_assert(DIM == 3 || !LOCAL, "the XCD-local loop is 3D only");
   __shared__ double sh[kBlock];
-  const int G = gridDim.x, w = blockIdx.x, t = threadIdx.x;
+  int G = gridDim.x, w = blockIdx.x;
+  if (LOCAL) {
+    w = xcd_group(Q.reg, Q.err, &G);
+    if (w < 0) return;
+  }
+  const int t = threadIdx.x;
   const int64_t stride = (int64_t)G * kBlock;
   Ctl *ctl = const_cast<Ctl *>(Q.P.ctl);
   unsigned nbar = 0;
+  auto arrive = [&]() {
+    ++nbar;
+    if (LOCAL) xcd_arrive(Q.bar);
+  };
+  auto wait = [&]() -> bool {
+    return LOCAL ? xcd_wait(Q.bar, nbar * (unsigned)G, Q.err)
+                 : grid_barrier(Q.bar, nbar * (unsigned)G, Q.err);
+  };
+  auto barrier = [&]() -> bool {
+    arrive();
+    return wait();
+  };
+  const bool onePass = LOCAL && Q.maxColorCnt <= stride;
   for (int it = 0; it < Q.maxIters; ++it) {
-    if (iteration_stopped(ctl)) break;
-    for (int c = 0; c < Q.numColors; ++c) {
+    if ((ldc<LOCAL>(&ctl->stopSolve) | ldc<LOCAL>(&ctl->endLayer)) != 0) break;
+    bool swept = false;
+    if constexpr (DIM == 3 && LOCAL) {
+      if (onePass) {
+        // slot of this thread in color c (colors fit one pass)
+        TetIn<T> cur;
+        int64_t sc = -1;
+        int c = 0;
+        while (c < Q.numColors && Q.colorCnt[c] == 0) ++c;
+        if (c < Q.numColors) {
+          const int64_t s0 = Q.colorOff[c] + (int64_t)w * kBlock + t;
+          if (s0 < Q.colorOff[c] + Q.colorCnt[c]) {
+            sc = s0;
+            tet_fetch<T>(Q.P, sc, cur);
+          }
+        }
+        while (c < Q.numColors) {
+          if (sc >= 0) tet_solve<T, true>(Q.P, sc, cur);
+          arrive();
+          int cn = c + 1;
+          while (cn < Q.numColors && Q.colorCnt[cn] == 0) ++cn;
+          sc = -1;
+          if (cn < Q.numColors) {
+            const int64_t s0 = Q.colorOff[cn] + (int64_t)w * kBlock + t;
+            if (s0 < Q.colorOff[cn] + Q.colorCnt[cn]) {
+              sc = s0;
+              tet_fetch<T>(Q.P, sc, cur);
+            }
+          }
+          if (!wait()) return;
+          c = cn;
+        }
+        swept = true;
+      }
+    }
+    for (int c = 0; !swept && c < Q.numColors; ++c) {
       const int cnt = Q.colorCnt[c];
       if (cnt == 0) continue;
       const int64_t off = Q.colorOff[c], end = off + cnt;
       for (int64_t s = off + (int64_t)w * kBlock + t; s < end; s += stride) {
-        if (DIM == 3)
-          project_tet<T>(Q.P, s);
-        else
+        if constexpr (DIM == 3) {
+          TetIn<T> in;
+          tet_fetch<T>(Q.P, s, in);
+          tet_solve<T, LOCAL>(Q.P, s, in);
+        } else {
           project_tri<T>(Q.P, s);
+        }
       }
-      if (!grid_barrier(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
+      if (!barrier()) return;
     }
-    if (w == 0 && (Q.nec > 0 || Q.nPin > 0)) {
-      for (int q = t; q < Q.nec; q += kBlock)
-        contact_elastic_one<T, DIM>(Q.ca, Q.ecIdx[q], Q.P.x, Q.minv, Q.resistance, Q.h);
-      __syncthreads();
-      for (int64_t q = t; q < Q.nPin; q += kBlock) pin_one<T, DIM>(Q.pinDof, Q.pinVal, Q.P.x, q);
+    // contacts (distinct vertices) over all workgroups, then the pins
+    if (Q.nec > 0) {
+      for (int64_t q = (int64_t)w * kBlock + t; q < Q.nec; q += stride)
+        contact_elastic_one<T, DIM, LOCAL>(Q.ca, Q.ecIdx[q], Q.P.x, Q.minv, Q.resistance, Q.h);
+      if (Q.nPin > 0 && !barrier()) return;
     }
-    if (!grid_barrier(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
+    for (int64_t q = (int64_t)w * kBlock + t; q < Q.nPin; q += stride)
+      pin_one<T, DIM>(Q.pinDof, Q.pinVal, Q.P.x, q);
+    if (!barrier()) return;
     if (Q.mp.computeResiduals) {
       double acc = 0;
       for (int64_t s = (int64_t)w * kBlock + t; s < Q.P.ld; s += stride)
-        if (Q.P.vid[s] >= 0) acc += residual_elem<T, DIM>(Q.P, s, Q.lamSlot2d);
-      if (w == 0)
-        for (int i = t; i < Q.ca.K; i += kBlock)
-          acc += residual_contact_one<T, DIM>(Q.ca, i, Q.P.x, Q.resistance, Q.h, Q.uniq, Q.ucnt,
-                                              Q.nUniq);
+        if (Q.P.vid[s] >= 0) acc += residual_elem<T, DIM, LOCAL>(Q.P, s, Q.lamSlot2d);
+      for (int64_t i = (int64_t)w * kBlock + t; i < Q.ca.K; i += stride)
+        acc += residual_contact_one<T, DIM, LOCAL>(Q.ca, (int)i, Q.P.x, Q.resistance, Q.h,
+                                                   Q.uniq, Q.ucnt, Q.nUniq);
       double v1[1] = {acc}, o[1];
       block_sum<1>(v1, sh, o);
       if (t == 0) Q.partials[w] = o[0];
-      if (!grid_barrier(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
+      if (!barrier()) return;
     }
     if (w == 0 && t == 0) {
+      // this thread is the only writer of the control block in the launch
       ctl->iters += 1;
       if (Q.mp.computeResiduals) {
         double sum = 0;
-        for (int q = 0; q < G; ++q) sum += Q.partials[q];
+        for (int q = 0; q < G; ++q) sum += ldc<LOCAL>(Q.partials + q);
         monitor_update(*ctl, Q.mp, sqrt(sum), Q.resid, Q.residCap);
       }
     }
-    if (!grid_barrier(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
+    if (!barrier()) return;
   }
 }
 

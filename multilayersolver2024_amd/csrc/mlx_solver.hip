@@ -103,6 +103,18 @@ struct SolveResult {
 };
 
 static constexpr int kMaxPartials = 65536;
+static constexpr int kXcds = 8;  // XCDs of an MI355X (workgroups are dealt round-robin over them)
+
+// The persistent-loop kernel for the mesh's dimension and barrier mode.
+// The XCD-local form is 3D only.
+static bool persist_local(const mlx_mesh &m) { return m.persistLocal && m.dim == 3; }
+
+template <typename T>
+static const void *persist_fn(const mlx_mesh &m) {
+  if (persist_local(m)) return reinterpret_cast<const void *>(&k_persist<T, 3, true>);
+  return m.dim == 3 ? reinterpret_cast<const void *>(&k_persist<T, 3, false>)
+                    : reinterpret_cast<const void *>(&k_persist<T, 2, false>);
+}
 
 // Upload the 1-based column-major contact arrays of a layerSolve call.
 // Contact arrays live at offsets fixed by the capacity (one contact per
@@ -452,7 +464,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
 
   const int64_t nPin = (int64_t)m.pinnedDOF.size();
   // persistent loop: color table on the device, barrier counter, grid size
-  int persistG = 0;
+  int persistG = 0, persistMaxCnt = 0;
   if (m.usePersistent) {
     std::vector<int32_t> ct(2 * (size_t)m.numColors);
     int64_t maxCnt = 1;
@@ -461,20 +473,23 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       ct[m.numColors + c] = (int32_t)m.colorCnt[c];
       maxCnt = std::max<int64_t>(maxCnt, m.colorCnt[c]);
     }
+    persistMaxCnt = (int)maxCnt;
     MLX_TRY(m.d_colors.reserve(sizeof(int32_t) * ct.size() + 4));
     MLX_CHECK_HIP(hipMemcpyAsync(m.d_colors.ptr, ct.data(), sizeof(int32_t) * ct.size(),
                                  hipMemcpyHostToDevice, st));
     MLX_TRY(m.d_sync.reserve(256));
+    // local: persistG workgroups per XCD are launched on each of the 8 XCDs
+    // and one XCD's group runs the loop; otherwise persistG in all
     persistG = m.persistWG > 0 ? m.persistWG
-                               : (int)std::min<int64_t>(32, std::max<int64_t>(4, (maxCnt + kBlock - 1) / kBlock));
+               : persist_local(m)
+                   ? (int)std::min<int64_t>(32, std::max<int64_t>(4, (ld + 8 * kBlock - 1) / (8 * kBlock)))
+                   : (int)std::min<int64_t>(32, std::max<int64_t>(4, (maxCnt + kBlock - 1) / kBlock));
     int perCU = 0, dev = 0;
-    const void *fn = m.dim == 3 ? reinterpret_cast<const void *>(&k_persist<T, 3>)
-                                : reinterpret_cast<const void *>(&k_persist<T, 2>);
-    MLX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, kBlock, 0));
+    MLX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, persist_fn<T>(m), kBlock, 0));
     MLX_CHECK_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     MLX_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
-    persistG = std::min(persistG, perCU * prop.multiProcessorCount);
+    persistG = std::min(persistG * (persist_local(m) ? kXcds : 1), perCU * prop.multiProcessorCount);
     if (persistG > kMaxPartials || persistG < 1) {
       set_error("persistent loop: no valid grid size");
       return MLX_ERR_STATE;
@@ -652,7 +667,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     const bool persist = m.usePersistent && !hasRigidEl && L.nb == 0;
     auto run_persistent = [&](int n) -> int {
       if (n <= 0) return MLX_OK;
-      MLX_CHECK_HIP(hipMemsetAsync(m.d_sync.ptr, 0, 8, st));
+      MLX_CHECK_HIP(hipMemsetAsync(m.d_sync.ptr, 0, 256, st));
       PersistParams<T> Q;
       Q.P = P;
       Q.P.flags = nullptr;
@@ -678,10 +693,11 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       Q.residCap = residCap;
       Q.bar = m.d_sync.as<unsigned>();
       Q.err = reinterpret_cast<int *>(m.d_sync.as<unsigned>() + 1);
+      Q.reg = m.d_sync.as<unsigned>() + 16;
       Q.maxIters = n;
+      Q.maxColorCnt = persistMaxCnt;
       void *args[] = {&Q};
-      const void *fn = m.dim == 3 ? reinterpret_cast<const void *>(&k_persist<T, 3>)
-                                  : reinterpret_cast<const void *>(&k_persist<T, 2>);
+      const void *fn = persist_fn<T>(m);
       const bool timed = m.prof.on && m.prof.pev[0] && m.prof.pev[1];
       if (timed) MLX_CHECK_HIP(hipEventRecord(m.prof.pev[0], st));
       MLX_CHECK_HIP(hipLaunchCooperativeKernel(fn, dim3(persistG), dim3(kBlock), args, 0, st));
@@ -1863,6 +1879,9 @@ int mlx_mesh_set_option(mlx_mesh *m, int32_t option, int64_t value) {
       return MLX_OK;
     case MLX_OPT_DEVICE_MAPS:
       m->deviceMaps = value != 0;
+      return MLX_OK;
+    case MLX_OPT_PERSIST_LOCAL:
+      m->persistLocal = value != 0;
       return MLX_OK;
     default:
       break;

@@ -199,24 +199,34 @@ def test_graph_replay_matches_eager(oracle_lib):
         b.close()
 
 
-@pytest.mark.parametrize("precision,wg", [(L.MLX_FP64, 0), (L.MLX_FP64, 7), (L.MLX_FP32, 0)])
-def test_persistent_loop_matches_per_color_launches(precision, wg, oracle_lib):
+@pytest.mark.parametrize("precision,wg,local,size", [
+    (L.MLX_FP64, 0, 0, "small"), (L.MLX_FP64, 7, 0, "small"), (L.MLX_FP32, 0, 0, "small"),
+    (L.MLX_FP64, 0, 1, "small"), (L.MLX_FP64, 7, 1, "small"), (L.MLX_FP32, 0, 1, "small"),
+    (L.MLX_FP64, 0, 1, "large"), (L.MLX_FP64, 3, 1, "large"), (L.MLX_FP64, 0, 0, "large")])
+def test_persistent_loop_matches_per_color_launches(precision, wg, local, size, oracle_lib):
     """The cooperative persistent loop (colors split over workgroups, grid
     barriers) runs the same element, contact and pin updates as per-color
     launches; only the residual is summed in another order.  In FP64 the
     iteration counts match and the states agree to 1e-12 (the two kernels
     may fuse multiply-adds differently); in FP32 the states agree to 1e-5.
-    Runs the stop rule (relative 1e-4) with ground contacts."""
+    Runs the stop rule (relative 1e-4) with ground contacts.  local=1 runs
+    the loop on one XCD's workgroups with L2-keeping barriers; the large
+    beam (19,200 tets, colors of ~480 tets) spreads every color over
+    several workgroups so each barrier carries positions between CUs."""
     import copy
-    m, integ, plane = beam_case(nx=10, ny=4, nz=4, z0=0.004, pct=99.99)
+    if size == "small":
+        m, integ, plane = beam_case(nx=10, ny=4, nz=4, z0=0.004, pct=99.99)
+    else:
+        m, integ, plane = beam_case(nx=60, ny=8, nz=8, z0=0.004, pct=99.99)
     a = Simulation(m, integ, 1 / 60, plane, precision=precision)
     b = Simulation(m, copy.deepcopy(integ), 1 / 60, plane, precision=precision)
     a.dmesh.set_option(L.MLX_OPT_PERSISTENT, 0)
     b.dmesh.set_option(L.MLX_OPT_PERSISTENT, 1)
     b.dmesh.set_option(L.MLX_OPT_PERSIST_WG, wg)
+    b.dmesh.set_option(L.MLX_OPT_PERSIST_LOCAL, local)
     try:
         kmax = 0
-        for k in range(8):
+        for k in range(8 if size == "small" else 5):
             ra, rb = a.step(), b.step()
             pa, va, la, ka = a.download()
             pb, vb, lb, kb = b.download()
