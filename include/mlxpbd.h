@@ -334,13 +334,21 @@ typedef struct {
 /* One integrate() substep of every scene (xpbdLayer3D.m:621-745 per scene),
  * the whole projectGS loop of all scenes in one kernel launch. */
 /* Splits large scenes over several workgroups (tetsPerWorkgroup > 0): a
- * scene of n tets gets round(n / tetsPerWorkgroup) workgroups (at least
- * one), which meet at a per-scene grid barrier after every color, and the
- * whole grid is one cooperative launch (the target grows until every
- * workgroup is resident).  0 (default): one workgroup per scene.  Results
- * follow the same per-element updates either way; the residual is summed in
- * another order (last-bit differences). */
+ * scene of n tets gets round(n / tetsPerWorkgroup) workgroups (1..64), all
+ * on one XCD (placed by the XCC_ID register), which meet at a per-scene
+ * barrier after every color; the whole grid is one cooperative launch (the
+ * target grows until every workgroup is resident).  If the workgroups were
+ * not dealt over the XCDs as planned, the launch does nothing and the
+ * substep runs with one workgroup per scene.  0 (default): one workgroup per
+ * scene.  Results follow the same per-element updates either way; the
+ * residual is summed in another order (last-bit differences). */
 int mlx_batch_set_split(mlx_batch *batch, int32_t tetsPerWorkgroup);
+
+/* With one workgroup per scene, scenes whose positions fit in the
+ * workgroup's LDS (3 values per vertex) keep them there from the first
+ * color to the last residual (enable = 1, default); 0 keeps every scene's
+ * positions in HBM.  Same updates in the same order either way. */
+int mlx_batch_set_lds(mlx_batch *batch, int32_t enable);
 
 int mlx_batch_integrate(mlx_batch *batch, mlx_settings *settings, double h,
                         int32_t frame, const mlx_plane *plane,

@@ -89,6 +89,10 @@ class BatchSimulation:
         L.check("mlx_batch_set_split", L.lib().mlx_batch_set_split(self.handle,
                                                                    int(tets_per_workgroup)))
 
+    def set_lds(self, enable):
+        """Keep positions of scenes that fit in LDS there (default on)."""
+        L.check("mlx_batch_set_lds", L.lib().mlx_batch_set_lds(self.handle, int(bool(enable))))
+
     def set_block(self, threads):
         L.check("mlx_batch_set_block", L.lib().mlx_batch_set_block(self.handle, int(threads)))
 

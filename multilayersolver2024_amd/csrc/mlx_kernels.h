@@ -253,15 +253,14 @@ __device__ __forceinline__ void tet_fetch(const SweepParams<T> &P, const int64_t
   for (int i = 0; i < 6; ++i) in.lam[i] = P.lam[i * ld + s];
 }
 
-// The projection proper, on fetched slot data and the current positions
-// (loaded with ldc<COH>).
-template <typename T, bool COH>
-__device__ __forceinline__ void tet_solve(const SweepParams<T> &P, const int64_t s,
-                                          const TetIn<T> &in) {
+// The projection proper, on fetched slot data and the current positions X
+// of the tet's four vertices; the moved vertex a is handed to st(a, x).
+template <typename T, typename Store>
+__device__ __forceinline__ void tet_solve_st(const SweepParams<T> &P, const int64_t s,
+                                             const TetIn<T> &in, const vec4_t<T> (&X)[4],
+                                             Store &&st) {
   const int64_t ld = P.ld;
-  const int v0 = in.v[0], v1 = in.v[1], v2 = in.v[2], v3 = in.v[3];
-  const vec4_t<T> X0 = ldc<COH>(P.x + v0), X1 = ldc<COH>(P.x + v1), X2 = ldc<COH>(P.x + v2),
-                  X3 = ldc<COH>(P.x + v3);
+  const vec4_t<T> X0 = X[0], X1 = X[1], X2 = X[2], X3 = X[3];
   const vec4_t<T> O0 = in.O[0], O1 = in.O[1], O2 = in.O[2], O3 = in.O[3];
   T D[9];
 #pragma unroll
@@ -353,7 +352,6 @@ __device__ __forceinline__ void tet_solve(const SweepParams<T> &P, const int64_t
 #pragma unroll
     for (int c = 0; c < 3; ++c) FS[r][c] = F[r][0] * S[0][c] + F[r][1] * S[1][c] + F[r][2] * S[2][c];
   const vec4_t<T> Xs[4] = {X0, X1, X2, X3};
-  const int vs[4] = {v0, v1, v2, v3};
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const T sc = w[a] * vol;
@@ -361,8 +359,30 @@ __device__ __forceinline__ void tet_solve(const SweepParams<T> &P, const int64_t
     o.x += sc * (FS[0][0] * G[a][0] + FS[0][1] * G[a][1] + FS[0][2] * G[a][2]);
     o.y += sc * (FS[1][0] * G[a][0] + FS[1][1] * G[a][1] + FS[1][2] * G[a][2]);
     o.z += sc * (FS[2][0] * G[a][0] + FS[2][1] * G[a][1] + FS[2][2] * G[a][2]);
-    P.x[vs[a]] = o;
+    st(a, o);
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void tet_solve_x(const SweepParams<T> &P, const int64_t s,
+                                            const TetIn<T> &in, const vec4_t<T> (&X)[4]) {
+  tet_solve_st<T>(P, s, in, X, [&](int a, const vec4_t<T> &o) { P.x[in.v[a]] = o; });
+}
+
+// Current positions of a fetched tet's vertices (ldc<COH>).
+template <typename T, bool COH>
+__device__ __forceinline__ void tet_gather(const SweepParams<T> &P, const TetIn<T> &in,
+                                           vec4_t<T> (&X)[4]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) X[a] = ldc<COH>(P.x + in.v[a]);
+}
+
+template <typename T, bool COH>
+__device__ __forceinline__ void tet_solve(const SweepParams<T> &P, const int64_t s,
+                                          const TetIn<T> &in) {
+  vec4_t<T> X[4];
+  tet_gather<T, COH>(P, in, X);
+  tet_solve_x<T>(P, s, in, X);
 }
 
 template <typename T>
@@ -370,6 +390,31 @@ __device__ __forceinline__ void project_tet(const SweepParams<T> &P, const int64
   TetIn<T> in;
   tet_fetch<T>(P, s, in);
   tet_solve<T, false>(P, s, in);
+}
+
+// The slots s, s + step, ... < e of one color for one thread, two at a time:
+// both tets' slot data and positions are loaded before either is solved.
+// The tets of a color share no vertex, so no other thread writes these
+// positions while the color runs, and this thread's first solve does not
+// touch the second tet's vertices.
+template <typename T, bool COH>
+__device__ __forceinline__ void color_slots(const SweepParams<T> &P, int64_t s, const int64_t e,
+                                            const int64_t step) {
+  for (; s + step < e; s += 2 * step) {
+    TetIn<T> a, b;
+    vec4_t<T> Xa[4], Xb[4];
+    tet_fetch<T>(P, s, a);
+    tet_fetch<T>(P, s + step, b);
+    tet_gather<T, COH>(P, a, Xa);
+    tet_gather<T, COH>(P, b, Xb);
+    tet_solve_x<T>(P, s, a, Xa);
+    tet_solve_x<T>(P, s + step, b, Xb);
+  }
+  if (s < e) {
+    TetIn<T> a;
+    tet_fetch<T>(P, s, a);
+    tet_solve<T, COH>(P, s, a);
+  }
 }
 
 template <typename T>
@@ -1093,16 +1138,39 @@ __global__ __launch_bounds__(kBlock) void k_pins(int64_t n, const int32_t *dof, 
 
 // Residual terms of one tetrahedron (xpbdLayer3D.m:555-568): sum over its six
 // constraints of (|C_i| - min(|C_i|, |(alpha~ lambda)_i|))^2, in double.
-template <typename T, bool COH = false>
-__device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const int64_t s) {
+// Inputs of one tet's residual terms.
+template <typename T>
+struct ResIn {
+  vec4_t<T> X[4];
+  T D[9];
+  T tad, tao, tas, vol;
+  T lam[6];
+};
+
+template <typename T, bool COH>
+__device__ __forceinline__ void res_fetch(const SweepParams<T> &P, const int64_t s,
+                                          ResIn<T> &in) {
   const int64_t ld = P.ld;
-  const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
-  const T vol = P.vol[s];
-  const vec4_t<T> X0 = ldc<COH>(P.x + P.vid[s]), X1 = ldc<COH>(P.x + P.vid[ld + s]),
-                  X2 = ldc<COH>(P.x + P.vid[2 * ld + s]), X3 = ldc<COH>(P.x + P.vid[3 * ld + s]);
+  in.tad = P.mat[s] * P.ih2;
+  in.tao = P.mat[ld + s] * P.ih2;
+  in.tas = P.mat[2 * ld + s] * P.ih2;
+  in.vol = P.vol[s];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) in.X[a] = ldc<COH>(P.x + P.vid[a * ld + s]);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) in.D[k] = P.dminv[k * ld + s];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) in.lam[i] = ldc<COH>(P.lam + i * ld + s);
+}
+
+template <typename T>
+__device__ __forceinline__ double res_compute(const ResIn<T> &in) {
+  const T tad = in.tad, tao = in.tao, tas = in.tas;
+  const T vol = in.vol;
+  const vec4_t<T> X0 = in.X[0], X1 = in.X[1], X2 = in.X[2], X3 = in.X[3];
   T D[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) D[k] = P.dminv[k * ld + s];
+  for (int k = 0; k < 9; ++k) D[k] = in.D[k];
   const T e[3][3] = {{X1.x - X0.x, X1.y - X0.y, X1.z - X0.z},
                      {X2.x - X0.x, X2.y - X0.y, X2.z - X0.z},
                      {X3.x - X0.x, X3.y - X0.y, X3.z - X0.z}};
@@ -1119,7 +1187,7 @@ __device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const in
     for (int k = 0; k < 3; ++k) Cg[j][k] = F[0][j] * F[0][k] + F[1][j] * F[1][k] + F[2][j] * F[2][k];
   T lam[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) lam[i] = ldc<COH>(P.lam + i * ld + s);
+  for (int i = 0; i < 6; ++i) lam[i] = in.lam[i];
   const T half = T(0.5);
   T C[6], al[6];
 #pragma unroll
@@ -1140,6 +1208,53 @@ __device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const in
     const double a = fabs((double)C[i]), cr = fabs((double)al[i]);
     const double t = a - (a < cr ? a : cr);
     acc += t * t;
+  }
+  return acc;
+}
+
+// res_fetch with the positions from pos(vertex id) instead of P.x.
+template <typename T, typename Pos>
+__device__ __forceinline__ void res_fetch_pos(const SweepParams<T> &P, const int64_t s,
+                                              ResIn<T> &in, Pos &&pos) {
+  const int64_t ld = P.ld;
+  in.tad = P.mat[s] * P.ih2;
+  in.tao = P.mat[ld + s] * P.ih2;
+  in.tas = P.mat[2 * ld + s] * P.ih2;
+  in.vol = P.vol[s];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) in.X[a] = pos(P.vid[a * ld + s]);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) in.D[k] = P.dminv[k * ld + s];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) in.lam[i] = P.lam[i * ld + s];
+}
+
+template <typename T, bool COH = false>
+__device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const int64_t s) {
+  ResIn<T> in;
+  res_fetch<T, COH>(P, s, in);
+  return res_compute<T>(in);
+}
+
+// Sum of residual_tet over the real slots s, s + step, ... < e of one
+// thread, four slots' inputs loaded before their terms are computed.
+template <typename T, bool COH>
+__device__ __forceinline__ double residual_slots(const SweepParams<T> &P, int64_t s,
+                                                 const int64_t e, const int64_t step) {
+  constexpr int U = 4;
+  double acc = 0;
+  for (; s < e; s += U * step) {
+    ResIn<T> in[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t su = s + u * step;
+      ok[u] = su < e && P.vid[su] >= 0;
+      if (ok[u]) res_fetch<T, COH>(P, su, in[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (ok[u]) acc += res_compute<T>(in[u]);
   }
   return acc;
 }
@@ -1560,6 +1675,8 @@ struct BatchParams {
   const int64_t *slot0;          // nScenes+1 slot range per scene
   const int64_t *cont0;          // nScenes+1 contact range per scene
   const int64_t *pin0;           // nScenes+1 pinned-DOF range per scene
+  const int64_t *vert0;          // nScenes+1 vertex range per scene
+  int ldsVerts;                  // scenes with at most this many vertices keep positions in LDS
   const int32_t *pinDof;         // 0-based DOF
   const T *pinVal;
   ContactArrays c;
@@ -1575,17 +1692,40 @@ struct BatchParams {
   int32_t *residualCount;
 };
 
-// One scene per workgroup (plain method: a single all-elastic layer): the
-// projectGS loop of xpbdLayer3D.m:315-341 with colors separated by workgroup
-// barriers, then contacts, pins and the scene's own residual/stop rule.
-// Every loop is bounded (hardCap) so the kernel always returns.
+// k_batch_plain for a scene whose positions fit in LDS: x, y, z of its nv
+// vertices live in LDS (structure of arrays, pos[d * nv + local vertex])
+// from the first color to the last residual, and go back to HBM at the end.
+// The inverse mass (w) comes from x_old, a copy of x taken at the start of
+// the substep.  Same updates in the same order as the HBM path.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
-  __shared__ double sh[NT];
+__device__ __forceinline__ void batch_scene_lds(const BatchParams<T> &B, const int sc, T *pos,
+                                                double *sh) {
   __shared__ int sStop;
   __shared__ double sRunUntil;
-  const int sc = blockIdx.x;
   const int tid = threadIdx.x;
+  const int64_t vb = B.vert0[sc];
+  const int nv = (int)(B.vert0[sc + 1] - vb);
+  for (int i = tid; i < nv; i += NT) {
+    const vec4_t<T> p = B.P.x[vb + i];
+    pos[i] = p.x;
+    pos[nv + i] = p.y;
+    pos[2 * nv + i] = p.z;
+  }
+  auto ld_pos = [&](int v) -> vec4_t<T> {
+    const int l = v - (int)vb;
+    vec4_t<T> p;
+    p.x = pos[l];
+    p.y = pos[nv + l];
+    p.z = pos[2 * nv + l];
+    p.w = B.P.xold[v].w;
+    return p;
+  };
+  auto st_pos = [&](int v, const vec4_t<T> &p) {
+    const int l = v - (int)vb;
+    pos[l] = p.x;
+    pos[nv + l] = p.y;
+    pos[2 * nv + l] = p.z;
+  };
   Ctl c;
   c.stopSolve = 0;
   c.endLayer = 0;
@@ -1613,7 +1753,150 @@ __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
   while (it < B.hardCap) {
     for (int k = 0; k < nc; ++k) {
       const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
-      for (int64_t s = a + tid; s < e; s += NT) project_tet<T>(B.P, s);
+      for (int64_t s = a + tid; s < e; s += NT) {
+        TetIn<T> in;
+        tet_fetch<T>(B.P, s, in);
+        vec4_t<T> X[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int l = in.v[q] - (int)vb;
+          X[q].x = pos[l];
+          X[q].y = pos[nv + l];
+          X[q].z = pos[2 * nv + l];
+          X[q].w = in.O[q].w;
+        }
+        tet_solve_st<T>(B.P, s, in, X, [&](int q, const vec4_t<T> &o) { st_pos(in.v[q], o); });
+      }
+      __syncthreads();
+    }
+    for (int64_t i = k0 + tid; i < k1; i += NT) {
+      const int v = B.c.vert[i];
+      const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
+      vec4_t<T> p = ld_pos(v);
+      const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
+      double phi = n[0] * (xv[0] - pc[0]) + n[1] * (xv[1] - pc[1]) + n[2] * (xv[2] - pc[2]);
+      if (phi > 0) phi = 0;
+      const double C = B.contactResistance * phi;
+      const double aT = B.c.compliance[i] / h2;
+      const double mi = B.minv[v];
+      const double dl = (-C - aT * B.c.lambda[i]) / (2 * mi + aT);
+      p.x = (T)(xv[0] + mi * (dl * n[0]));
+      p.y = (T)(xv[1] + mi * (dl * n[1]));
+      p.z = (T)(xv[2] + mi * (dl * n[2]));
+      B.c.lambda[i] += dl;
+      st_pos(v, p);
+    }
+    __syncthreads();
+    for (int64_t q = q0 + tid; q < q1; q += NT) {
+      const int d = B.pinDof[q];
+      pos[(d % 3) * nv + (d / 3 - (int)vb)] = B.pinVal[q];
+    }
+    __syncthreads();
+    ++it;
+    if (B.mp.computeResiduals) {
+      double acc = 0;
+      for (int64_t s = s0 + tid; s < s1; s += NT)
+        if (B.P.vid[s] >= 0) {
+          ResIn<T> in;
+          res_fetch_pos<T>(B.P, s, in, ld_pos);
+          acc += res_compute<T>(in);
+        }
+      const double Ks = (double)(k1 - k0);
+      for (int64_t i = k0 + tid; i < k1; i += NT) {
+        const int v = B.c.vert[i];
+        const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
+        const vec4_t<T> p = ld_pos(v);
+        double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]) +
+                     n[2] * ((double)p.z - pc[2]);
+        if (phi > 0) phi = 0;
+        const double a = fabs(B.contactResistance * phi);
+        const double cr = fabs((B.contactCompliance / h2) * B.c.lambda[i]);
+        const double t = a - (a < cr ? a : cr);
+        acc += Ks * (t * t);
+      }
+      sh[tid] = acc;
+      __syncthreads();
+      for (int st = NT / 2; st > 0; st >>= 1) {
+        if (tid < st) sh[tid] += sh[tid + st];
+        __syncthreads();
+      }
+      if (tid == 0) {
+        monitor_update(c, B.mp, sqrt(sh[0]), nullptr, 0);
+        sStop = c.stopSolve | c.endLayer;
+        sRunUntil = c.runUntil;
+      }
+      __syncthreads();
+    }
+    const int stop = sStop;
+    const double ru = sRunUntil;
+    __syncthreads();
+    if (stop) break;
+    if (!(it < B.iterations || ru >= 0)) break;
+  }
+  for (int i = tid; i < nv; i += NT) {
+    vec4_t<T> p = B.P.x[vb + i];
+    p.x = pos[i];
+    p.y = pos[nv + i];
+    p.z = pos[2 * nv + i];
+    B.P.x[vb + i] = p;
+  }
+  if (tid == 0) {
+    B.iters[sc] = it;
+    B.lastResidual[sc] = c.lastResidual;
+    B.runUntilOut[sc] = c.runUntil;
+    B.residualCount[sc] = c.residualCount;
+  }
+}
+
+// One scene per workgroup (plain method: a single all-elastic layer): the
+// projectGS loop of xpbdLayer3D.m:315-341 with colors separated by workgroup
+// barriers, then contacts, pins and the scene's own residual/stop rule.
+// Every loop is bounded (hardCap) so the kernel always returns.  Scenes
+// with at most ldsVerts vertices run batch_scene_lds on the dynamic LDS.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
+  __shared__ double sh[NT];
+  __shared__ int sStop;
+  __shared__ double sRunUntil;
+  extern __shared__ __align__(16) unsigned char mlx_dyn_lds[];
+  const int sc = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (B.vert0[sc + 1] - B.vert0[sc] <= B.ldsVerts) {
+    batch_scene_lds<T, NT>(B, sc, reinterpret_cast<T *>(mlx_dyn_lds), sh);
+    return;
+  }
+  Ctl c;
+  c.stopSolve = 0;
+  c.endLayer = 0;
+  c.iters = 0;
+  c.errorChangeDecreased = 0;
+  c.residualCount = 1;
+  c.stopLayer = -1;
+  c.isLastLayer = 1;
+  c.layer = 0;
+  c.runUntil = B.runUntil0;
+  c.setPercentStop = B.stopPct;
+  c.normresv = 0;
+  c.lastResidual = 0;
+  if (tid == 0) {
+    sStop = 0;
+    sRunUntil = B.runUntil0;
+  }
+  __syncthreads();
+  const int nc = B.ncolor[sc], cb = B.color0[sc];
+  const int64_t k0 = B.cont0[sc], k1 = B.cont0[sc + 1];
+  const int64_t q0 = B.pin0[sc], q1 = B.pin0[sc + 1];
+  const int64_t s0 = B.slot0[sc], s1 = B.slot0[sc + 1];
+  const double h2 = B.h * B.h;
+  int it = 0;
+  while (it < B.hardCap) {
+    for (int k = 0; k < nc; ++k) {
+      const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
+      if constexpr (NT <= 256) {
+        color_slots<T, false>(B.P, a + tid, e, NT);
+      } else {
+        for (int64_t s = a + tid; s < e; s += NT) project_tet<T>(B.P, s);
+      }
       __syncthreads();
     }
     for (int64_t i = k0 + tid; i < k1; i += NT) {
@@ -1643,8 +1926,12 @@ __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
     ++it;
     if (B.mp.computeResiduals) {
       double acc = 0;
-      for (int64_t s = s0 + tid; s < s1; s += NT)
-        if (B.P.vid[s] >= 0) acc += residual_tet<T>(B.P, s);
+      if constexpr (NT <= 256) {
+        acc = residual_slots<T, false>(B.P, s0 + tid, s1, NT);
+      } else {
+        for (int64_t s = s0 + tid; s < s1; s += NT)
+          if (B.P.vid[s] >= 0) acc += residual_tet<T>(B.P, s);
+      }
       const double Ks = (double)(k1 - k0);
       for (int64_t i = k0 + tid; i < k1; i += NT) {
         const int v = B.c.vert[i];
@@ -1990,14 +2277,11 @@ __global__ __launch_bounds__(kBlock) void k_persist(PersistParams<T> Q) {
       const int cnt = Q.colorCnt[c];
       if (cnt == 0) continue;
       const int64_t off = Q.colorOff[c], end = off + cnt;
-      for (int64_t s = off + (int64_t)w * kBlock + t; s < end; s += stride) {
-        if constexpr (DIM == 3) {
-          TetIn<T> in;
-          tet_fetch<T>(Q.P, s, in);
-          tet_solve<T, LOCAL>(Q.P, s, in);
-        } else {
+      if constexpr (DIM == 3) {
+        color_slots<T, LOCAL>(Q.P, off + (int64_t)w * kBlock + t, end, stride);
+      } else {
+        for (int64_t s = off + (int64_t)w * kBlock + t; s < end; s += stride)
           project_tri<T>(Q.P, s);
-        }
       }
       if (!barrier()) return;
     }
@@ -2012,8 +2296,12 @@ __global__ __launch_bounds__(kBlock) void k_persist(PersistParams<T> Q) {
     if (!barrier()) return;
     if (Q.mp.computeResiduals) {
       double acc = 0;
-      for (int64_t s = (int64_t)w * kBlock + t; s < Q.P.ld; s += stride)
-        if (Q.P.vid[s] >= 0) acc += residual_elem<T, DIM, LOCAL>(Q.P, s, Q.lamSlot2d);
+      if constexpr (DIM == 3) {
+        acc = residual_slots<T, LOCAL>(Q.P, (int64_t)w * kBlock + t, Q.P.ld, stride);
+      } else {
+        for (int64_t s = (int64_t)w * kBlock + t; s < Q.P.ld; s += stride)
+          if (Q.P.vid[s] >= 0) acc += residual_elem<T, DIM, LOCAL>(Q.P, s, Q.lamSlot2d);
+      }
       for (int64_t i = (int64_t)w * kBlock + t; i < Q.ca.K; i += stride)
         acc += residual_contact_one<T, DIM, LOCAL>(Q.ca, (int)i, Q.P.x, Q.resistance, Q.h,
                                                    Q.uniq, Q.ucnt, Q.nUniq);
@@ -2043,32 +2331,40 @@ namespace mlx {
 // batched scenes split over several workgroups
 // ---------------------------------------------------------------------------
 
-// Work split of k_batch_split: workgroup w serves scene wgScene[w] as rank
-// wgRank[w] of sceneG[scene] workgroups (a scene's workgroups are
-// consecutive).  Scenes with more than one workgroup meet at a per-scene
-// grid barrier (grid_barrier on bar[scene]) after every color, after the
-// contacts and pins, after the residual partials and after the monitor.
+// Work split of k_batch_split.  The grid is 8 * P workgroups, dealt
+// round-robin over the 8 XCDs; a workgroup takes a rank k on its XCD's
+// counter and serves entry[xcd * P + k]: scene * 64 + rank of that scene's
+// G workgroups, or -1 (idle).  All workgroups of a scene are on one XCD,
+// so they share its L2 and meet at xcd_arrive/xcd_wait on bar[scene].
+// After the ranks are taken, one grid barrier lets every workgroup check
+// that each XCD received exactly P workgroups; otherwise all return at
+// once and *mismatch is set (nothing has been written).
 struct SplitParams {
-  const int32_t *wgScene, *wgRank, *sceneG;
+  const int32_t *entry;   // 8 * P entries
+  const int32_t *sceneG;  // workgroups per scene (1..64)
+  const int32_t *pbase;   // first partial of each scene
+  int P;                  // workgroups per XCD
+  unsigned *reg;          // 17 counters (ranks per XCD, grid barrier), zero at launch
   unsigned *bar;          // per scene arrival counter, zero at launch
   int32_t *sceneStop;     // per scene stop flag published by rank 0
   double *sceneRunUntil;  // per scene runUntil published by rank 0
-  double *partials;       // per workgroup residual partial
+  double *partials;       // per (scene, rank) residual partial
   int *err;               // barrier failure flag
+  int *mismatch;          // XCD placement differed from the plan
 };
 
-// The plain-method projectGS loop of k_batch_plain with a scene's elements
-// of each color, its residual and its pins/contacts split over the scene's
-// workgroups.  Rank 0 owns the scene's control block (monitor and stop
-// rule) and runs the contacts then the pins; all ranks stop together on the
-// flag it publishes.  One workgroup per scene reduces to workgroup barriers.
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_batch_split(BatchParams<T> B, SplitParams S) {
-  __shared__ double sh[NT];
+// One scene's plain-method layerSolve (the loop of k_batch_plain) on rank r
+// of its G workgroups: each color split over the ranks, the contacts
+// (distinct vertices) then the pins, the residual from all ranks, the
+// monitor on rank 0, which publishes the stop decision.  COH (G > 1): data
+// other ranks write is read with ldc<true> and the ranks meet at the
+// scene's XCD barrier; otherwise a workgroup barrier.
+template <typename T, int NT, bool COH>
+__device__ __forceinline__ void batch_scene(const BatchParams<T> &B, const SplitParams &S,
+                                            const int sc, const int r, const int G, double *sh) {
   __shared__ int sStop;
   __shared__ double sRunUntil;
-  const int w = blockIdx.x, tid = threadIdx.x;
-  const int sc = S.wgScene[w], r = S.wgRank[w], G = S.sceneG[sc];
+  const int tid = threadIdx.x;
   unsigned nbar = 0;
   Ctl c;
   c.stopSolve = 0;
@@ -2087,70 +2383,81 @@ __global__ __launch_bounds__(NT) void k_batch_split(BatchParams<T> B, SplitParam
   const int64_t k0 = B.cont0[sc], k1 = B.cont0[sc + 1];
   const int64_t q0 = B.pin0[sc], q1 = B.pin0[sc + 1];
   const int64_t s0 = B.slot0[sc], s1 = B.slot0[sc + 1];
-  const int64_t stride = (int64_t)G * NT;
+  const int64_t stride = (int64_t)G * NT, first = (int64_t)r * NT + tid;
+  double *part = S.partials + S.pbase[sc];
+  auto sync = [&]() -> bool {
+    if (!COH) {
+      __syncthreads();
+      return true;
+    }
+    ++nbar;
+    xcd_arrive(S.bar + sc);
+    return xcd_wait(S.bar + sc, nbar * (unsigned)G, S.err);
+  };
   const double h2 = B.h * B.h;
-#define MLX_SCENE_SYNC()                                                  \
-  do {                                                                    \
-    if (G == 1) {                                                         \
-      __syncthreads();                                                    \
-    } else if (!grid_barrier(S.bar + sc, ++nbar * (unsigned)G, S.err)) {  \
-      return;                                                             \
-    }                                                                     \
-  } while (0)
+  const double Ks = (double)(k1 - k0);
   int it = 0;
   double ru = B.runUntil0;
   while (it < B.hardCap) {
     for (int k = 0; k < nc; ++k) {
       const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
-      for (int64_t s = a + (int64_t)r * NT + tid; s < e; s += stride) project_tet<T>(B.P, s);
-      MLX_SCENE_SYNC();
-    }
-    if (r == 0) {
-      for (int64_t i = k0 + tid; i < k1; i += NT) {
-        const int v = B.c.vert[i];
-        const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
-        vec4_t<T> p = B.P.x[v];
-        const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
-        double phi = n[0] * (xv[0] - pc[0]) + n[1] * (xv[1] - pc[1]) + n[2] * (xv[2] - pc[2]);
-        if (phi > 0) phi = 0;
-        const double C = B.contactResistance * phi;
-        const double aT = B.c.compliance[i] / h2;
-        const double mi = B.minv[v];
-        const double dl = (-C - aT * B.c.lambda[i]) / (2 * mi + aT);
-        p.x = (T)(xv[0] + mi * (dl * n[0]));
-        p.y = (T)(xv[1] + mi * (dl * n[1]));
-        p.z = (T)(xv[2] + mi * (dl * n[2]));
-        B.c.lambda[i] += dl;
-        B.P.x[v] = p;
+      if constexpr (NT <= 256) {
+        color_slots<T, COH>(B.P, a + first, e, stride);
+      } else {
+        for (int64_t s = a + first; s < e; s += stride) {
+          TetIn<T> in;
+          tet_fetch<T>(B.P, s, in);
+          tet_solve<T, COH>(B.P, s, in);
+        }
       }
-      __syncthreads();
-      for (int64_t q = q0 + tid; q < q1; q += NT) {
-        const int d = B.pinDof[q];
-        T *px = reinterpret_cast<T *>(&B.P.x[d / 3]);
-        px[d % 3] = B.pinVal[q];
-      }
+      if (!sync()) return;
     }
-    MLX_SCENE_SYNC();
+    for (int64_t i = k0 + first; i < k1; i += stride) {
+      const int v = B.c.vert[i];
+      const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
+      vec4_t<T> p = ldc<COH>(B.P.x + v);
+      const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
+      double phi = n[0] * (xv[0] - pc[0]) + n[1] * (xv[1] - pc[1]) + n[2] * (xv[2] - pc[2]);
+      if (phi > 0) phi = 0;
+      const double C = B.contactResistance * phi;
+      const double aT = B.c.compliance[i] / h2;
+      const double mi = B.minv[v];
+      const double li = B.c.lambda[i];  // written only by this thread
+      const double dl = (-C - aT * li) / (2 * mi + aT);
+      p.x = (T)(xv[0] + mi * (dl * n[0]));
+      p.y = (T)(xv[1] + mi * (dl * n[1]));
+      p.z = (T)(xv[2] + mi * (dl * n[2]));
+      B.c.lambda[i] = li + dl;
+      B.P.x[v] = p;
+    }
+    if (k1 > k0 && q1 > q0 && !sync()) return;
+    for (int64_t q = q0 + first; q < q1; q += stride) {
+      const int d = B.pinDof[q];
+      T *px = reinterpret_cast<T *>(&B.P.x[d / 3]);
+      px[d % 3] = B.pinVal[q];
+    }
+    if (!sync()) return;
     ++it;
     int stop = 0;
     if (B.mp.computeResiduals) {
       double acc = 0;
-      for (int64_t s = s0 + (int64_t)r * NT + tid; s < s1; s += stride)
-        if (B.P.vid[s] >= 0) acc += residual_tet<T>(B.P, s);
-      if (r == 0) {
-        const double Ks = (double)(k1 - k0);
-        for (int64_t i = k0 + tid; i < k1; i += NT) {
-          const int v = B.c.vert[i];
-          const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
-          const vec4_t<T> p = B.P.x[v];
-          double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]) +
-                       n[2] * ((double)p.z - pc[2]);
-          if (phi > 0) phi = 0;
-          const double a = fabs(B.contactResistance * phi);
-          const double cr = fabs((B.contactCompliance / h2) * B.c.lambda[i]);
-          const double t = a - (a < cr ? a : cr);
-          acc += Ks * (t * t);
-        }
+      if constexpr (NT <= 256) {
+        acc = residual_slots<T, COH>(B.P, s0 + first, s1, stride);
+      } else {
+        for (int64_t s = s0 + first; s < s1; s += stride)
+          if (B.P.vid[s] >= 0) acc += residual_tet<T, COH>(B.P, s);
+      }
+      for (int64_t i = k0 + first; i < k1; i += stride) {
+        const int v = B.c.vert[i];
+        const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
+        const vec4_t<T> p = ldc<COH>(B.P.x + v);
+        double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]) +
+                     n[2] * ((double)p.z - pc[2]);
+        if (phi > 0) phi = 0;
+        const double a = fabs(B.contactResistance * phi);
+        const double cr = fabs((B.contactCompliance / h2) * B.c.lambda[i]);
+        const double t = a - (a < cr ? a : cr);
+        acc += Ks * (t * t);
       }
       sh[tid] = acc;
       __syncthreads();
@@ -2158,19 +2465,19 @@ __global__ __launch_bounds__(NT) void k_batch_split(BatchParams<T> B, SplitParam
         if (tid < st) sh[tid] += sh[tid + st];
         __syncthreads();
       }
-      if (tid == 0) S.partials[w] = sh[0];
-      MLX_SCENE_SYNC();
+      if (tid == 0) part[r] = sh[0];
+      if (!sync()) return;
       if (r == 0 && tid == 0) {
         double sum = 0;
-        for (int q = 0; q < G; ++q) sum += S.partials[w + q];
+        for (int q = 0; q < G; ++q) sum += ldc<COH>(part + q);
         monitor_update(c, B.mp, sqrt(sum), nullptr, 0);
         S.sceneStop[sc] = c.stopSolve | c.endLayer;
         S.sceneRunUntil[sc] = c.runUntil;
       }
-      MLX_SCENE_SYNC();
+      if (!sync()) return;
       if (tid == 0) {
-        sStop = S.sceneStop[sc];
-        sRunUntil = S.sceneRunUntil[sc];
+        sStop = ldc<COH>(S.sceneStop + sc);
+        sRunUntil = ldc<COH>(S.sceneRunUntil + sc);
       }
       __syncthreads();
       stop = sStop;
@@ -2180,13 +2487,41 @@ __global__ __launch_bounds__(NT) void k_batch_split(BatchParams<T> B, SplitParam
     if (stop) break;
     if (!(it < B.iterations || ru >= 0)) break;
   }
-#undef MLX_SCENE_SYNC
   if (r == 0 && tid == 0) {
     B.iters[sc] = it;
     B.lastResidual[sc] = c.lastResidual;
     B.runUntilOut[sc] = c.runUntil;
     B.residualCount[sc] = c.residualCount;
   }
+}
+
+// Scenes of uneven size split over workgroups of one XCD each (SplitParams).
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_batch_split(BatchParams<T> B, SplitParams S) {
+  __shared__ double sh[NT];
+  __shared__ int sEntry, sOk;
+  if (threadIdx.x == 0) {
+    const unsigned x = xcd_id();
+    const unsigned k =
+        __hip_atomic_fetch_add(&S.reg[x], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sEntry = (x < 8u && k < (unsigned)S.P) ? S.entry[x * S.P + k] : -1;
+  }
+  if (!grid_barrier(S.reg + 16, gridDim.x, S.err)) return;
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    for (int x = 0; x < 8; ++x)
+      ok &= __hip_atomic_load(&S.reg[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            (unsigned)S.P;
+    sOk = ok;
+    if (!ok && blockIdx.x == 0) *S.mismatch = 1;
+  }
+  __syncthreads();
+  if (!sOk || sEntry < 0) return;
+  const int sc = sEntry >> 6, r = sEntry & 63, G = S.sceneG[sc];
+  if (G > 1)
+    batch_scene<T, NT, true>(B, S, sc, r, G, sh);
+  else
+    batch_scene<T, NT, false>(B, S, sc, 0, 1, sh);
 }
 
 }  // namespace mlx

@@ -1915,6 +1915,9 @@ struct mlx_batch {
   DevBuf d_split;    // split schedule, per-scene barriers and flags, partials
   int split = 0;     // target tets per workgroup (0: one workgroup per scene)
   int splitWGs = 0;  // workgroups of the last split launch
+  int64_t splitFallbacks = 0;  // split launches redone unsplit (XCD placement differed)
+  int useLds = 1;              // scenes that fit keep their positions in LDS (k_batch_plain)
+  int ldsVerts = 0;            // vertex limit of the LDS path in the last launch
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -1943,8 +1946,38 @@ __global__ void k_batch_prologue(int64_t nV, const T *v, vec4_t<T> *x, const dou
 }
 
 template <typename T, int NT>
-static void launch_batch(const BatchParams<T> &B, int n, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_plain<T, NT>), dim3(n), dim3(NT), 0, s, B);
+static void launch_batch(const BatchParams<T> &B, int n, size_t ldsBytes, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch_plain<T, NT>), dim3(n), dim3(NT), ldsBytes, s, B);
+}
+
+// Scenes with at most this many vertices keep their positions in LDS
+// (3 values of T per vertex), within the device's per-workgroup LDS less
+// the kernel's static LDS; 0 disables the LDS path.
+template <typename T, int NT>
+static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
+  *bytes = 0;
+  if (!enabled) return 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+  hipFuncAttributes fa;
+  if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_batch_plain<T, NT>)) !=
+      hipSuccess)
+    return 0;
+  if (prop.sharedMemPerBlock <= fa.sharedSizeBytes + 1024) return 0;
+  const size_t avail = prop.sharedMemPerBlock - fa.sharedSizeBytes - 256;
+  const int64_t fit = (int64_t)(avail / (3 * sizeof(T)));
+  const int64_t nv = std::min<int64_t>(fit, maxSceneVerts);
+  if (nv <= 0) return 0;
+  *bytes = (size_t)nv * 3 * sizeof(T);
+  if (*bytes > 65536 &&
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_plain<T, NT>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)*bytes) != hipSuccess) {
+    *bytes = 0;
+    return 0;
+  }
+  return (int)nv;
 }
 
 template <typename T>
@@ -2017,12 +2050,13 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
                        const_cast<double *>(ca.compliance));
     MLX_CHECK_HIP(hipMemsetAsync(ca.lambda, 0, sizeof(double) * K, s));
   }
-  // per-scene tables: color0 | ncolor | colorOff | colorCnt | slot0 | cont0 | pin0
+  // per-scene tables: color0 | ncolor | colorOff | colorCnt | slot0 | cont0 | pin0 | vert0
   const int ns = b.nScenes, ncT = (int)m.colorCnt.size();
   const size_t o_color0 = 0, o_ncolor = o_color0 + 4 * (size_t)ns,
                o_coff = (o_ncolor + 4 * (size_t)ns + 7) / 8 * 8, o_ccnt = o_coff + 8 * (size_t)ncT,
                o_slot = o_ccnt + 8 * (size_t)ncT, o_cont = o_slot + 8 * (size_t)(ns + 1),
-               o_pin = o_cont + 8 * (size_t)(ns + 1), o_end = o_pin + 8 * (size_t)(ns + 1);
+               o_pin = o_cont + 8 * (size_t)(ns + 1), o_vert = o_pin + 8 * (size_t)(ns + 1),
+               o_end = o_vert + 8 * (size_t)(ns + 1);
   std::vector<char> tab(o_end);
   std::memcpy(tab.data() + o_color0, m.sceneColor0.data(), 4 * (size_t)ns);
   std::memcpy(tab.data() + o_ncolor, m.sceneNColors.data(), 4 * (size_t)ns);
@@ -2031,6 +2065,7 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   std::memcpy(tab.data() + o_slot, m.sceneSlot0.data(), 8 * (size_t)(ns + 1));
   std::memcpy(tab.data() + o_cont, b.cont0.data(), 8 * (size_t)(ns + 1));
   std::memcpy(tab.data() + o_pin, b.pin0.data(), 8 * (size_t)(ns + 1));
+  std::memcpy(tab.data() + o_vert, b.vertOff.data(), 8 * (size_t)(ns + 1));
   MLX_TRY(b.d_tables.reserve(o_end));
   MLX_CHECK_HIP(hipMemcpy(b.d_tables.ptr, tab.data(), o_end, hipMemcpyHostToDevice));
   // shift, prologue, lambda = 0
@@ -2052,6 +2087,15 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   B.slot0 = reinterpret_cast<const int64_t *>(tb + o_slot);
   B.cont0 = reinterpret_cast<const int64_t *>(tb + o_cont);
   B.pin0 = reinterpret_cast<const int64_t *>(tb + o_pin);
+  B.vert0 = reinterpret_cast<const int64_t *>(tb + o_vert);
+  int64_t maxSceneVerts = 0;
+  for (int sc = 0; sc < ns; ++sc)
+    maxSceneVerts = std::max<int64_t>(maxSceneVerts, b.vertOff[sc + 1] - b.vertOff[sc]);
+  size_t ldsBytes = 0;
+  B.ldsVerts = b.block == 256    ? batch_lds_verts<T, 256>(maxSceneVerts, b.useLds, &ldsBytes)
+               : b.block == 1024 ? batch_lds_verts<T, 1024>(maxSceneVerts, b.useLds, &ldsBytes)
+                                 : batch_lds_verts<T, 512>(maxSceneVerts, b.useLds, &ldsBytes);
+  b.ldsVerts = B.ldsVerts;
   B.pinDof = m.d_pinned.as<int32_t>();
   B.pinVal = m.d_pinnedx.as<T>();
   B.c = ca;
@@ -2080,10 +2124,12 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     MLX_CHECK_HIP(hipEventCreate(&b.ev0));
     MLX_CHECK_HIP(hipEventCreate(&b.ev1));
   }
-  int *splitErr = nullptr;
+  int *splitErr = nullptr, *splitMismatch = nullptr;
   if (b.split > 0) {
-    // split schedule: scene sc gets G = round(nT_sc / split) workgroups (at
-    // least 1), scaled down until every workgroup of the grid is resident
+    // split schedule: scene sc gets G = round(nT_sc / split) workgroups (1..64)
+    // on one XCD; scenes go to XCDs largest first, each to the XCD with the
+    // fewest tets so far; P = the largest XCD's workgroup count, and the grid
+    // of 8 P workgroups must be resident (else the split is coarsened)
     const void *fn = b.block == 256 ? reinterpret_cast<const void *>(&k_batch_split<T, 256>)
                      : b.block == 1024 ? reinterpret_cast<const void *>(&k_batch_split<T, 1024>)
                                        : reinterpret_cast<const void *>(&k_batch_split<T, 512>);
@@ -2093,64 +2139,94 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     hipDeviceProp_t prop;
     MLX_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
     const int64_t cap = (int64_t)perCU * prop.multiProcessorCount;
-    if (cap < ns) {
-      set_error("batched split: %d scenes do not fit %lld resident workgroups", ns, (long long)cap);
-      return MLX_ERR_UNSUPPORTED;
-    }
+    std::vector<int> order(ns);
+    for (int sc = 0; sc < ns; ++sc) order[sc] = sc;
+    std::stable_sort(order.begin(), order.end(), [&](int p_, int q_) {
+      return b.elemOff[p_ + 1] - b.elemOff[p_] > b.elemOff[q_ + 1] - b.elemOff[q_];
+    });
     double per = (double)b.split;
-    std::vector<int32_t> G(ns);
-    int64_t total = 0;
+    std::vector<int32_t> G(ns), xcdOf(ns);
+    int64_t P = 0;
     for (int guard = 0; guard < 64; ++guard) {
-      total = 0;
-      for (int sc = 0; sc < ns; ++sc) {
+      int64_t tets[kXcds] = {0}, wgs[kXcds] = {0};
+      for (int sc : order) {
         const double nt = (double)(b.elemOff[sc + 1] - b.elemOff[sc]);
         G[sc] = (int32_t)std::max(1.0, std::min(64.0, std::floor(nt / per + 0.5)));
-        total += G[sc];
+        int best = 0;
+        for (int x = 1; x < kXcds; ++x)
+          if (tets[x] < tets[best]) best = x;
+        xcdOf[sc] = best;
+        tets[best] += (int64_t)nt;
+        wgs[best] += G[sc];
       }
-      if (total <= cap) break;
+      P = *std::max_element(wgs, wgs + kXcds);
+      if (kXcds * P <= cap) break;
       per *= 1.25;
     }
-    if (total > cap) {
+    if (kXcds * P > cap || P < 1) {
       set_error("batched split: no schedule fits %lld resident workgroups", (long long)cap);
       return MLX_ERR_UNSUPPORTED;
     }
-    std::vector<int32_t> wgScene(total), wgRank(total);
-    int64_t w = 0;
-    for (int sc = 0; sc < ns; ++sc)
-      for (int q = 0; q < G[sc]; ++q, ++w) {
-        wgScene[w] = sc;
-        wgRank[w] = q;
-      }
-    const size_t o_scene = 0, o_rank = o_scene + 4 * total, o_G = o_rank + 4 * total,
-                 o_bar = (o_G + 4 * (size_t)ns + 255) / 256 * 256, o_stop = o_bar + 4 * (size_t)ns,
-                 o_err = o_stop + 4 * (size_t)ns, o_ru = (o_err + 4 + 7) / 8 * 8,
-                 o_part = o_ru + 8 * (size_t)ns, o_end = o_part + 8 * total;
+    std::vector<int32_t> entry((size_t)kXcds * P, -1), pbase(ns);
+    std::vector<int64_t> fill(kXcds, 0);
+    int64_t np = 0;
+    for (int sc = 0; sc < ns; ++sc) {
+      pbase[sc] = (int32_t)np;
+      np += G[sc];
+      const int x = xcdOf[sc];
+      for (int q = 0; q < G[sc]; ++q) entry[(size_t)x * P + fill[x]++] = sc * 64 + q;
+    }
+    const size_t nE = entry.size();
+    const size_t o_entry = 0, o_G = o_entry + 4 * nE, o_pb = o_G + 4 * (size_t)ns,
+                 o_zero = (o_pb + 4 * (size_t)ns + 255) / 256 * 256, o_reg = o_zero,
+                 o_bar = o_reg + 4 * 32, o_stop = o_bar + 4 * (size_t)ns,
+                 o_err = o_stop + 4 * (size_t)ns, o_mis = o_err + 4,
+                 o_zend = (o_mis + 4 + 7) / 8 * 8, o_ru = o_zend, o_part = o_ru + 8 * (size_t)ns,
+                 o_end = o_part + 8 * (size_t)np;
     MLX_TRY(b.d_split.reserve(o_end));
     char *sb = b.d_split.as<char>();
-    MLX_CHECK_HIP(hipMemcpy(sb + o_scene, wgScene.data(), 4 * total, hipMemcpyHostToDevice));
-    MLX_CHECK_HIP(hipMemcpy(sb + o_rank, wgRank.data(), 4 * total, hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemcpy(sb + o_entry, entry.data(), 4 * nE, hipMemcpyHostToDevice));
     MLX_CHECK_HIP(hipMemcpy(sb + o_G, G.data(), 4 * (size_t)ns, hipMemcpyHostToDevice));
-    MLX_CHECK_HIP(hipMemsetAsync(sb + o_bar, 0, o_ru - o_bar, s));
+    MLX_CHECK_HIP(hipMemcpy(sb + o_pb, pbase.data(), 4 * (size_t)ns, hipMemcpyHostToDevice));
+    MLX_CHECK_HIP(hipMemsetAsync(sb + o_zero, 0, o_zend - o_zero, s));
     SplitParams SP;
-    SP.wgScene = reinterpret_cast<const int32_t *>(sb + o_scene);
-    SP.wgRank = reinterpret_cast<const int32_t *>(sb + o_rank);
+    SP.entry = reinterpret_cast<const int32_t *>(sb + o_entry);
     SP.sceneG = reinterpret_cast<const int32_t *>(sb + o_G);
+    SP.pbase = reinterpret_cast<const int32_t *>(sb + o_pb);
+    SP.P = (int)P;
+    SP.reg = reinterpret_cast<unsigned *>(sb + o_reg);
     SP.bar = reinterpret_cast<unsigned *>(sb + o_bar);
     SP.sceneStop = reinterpret_cast<int32_t *>(sb + o_stop);
     SP.err = reinterpret_cast<int *>(sb + o_err);
+    SP.mismatch = reinterpret_cast<int *>(sb + o_mis);
     SP.sceneRunUntil = reinterpret_cast<double *>(sb + o_ru);
     SP.partials = reinterpret_cast<double *>(sb + o_part);
     splitErr = SP.err;
-    b.splitWGs = (int)total;
+    splitMismatch = SP.mismatch;
+    b.splitWGs = (int)(kXcds * P);
     void *args[] = {&B, &SP};
     MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
-    MLX_CHECK_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)total), dim3(b.block), args, 0, s));
+    MLX_CHECK_HIP(
+        hipLaunchCooperativeKernel(fn, dim3((unsigned)(kXcds * P)), dim3(b.block), args, 0, s));
     MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
+    // workgroups dealt other than round-robin over the XCDs: nothing was
+    // written; run the substep with one workgroup per scene instead
+    int mis = 0;
+    MLX_CHECK_HIP(hipMemcpyAsync(&mis, splitMismatch, 4, hipMemcpyDeviceToHost, s));
+    MLX_CHECK_HIP(hipStreamSynchronize(s));
+    if (mis) {
+      ++b.splitFallbacks;
+      MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
+      if (b.block == 256) launch_batch<T, 256>(B, ns, ldsBytes, s);
+      else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
+      else launch_batch<T, 512>(B, ns, ldsBytes, s);
+      MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
+    }
   } else {
     MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
-    if (b.block == 256) launch_batch<T, 256>(B, ns, s);
-    else if (b.block == 1024) launch_batch<T, 1024>(B, ns, s);
-    else launch_batch<T, 512>(B, ns, s);
+    if (b.block == 256) launch_batch<T, 256>(B, ns, ldsBytes, s);
+    else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
+    else launch_batch<T, 512>(B, ns, ldsBytes, s);
     MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
   }
   hipLaunchKernelGGL((k_velocity<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x, xo,
@@ -2263,6 +2339,15 @@ int mlx_batch_destroy(mlx_batch *b) {
   b->d_split.release();
   mlx_mesh_destroy(b->m);
   delete b;
+  return MLX_OK;
+}
+
+int mlx_batch_set_lds(mlx_batch *b, int32_t enable) {
+  if (!b) {
+    set_error("mlx_batch_set_lds: null batch");
+    return MLX_ERR_ARG;
+  }
+  b->useLds = enable != 0;
   return MLX_OK;
 }
 

@@ -27,12 +27,15 @@ def _integ(iters=12, pct=0.0):
     return it
 
 
-@pytest.mark.parametrize("pct,block,split", [(0.0, 256, 0), (99.99, 1024, 0), (0.0, 256, 40),
-                                             (99.99, 512, 30)])
-def test_batch_fp64_matches_oracle_per_scene(pct, block, split, oracle_lib):
+@pytest.mark.parametrize("pct,block,split,lds", [(0.0, 256, 0, 1), (99.99, 1024, 0, 1),
+                                                 (99.99, 512, 0, 1), (0.0, 256, 0, 0),
+                                                 (99.99, 1024, 0, 0), (0.0, 256, 40, 1),
+                                                 (99.99, 512, 30, 1)])
+def test_batch_fp64_matches_oracle_per_scene(pct, block, split, lds, oracle_lib):
     """Each scene of the batch equals the oracle run alone, with one
-    workgroup per scene and with scenes split over several workgroups
-    (split = target tets per workgroup: the 80..240-tet scenes get 2..8)."""
+    workgroup per scene (positions in LDS or in HBM) and with scenes split
+    over several workgroups (split = target tets per workgroup: the
+    80..240-tet scenes get 2..8)."""
     scs = [scenes.beam3d(4, 2, 2, seed=1, z0=0.01), scenes.beam3d(6, 3, 2, seed=2, z0=0.02),
            scenes.beam3d(3, 3, 3, seed=3, z0=-0.01), scenes.beam3d(8, 2, 3, seed=4, z0=0.03)]
     meshes = [Mesh(s) for s in scs]
@@ -40,6 +43,7 @@ def test_batch_fp64_matches_oracle_per_scene(pct, block, split, oracle_lib):
     plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
     it = _integ(pct=pct)
     b = BatchSimulation(merged, it, 1 / 60, plane, precision=L.MLX_FP64, block=block, split=split)
+    b.set_lds(lds)
     osims = [O.OracleSim(m, copy.deepcopy(it), 1 / 60, plane) for m in meshes]
     try:
         for k in range(5):
@@ -96,3 +100,33 @@ def test_batch_split_matches_one_workgroup_per_scene(precision):
         a.close()
         b.close()
 
+
+
+@pytest.mark.parametrize("precision,block", [(L.MLX_FP64, 512), (L.MLX_FP32, 512),
+                                             (L.MLX_FP32, 1024), (L.MLX_FP32, 256)])
+def test_batch_lds_positions_equal_hbm_positions(precision, block):
+    """Scenes whose positions live in LDS for the whole solve run the same
+    arithmetic in the same order as with positions in HBM: bit-identical
+    positions, velocities, lambdas and iteration counts, on uneven scenes
+    (600..9000 tets) with ground contacts and the relative stop rule."""
+    scs = random_beams(10, seed=5, tets=(600, 9000))
+    merged = merge_meshes([Mesh(sc) for sc in scs])
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    it = _integ(iters=20, pct=99.99)
+    it.hangingStop = 200
+    a = BatchSimulation(merged, it, 1 / 60, plane, precision=precision, block=block)
+    b = BatchSimulation(merged, copy.deepcopy(it), 1 / 60, plane, precision=precision, block=block)
+    a.set_lds(0)
+    b.set_lds(1)
+    try:
+        for k in range(8):
+            a.step()
+            b.step()
+            pa, va, la = a.download()
+            pb, vb, lb = b.download()
+            print("lds", k, list(a.iters), list(b.iters), rel(pb, pa), flush=True)
+            assert np.array_equal(a.iters, b.iters) and np.array_equal(a.contacts, b.contacts)
+            assert np.array_equal(pa, pb) and np.array_equal(va, vb) and np.array_equal(la, lb)
+    finally:
+        a.close()
+        b.close()
