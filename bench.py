@@ -113,7 +113,7 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
     frame0 = b.frame
     tune = {}
     for blk in (256, 512, 1024):
-        for split in (0, 1500, 3000, 6000):
+        for split in (0, 8000, 25000):
             b.set_block(blk)
             b.set_split(split)
             b.set_state(p0, v0)

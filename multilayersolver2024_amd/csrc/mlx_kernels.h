@@ -1692,6 +1692,23 @@ struct BatchParams {
   int32_t *residualCount;
 };
 
+// Workgroup sum of one double per thread in a fixed order: a butterfly
+// within each wavefront (lane 0's value), then the wavefront sums in
+// wavefront order.  Needs NT / 64 doubles of LDS; the result is valid in
+// every thread.
+template <int NT>
+__device__ __forceinline__ double wg_sum(double v, double *shw) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) shw[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) t += shw[w];
+  __syncthreads();
+  return t;
+}
+
 // k_batch_plain for a scene whose positions fit in LDS: x, y, z of its nv
 // vertices live in LDS (structure of arrays, pos[d * nv + local vertex])
 // from the first color to the last residual, and go back to HBM at the end.
@@ -1814,14 +1831,9 @@ __device__ __forceinline__ void batch_scene_lds(const BatchParams<T> &B, const i
         const double t = a - (a < cr ? a : cr);
         acc += Ks * (t * t);
       }
-      sh[tid] = acc;
-      __syncthreads();
-      for (int st = NT / 2; st > 0; st >>= 1) {
-        if (tid < st) sh[tid] += sh[tid + st];
-        __syncthreads();
-      }
+      const double tot = wg_sum<NT>(acc, sh);
       if (tid == 0) {
-        monitor_update(c, B.mp, sqrt(sh[0]), nullptr, 0);
+        monitor_update(c, B.mp, sqrt(tot), nullptr, 0);
         sStop = c.stopSolve | c.endLayer;
         sRunUntil = c.runUntil;
       }
@@ -1855,7 +1867,7 @@ __device__ __forceinline__ void batch_scene_lds(const BatchParams<T> &B, const i
 // with at most ldsVerts vertices run batch_scene_lds on the dynamic LDS.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
-  __shared__ double sh[NT];
+  __shared__ double sh[NT / 64];
   __shared__ int sStop;
   __shared__ double sRunUntil;
   extern __shared__ __align__(16) unsigned char mlx_dyn_lds[];
@@ -1892,11 +1904,7 @@ __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
   while (it < B.hardCap) {
     for (int k = 0; k < nc; ++k) {
       const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
-      if constexpr (NT <= 256) {
-        color_slots<T, false>(B.P, a + tid, e, NT);
-      } else {
-        for (int64_t s = a + tid; s < e; s += NT) project_tet<T>(B.P, s);
-      }
+      for (int64_t s = a + tid; s < e; s += NT) project_tet<T>(B.P, s);
       __syncthreads();
     }
     for (int64_t i = k0 + tid; i < k1; i += NT) {
@@ -1926,12 +1934,8 @@ __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
     ++it;
     if (B.mp.computeResiduals) {
       double acc = 0;
-      if constexpr (NT <= 256) {
-        acc = residual_slots<T, false>(B.P, s0 + tid, s1, NT);
-      } else {
-        for (int64_t s = s0 + tid; s < s1; s += NT)
-          if (B.P.vid[s] >= 0) acc += residual_tet<T>(B.P, s);
-      }
+      for (int64_t s = s0 + tid; s < s1; s += NT)
+        if (B.P.vid[s] >= 0) acc += residual_tet<T>(B.P, s);
       const double Ks = (double)(k1 - k0);
       for (int64_t i = k0 + tid; i < k1; i += NT) {
         const int v = B.c.vert[i];
@@ -1945,17 +1949,9 @@ __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
         const double t = a - (a < cr ? a : cr);
         acc += Ks * (t * t);
       }
-      double v1[1] = {acc}, o[1];
-      // block_sum with NT threads
-      sh[tid] = v1[0];
-      __syncthreads();
-      for (int st = NT / 2; st > 0; st >>= 1) {
-        if (tid < st) sh[tid] += sh[tid + st];
-        __syncthreads();
-      }
-      o[0] = sh[0];
+      const double tot = wg_sum<NT>(acc, sh);
       if (tid == 0) {
-        monitor_update(c, B.mp, sqrt(o[0]), nullptr, 0);
+        monitor_update(c, B.mp, sqrt(tot), nullptr, 0);
         sStop = c.stopSolve | c.endLayer;
         sRunUntil = c.runUntil;
       }
@@ -2401,14 +2397,10 @@ __device__ __forceinline__ void batch_scene(const BatchParams<T> &B, const Split
   while (it < B.hardCap) {
     for (int k = 0; k < nc; ++k) {
       const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
-      if constexpr (NT <= 256) {
-        color_slots<T, COH>(B.P, a + first, e, stride);
-      } else {
-        for (int64_t s = a + first; s < e; s += stride) {
-          TetIn<T> in;
-          tet_fetch<T>(B.P, s, in);
-          tet_solve<T, COH>(B.P, s, in);
-        }
+      for (int64_t s = a + first; s < e; s += stride) {
+        TetIn<T> in;
+        tet_fetch<T>(B.P, s, in);
+        tet_solve<T, COH>(B.P, s, in);
       }
       if (!sync()) return;
     }
@@ -2441,12 +2433,8 @@ __device__ __forceinline__ void batch_scene(const BatchParams<T> &B, const Split
     int stop = 0;
     if (B.mp.computeResiduals) {
       double acc = 0;
-      if constexpr (NT <= 256) {
-        acc = residual_slots<T, COH>(B.P, s0 + first, s1, stride);
-      } else {
-        for (int64_t s = s0 + first; s < s1; s += stride)
-          if (B.P.vid[s] >= 0) acc += residual_tet<T, COH>(B.P, s);
-      }
+      for (int64_t s = s0 + first; s < s1; s += stride)
+        if (B.P.vid[s] >= 0) acc += residual_tet<T, COH>(B.P, s);
       for (int64_t i = k0 + first; i < k1; i += stride) {
         const int v = B.c.vert[i];
         const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
@@ -2459,13 +2447,8 @@ __device__ __forceinline__ void batch_scene(const BatchParams<T> &B, const Split
         const double t = a - (a < cr ? a : cr);
         acc += Ks * (t * t);
       }
-      sh[tid] = acc;
-      __syncthreads();
-      for (int st = NT / 2; st > 0; st >>= 1) {
-        if (tid < st) sh[tid] += sh[tid + st];
-        __syncthreads();
-      }
-      if (tid == 0) part[r] = sh[0];
+      const double tot = wg_sum<NT>(acc, sh);
+      if (tid == 0) part[r] = tot;
       if (!sync()) return;
       if (r == 0 && tid == 0) {
         double sum = 0;
@@ -2498,8 +2481,9 @@ __device__ __forceinline__ void batch_scene(const BatchParams<T> &B, const Split
 // Scenes of uneven size split over workgroups of one XCD each (SplitParams).
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_batch_split(BatchParams<T> B, SplitParams S) {
-  __shared__ double sh[NT];
+  __shared__ double sh[NT / 64];
   __shared__ int sEntry, sOk;
+  extern __shared__ __align__(16) unsigned char mlx_dyn_lds[];
   if (threadIdx.x == 0) {
     const unsigned x = xcd_id();
     const unsigned k =
@@ -2520,6 +2504,8 @@ __global__ __launch_bounds__(NT) void k_batch_split(BatchParams<T> B, SplitParam
   const int sc = sEntry >> 6, r = sEntry & 63, G = S.sceneG[sc];
   if (G > 1)
     batch_scene<T, NT, true>(B, S, sc, r, G, sh);
+  else if (B.vert0[sc + 1] - B.vert0[sc] <= B.ldsVerts)
+    batch_scene_lds<T, NT>(B, sc, reinterpret_cast<T *>(mlx_dyn_lds), sh);
   else
     batch_scene<T, NT, false>(B, S, sc, 0, 1, sh);
 }

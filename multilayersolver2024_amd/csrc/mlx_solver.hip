@@ -1961,22 +1961,27 @@ static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-  hipFuncAttributes fa;
-  if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_batch_plain<T, NT>)) !=
-      hipSuccess)
-    return 0;
-  if (prop.sharedMemPerBlock <= fa.sharedSizeBytes + 1024) return 0;
-  const size_t avail = prop.sharedMemPerBlock - fa.sharedSizeBytes - 256;
+  const void *fns[2] = {reinterpret_cast<const void *>(&k_batch_plain<T, NT>),
+                        reinterpret_cast<const void *>(&k_batch_split<T, NT>)};
+  size_t stat = 0;
+  for (const void *fn : fns) {
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, fn) != hipSuccess) return 0;
+    stat = std::max(stat, fa.sharedSizeBytes);
+  }
+  if (prop.sharedMemPerBlock <= stat + 1024) return 0;
+  const size_t avail = prop.sharedMemPerBlock - stat - 256;
   const int64_t fit = (int64_t)(avail / (3 * sizeof(T)));
   const int64_t nv = std::min<int64_t>(fit, maxSceneVerts);
   if (nv <= 0) return 0;
   *bytes = (size_t)nv * 3 * sizeof(T);
-  if (*bytes > 65536 &&
-      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_plain<T, NT>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)*bytes) != hipSuccess) {
-    *bytes = 0;
-    return 0;
-  }
+  if (*bytes > 65536)
+    for (const void *fn : fns)
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)*bytes) !=
+          hipSuccess) {
+        *bytes = 0;
+        return 0;
+      }
   return (int)nv;
 }
 
@@ -2133,38 +2138,53 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     const void *fn = b.block == 256 ? reinterpret_cast<const void *>(&k_batch_split<T, 256>)
                      : b.block == 1024 ? reinterpret_cast<const void *>(&k_batch_split<T, 1024>)
                                        : reinterpret_cast<const void *>(&k_batch_split<T, 512>);
-    int perCU = 0, dev = 0;
-    MLX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, b.block, 0));
+    int dev = 0;
     MLX_CHECK_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     MLX_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
-    const int64_t cap = (int64_t)perCU * prop.multiProcessorCount;
     std::vector<int> order(ns);
     for (int sc = 0; sc < ns; ++sc) order[sc] = sc;
     std::stable_sort(order.begin(), order.end(), [&](int p_, int q_) {
       return b.elemOff[p_ + 1] - b.elemOff[p_] > b.elemOff[q_ + 1] - b.elemOff[q_];
     });
-    double per = (double)b.split;
     std::vector<int32_t> G(ns), xcdOf(ns);
     int64_t P = 0;
-    for (int guard = 0; guard < 64; ++guard) {
-      int64_t tets[kXcds] = {0}, wgs[kXcds] = {0};
-      for (int sc : order) {
-        const double nt = (double)(b.elemOff[sc + 1] - b.elemOff[sc]);
-        G[sc] = (int32_t)std::max(1.0, std::min(64.0, std::floor(nt / per + 0.5)));
-        int best = 0;
-        for (int x = 1; x < kXcds; ++x)
-          if (tets[x] < tets[best]) best = x;
-        xcdOf[sc] = best;
-        tets[best] += (int64_t)nt;
-        wgs[best] += G[sc];
+    bool fits = false;
+    // the LDS of single-workgroup scenes limits residency: halve it (fewer
+    // scenes keep positions in LDS) until a schedule fits
+    for (;;) {
+      int perCU = 0;
+      MLX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, b.block, ldsBytes));
+      const int64_t room = (int64_t)perCU * prop.multiProcessorCount / kXcds;
+      double per = (double)b.split;
+      for (int guard = 0; guard < 64 && !fits; ++guard, per *= 1.25) {
+        int64_t tets[kXcds] = {0}, wgs[kXcds] = {0};
+        bool ok = true;
+        for (int sc : order) {
+          const double nt = (double)(b.elemOff[sc + 1] - b.elemOff[sc]);
+          G[sc] = (int32_t)std::max(1.0, std::min(64.0, std::floor(nt / per + 0.5)));
+          int best = -1;
+          for (int x = 0; x < kXcds; ++x)
+            if (wgs[x] + G[sc] <= room && (best < 0 || tets[x] < tets[best])) best = x;
+          if (best < 0) {
+            ok = false;
+            break;
+          }
+          xcdOf[sc] = best;
+          tets[best] += (int64_t)nt;
+          wgs[best] += G[sc];
+        }
+        if (!ok) continue;
+        P = *std::max_element(wgs, wgs + kXcds);
+        fits = P >= 1;
       }
-      P = *std::max_element(wgs, wgs + kXcds);
-      if (kXcds * P <= cap) break;
-      per *= 1.25;
+      if (fits || ldsBytes == 0) break;
+      ldsBytes = ldsBytes / 2 < 4096 ? 0 : ldsBytes / 2;
+      B.ldsVerts = (int)(ldsBytes / (3 * sizeof(T)));
+      b.ldsVerts = B.ldsVerts;
     }
-    if (kXcds * P > cap || P < 1) {
-      set_error("batched split: no schedule fits %lld resident workgroups", (long long)cap);
+    if (!fits) {
+      set_error("batched split: no schedule fits the resident workgroups");
       return MLX_ERR_UNSUPPORTED;
     }
     std::vector<int32_t> entry((size_t)kXcds * P, -1), pbase(ns);
@@ -2206,8 +2226,8 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     b.splitWGs = (int)(kXcds * P);
     void *args[] = {&B, &SP};
     MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
-    MLX_CHECK_HIP(
-        hipLaunchCooperativeKernel(fn, dim3((unsigned)(kXcds * P)), dim3(b.block), args, 0, s));
+    MLX_CHECK_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)(kXcds * P)), dim3(b.block), args,
+                                             ldsBytes, s));
     MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
     // workgroups dealt other than round-robin over the XCDs: nothing was
     // written; run the substep with one workgroup per scene instead
