@@ -140,7 +140,10 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
     wall = time.perf_counter() - t0
     p_end, v_end, _ = b.download()
     b.close()
-    bpt = BYTES_PER_TET[precision]
+    # one element-iteration of the batched kernel = a projection and a
+    # residual evaluation of that element (every element is evaluated every
+    # iteration)
+    bpt = BYTES_PER_TET[precision] + RESIDUAL_BYTES_PER_TET[precision]
     achieved = proj * bpt / (kms / 1e3) / 1e9
     return {
         "value": proj / wall, "unit": "tet-proj/s", "ms_per_step": 1e3 * wall / steps,
@@ -156,7 +159,8 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
         "finite": bool(np.isfinite(p_end).all() and np.isfinite(v_end).all()),
         "roofline": {"bound": "hbm", "kernel": "k_batch_plain", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "bytes_per_unit": bpt, "avg_launch_ms": kms / steps},
+                     "bytes_per_unit": bpt, "unit_of_work": "element-iteration (projection + "
+                     "residual evaluation)", "avg_launch_ms": kms / steps},
     }
 
 
@@ -203,6 +207,8 @@ def main():
                     help="workgroups of the persistent iteration loop (0 = from the mesh size)")
     ap.add_argument("--persist-local", type=int, default=1,
                     help="persistent loop on one XCD's workgroups with L2-keeping barriers")
+    ap.add_argument("--sweep-block", type=int, default=64, choices=[64, 128, 256],
+                    help="threads per workgroup of the per-color sweep launches")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-warmup", type=int, default=6)
     args = ap.parse_args()
@@ -224,6 +230,7 @@ def main():
     sim.dmesh.set_option(L.MLX_OPT_PERSISTENT, args.persistent)
     sim.dmesh.set_option(L.MLX_OPT_PERSIST_WG, args.persist_wg)
     sim.dmesh.set_option(L.MLX_OPT_PERSIST_LOCAL, args.persist_local)
+    sim.dmesh.set_option(L.MLX_OPT_SWEEP_BLOCK, args.sweep_block)
     ncol, _ = sim.dmesh.info()
     for _ in range(args.warmup):
         sim.step()
@@ -312,6 +319,7 @@ def main():
             "form": "plain",
             "precision": args.precision,
             "graphs": bool(args.graphs),
+            "sweep_block": args.sweep_block,
             "persistent": bool(args.persistent),
             "persist_local": bool(args.persist_local) if args.persistent else None,
             "colors": ncol,

@@ -424,6 +424,10 @@ int mlx_profile_read_persistent(mlx_mesh *mesh, double *ms, int64_t *launches,
  * workgroup).  0: all launched workgroups take part, with agent-scope
  * fences. */
 #define MLX_OPT_PERSIST_LOCAL 6
+/* MLX_OPT_SWEEP_BLOCK (64, 128 or 256; default 64): threads per workgroup
+ * of the per-color sweep launches (default 64: a color of ~2,000 elements
+ * spreads over ~32 CUs).  Same results for every value. */
+#define MLX_OPT_SWEEP_BLOCK 7
 int mlx_mesh_set_option(mlx_mesh *mesh, int32_t option, int64_t value);
 
 /* hipDeviceSynchronize on the library's device. */

@@ -130,6 +130,7 @@ struct mlx_mesh {
   mlx::DevBuf d_ctl;              // Ctl
   mlx::DevBuf d_resid;            // double residual array
   mlx::DevBuf d_tmp;              // misc
+  mlx::DevBuf d_count;            // uint32 arrival counter of k_residual_tail (kept at 0)
   mlx::DevBuf d_nh;               // T 4*ld: 2D NeoHookean mu | lambda | lambda_D | lambda_H
   mlx::Ctl *h_ctl = nullptr;      // pinned host mirror
   int32_t *h_counts = nullptr;    // pinned {K, nec, nrc, nUniq} staging for the device counts
@@ -149,6 +150,7 @@ struct mlx_mesh {
   int usePersistent = 0;
   int persistWG = 0;               // 0 = from the largest color (local: from the slot count)
   int persistLocal = 1;            // MLX_OPT_PERSIST_LOCAL: one XCD's workgroups, xcd_barrier
+  int sweepBlock = 64;             // MLX_OPT_SWEEP_BLOCK: threads per workgroup of a color sweep
   mlx::DevBuf d_colors;            // int32 colorOff[numColors] | colorCnt[numColors]
   // layer maps on the device (mlx_maps.hip)
   int deviceMaps = 1;              // MLX_OPT_DEVICE_MAPS

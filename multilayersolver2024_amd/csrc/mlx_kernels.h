@@ -420,7 +420,7 @@ __device__ __forceinline__ void color_slots(const SweepParams<T> &P, int64_t s, 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sweep3d(SweepParams<T> P) {
   if (P.ctl && iteration_stopped(P.ctl)) return;
-  const int64_t s = P.begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t s = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.end) return;
   if (P.flags && !P.flags[s]) return;
   project_tet<T>(P, s);
@@ -518,7 +518,7 @@ __device__ __forceinline__ void project_tri(const SweepParams<T> &P, const int64
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sweep2d(SweepParams<T> P) {
   if (P.ctl && iteration_stopped(P.ctl)) return;
-  const int64_t s = P.begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t s = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.end) return;
   if (P.flags && !P.flags[s]) return;
   project_tri<T>(P, s);
@@ -1399,6 +1399,58 @@ __device__ __forceinline__ void monitor_update(Ctl &c, const MonitorParams &mp, 
     c.stopSolve = 1;
     c.stopLayer = c.layer;
   }
+}
+
+// The residual, the monitor and the iteration count of one projectGS
+// iteration in one launch (k_residual_elems + k_residual_contacts +
+// k_iter_end): blocks [0, nE) hold the element terms of their slots, the
+// rest the contact terms; each block stores its partial past the caches
+// (device-scope store), waits for it, and counts itself on *done.  The
+// block that counts last reads every partial the same way, sums them
+// exactly as k_iter_end does, runs the monitor and re-arms *done to 0 (so
+// a captured graph can replay the launch).  Every block returns at once
+// when the layer has stopped, leaving *done at 0.
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void k_residual_tail(
+    SweepParams<T> P, int64_t nSlots, const int32_t *lamSlot2d, int nE, ContactArrays c,
+    double resistance, double h, const double *uniqAlpha, const double *uniqCount,
+    double *partials, unsigned *done, Ctl *ctl, MonitorParams mp, double *resid, int residCap) {
+  if (iteration_stopped(ctl)) return;
+  __shared__ double sh[kBlock];
+  __shared__ int sLast;
+  const int t = threadIdx.x;
+  double acc = 0;
+  if ((int)blockIdx.x < nE) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + t;
+    if (s < nSlots && P.vid[s] >= 0) acc = residual_elem<T, DIM>(P, s, lamSlot2d);
+  } else {
+    const int i = ((int)blockIdx.x - nE) * kBlock + t;
+    if (i < c.counts[0])
+      acc = residual_contact_one<T, DIM>(c, i, P.x, resistance, h, uniqAlpha, uniqCount,
+                                         c.counts[3]);
+  }
+  double o[1];
+  double v1[1] = {acc};
+  block_sum<1>(v1, sh, o);
+  if (t == 0) {
+    __hip_atomic_store(partials + blockIdx.x, o[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    const unsigned prev =
+        __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sLast = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!sLast) return;
+  double a2 = 0;
+  for (int q = t; q < (int)gridDim.x; q += kBlock)
+    a2 += __hip_atomic_load(partials + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double o2[1];
+  double w1[1] = {a2};
+  block_sum<1>(w1, sh, o2);
+  if (t != 0) return;
+  __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  ctl->iters += 1;
+  monitor_update(*ctl, mp, sqrt(o2[0]), resid, residCap);
 }
 
 // End of one projectGS iteration: when residuals are monitored, sum the

@@ -244,16 +244,16 @@ static void enqueue_sweeps(mlx_mesh &m, SweepParams<T> P, const uint8_t *flags,
       (void)hipEventRecord(e0, m.stream);
     }
   }
+  const int nb = m.sweepBlock;
   for (int c = 0; c < m.numColors; ++c) {
     P.begin = m.colorOff[c];
     P.end = m.colorOff[c] + m.colorCnt[c];
     if (P.end <= P.begin) continue;
+    const unsigned grid = (unsigned)((P.end - P.begin + nb - 1) / nb);
     if (m.dim == 3)
-      hipLaunchKernelGGL(k_sweep3d<T>, dim3(blocks_for(P.end - P.begin)), dim3(kBlock), 0,
-                         m.stream, P);
+      hipLaunchKernelGGL(k_sweep3d<T>, dim3(grid), dim3(nb), 0, m.stream, P);
     else
-      hipLaunchKernelGGL(k_sweep2d<T>, dim3(blocks_for(P.end - P.begin)), dim3(kBlock), 0,
-                         m.stream, P);
+      hipLaunchKernelGGL(k_sweep2d<T>, dim3(grid), dim3(nb), 0, m.stream, P);
   }
   if (e0 && e1) (void)hipEventRecord(e1, m.stream);
 }
@@ -388,6 +388,11 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   const int residCap = 1 << 16;
   MLX_TRY(m.d_resid.reserve(sizeof(double) * residCap));
   MLX_TRY(m.d_ctl.reserve(sizeof(Ctl)));
+  if (!m.d_count.ptr) {
+    // arrival counter of k_residual_tail: zero here, re-armed by the kernel
+    MLX_TRY(m.d_count.reserve(256));
+    MLX_CHECK_HIP(hipMemsetAsync(m.d_count.ptr, 0, 256, st));
+  }
   const int K = hc.K;
   ContactArrays ca = contact_arrays(m, K);
 
@@ -597,29 +602,23 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
           hipLaunchKernelGGL((k_pins<T, 2>), dim3(blocks_for(nPin)), dim3(kBlock), 0, st, nPin,
                              m.d_pinned.as<int32_t>(), m.d_pinnedx.as<T>(), x, ctl);
       }
-      int nPart = 0;
       if (sp.computeResiduals) {
+        const int nBlk = nResElemBlocks + (K > 0 ? nResContactBlocks : 0);
         if (m.dim == 3)
-          hipLaunchKernelGGL((k_residual_elems<T, 3>), dim3(nResElemBlocks), dim3(kBlock), 0, st,
-                             P, ld, (const int32_t *)nullptr, partials);
+          hipLaunchKernelGGL((k_residual_tail<T, 3>), dim3(nBlk), dim3(kBlock), 0, st, P, ld,
+                             (const int32_t *)nullptr, nResElemBlocks, ca, sp.contactResistance,
+                             sp.h, d_uniq, d_cnt, partials, m.d_count.as<unsigned>(), ctl, mp,
+                             m.d_resid.as<double>(), residCap);
         else
-          hipLaunchKernelGGL((k_residual_elems<T, 2>), dim3(nResElemBlocks), dim3(kBlock), 0, st,
-                             P, ld, (const int32_t *)d_lamSlot2d, partials);
-        nPart = nResElemBlocks;
-        if (K > 0) {
-          if (m.dim == 3)
-            hipLaunchKernelGGL((k_residual_contacts<T, 3>), dim3(nResContactBlocks), dim3(kBlock),
-                               0, st, ca, x, sp.contactResistance, sp.h, d_uniq, d_cnt,
-                               partials + nResElemBlocks, ctl);
-          else
-            hipLaunchKernelGGL((k_residual_contacts<T, 2>), dim3(nResContactBlocks), dim3(kBlock),
-                               0, st, ca, x, sp.contactResistance, sp.h, d_uniq, d_cnt,
-                               partials + nResElemBlocks, ctl);
-          nPart += nResContactBlocks;
-        }
+          hipLaunchKernelGGL((k_residual_tail<T, 2>), dim3(nBlk), dim3(kBlock), 0, st, P, ld,
+                             (const int32_t *)d_lamSlot2d, nResElemBlocks, ca,
+                             sp.contactResistance, sp.h, d_uniq, d_cnt, partials,
+                             m.d_count.as<unsigned>(), ctl, mp, m.d_resid.as<double>(),
+                             residCap);
+      } else {
+        hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(kBlock), 0, st, ctl, mp, partials, 0,
+                           m.d_resid.as<double>(), residCap);
       }
-      hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(kBlock), 0, st, ctl, mp, partials, nPart,
-                         m.d_resid.as<double>(), residCap);
     };
 
     // Iterations run as replays of a captured chunk of G iterations when
@@ -631,7 +630,8 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
         return u;
       };
       std::vector<uint64_t> sig = {
-          (uint64_t)col, (uint64_t)G, (uint64_t)L.nb, (uint64_t)L.nbnd, (uint64_t)(L.nec > 0),
+          (uint64_t)col, (uint64_t)G, (uint64_t)m.sweepBlock, (uint64_t)L.nb, (uint64_t)L.nbnd,
+          (uint64_t)(L.nec > 0),
           (uint64_t)L.nrc, (uint64_t)(K > 0), (uint64_t)uniqCap, (uint64_t)L.o_rig_vert,
           (uint64_t)L.o_bnd_off, (uint64_t)L.o_bnd_vert, (uint64_t)L.o_rc_off,
           (uint64_t)L.o_rc_idx, (uint64_t)L.o_ec, (uint64_t)sp.computeResiduals,
@@ -642,6 +642,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
           (uint64_t)(uintptr_t)m.d_xold.ptr, (uint64_t)(uintptr_t)m.d_lam.ptr,
           (uint64_t)(uintptr_t)m.d_partials.ptr, (uint64_t)(uintptr_t)m.d_resid.ptr,
           (uint64_t)(uintptr_t)m.d_pinnedx.ptr, (uint64_t)(uintptr_t)m.d_ctl.ptr,
+          (uint64_t)(uintptr_t)m.d_count.ptr,
           (uint64_t)(uintptr_t)m.d_vbody.ptr, bits(sp.h), bits(sp.boundaryCompliance),
           bits(sp.contactResistance), bits(sp.giveUpThreshold), (uint64_t)sp.hangingStop,
           (uint64_t)sp.giveUpEnabled, (uint64_t)sp.lastLayerGiveUpEnabled};
@@ -1172,7 +1173,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
                          &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced,
-                         &m->d_maps, &m->d_scan};
+                         &m->d_maps, &m->d_scan, &m->d_count};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
@@ -1882,6 +1883,10 @@ int mlx_mesh_set_option(mlx_mesh *m, int32_t option, int64_t value) {
       return MLX_OK;
     case MLX_OPT_PERSIST_LOCAL:
       m->persistLocal = value != 0;
+      return MLX_OK;
+    case MLX_OPT_SWEEP_BLOCK:
+      if (value != 64 && value != 128 && value != 256) break;
+      m->sweepBlock = (int)value;
       return MLX_OK;
     default:
       break;
