@@ -85,6 +85,22 @@ __device__ __forceinline__ int32_t cc_find(const int32_t *parent, int32_t x) {
   return x;
 }
 
+// Root of x with path splitting: every non-root met on the way is pointed
+// at its grandparent.  Only roots are ever hooked (k_cc_union's CAS), a
+// non-root never becomes a root again, and a grandparent is an ancestor,
+// so concurrent finds and unions keep a forest with the same components.
+__device__ __forceinline__ int32_t cc_find_split(int32_t *parent, int32_t x) {
+  int32_t cur = cc_load(parent + x);
+  if (cur == x) return x;
+  int32_t prev = x, next;
+  while (cur != (next = cc_load(parent + cur))) {
+    __hip_atomic_store(parent + prev, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    prev = cur;
+    cur = next;
+  }
+  return cur;
+}
+
 // Union of e with vmin[v] for each of its vertices.  Roots only ever get a
 // smaller parent, so every retry either finds the two in one tree or hooks
 // one root; the loop ends.
@@ -95,8 +111,8 @@ __global__ void k_cc_union(int64_t nT, int vpe, const int32_t *T, const int32_t 
   for (int k = 0; k < vpe; ++k) {
     int32_t a = (int32_t)e, b = vmin[T[e * vpe + k]];
     while (true) {
-      a = cc_find(parent, a);
-      b = cc_find(parent, b);
+      a = cc_find_split(parent, a);
+      b = cc_find_split(parent, b);
       if (a == b) break;
       if (a < b) {
         const int32_t t = a;
@@ -123,7 +139,8 @@ __global__ void k_cc_flatten(int64_t nT, int32_t *parent, int32_t *isRoot) {
     isRoot[e] = 0;
     return;
   }
-  const int32_t r = cc_find(parent, (int32_t)e);
+  const int32_t r = cc_find_split(parent, (int32_t)e);
+  if (r != e) __hip_atomic_store(parent + e, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   isRoot[e] = r == e ? 1 : 0;
 }
 
