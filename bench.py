@@ -52,12 +52,22 @@ BYTES_PER_TET = {"fp32": 280, "fp64": 544}
 # ids 16 + x read 48 + DmInv 36 + vol 4 + (alpha_d, alpha_o, alpha_s) 12 +
 # lambda read 24 (FP64: 16 + 96 + 72 + 8 + 24 + 48).
 RESIDUAL_BYTES_PER_TET = {"fp32": 140, "fp64": 264}
-# HBM-side bytes per k_sweep3d<float> dispatch on this workload, from the
-# rocprofv3 FETCH_SIZE / WRITE_SIZE runs (KiB per dispatch, averaged over
-# 8,200 dispatches), scaled by what the same counters read for a 1 GiB
-# device copy (FETCH_SIZE reads half the bytes fetched, WRITE_SIZE all).
-TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 167.92, "fetch_scale": 2.0,
-                                    "write": 94.74, "write_scale": 1.0}}
+# HBM-side bytes per dispatch on the default workloads, from the rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs (KiB per dispatch; k_sweep3d averaged over
+# 8,200 dispatches of the 81,920-tet beam with 64-thread workgroups,
+# k_batch_plain over the 256-beam substeps at 1,024 threads), scaled by what
+# the same counters read for a 1 GiB device copy (FETCH_SIZE reads half the
+# bytes fetched, WRITE_SIZE all).
+TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 197.73, "fetch_scale": 2.0,
+                                    "write": 96.19, "write_scale": 1.0},
+               "k_batch_plain<float,1024>": {"fetch": 15336905.12, "fetch_scale": 2.0,
+                                             "write": 3388201.03, "write_scale": 1.0}}
+
+
+def traffic_bytes(kernel):
+    tk = TRAFFIC_KIB.get(kernel)
+    return (1024 * (tk["fetch"] * tk["fetch_scale"] + tk["write"] * tk["write_scale"])
+            if tk else None)
 
 
 def log(*a):
@@ -143,6 +153,7 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
     # one element-iteration of the batched kernel = a projection and a
     # residual evaluation of that element (every element is evaluated every
     # iteration)
+    ftype = "float" if precision == "fp32" else "double"
     bpt = BYTES_PER_TET[precision] + RESIDUAL_BYTES_PER_TET[precision]
     achieved = proj * bpt / (kms / 1e3) / 1e9
     return {
@@ -157,8 +168,12 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
         "mean_iterations_per_scene_substep": float(np.mean(iters)),
         "max_iterations_per_scene_substep": int(np.max(iters_max)),
         "finite": bool(np.isfinite(p_end).all() and np.isfinite(v_end).all()),
-        "roofline": {"bound": "hbm", "kernel": "k_batch_plain", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": f"k_batch_plain<{ftype},{best[0]}>"
+                     if best[1] == 0 else f"k_batch_split<{ftype},{best[0]}>",
+                     "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_bytes(f"k_batch_plain<{ftype},{best[0]}>")
+                     if best[1] == 0 and nscenes == 256 else None,
                      "bytes_per_unit": bpt, "unit_of_work": "element-iteration (projection + "
                      "residual evaluation)", "avg_launch_ms": kms / steps},
     }
@@ -296,9 +311,8 @@ def main():
         roof = {"kernel": sweeps["kernel"], "bytes_per_unit": bpt,
                 "avg_launch_us": sweeps["avg_launch_us"], "launches": launches.value,
                 "elements_per_launch": elems.value / max(launches.value, 1)}
-    tk = TRAFFIC_KIB.get(roof["kernel"]) if (args.nx, args.ny, args.nz) == (64, 16, 16) else None
-    traffic = (1024 * (tk["fetch"] * tk["fetch_scale"] + tk["write"] * tk["write_scale"])
-               if tk else None)
+    traffic = (traffic_bytes(roof["kernel"]) if (args.nx, args.ny, args.nz) == (64, 16, 16)
+               and args.sweep_block == 64 else None)
     line = {
         "metric": "ms/substep & tet-proj/s to rel. residual 1e-4: plain vs learned+batched, 1 GPU",
         "value": proj / wall,

@@ -284,6 +284,13 @@ int mlx_integrate(mlx_state *state, mlx_settings *settings, double h,
                   int32_t frame, const mlx_plane *plane,
                   mlx_layer_solve_out *out);
 
+/* Prescribed values of the pinned DOFs (nPinnedDOFs, in pinnedDOFs order)
+ * for the following substeps of frame >= 2 (3D).  No counterpart in the
+ * reference, which freezes pinnedx at frame 1 (xpbdLayer3D.m:629-632); it
+ * drives scripted pins such as a twisted end cap.  Frame 1 still freezes
+ * pinnedx from the current positions. */
+int mlx_state_set_pinned_targets(mlx_state *state, const double *pinnedx);
+
 /* ------------------------------------------------------------------ */
 /* Coarse-layer linear solve                                            */
 /* ------------------------------------------------------------------ */

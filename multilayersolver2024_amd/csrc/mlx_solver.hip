@@ -1761,6 +1761,25 @@ int mlx_state_download(mlx_state *S, double *p, double *v, double *lambdai, int3
   return MLX_OK;
 }
 
+int mlx_state_set_pinned_targets(mlx_state *S, const double *pinnedx) {
+  if (!S || !pinnedx) {
+    set_error("mlx_state_set_pinned_targets: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *S->mesh;
+  if (m.dim != 3) {
+    set_error("mlx_state_set_pinned_targets: 3D meshes only (2D pins follow oldp)");
+    return MLX_ERR_UNSUPPORTED;
+  }
+  const int64_t nPin = (int64_t)m.pinnedDOF.size();
+  if (nPin == 0) return MLX_OK;
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  if (m.precision == MLX_FP64) MLX_TRY(upload_scalars<double>(m.d_pinnedx, pinnedx, nPin));
+  else MLX_TRY(upload_scalars<float>(m.d_pinnedx, pinnedx, nPin));
+  return MLX_OK;
+}
+
 int mlx_integrate(mlx_state *S, mlx_settings *st, double h, int32_t frame, const mlx_plane *plane,
                   mlx_layer_solve_out *out) {
   if (!S || !st || !(h > 0) || frame < 1 || st->nLayers < 1 || st->nLayers > MLX_MAX_LAYERS) {

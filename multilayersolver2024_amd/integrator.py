@@ -161,6 +161,16 @@ class Simulation:
         n = min(self._out.residualCount, len(self._resid))
         return StepResult(self._out, self._resid[:n].copy())
 
+    def set_pinned_targets(self, pinnedx):
+        """Prescribed positions of the pinned DOFs (pinnedDOFs order) for the
+        next substeps (3D, frame >= 2): scripted pins such as a twisted end
+        cap.  The reference freezes them at frame 1."""
+        x = L.f64(pinnedx)
+        if len(x) != len(self.mesh.pinnedDOFs):
+            raise ValueError("set_pinned_targets: one value per pinned DOF")
+        L.check("mlx_state_set_pinned_targets",
+                L.lib().mlx_state_set_pinned_targets(self.state, L.ptr(x)))
+
     def download(self):
         m = self.mesh
         p = np.zeros(m.dim * m.nV)

@@ -84,13 +84,15 @@ def _tet_vol6(P, tet):
 
 
 def beam3d(nx=64, ny=16, nz=16, cell=0.02, z0=0.05, seed=0, E=None, nu=None,
-           E_range=(1e4, 1e7), nu_range=(0.30, 0.45), rho=1000.0, alpha1=0.02):
+           E_range=(1e4, 1e7), nu_range=(0.30, 0.45), rho=1000.0, alpha1=0.02,
+           pin_far_end=False):
     """3-D box beam: (nx+1)(ny+1)(nz+1) vertices, vertex (i,j,k) at index
     (i*(ny+1)+j)*(nz+1)+k, position (i, j, k)*cell + (0, 0, z0); 5 tets per
     cube (alternating parity), each positively oriented (det(Dm) > 0).
     Per-element E is log-uniform in E_range and nu uniform in nu_range, drawn
     from Pcg32(seed) in element order as (E draw, nu draw) pairs, unless E and
-    nu are given as scalars.  The x = 0 face is pinned."""
+    nu are given as scalars.  The x = 0 face is pinned (and the x = nx*cell
+    face too with pin_far_end, for the twisting study)."""
     NY, NZ = ny + 1, nz + 1
     vid = lambda i, j, k: (i * NY + j) * NZ + k
     ii, jj, kk = np.meshgrid(np.arange(nx + 1), np.arange(NY), np.arange(NZ), indexing="ij")
@@ -130,9 +132,34 @@ def beam3d(nx=64, ny=16, nz=16, cell=0.02, z0=0.05, seed=0, E=None, nu=None,
         Es = np.full(nT, float(E))
         nus = np.full(nT, float(nu))
     mu, lam = to_lame(nus, Es)
-    pinned = np.array([vid(0, j, k) for j in range(NY) for k in range(NZ)], dtype=np.int32)
+    ends = (0, nx) if pin_far_end else (0,)
+    pinned = np.array([vid(i, j, k) for i in ends for j in range(NY) for k in range(NZ)],
+                      dtype=np.int32)
     return dict(
         dim=3, p=P, t=T, rho=np.full(nT, rho), mu=mu, lam=lam,
         alpha1=np.full(nT, alpha1), pinned=pinned, E=Es, nu=nus,
         name=f"beam3d_{nx}x{ny}x{nz}",
     )
+
+
+def twist_pinned_targets(mesh, angle, axis_dim=0):
+    """Pinned-DOF targets (pinnedDOFs order) of a beam whose far end cap
+    (the pinned vertices at the largest coordinate along axis_dim) is turned
+    by `angle` radians about the beam axis (through the centre of the
+    cross-section), the other pinned vertices kept at their rest positions.
+    The twisting animation of the precision study (end cap at 1 rad/s:
+    angle = t).  3D meshes; uses mesh.p as the rest positions."""
+    P = np.asarray(mesh.p, dtype=np.float64).reshape(-1, 3)
+    pv = np.asarray(mesh.pinnedInds, dtype=np.int64)
+    a = [d for d in range(3) if d != axis_dim]
+    far = P[pv, axis_dim] >= P[:, axis_dim].max() - 1e-12
+    cy = P[:, a[0]].min() + np.ptp(P[:, a[0]]) / 2
+    cz = P[:, a[1]].min() + np.ptp(P[:, a[1]]) / 2
+    Q = P.copy()
+    c, s_ = np.cos(angle), np.sin(angle)
+    idx = pv[far]
+    y, z = P[idx, a[0]] - cy, P[idx, a[1]] - cz
+    Q[idx, a[0]] = cy + c * y - s_ * z
+    Q[idx, a[1]] = cz + s_ * y + c * z
+    dofs = np.asarray(mesh.pinnedDOFs, dtype=np.int64) - 1  # 1-based
+    return Q.reshape(-1)[dofs]
