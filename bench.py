@@ -224,6 +224,9 @@ def main():
                     help="persistent loop on one XCD's workgroups with L2-keeping barriers")
     ap.add_argument("--sweep-block", type=int, default=64, choices=[64, 128, 256],
                     help="threads per workgroup of the per-color sweep launches")
+    ap.add_argument("--sweep-wg", type=int, default=0,
+                    help="sparse rigid layers sweep all colors in one workgroup (1) or launch "
+                         "per color (0)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-warmup", type=int, default=6)
     args = ap.parse_args()
@@ -246,6 +249,7 @@ def main():
     sim.dmesh.set_option(L.MLX_OPT_PERSIST_WG, args.persist_wg)
     sim.dmesh.set_option(L.MLX_OPT_PERSIST_LOCAL, args.persist_local)
     sim.dmesh.set_option(L.MLX_OPT_SWEEP_BLOCK, args.sweep_block)
+    sim.dmesh.set_option(L.MLX_OPT_SWEEP_WG, args.sweep_wg)
     ncol, _ = sim.dmesh.info()
     for _ in range(args.warmup):
         sim.step()
@@ -333,7 +337,7 @@ def main():
             "form": "plain",
             "precision": args.precision,
             "graphs": bool(args.graphs),
-            "sweep_block": args.sweep_block,
+            "sweep_block": args.sweep_block, "sweep_wg": bool(args.sweep_wg),
             "persistent": bool(args.persistent),
             "persist_local": bool(args.persist_local) if args.persistent else None,
             "colors": ncol,

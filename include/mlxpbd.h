@@ -435,6 +435,15 @@ int mlx_profile_read_persistent(mlx_mesh *mesh, double *ms, int64_t *launches,
  * of the per-color sweep launches (default 64: a color of ~2,000 elements
  * spreads over ~32 CUs).  Same results for every value. */
 #define MLX_OPT_SWEEP_BLOCK 7
+/* MLX_OPT_SWEEP_WG (default 0): a 3D layer with rigid elements whose every
+ * color has at most 1,024 elastic elements runs each iteration's sweep of
+ * all colors in one workgroup (compacted per-color lists of elastic slots,
+ * colors separated by workgroup barriers) instead of one launch per color.
+ * Same projections in the same order.  Off by default: on the 81,920-tet
+ * beam it is slower (8.62 against 8.31 ms per substep), a workgroup's
+ * dependent gather-project-store chain per color costing as much as a
+ * pipelined launch. */
+#define MLX_OPT_SWEEP_WG 8
 int mlx_mesh_set_option(mlx_mesh *mesh, int32_t option, int64_t value);
 
 /* hipDeviceSynchronize on the library's device. */

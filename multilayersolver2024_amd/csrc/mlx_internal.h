@@ -151,6 +151,8 @@ struct mlx_mesh {
   int persistWG = 0;               // 0 = from the largest color (local: from the slot count)
   int persistLocal = 1;            // MLX_OPT_PERSIST_LOCAL: one XCD's workgroups, xcd_barrier
   int sweepBlock = 64;             // MLX_OPT_SWEEP_BLOCK: threads per workgroup of a color sweep
+  int useSweepWg = 0;              // MLX_OPT_SWEEP_WG: one-workgroup sweeps for sparse layers
+  mlx::DevBuf d_elist;             // colorOff i64 | colorCnt i64 | count i32 | elastic slot lists i32
   mlx::DevBuf d_colors;            // int32 colorOff[numColors] | colorCnt[numColors]
   // layer maps on the device (mlx_maps.hip)
   int deviceMaps = 1;              // MLX_OPT_DEVICE_MAPS

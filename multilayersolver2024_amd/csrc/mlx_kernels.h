@@ -426,6 +426,57 @@ __global__ __launch_bounds__(kBlock) void k_sweep3d(SweepParams<T> P) {
   project_tet<T>(P, s);
 }
 
+// Elastic slots of each color of a layer, compacted in ascending slot order
+// into list[colorOff[c] ..) (a color's list fits in its own slot range),
+// their number into cnt[c].  One workgroup per color.
+__global__ __launch_bounds__(kBlock) void k_color_lists(const int64_t *colorOff,
+                                                        const int64_t *colorCnt,
+                                                        const uint8_t *flags, int32_t *list,
+                                                        int32_t *cnt) {
+  __shared__ int32_t sh[kBlock];
+  __shared__ int32_t base;
+  const int c = blockIdx.x, t = threadIdx.x;
+  const int64_t a = colorOff[c], n = colorCnt[c];
+  if (t == 0) base = 0;
+  __syncthreads();
+  for (int64_t s0 = 0; s0 < n; s0 += kBlock) {
+    const int64_t s = a + s0 + t;
+    const int f = (s0 + t < n && flags[s]) ? 1 : 0;
+    sh[t] = f;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele) over the workgroup
+    for (int o = 1; o < kBlock; o <<= 1) {
+      const int32_t v = t >= o ? sh[t - o] : 0;
+      __syncthreads();
+      sh[t] += v;
+      __syncthreads();
+    }
+    if (f) list[a + base + sh[t] - 1] = (int32_t)s;
+    __syncthreads();
+    if (t == kBlock - 1) base += sh[t];
+    __syncthreads();
+  }
+  if (t == 0) cnt[c] = base;
+}
+
+// One projectGS sweep over all colors of a layer with few elastic elements
+// per color, in one workgroup: the compacted elastic slots of each color
+// (k_color_lists), colors separated by workgroup barriers (which wait for
+// the workgroup's stores; its waves share one L1).  Same projections in the
+// same color order as one k_sweep3d launch per color.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_sweep_wg(SweepParams<T> P, const int64_t *colorOff,
+                                                 const int32_t *list, const int32_t *cnt,
+                                                 int numColors) {
+  if (P.ctl && iteration_stopped(P.ctl)) return;
+  for (int c = 0; c < numColors; ++c) {
+    const int n = cnt[c];
+    const int32_t *l = list + colorOff[c];
+    for (int i = threadIdx.x; i < n; i += NT) project_tet<T>(P, l[i]);
+    __syncthreads();
+  }
+}
+
 // StVK-Voigt triangle (matStvkVoigtConstraint.m:1-34): same algebra in 2D
 // with constraints (E11, E22, E12) and the full 3x3 solve of justSolve.
 template <typename T>

@@ -104,6 +104,7 @@ struct SolveResult {
 
 static constexpr int kMaxPartials = 65536;
 static constexpr int kXcds = 8;  // XCDs of an MI355X (workgroups are dealt round-robin over them)
+static constexpr int kSweepWg = 1024;  // threads of the single-workgroup sweep (k_sweep_wg)
 
 // The persistent-loop kernel for the mesh's dimension and barrier mode.
 // The XCD-local form is 3D only.
@@ -258,6 +259,19 @@ static void enqueue_sweeps(mlx_mesh &m, SweepParams<T> P, const uint8_t *flags,
   if (e0 && e1) (void)hipEventRecord(e1, m.stream);
 }
 
+// One iteration's sweep of a layer with few elastic elements per color:
+// a single k_sweep_wg workgroup over the compacted color lists.  Not in the
+// sweep timing of mlx_profile_read, which covers k_sweep3d/k_sweep2d.
+template <typename T>
+static void enqueue_sweep_wg(mlx_mesh &m, SweepParams<T> P) {
+  const char *eb = m.d_elist.as<char>();
+  const int nc = m.numColors;
+  hipLaunchKernelGGL((k_sweep_wg<T, kSweepWg>), dim3(1), dim3(kSweepWg), 0, m.stream, P,
+                     reinterpret_cast<const int64_t *>(eb),
+                     reinterpret_cast<const int32_t *>(eb + 20 * (size_t)nc),
+                     reinterpret_cast<const int32_t *>(eb + 16 * (size_t)nc), nc);
+}
+
 // After a stream sync: fold the recorded iterations whose index is below
 // itersDone (the rest returned at once: the layer had stopped).
 static void prof_harvest(mlx_mesh &m, int itersDone) {
@@ -375,6 +389,28 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
         nElastic[l] += f;
         perColor[(size_t)m.numColors * l + c] += f;
       }
+  // layers with rigid elements and at most kSweepWg elastic elements in
+  // every color sweep all colors in one workgroup (k_sweep_wg)
+  std::vector<uint8_t> wgLayer(ns, 0);
+  bool anyWg = false;
+  if (m.useSweepWg && m.dim == 3) {
+    for (int l = 0; l < ns; ++l) {
+      int64_t mx = 0;
+      for (int c = 0; c < m.numColors; ++c) mx = std::max(mx, perColor[(size_t)m.numColors * l + c]);
+      wgLayer[l] = nElastic[l] < nT && mx <= kSweepWg;
+      anyWg = anyWg || wgLayer[l];
+    }
+  }
+  if (anyWg) {
+    const size_t nc = (size_t)m.numColors;
+    MLX_TRY(m.d_elist.reserve(20 * nc + 4 * (size_t)ld));
+    std::vector<int64_t> oc(2 * nc);
+    for (size_t c = 0; c < nc; ++c) {
+      oc[c] = m.colorOff[c];
+      oc[nc + c] = m.colorCnt[c];
+    }
+    MLX_CHECK_HIP(hipMemcpyAsync(m.d_elist.ptr, oc.data(), 16 * nc, hipMemcpyHostToDevice, st));
+  }
   MLX_TRY(m.d_flags.reserve(flags.size()));
   MLX_TRY(m.d_vbody.reserve(sizeof(int32_t) * maps.vbody.size()));
   MLX_CHECK_HIP(hipMemcpyAsync(m.d_flags.ptr, flags.data(), flags.size(), hipMemcpyHostToDevice, st));
@@ -570,9 +606,22 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     }
 
     hc_ctl.iters = 0;  // k_layer_begin cleared the device count
+    const bool sweepWg = wgLayer[col] != 0;
+    if (sweepWg) {
+      char *eb = m.d_elist.as<char>();
+      const size_t nc = (size_t)m.numColors;
+      hipLaunchKernelGGL(k_color_lists, dim3(m.numColors), dim3(kBlock), 0, st,
+                         reinterpret_cast<const int64_t *>(eb),
+                         reinterpret_cast<const int64_t *>(eb + 8 * nc), fl,
+                         reinterpret_cast<int32_t *>(eb + 20 * nc),
+                         reinterpret_cast<int32_t *>(eb + 16 * nc));
+    }
     int iterEnqueued = 0;
     auto enqueue_iteration = [&]() {
-      enqueue_sweeps<T>(m, P, fl, perColor.data() + (size_t)m.numColors * col, iterEnqueued++);
+      if (sweepWg)
+        enqueue_sweep_wg<T>(m, P);
+      else
+        enqueue_sweeps<T>(m, P, fl, perColor.data() + (size_t)m.numColors * col, iterEnqueued++);
       if (m.dim == 3 && L.nbnd > 0)
         hipLaunchKernelGGL(k_boundary<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, minv, bodies,
                            ri, m.d_lamB.as<double>(), sp.boundaryCompliance, sp.h, ctl);
@@ -642,7 +691,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
           (uint64_t)(uintptr_t)m.d_xold.ptr, (uint64_t)(uintptr_t)m.d_lam.ptr,
           (uint64_t)(uintptr_t)m.d_partials.ptr, (uint64_t)(uintptr_t)m.d_resid.ptr,
           (uint64_t)(uintptr_t)m.d_pinnedx.ptr, (uint64_t)(uintptr_t)m.d_ctl.ptr,
-          (uint64_t)(uintptr_t)m.d_count.ptr,
+          (uint64_t)(uintptr_t)m.d_count.ptr, (uint64_t)sweepWg, (uint64_t)(uintptr_t)m.d_elist.ptr,
           (uint64_t)(uintptr_t)m.d_vbody.ptr, bits(sp.h), bits(sp.boundaryCompliance),
           bits(sp.contactResistance), bits(sp.giveUpThreshold), (uint64_t)sp.hangingStop,
           (uint64_t)sp.giveUpEnabled, (uint64_t)sp.lastLayerGiveUpEnabled};
@@ -1173,7 +1222,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
                          &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced,
-                         &m->d_maps, &m->d_scan, &m->d_count};
+                         &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
@@ -1902,6 +1951,9 @@ int mlx_mesh_set_option(mlx_mesh *m, int32_t option, int64_t value) {
       return MLX_OK;
     case MLX_OPT_PERSIST_LOCAL:
       m->persistLocal = value != 0;
+      return MLX_OK;
+    case MLX_OPT_SWEEP_WG:
+      m->useSweepWg = value != 0;
       return MLX_OK;
     case MLX_OPT_SWEEP_BLOCK:
       if (value != 64 && value != 128 && value != 256) break;

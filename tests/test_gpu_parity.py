@@ -199,6 +199,32 @@ def test_graph_replay_matches_eager(oracle_lib):
         b.close()
 
 
+@pytest.mark.parametrize("precision", [L.MLX_FP64, L.MLX_FP32])
+def test_single_workgroup_sweeps_equal_per_color_launches(precision, oracle_lib):
+    """Rigid layers with few elastic elements per color sweep all colors in
+    one workgroup over compacted lists (MLX_OPT_SWEEP_WG); the projections
+    and their order within the layer are the same as with one launch per
+    color, so every state is bit-identical, with ground contacts, the
+    relative stop rule and bodies in both coarse layers."""
+    import copy
+    m, integ, plane = beam_case(nx=24, ny=6, nz=6, z0=0.004, pct=99.99)
+    a = Simulation(m, integ, 1 / 60, plane, precision=precision)
+    b = Simulation(m, copy.deepcopy(integ), 1 / 60, plane, precision=precision)
+    a.dmesh.set_option(L.MLX_OPT_SWEEP_WG, 0)
+    b.dmesh.set_option(L.MLX_OPT_SWEEP_WG, 1)
+    try:
+        for k in range(8):
+            ra, rb = a.step(), b.step()
+            pa, va, la, ka = a.download()
+            pb, vb, lb, kb = b.download()
+            _say("sweep_wg", k, ra.itersPerLayer, rb.itersPerLayer, ka)
+            assert ra.itersPerLayer == rb.itersPerLayer and ka == kb
+            assert np.array_equal(pa, pb) and np.array_equal(la, lb) and np.array_equal(va, vb)
+    finally:
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("precision,wg,local,size", [
     (L.MLX_FP64, 0, 0, "small"), (L.MLX_FP64, 7, 0, "small"), (L.MLX_FP32, 0, 0, "small"),
     (L.MLX_FP64, 0, 1, "small"), (L.MLX_FP64, 7, 1, "small"), (L.MLX_FP32, 0, 1, "small"),
