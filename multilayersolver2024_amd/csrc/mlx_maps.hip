@@ -101,32 +101,100 @@ __device__ __forceinline__ int32_t cc_find_split(int32_t *parent, int32_t x) {
   return cur;
 }
 
-// Union of e with vmin[v] for each of its vertices.  Roots only ever get a
-// smaller parent, so every retry either finds the two in one tree or hooks
-// one root; the loop ends.
+// Join the trees of a and b.  Roots only ever get a smaller parent, so
+// every retry either finds the two in one tree or hooks one root; the loop
+// ends.
+__device__ __forceinline__ void cc_union_pair(int32_t *parent, int32_t a, int32_t b) {
+  while (true) {
+    a = cc_find_split(parent, a);
+    b = cc_find_split(parent, b);
+    if (a == b) return;
+    if (a < b) {
+      const int32_t t = a;
+      a = b;
+      b = t;
+    }
+    // hook root a (larger) under b (smaller)
+    int32_t expected = a;
+    if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return;
+    a = expected;
+  }
+}
+
+// Union of e with vmin[v] for each of its vertices (every inserted element).
 __global__ void k_cc_union(int64_t nT, int vpe, const int32_t *T, const int32_t *vmin,
                            int32_t *parent) {
   const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
   if (e >= nT || cc_load(parent + e) < 0) return;
+  for (int k = 0; k < vpe; ++k) cc_union_pair(parent, (int32_t)e, vmin[T[e * vpe + k]]);
+}
+
+// Incremental layers (sorted percentages ascending, so each layer's
+// inserted set contains the previous one's): the elements with rank in
+// [prev, n) (and not forced elastic) become singleton roots; the earlier
+// ones keep their forest.
+__global__ void k_cc_add(int64_t nT, const int32_t *rank, const uint8_t *forced, int32_t prev,
+                         int32_t n, int32_t *parent) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int32_t r = rank[e];
+  if (r >= prev && r < n && !forced[e]) parent[e] = (int32_t)e;
+}
+
+// vmin over the newly inserted elements (vmin already holds the minimum
+// over the earlier ones).
+__global__ void k_cc_vmin_new(int64_t nT, int vpe, const int32_t *T, const int32_t *rank,
+                              const uint8_t *forced, int32_t prev, int32_t n, int32_t *vmin) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int32_t r = rank[e];
+  if (r < prev || r >= n || forced[e]) return;
+  for (int k = 0; k < vpe; ++k) atomicMin(&vmin[T[e * vpe + k]], (int32_t)e);
+}
+
+// Union of a newly inserted element e with, for each of its vertices v,
+// the smallest earlier element containing v (vminOld: every earlier element
+// containing v is already in that one's tree) and the smallest element of
+// the layer containing v (vmin: joins the new elements sharing v).  With
+// the earlier layer's forest this gives the components of the whole
+// inserted set: two earlier elements sharing a vertex were joined before.
+__global__ void k_cc_union_new(int64_t nT, int vpe, const int32_t *T, const int32_t *rank,
+                               const uint8_t *forced, int32_t prev, int32_t n,
+                               const int32_t *vminOld, const int32_t *vmin, int32_t *parent) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int32_t r = rank[e];
+  if (r < prev || r >= n || forced[e]) return;
   for (int k = 0; k < vpe; ++k) {
-    int32_t a = (int32_t)e, b = vmin[T[e * vpe + k]];
-    while (true) {
-      a = cc_find_split(parent, a);
-      b = cc_find_split(parent, b);
-      if (a == b) break;
-      if (a < b) {
-        const int32_t t = a;
-        a = b;
-        b = t;
-      }
-      // hook root a (larger) under b (smaller)
-      int32_t expected = a;
-      if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        break;
-      a = expected;
-    }
+    const int32_t v = T[e * vpe + k];
+    const int32_t o = vminOld[v];
+    if (o != INT_MAX) cc_union_pair(parent, (int32_t)e, o);
+    cc_union_pair(parent, (int32_t)e, vmin[v]);
   }
+}
+
+// Elastic flags of every sorted layer in slot order (flags pre-zeroed, so
+// padding slots stay 0) and elastic counts per (layer, color).
+__global__ void k_slot_flags(int64_t nT, int ns, int64_t ld, int nc, const int32_t *sets,
+                             const int32_t *slotOf, const int32_t *ecolor, uint8_t *flags,
+                             int32_t *counts) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int64_t s = slotOf[e];
+  const int c = ecolor[e];
+  for (int l = 0; l < ns; ++l) {
+    const bool f = sets[(int64_t)l * nT + e] == 0;
+    flags[l * ld + s] = f ? 1 : 0;
+    if (f) atomicAdd(&counts[l * nc + c], 1);
+  }
+}
+
+// Body count of a layer: last scan entry + last root flag.
+__global__ void k_cc_count(int64_t nT, const int32_t *scan, const int32_t *isRoot,
+                           int32_t *count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *count = scan[nT - 1] + isRoot[nT - 1];
 }
 
 // After all unions: each inserted element points at its root, and roots
@@ -194,13 +262,14 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 }  // namespace
 
 int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int ns,
-                   HostMaps &maps, std::vector<int32_t> *setsOut) {
+                   HostMaps &maps, std::vector<int32_t> *setsOut, bool deviceFlags) {
   const int64_t nT = m.nT, nV = m.nV;
   const int vpe = m.vpe;
   hipStream_t st = m.stream;
   maps.nSorted = ns;
   maps.numRigids.assign(ns, 0);
-  maps.elastic.assign((size_t)ns * nT, 1);
+  maps.deviceFlags = deviceFlags;
+  if (!deviceFlags) maps.elastic.assign((size_t)ns * nT, 1);
   maps.vbody.assign((size_t)ns * nV, 0);
   maps.boundary.assign((size_t)ns * nV, 0);
   if (nT == 0) return MLX_OK;
@@ -224,7 +293,9 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
                oOrder = take(4 * nT), oRank = take(4 * nT), oParent = take(4 * nT),
                oVmin = take(4 * nV), oRoot = take(4 * nT), oScan = take(4 * nT),
                oSets = take(4 * nT * (size_t)ns), oVb = take(4 * nV * (size_t)ns),
-               oRig = take(nV), oOfEl = take(nV), oBnd = take(nV * (size_t)ns);
+               oRig = take(nV), oOfEl = take(nV), oBnd = take(nV * (size_t)ns),
+               oVminOld = take(4 * nV), oCount = take(4 * (size_t)ns),
+               oColorCnt = take(4 * (size_t)ns * (size_t)std::max(m.numColors, 1));
   MLX_TRY(m.d_maps.reserve(off));
   char *base = m.d_maps.as<char>();
   auto at32 = [&](size_t o) { return reinterpret_cast<int32_t *>(base + o); };
@@ -236,6 +307,7 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
   uint8_t *rig = reinterpret_cast<uint8_t *>(base + oRig);
   uint8_t *ofEl = reinterpret_cast<uint8_t *>(base + oOfEl);
   uint8_t *bnd = reinterpret_cast<uint8_t *>(base + oBnd);
+  int32_t *vminOld = at32(oVminOld), *count = at32(oCount);
   const int32_t *T = m.d_T.as<int32_t>();
   const uint8_t *forced = m.d_forced.as<uint8_t>();
 
@@ -252,18 +324,33 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
                                           (unsigned int)nT, 0, 64, st));
   hipLaunchKernelGGL(k_rank, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, order, rank);
 
-  // 2. bodies per sorted layer
+  // 2. bodies per sorted layer.  The percentages are ascending, so each
+  // layer's inserted set contains the previous one's: its forest is kept
+  // and only the newly inserted elements are added and joined.
   const int64_t nMax = std::max(nT, nV);
+  hipLaunchKernelGGL(k_cc_init, dim3(grid_for(nMax)), dim3(kMapBlock), 0, st, nT, nV, rank,
+                     forced, 0, parent, vmin);
+  int32_t prevInsert = 0;
   for (int l = 0; l < ns; ++l) {
     const double end = std::floor((sorted[l] / 100.0) * (double)nT);
     const int32_t nInsert = end <= 0 ? 0 : (end >= (double)nT ? (int32_t)nT : (int32_t)end);
     int32_t *setsL = sets + (size_t)nT * l;
-    hipLaunchKernelGGL(k_cc_init, dim3(grid_for(nMax)), dim3(kMapBlock), 0, st, nT, nV, rank,
-                       forced, nInsert, parent, vmin);
-    hipLaunchKernelGGL(k_cc_vmin, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T, parent,
-                       vmin);
-    hipLaunchKernelGGL(k_cc_union, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T, vmin,
-                       parent);
+    if (nInsert < prevInsert) {
+      // percentages given out of order: start this layer from scratch
+      hipLaunchKernelGGL(k_cc_init, dim3(grid_for(nMax)), dim3(kMapBlock), 0, st, nT, nV, rank,
+                         forced, 0, parent, vmin);
+      prevInsert = 0;
+    }
+    if (nInsert > prevInsert) {
+      hipLaunchKernelGGL(k_cc_add, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, rank, forced,
+                         prevInsert, nInsert, parent);
+      MLX_CHECK_HIP(hipMemcpyAsync(vminOld, vmin, 4 * (size_t)nV, hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_cc_vmin_new, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T,
+                         rank, forced, prevInsert, nInsert, vmin);
+      hipLaunchKernelGGL(k_cc_union_new, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T,
+                         rank, forced, prevInsert, nInsert, vminOld, vmin, parent);
+      prevInsert = nInsert;
+    }
     hipLaunchKernelGGL(k_cc_flatten, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, parent,
                        isRoot);
     size_t tb = tmpScan;
@@ -281,21 +368,51 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
     hipLaunchKernelGGL(k_boundary_flags, dim3(grid_for(nV)), dim3(kMapBlock), 0, st, nV, rig, ofEl,
                        bndL);
     // body count = sum of root flags = last scan entry + last flag
-    int32_t tail[2];
-    MLX_CHECK_HIP(hipMemcpyAsync(&tail[0], scan + (nT - 1), 4, hipMemcpyDeviceToHost, st));
-    MLX_CHECK_HIP(hipMemcpyAsync(&tail[1], isRoot + (nT - 1), 4, hipMemcpyDeviceToHost, st));
-    MLX_CHECK_HIP(hipStreamSynchronize(st));
-    maps.numRigids[l] = tail[0] + tail[1];
+    hipLaunchKernelGGL(k_cc_count, dim3(1), dim3(64), 0, st, nT, scan, isRoot, count + l);
   }
   MLX_CHECK_HIP(hipGetLastError());
-  std::vector<int32_t> hs((size_t)nT * ns);
-  MLX_CHECK_HIP(hipMemcpyAsync(hs.data(), sets, 4 * hs.size(), hipMemcpyDeviceToHost, st));
+  MLX_CHECK_HIP(hipMemcpyAsync(maps.numRigids.data(), count, 4 * (size_t)ns,
+                               hipMemcpyDeviceToHost, st));
+  std::vector<int32_t> colorCnt;
+  if (deviceFlags) {
+    // 4. elastic flags in slot order straight into m.d_flags
+    const int nc = m.numColors;
+    if (m.d_ecolor.bytes < 4 * (size_t)nT) {
+      std::vector<int32_t> ec(nT);
+      for (int64_t e = 0; e < nT; ++e) ec[e] = m.color[e] - 1;
+      MLX_TRY(m.d_ecolor.reserve(4 * (size_t)nT));
+      MLX_CHECK_HIP(hipMemcpyAsync(m.d_ecolor.ptr, ec.data(), 4 * (size_t)nT,
+                                   hipMemcpyHostToDevice, st));
+      MLX_CHECK_HIP(hipStreamSynchronize(st));
+    }
+    MLX_TRY(m.d_flags.reserve((size_t)ns * m.ld));
+    int32_t *cc = at32(oColorCnt);
+    MLX_CHECK_HIP(hipMemsetAsync(m.d_flags.ptr, 0, (size_t)ns * m.ld, st));
+    MLX_CHECK_HIP(hipMemsetAsync(cc, 0, 4 * (size_t)ns * nc, st));
+    hipLaunchKernelGGL(k_slot_flags, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, ns, m.ld, nc,
+                       sets, m.d_slotOf.as<int32_t>(), m.d_ecolor.as<int32_t>(),
+                       m.d_flags.as<uint8_t>(), cc);
+    colorCnt.resize((size_t)ns * nc);
+    MLX_CHECK_HIP(hipMemcpyAsync(colorCnt.data(), cc, 4 * colorCnt.size(), hipMemcpyDeviceToHost,
+                                 st));
+  }
+  std::vector<int32_t> hs(deviceFlags && !setsOut ? 0 : (size_t)nT * ns);
+  if (!hs.empty())
+    MLX_CHECK_HIP(hipMemcpyAsync(hs.data(), sets, 4 * hs.size(), hipMemcpyDeviceToHost, st));
   MLX_CHECK_HIP(hipMemcpyAsync(maps.vbody.data(), vb, 4 * maps.vbody.size(), hipMemcpyDeviceToHost,
                                st));
   MLX_CHECK_HIP(hipMemcpyAsync(maps.boundary.data(), bnd, maps.boundary.size(),
                                hipMemcpyDeviceToHost, st));
   MLX_CHECK_HIP(hipStreamSynchronize(st));
-  for (size_t i = 0; i < hs.size(); ++i) maps.elastic[i] = hs[i] == 0;
+  if (deviceFlags) {
+    const int nc = m.numColors;
+    maps.perColor.assign(colorCnt.begin(), colorCnt.end());
+    maps.nElastic.assign(ns, 0);
+    for (int l = 0; l < ns; ++l)
+      for (int c = 0; c < nc; ++c) maps.nElastic[l] += colorCnt[(size_t)l * nc + c];
+  } else {
+    for (size_t i = 0; i < hs.size(); ++i) maps.elastic[i] = hs[i] == 0;
+  }
   if (setsOut) setsOut->swap(hs);
   return MLX_OK;
 }

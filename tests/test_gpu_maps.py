@@ -65,6 +65,9 @@ def _keys(rng, n, kind):
 @pytest.mark.parametrize("case", ["beam", "beam_pinned", "cantilever"])
 @pytest.mark.parametrize("kind", ["smooth", "ties"])
 def test_device_layer_maps_bit_exact(case, kind, oracle_lib):
+    """The device builds each layer's bodies from the previous layer's
+    forest (the percentages ascend, so the inserted sets are nested): six
+    layers, five increments."""
     rng = np.random.default_rng(zlib.crc32(f"{case}-{kind}".encode()))
     if case.startswith("beam"):
         sc = scenes.beam3d(12, 5, 4, seed=2)
