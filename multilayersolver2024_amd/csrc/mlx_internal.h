@@ -160,6 +160,7 @@ struct mlx_mesh {
   mlx::DevBuf d_maps;              // sort keys/values, ranks, parents, labels, outputs
   mlx::DevBuf d_scan;              // rocPRIM temporary storage
   mlx::DevBuf d_sync;              // uint32 barrier counter | int32 error flag
+  mlx::DevBuf d_ecolor;            // int32 nT: 0-based color of each element
 
   // sweep timing (mlx_profile_enable)
   struct Prof {
@@ -203,6 +204,12 @@ struct HostMaps {
   std::vector<int32_t> vbody;     // nSorted x nV
   std::vector<uint8_t> boundary;  // nSorted x nV
   std::vector<int32_t> numRigids; // nSorted
+  // set by layer_maps_dev(deviceFlags): elastic flags already in m.d_flags
+  // (slot order, one column of ld per sorted layer); elastic element counts
+  // per (layer, color) and per layer; `elastic` is then left empty
+  bool deviceFlags = false;
+  std::vector<int64_t> perColor;  // nSorted x numColors
+  std::vector<int64_t> nElastic;  // nSorted
 };
 
 // buildLayers + findBoundaryVertices on the device from per-element
@@ -211,6 +218,7 @@ struct HostMaps {
 // ascending minimum element, vertex maps.  Results are copied to `maps`
 // (and the per-layer body ids to *setsOut, nT x ns, when given).
 int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int ns,
-                   HostMaps &maps, std::vector<int32_t> *setsOut = nullptr);
+                   HostMaps &maps, std::vector<int32_t> *setsOut = nullptr,
+                   bool deviceFlags = false);
 
 }  // namespace mlx

@@ -377,18 +377,25 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   const int64_t nV = m.nV, nT = m.nT, ld = m.ld;
   const int ns = maps.nSorted;
   hipStream_t st = m.stream;
-  // elastic flags in slot order, one column per sorted layer
-  std::vector<uint8_t> flags((size_t)ns * ld, 0);
+  // elastic flags in slot order, one column per sorted layer (already on
+  // the device when the maps were built there)
+  std::vector<uint8_t> flags;
   std::vector<int64_t> nElastic(ns, 0), perColor((size_t)ns * m.numColors, 0);
-  for (int l = 0; l < ns; ++l)
-    for (int c = 0; c < m.numColors; ++c)
-      for (int64_t s = m.colorOff[c]; s < m.colorOff[c] + m.colorCnt[c]; ++s) {
-        const int32_t e = m.perm[s];
-        const uint8_t f = maps.elastic[(size_t)nT * l + e] ? 1 : 0;
-        flags[(size_t)ld * l + s] = f;
-        nElastic[l] += f;
-        perColor[(size_t)m.numColors * l + c] += f;
-      }
+  if (maps.deviceFlags) {
+    nElastic = maps.nElastic;
+    perColor = maps.perColor;
+  } else {
+    flags.assign((size_t)ns * ld, 0);
+    for (int l = 0; l < ns; ++l)
+      for (int c = 0; c < m.numColors; ++c)
+        for (int64_t s = m.colorOff[c]; s < m.colorOff[c] + m.colorCnt[c]; ++s) {
+          const int32_t e = m.perm[s];
+          const uint8_t f = maps.elastic[(size_t)nT * l + e] ? 1 : 0;
+          flags[(size_t)ld * l + s] = f;
+          nElastic[l] += f;
+          perColor[(size_t)m.numColors * l + c] += f;
+        }
+  }
   // layers with rigid elements and at most kSweepWg elastic elements in
   // every color sweep all colors in one workgroup (k_sweep_wg)
   std::vector<uint8_t> wgLayer(ns, 0);
@@ -411,9 +418,12 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     }
     MLX_CHECK_HIP(hipMemcpyAsync(m.d_elist.ptr, oc.data(), 16 * nc, hipMemcpyHostToDevice, st));
   }
-  MLX_TRY(m.d_flags.reserve(flags.size()));
+  if (!maps.deviceFlags) {
+    MLX_TRY(m.d_flags.reserve(flags.size()));
+    MLX_CHECK_HIP(hipMemcpyAsync(m.d_flags.ptr, flags.data(), flags.size(), hipMemcpyHostToDevice,
+                                 st));
+  }
   MLX_TRY(m.d_vbody.reserve(sizeof(int32_t) * maps.vbody.size()));
-  MLX_CHECK_HIP(hipMemcpyAsync(m.d_flags.ptr, flags.data(), flags.size(), hipMemcpyHostToDevice, st));
   MLX_CHECK_HIP(hipMemcpyAsync(m.d_vbody.ptr, maps.vbody.data(), sizeof(int32_t) * maps.vbody.size(),
                                hipMemcpyHostToDevice, st));
   int maxNb = 1;
@@ -1221,7 +1231,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_slotOf, &m->d_x, &m->d_xold, &m->d_rv, &m->d_pinnedx,
                          &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
-                         &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced,
+                         &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced, &m->d_ecolor,
                          &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
@@ -1532,7 +1542,8 @@ static int build_layer_maps(mlx_state &S, const double *sorted, int ns, double h
     hipLaunchKernelGGL(k_ediff2d<T>, dim3(blocks_for(nT)), dim3(kBlock), 0, m.stream, nT,
                        m.d_T.as<int32_t>(), m.d_dminv64.as<double>(), m.d_x.as<vec4_t<T>>(),
                        m.d_xold.as<vec4_t<T>>(), h, S.d_ediff.as<double>());
-  if (m.deviceMaps) return layer_maps_dev(m, S.d_ediff.as<double>(), sorted, ns, maps);
+  if (m.deviceMaps)
+    return layer_maps_dev(m, S.d_ediff.as<double>(), sorted, ns, maps, nullptr, true);
   S.h_ediff.resize(nT);
   MLX_CHECK_HIP(hipMemcpyAsync(S.h_ediff.data(), S.d_ediff.ptr, sizeof(double) * nT,
                                hipMemcpyDeviceToHost, m.stream));
