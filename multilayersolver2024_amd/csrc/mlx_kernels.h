@@ -1350,21 +1350,6 @@ __device__ __forceinline__ double residual_elem(const SweepParams<T> &P, int64_t
   return acc;
 }
 
-template <typename T, int DIM>
-__global__ __launch_bounds__(kBlock) void k_residual_elems(SweepParams<T> P, int64_t nSlots,
-                                                           const int32_t *lamSlot2d,
-                                                           double *partials) {
-  if (iteration_stopped(P.ctl)) return;
-  __shared__ double sh[kBlock];
-  const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  double acc = 0;
-  if (s < nSlots && P.vid[s] >= 0) acc = residual_elem<T, DIM>(P, s, lamSlot2d);
-  double o[1];
-  double v1[1] = {acc};
-  block_sum<1>(v1, sh, o);
-  if (threadIdx.x == 0) partials[blockIdx.x] = o[0];
-}
-
 // Contact part of the residual (xpbdLayer3D.m:573-583).  In 3D the
 // compliance enters as a 1x1xK page array times lambda(:), which Matlab
 // expands to K x 1 x K: every contact i contributes once per compliance
@@ -1397,24 +1382,6 @@ __device__ __forceinline__ double residual_contact_one(const ContactArrays &c, i
     acc += t * t;
   }
   return acc;
-}
-
-template <typename T, int DIM>
-__global__ __launch_bounds__(kBlock) void k_residual_contacts(ContactArrays c, const vec4_t<T> *x,
-                                                              double resistance, double h,
-                                                              const double *uniqAlpha,
-                                                              const double *uniqCount,
-                                                              double *partials, const Ctl *ctl) {
-  if (iteration_stopped(ctl)) return;
-  __shared__ double sh[kBlock];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  double acc = 0;
-  if (i < c.counts[0])
-    acc = residual_contact_one<T, DIM>(c, i, x, resistance, h, uniqAlpha, uniqCount, c.counts[3]);
-  double o[1];
-  double v1[1] = {acc};
-  block_sum<1>(v1, sh, o);
-  if (threadIdx.x == 0) partials[blockIdx.x] = o[0];
 }
 
 // simpleMonitorError (xpbdLayer3D.m:515-544; 2D xpbdLayer2D.m:520-549) on
@@ -1452,9 +1419,10 @@ __device__ __forceinline__ void monitor_update(Ctl &c, const MonitorParams &mp, 
   }
 }
 
-// The residual, the monitor and the iteration count of one projectGS
-// iteration in one launch (k_residual_elems + k_residual_contacts +
-// k_iter_end): blocks [0, nE) hold the element terms of their slots, the
+// The residual (simpleEvaluateAllConstraintResiduals, xpbdLayer3D.m:546-586;
+// 2D xpbdLayer2D.m:477-518), the monitor and the iteration count of one
+// projectGS iteration in one launch: blocks [0, nE) hold the element terms
+// of their slots (one thread per slot), the
 // rest the contact terms; each block stores its partial past the caches
 // (device-scope store), waits for it, and counts itself on *done.  The
 // block that counts last reads every partial the same way, sums them
