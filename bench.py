@@ -122,7 +122,7 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
     p0, v0, _ = b.download()
     frame0 = b.frame
     tune = {}
-    for blk in (256, 512, 1024):
+    for blk in (256, 512, 768, 1024):
         for split in (0, 8000, 25000):
             b.set_block(blk)
             b.set_split(split)

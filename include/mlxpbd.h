@@ -324,7 +324,7 @@ int mlx_batch_create(const mlx_mesh_desc *merged, int32_t nScenes,
                      const int64_t *sceneVertOff, const int64_t *sceneElemOff,
                      int32_t precision, mlx_batch **out);
 int mlx_batch_destroy(mlx_batch *batch);
-/* Threads per scene workgroup: 256, 512 (default) or 1024. */
+/* Threads per scene workgroup: 256, 512 (default), 768 or 1024. */
 int mlx_batch_set_block(mlx_batch *batch, int32_t threads);
 /* (Re)starts all scenes from p, v (3nV of the merged mesh). */
 int mlx_batch_set_state(mlx_batch *batch, const double *p, const double *v);

@@ -2185,6 +2185,7 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     maxSceneVerts = std::max<int64_t>(maxSceneVerts, b.vertOff[sc + 1] - b.vertOff[sc]);
   size_t ldsBytes = 0;
   B.ldsVerts = b.block == 256    ? batch_lds_verts<T, 256>(maxSceneVerts, b.useLds, &ldsBytes)
+               : b.block == 768  ? batch_lds_verts<T, 768>(maxSceneVerts, b.useLds, &ldsBytes)
                : b.block == 1024 ? batch_lds_verts<T, 1024>(maxSceneVerts, b.useLds, &ldsBytes)
                                  : batch_lds_verts<T, 512>(maxSceneVerts, b.useLds, &ldsBytes);
   b.ldsVerts = B.ldsVerts;
@@ -2222,7 +2223,8 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     // on one XCD; scenes go to XCDs largest first, each to the XCD with the
     // fewest tets so far; P = the largest XCD's workgroup count, and the grid
     // of 8 P workgroups must be resident (else the split is coarsened)
-    const void *fn = b.block == 256 ? reinterpret_cast<const void *>(&k_batch_split<T, 256>)
+    const void *fn = b.block == 256    ? reinterpret_cast<const void *>(&k_batch_split<T, 256>)
+                     : b.block == 768  ? reinterpret_cast<const void *>(&k_batch_split<T, 768>)
                      : b.block == 1024 ? reinterpret_cast<const void *>(&k_batch_split<T, 1024>)
                                        : reinterpret_cast<const void *>(&k_batch_split<T, 512>);
     int dev = 0;
@@ -2325,13 +2327,16 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
       ++b.splitFallbacks;
       MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
       if (b.block == 256) launch_batch<T, 256>(B, ns, ldsBytes, s);
-      else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
+      else if (b.block == 768) launch_batch<T, 768>(B, ns, ldsBytes, s);
+      else if (b.block == 768) launch_batch<T, 768>(B, ns, ldsBytes, s);
+    else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
       else launch_batch<T, 512>(B, ns, ldsBytes, s);
       MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
     }
   } else {
     MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
     if (b.block == 256) launch_batch<T, 256>(B, ns, ldsBytes, s);
+    else if (b.block == 768) launch_batch<T, 768>(B, ns, ldsBytes, s);
     else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
     else launch_batch<T, 512>(B, ns, ldsBytes, s);
     MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
@@ -2468,8 +2473,8 @@ int mlx_batch_set_split(mlx_batch *b, int32_t tetsPerWorkgroup) {
 }
 
 int mlx_batch_set_block(mlx_batch *b, int32_t threads) {
-  if (!b || (threads != 256 && threads != 512 && threads != 1024)) {
-    set_error("mlx_batch_set_block: threads must be 256, 512 or 1024");
+  if (!b || (threads != 256 && threads != 512 && threads != 768 && threads != 1024)) {
+    set_error("mlx_batch_set_block: threads must be 256, 512, 768 or 1024");
     return MLX_ERR_ARG;
   }
   b->block = threads;

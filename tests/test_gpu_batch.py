@@ -28,7 +28,8 @@ def _integ(iters=12, pct=0.0):
 
 
 @pytest.mark.parametrize("pct,block,split,lds", [(0.0, 256, 0, 1), (99.99, 1024, 0, 1),
-                                                 (99.99, 512, 0, 1), (0.0, 256, 0, 0),
+                                                 (99.99, 512, 0, 1), (99.99, 768, 0, 1),
+                                                 (0.0, 256, 0, 0),
                                                  (99.99, 1024, 0, 0), (0.0, 256, 40, 1),
                                                  (99.99, 512, 30, 1)])
 def test_batch_fp64_matches_oracle_per_scene(pct, block, split, lds, oracle_lib):
@@ -103,7 +104,8 @@ def test_batch_split_matches_one_workgroup_per_scene(precision):
 
 
 @pytest.mark.parametrize("precision,block", [(L.MLX_FP64, 512), (L.MLX_FP32, 512),
-                                             (L.MLX_FP32, 1024), (L.MLX_FP32, 256)])
+                                             (L.MLX_FP32, 1024), (L.MLX_FP32, 256),
+                                             (L.MLX_FP32, 768)])
 def test_batch_lds_positions_equal_hbm_positions(precision, block):
     """Scenes whose positions live in LDS for the whole solve run the same
     arithmetic in the same order as with positions in HBM: bit-identical
