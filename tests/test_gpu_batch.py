@@ -132,3 +132,34 @@ def test_batch_lds_positions_equal_hbm_positions(precision, block):
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("layout", ["single_scene", "with_one_cube_scenes"])
+def test_batch_small_scenes_match_oracle(layout, oracle_lib):
+    """A batch of one scene, and a batch mixing single-cube scenes (5 tets)
+    with a larger one: each scene equals the oracle run alone (FP64)."""
+    if layout == "single_scene":
+        scs = [scenes.beam3d(6, 2, 2, seed=6, z0=0.01)]
+    else:
+        scs = [scenes.beam3d(1, 1, 1, seed=7, z0=0.004), scenes.beam3d(8, 3, 2, seed=8, z0=0.02),
+               scenes.beam3d(1, 1, 1, seed=9, z0=0.03)]
+    meshes = [Mesh(s) for s in scs]
+    merged = merge_meshes(meshes)
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    it = _integ(pct=99.99)
+    b = BatchSimulation(merged, it, 1 / 60, plane, precision=L.MLX_FP64, block=256)
+    osims = [O.OracleSim(m, copy.deepcopy(it), 1 / 60, plane) for m in meshes]
+    try:
+        for k in range(6):
+            b.step()
+            res = [o.step() for o in osims]
+            p, v, lam = b.download()
+            for s, (o, r) in enumerate(zip(osims, res)):
+                v0, v1 = merged.vertOff[s], merged.vertOff[s + 1]
+                e0, e1 = merged.elemOff[s], merged.elemOff[s + 1]
+                assert b.iters[s] == sum(r["iters"])
+                assert b.contacts[s] == o.last_contacts
+                assert rel(p[3 * v0:3 * v1], o.p) < 1e-9
+                assert rel(lam[e0:e1], o.lambdas()) < 1e-9
+    finally:
+        b.close()

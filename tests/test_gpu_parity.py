@@ -59,6 +59,73 @@ def test_beam_fp64_matches_oracle(steps, oracle_lib):
         sim.close()
 
 
+@pytest.mark.parametrize("case", ["one_cube", "no_pins_no_contacts", "all_rigid_top"])
+def test_small_and_degenerate_scenes_fp64_match_oracle(case, oracle_lib):
+    """Boundary cases of the substep: a single cube (5 tets, one color
+    each) that falls as one rigid body, lands and bounces; a free-falling
+    beam with no pinned vertex and no contact; a pinned beam whose coarsest
+    layer makes every insertable element rigid (100 %) before it hits the
+    ground.
+
+    The layer stacks are chosen so that the substep is well conditioned:
+    the oracle moves by about 1e-15 (relative) after 8 substeps when its
+    start positions are perturbed by 1e-15.  A stack that sorts a partial
+    layer (50 %, 87.5 %) over a mesh at rest orders strain-rate keys that
+    are pure rounding noise, so rounding-level differences between two
+    correct implementations change the rigid sets (the oracle alone then
+    moves by 1e-8 after two substeps and by 1e-1 after a cube lands); such
+    stacks are covered from bit-identical start states by the beam cases.
+    λ of an undeformed tet is rounding noise (|λ| ~ 1e-19 here, against
+    1e-4 .. 1e-1 for a deformed one), so λ is compared with an absolute
+    floor of 1e-15 besides the 1e-9 relative bound."""
+    import copy
+    from multilayersolver2024_amd import scenes
+    from multilayersolver2024_amd.integrator import PlaneContactFinder, XpbdLayer3D
+    from multilayersolver2024_amd.mesh import Mesh
+    if case == "one_cube":
+        sc = scenes.beam3d(1, 1, 1, seed=3, z0=0.004)
+        sc["pinned"] = np.zeros(0, np.int32)
+        layers, iters = (100.0, 0), (3, 14)
+    elif case == "no_pins_no_contacts":
+        sc = scenes.beam3d(6, 2, 2, seed=4, z0=5.0)
+        sc["pinned"] = np.zeros(0, np.int32)
+        layers, iters = (98.4375, 87.5, 0), (2, 4, 14)
+    else:
+        sc = scenes.beam3d(6, 3, 2, seed=5, z0=0.004)
+        layers, iters = (100.0, 0), (3, 10)
+    m = Mesh(sc)
+    it = XpbdLayer3D()
+    it.Gravity = -9.81
+    it.setLayers(list(layers))
+    it.iterations = list(iters)
+    it.computeResiduals = True
+    it.StopAtImprovementPercent = 99.99
+    it.hangingStop = 200
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    sim = Simulation(m, it, 1 / 60, plane, precision=L.MLX_FP64)
+    osim = O.OracleSim(m, copy.deepcopy(it), 1 / 60, plane)
+    contacts = 0
+    try:
+        for k in range(8):
+            r = sim.step()
+            o = osim.step()
+            p, v, lam, K = sim.download()
+            olam = osim.lambdas()
+            dl = float(np.linalg.norm(lam - olam))
+            _say(case, k, r.itersPerLayer, o["iters"], K, osim.last_contacts, rel(p, osim.p), dl)
+            assert K == osim.last_contacts
+            assert list(r.itersPerLayer) == list(o["iters"])
+            assert rel(p, osim.p) < 1e-9
+            assert dl <= 1e-9 * np.linalg.norm(olam) + 1e-15
+            contacts += K
+        if case == "no_pins_no_contacts":
+            assert contacts == 0
+        else:
+            assert contacts > 0
+    finally:
+        sim.close()
+
+
 def test_cantilever2d_fp64_200_steps(oracle_lib):
     m, integ = cantilever_case()
     sim = Simulation(m, integ, 0.01, None, precision=L.MLX_FP64)
