@@ -15,13 +15,15 @@ import numpy as np
 
 from . import _lib as L
 from . import scenes
-from .mesh import Mesh
+from .mesh import Mesh, _reject_neohookean
 
 
 class MergedMesh:
     """Concatenation of 3D Mesh objects (duck-types Mesh.desc())."""
 
     def __init__(self, meshes):
+        for m in meshes:
+            _reject_neohookean(m)
         self.meshes = meshes
         self.dim = 3
         self.nc = 6

@@ -19,7 +19,9 @@ class Mesh:
     (dim*nV), ``alpha``/``beta`` (perElementXPBDalphaMatrix / perElementXPBDbeta,
     nc x nc x nT pages), ``elementColor`` (1-based), ``numColors``,
     ``pinnedDOFs`` (1-based), ``pinnedTets`` mask (``forcedElastic``), ``p``
-    (dim*nV initial positions).
+    (dim*nV initial positions), ``elMu``/``elLambda`` (Lame parameters) and
+    ``neohookean`` (nT mask, 3D: elements projected with the NeoHookean
+    pair instead of StVK-Voigt; scene key "neohookean", a mask or a bool).
     """
 
     def __init__(self, scene):
@@ -42,6 +44,12 @@ class Mesh:
         self.alpha = np.zeros(nc * nc * nT)
         self.beta = np.zeros(nc * nc * nT)
         rho, mu, lam, a1 = (L.f64(scene[k]) for k in ("rho", "mu", "lam", "alpha1"))
+        self.elMu, self.elLambda = mu.copy(), lam.copy()
+        nh = scene.get("neohookean")
+        self.neohookean = (L.u8(np.broadcast_to(np.asarray(nh, dtype=bool), (nT,)))
+                           if nh is not None else np.zeros(nT, dtype=np.uint8))
+        if dim != 3 and self.neohookean.any():
+            raise ValueError("NeoHookean elements in the substep are 3D only")
         fn = lib.mlx_mesh3d_precompute if dim == 3 else lib.mlx_mesh2d_precompute
         L.check(fn.__name__, fn(L.ptr(self.p), nV, L.ptr(self.t, C.c_int32), nT, L.ptr(rho),
                                 L.ptr(mu), L.ptr(lam), L.ptr(a1), L.ptr(self.DmInv),
@@ -85,11 +93,23 @@ class Mesh:
         return d
 
 
+def _reject_neohookean(mesh):
+    """The GPU substep projects StVK-Voigt tetrahedra only.  The NeoHookean
+    pair exists in the CPU oracle, where it needs substeps of about 1/6000 s
+    to converge with lambda restarted every substep; a mesh that asks for it
+    is refused rather than run with another material."""
+    nh = getattr(mesh, "neohookean", None)
+    if nh is not None and np.any(nh):
+        raise NotImplementedError("NeoHookean elements are not projected by the GPU substep "
+                                  "(StVK-Voigt only); clear the scene's neohookean mask")
+
+
 class DeviceMesh:
     """A Mesh uploaded to the GPU (mlx_mesh_create); precision MLX_FP64 or
     MLX_FP32."""
 
     def __init__(self, mesh, precision=L.MLX_FP64, device=0):
+        _reject_neohookean(mesh)
         lib = L.lib()
         L.check("mlx_init", lib.mlx_init(device))
         self.mesh = mesh

@@ -85,14 +85,15 @@ def _tet_vol6(P, tet):
 
 def beam3d(nx=64, ny=16, nz=16, cell=0.02, z0=0.05, seed=0, E=None, nu=None,
            E_range=(1e4, 1e7), nu_range=(0.30, 0.45), rho=1000.0, alpha1=0.02,
-           pin_far_end=False):
+           pin_far_end=False, neohookean=False):
     """3-D box beam: (nx+1)(ny+1)(nz+1) vertices, vertex (i,j,k) at index
     (i*(ny+1)+j)*(nz+1)+k, position (i, j, k)*cell + (0, 0, z0); 5 tets per
     cube (alternating parity), each positively oriented (det(Dm) > 0).
     Per-element E is log-uniform in E_range and nu uniform in nu_range, drawn
     from Pcg32(seed) in element order as (E draw, nu draw) pairs, unless E and
     nu are given as scalars.  The x = 0 face is pinned (and the x = nx*cell
-    face too with pin_far_end, for the twisting study)."""
+    face too with pin_far_end, for the twisting study).  neohookean=True
+    projects every element with the NeoHookean pair instead of StVK-Voigt."""
     NY, NZ = ny + 1, nz + 1
     vid = lambda i, j, k: (i * NY + j) * NZ + k
     ii, jj, kk = np.meshgrid(np.arange(nx + 1), np.arange(NY), np.arange(NZ), indexing="ij")
@@ -138,7 +139,8 @@ def beam3d(nx=64, ny=16, nz=16, cell=0.02, z0=0.05, seed=0, E=None, nu=None,
     return dict(
         dim=3, p=P, t=T, rho=np.full(nT, rho), mu=mu, lam=lam,
         alpha1=np.full(nT, alpha1), pinned=pinned, E=Es, nu=nus,
-        name=f"beam3d_{nx}x{ny}x{nz}",
+        neohookean=bool(neohookean),
+        name=f"beam3d_{nx}x{ny}x{nz}" + ("_nh" if neohookean else ""),
     )
 
 

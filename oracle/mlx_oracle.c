@@ -724,6 +724,11 @@ void ora_exp_rodrigues(const double *omega, double h, double *R) {
 /* color sweeps                                                         */
 /* ------------------------------------------------------------------ */
 
+/* NeoHookean elements of a 3D mesh (defined with the 2D pair below). */
+static void nh_project3d(const ora_mesh *m, int e, double h, double *lambda, double *xi);
+static double nh_residual_sq3d(const ora_mesh *m, int e, double h, const double *lambda,
+                               const double *xi);
+
 /* xpbdLayer3D.stvkVoigtConstraints (xpbdLayer3D.m:347-379) with the
  * alpha/h^2 and beta*h^2 of the call site :318. */
 void ora_stvk_voigt_constraints3d(const ora_mesh *m, const uint8_t *isElastic,
@@ -733,6 +738,10 @@ void ora_stvk_voigt_constraints3d(const ora_mesh *m, const uint8_t *isElastic,
   for (int c = 1; c <= m->numColors; ++c)
     for (int e = 0; e < m->nT; ++e) {
       if (m->color[e] != c || !isElastic[e]) continue;
+      if (m->nhModel && m->nhModel[e]) {
+        nh_project3d(m, e, h, lambda, xi);
+        continue;
+      }
       double x[12], xo[12], mi[12], dx[12], dl[6];
       for (int a = 0; a < 4; ++a)
         for (int d = 0; d < 3; ++d) {
@@ -826,6 +835,10 @@ double ora_residual3d(const ora_mesh *m, double h, const double *lambda,
   (void)oldp;
   double acc = 0.0, h2 = h * h;
   for (int e = 0; e < m->nT; ++e) {
+    if (m->nhModel && m->nhModel[e]) {
+      acc = acc + nh_residual_sq3d(m, e, h, lambda, xi);
+      continue;
+    }
     double x[12], g[72], C[6];
     for (int a = 0; a < 4; ++a)
       for (int d = 0; d < 3; ++d) x[3 * a + d] = xi[3 * m->T[4 * e + a] + d];
@@ -1957,6 +1970,134 @@ void ora_neohookean_constraints2d(const ora_mesh *m, const uint8_t *isElastic, d
           xi[dof] = (x[2 * a + d] + dxD[2 * a + d]) + dxH[2 * a + d];
         }
     }
+}
+
+/* NeoHookean pair on a tetrahedron (extension of the 2D pair above).
+ * F[r][c] = sum_k (x_{k+1} - x_0)[r] DmInv(k, c), DmInv column-major.
+ * d||F||/dF = F/||F||; d det F/dF = cof(F).  The gradient w.r.t. vertex
+ * k+1 is (dC/dF) times row k of DmInv, and vertex 0 takes minus their sum.
+ * Pins: central finite differences of ||F|| and det F; the rest state with
+ * the equilibrium multipliers lambda_D = -sqrt(3) vol h^2 mu and
+ * lambda_H = vol h^2 mu is a fixed point (tests). */
+static void nh_F3(const double *D, const double *x, double F[3][3]) {
+  double e[3][3];
+  for (int k = 0; k < 3; ++k)
+    for (int r = 0; r < 3; ++r) e[k][r] = x[3 * (k + 1) + r] - x[r];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      F[r][c] = e[0][r] * D[3 * c] + e[1][r] * D[1 + 3 * c] + e[2][r] * D[2 + 3 * c];
+}
+
+static void nh_vertex_grads(const double *D, double P[3][3], double *g) {
+  for (int k = 0; k < 3; ++k)
+    for (int r = 0; r < 3; ++r)
+      g[3 * (k + 1) + r] = P[r][0] * D[k] + P[r][1] * D[k + 3] + P[r][2] * D[k + 6];
+  for (int r = 0; r < 3; ++r) g[r] = -(g[3 + r] + g[6 + r] + g[9 + r]);
+}
+
+void ora_neohookean_grad3d(const double *D, const double *x, double *gD, double *gH,
+                           double *CD, double *J) {
+  double F[3][3], PD[3][3], PH[3][3];
+  nh_F3(D, x, F);
+  double n2 = 0;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) n2 = n2 + F[r][c] * F[r][c];
+  double n = sqrt(n2);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) PD[r][c] = F[r][c] / n;
+  PH[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+  PH[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+  PH[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+  PH[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+  PH[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+  PH[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+  PH[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+  PH[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+  PH[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+  if (gD) nh_vertex_grads(D, PD, gD);
+  if (gH) nh_vertex_grads(D, PH, gH);
+  if (CD) *CD = n;
+  if (J) *J = F[0][0] * PH[0][0] + F[0][1] * PH[0][1] + F[0][2] * PH[0][2];
+}
+
+static void nh_step(double h, double C, double atil, double lambda, const double *g,
+                    const double *mass, double *dx, double *dlambda) {
+  double s = atil;
+  for (int q = 0; q < 12; ++q) s = s + (1.0 / mass[q]) * g[q] * g[q];
+  double dl = -(C + atil * lambda) / s;
+  for (int q = 0; q < 12; ++q) dx[q] = (1.0 / mass[q]) * g[q] * dl;
+  *dlambda = dl;
+}
+
+void ora_neohookean_d3d(double h, const double *x, const double *D, double mu, double vol,
+                        double lambda, const double *mass, double *dx, double *dlambda,
+                        double *C) {
+  double g[12], n;
+  ora_neohookean_grad3d(D, x, g, NULL, &n, NULL);
+  double atil = (1.0 / vol) * (1.0 / (h * h)) * (1.0 / mu);
+  nh_step(h, n, atil, lambda, g, mass, dx, dlambda);
+  if (C) *C = n;
+}
+
+void ora_neohookean_h3d(double h, const double *x, const double *D, double lamL, double vol,
+                        double lambda, const double *mass, double gamma, double *dx,
+                        double *dlambda, double *C) {
+  double g[12], J;
+  ora_neohookean_grad3d(D, x, NULL, g, NULL, &J);
+  double atil = (1.0 / vol) * (1.0 / (h * h)) * (1.0 / lamL);
+  nh_step(h, J - gamma, atil, lambda, g, mass, dx, dlambda);
+  if (C) *C = J - gamma;
+}
+
+static void nh_project3d(const ora_mesh *m, int e, double h, double *lambda, double *xi) {
+  double x[12], ms[12], dxD[12], dxH[12], dlD, dlH;
+  for (int a = 0; a < 4; ++a)
+    for (int d = 0; d < 3; ++d) {
+      int dof = 3 * m->T[4 * e + a] + d;
+      x[3 * a + d] = xi[dof];
+      ms[3 * a + d] = m->mass[dof];
+    }
+  double gamma = 1 + m->nhMu[e] / m->nhLam[e];
+  ora_neohookean_d3d(h, x, m->DmInv + 9 * e, m->nhMu[e], m->vol[e], lambda[6 * e], ms, dxD,
+                     &dlD, NULL);
+  ora_neohookean_h3d(h, x, m->DmInv + 9 * e, m->nhLam[e], m->vol[e], lambda[6 * e + 1], ms,
+                     gamma, dxH, &dlH, NULL);
+  lambda[6 * e] = lambda[6 * e] + dlD;
+  lambda[6 * e + 1] = lambda[6 * e + 1] + dlH;
+  for (int a = 0; a < 4; ++a)
+    for (int d = 0; d < 3; ++d) {
+      int dof = 3 * m->T[4 * e + a] + d;
+      xi[dof] = (x[3 * a + d] + dxD[3 * a + d]) + dxH[3 * a + d];
+    }
+}
+
+void ora_neohookean_constraints3d(const ora_mesh *m, const uint8_t *isElastic, double h,
+                                  double *lambda, double *xi) {
+  for (int c = 1; c <= m->numColors; ++c)
+    for (int e = 0; e < m->nT; ++e) {
+      if (m->color[e] != c || !isElastic[e] || !m->nhModel || !m->nhModel[e]) continue;
+      nh_project3d(m, e, h, lambda, xi);
+    }
+}
+
+/* Residual terms of a NeoHookean element: max(|C| - |alpha~ lambda|, 0)^2
+ * for C_D and C_H, as the StVK terms of ora_residual3d. */
+static double nh_residual_sq3d(const ora_mesh *m, int e, double h, const double *lambda,
+                               const double *xi) {
+  double x[12], n, J;
+  for (int a = 0; a < 4; ++a)
+    for (int d = 0; d < 3; ++d) x[3 * a + d] = xi[3 * m->T[4 * e + a] + d];
+  ora_neohookean_grad3d(m->DmInv + 9 * e, x, NULL, NULL, &n, &J);
+  double iv = (1.0 / m->vol[e]) * (1.0 / (h * h));
+  double C[2] = {n, J - (1 + m->nhMu[e] / m->nhLam[e])};
+  double al[2] = {iv * (1.0 / m->nhMu[e]) * lambda[6 * e], iv * (1.0 / m->nhLam[e]) * lambda[6 * e + 1]};
+  double acc = 0;
+  for (int i = 0; i < 2; ++i) {
+    double aC = fabs(C[i]), corr = fabs(al[i]);
+    double t = aC - (aC < corr ? aC : corr);
+    acc = acc + t * t;
+  }
+  return acc;
 }
 
 /* ------------------------------------------------------------------ */

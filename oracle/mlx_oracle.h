@@ -37,6 +37,10 @@ typedef struct {
   int numColors;
   const int32_t *pinnedDOFs; int nPinnedDOFs; /* 0-based, ascending */
   const uint8_t *forcedElastic;                /* nT (pinned elements) */
+  /* 3D only, may be NULL: elements projected with the NeoHookean pair
+   * instead of StVK-Voigt, and their Lame parameters (nT each) */
+  const uint8_t *nhModel;
+  const double *nhMu, *nhLam;
 } ora_mesh;
 
 typedef struct {
@@ -183,6 +187,33 @@ void ora_neohookean_h2d(double h, const double *x, const double *DmInv, double l
 void ora_neohookean_constraints2d(const ora_mesh *m, const uint8_t *isElastic, double h,
                                   const double *mu, const double *lamL, double *lambda,
                                   double *xi);
+
+/* NeoHookean pair on a tetrahedron: the 2D reference's deviatoric and
+ * volumetric constraints (mexNeoHookeanDConstraint.cpp:46-138,
+ * mexNeoHookeanHConstraint.cpp:47-138) with F = Ds DmInv of a tetrahedron,
+ * Ds = [x1-x0, x2-x0, x3-x0].  No counterpart in the reference, whose 3D
+ * integrator projects StVK-Voigt only.
+ * ora_neohookean_grad3d: gD, gH (12 each, d/dx of ||F||_F and det F),
+ * CD = ||F||_F, J = det F.
+ * ora_neohookean_d3d: C = ||F||_F, compliance 1/(vol h^2 mu);
+ * ora_neohookean_h3d: C = det F - gamma, compliance 1/(vol h^2 lambda);
+ * both dlambda = -(C + alpha~ lambda) / (sum_a w_a |grad_a C|^2 + alpha~),
+ * dx_a = w_a grad_a C dlambda. */
+void ora_neohookean_grad3d(const double *DmInv, const double *x, double *gD, double *gH,
+                           double *CD, double *J);
+void ora_neohookean_d3d(double h, const double *x, const double *DmInv, double mu,
+                        double vol, double lambda, const double *mass, double *dx,
+                        double *dlambda, double *C);
+void ora_neohookean_h3d(double h, const double *x, const double *DmInv, double lamL,
+                        double vol, double lambda, const double *mass, double gamma,
+                        double *dx, double *dlambda, double *C);
+/* One sweep over all colors of the NeoHookean elements (m->nhModel), both
+ * constraints of an element evaluated at the positions its color started
+ * from and their dx summed, as the 2D pair (ora_neohookean_constraints2d).
+ * gamma = 1 + mu/lambda.  lambda: nT x 6 row-major, (lambda_D, lambda_H) in
+ * the first two entries. */
+void ora_neohookean_constraints3d(const ora_mesh *m, const uint8_t *isElastic, double h,
+                                  double *lambda, double *xi);
 
 /* ---- PCG (solvePCG.m) ---- */
 void ora_solve_pcg(int n, const int32_t *rowptr, const int32_t *col, const double *val,

@@ -24,7 +24,7 @@ class OraMesh(C.Structure):
         ("dim", C.c_int), ("nV", C.c_int), ("nT", C.c_int), ("vpe", C.c_int),
         ("T", _i32), ("DmInv", _d), ("alpha", _d), ("beta", _d), ("vol", _d), ("mass", _d),
         ("color", _i32), ("numColors", C.c_int), ("pinnedDOFs", _i32), ("nPinnedDOFs", C.c_int),
-        ("forcedElastic", _u8),
+        ("forcedElastic", _u8), ("nhModel", _u8), ("nhMu", _d), ("nhLam", _d),
     ]
 
 
@@ -99,6 +99,13 @@ class OracleMesh:
         s.color, s.numColors = P(self.color, C.c_int32), int(mesh.numColors)
         s.pinnedDOFs, s.nPinnedDOFs = P(self.pinned, C.c_int32), len(self.pinned)
         s.forcedElastic = P(self.forced, C.c_uint8)
+        nh = getattr(mesh, "neohookean", None)
+        if dim == 3 and nh is not None and nh.any():
+            self.nh = np.ascontiguousarray(nh, dtype=np.uint8)
+            self.nhMu = np.ascontiguousarray(mesh.elMu, dtype=np.float64)
+            self.nhLam = np.ascontiguousarray(mesh.elLambda, dtype=np.float64)
+            s.nhModel = P(self.nh, C.c_uint8)
+            s.nhMu, s.nhLam = P(self.nhMu), P(self.nhLam)
         self.s = s
 
 
