@@ -221,4 +221,10 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
                    HostMaps &maps, std::vector<int32_t> *setsOut = nullptr,
                    bool deviceFlags = false);
 
+// Ordered list of the flagged entries of flag[0..n) (int32 0/1) on stream
+// st: list[k] = the k-th flagged index in ascending order, lambda[k] = 0,
+// count[0] = number flagged.  pos: n int32 of scratch (the exclusive scan).
+int compact_flags_dev(mlx_mesh &m, const int32_t *flag, int64_t n, int32_t *pos, int32_t *list,
+                      double *lambda, int32_t *count, hipStream_t st);
+
 }  // namespace mlx

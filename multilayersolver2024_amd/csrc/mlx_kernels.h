@@ -1694,11 +1694,10 @@ __global__ void k_velocity(int64_t nV, const vec4_t<T> *x, const vec4_t<T> *xold
   if (DIM == 3) v[DIM * i + 2] = (T)(((double)p.z - (double)o.z) / h);
 }
 
-// contactVelocityUpdate (xpbdLayer2D.m:577-603).
+// contactVelocityUpdate (xpbdLayer2D.m:577-603), contact i.
 template <typename T, int DIM>
-__global__ void k_contact_velocity(ContactArrays c, T *v, double h, double g, double e) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= c.K) return;
+__device__ __forceinline__ void contact_velocity_one(const ContactArrays &c, int i, T *v, double h,
+                                                     double g, double e) {
   const int id = c.vert[i];
   double vn[3] = {0, 0, 0}, nrm = 0;
   for (int d = 0; d < DIM; ++d) {
@@ -1713,6 +1712,22 @@ __global__ void k_contact_velocity(ContactArrays c, T *v, double h, double g, do
   double mn = -ei * vt;
   if (mn > 0) mn = 0;
   for (int d = 0; d < DIM; ++d) v[DIM * id + d] = (T)(vn[d] + c.normal[3 * i + d] * (-vn[d] + mn));
+}
+
+template <typename T, int DIM>
+__global__ void k_contact_velocity(ContactArrays c, T *v, double h, double g, double e) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= c.K) return;
+  contact_velocity_one<T, DIM>(c, i, v, h, g, e);
+}
+
+// The same with the contact count read from the device (c.counts[0]); the
+// grid covers every possible contact.
+template <typename T, int DIM>
+__global__ void k_contact_velocity_dev(ContactArrays c, T *v, double h, double g, double e) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= c.counts[0]) return;
+  contact_velocity_one<T, DIM>(c, i, v, h, g, e);
 }
 
 // Plane contact test (PlaneContactFinder3D.m:40-43; 2D strict).

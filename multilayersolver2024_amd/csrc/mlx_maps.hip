@@ -417,4 +417,40 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
   return MLX_OK;
 }
 
+namespace {
+
+// Ordered compaction: flagged vertex v goes to slot pos[v] (pos = exclusive
+// scan of the flags), so the list ascends in vertex id as a host loop over
+// the flags would build it; its multiplier starts at 0.  The last vertex
+// writes the count.
+__global__ void k_compact_flags(int64_t n, const int32_t *flag, const int32_t *pos,
+                                int32_t *list, double *lambda, int32_t *count) {
+  const int64_t v = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (v >= n) return;
+  if (flag[v]) {
+    list[pos[v]] = (int32_t)v;
+    lambda[pos[v]] = 0.0;
+  }
+  if (v == n - 1) count[0] = pos[v] + flag[v];
+}
+
+}  // namespace
+
+int compact_flags_dev(mlx_mesh &m, const int32_t *flag, int64_t n, int32_t *pos, int32_t *list,
+                      double *lambda, int32_t *count, hipStream_t st) {
+  if (n <= 0) {
+    MLX_CHECK_HIP(hipMemsetAsync(count, 0, sizeof(int32_t), st));
+    return MLX_OK;
+  }
+  size_t tmp = 0;
+  MLX_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp, flag, pos, 0, (size_t)n,
+                                        rocprim::plus<int32_t>(), st));
+  MLX_TRY(m.d_scan.reserve(std::max<size_t>(tmp, 256)));
+  MLX_CHECK_HIP(rocprim::exclusive_scan(m.d_scan.ptr, tmp, flag, pos, 0, (size_t)n,
+                                        rocprim::plus<int32_t>(), st));
+  hipLaunchKernelGGL(k_compact_flags, dim3(grid_for(n)), dim3(kMapBlock), 0, st, n, flag, pos,
+                     list, lambda, count);
+  return MLX_OK;
+}
+
 }  // namespace mlx

@@ -1501,11 +1501,10 @@ __global__ void k_reset_oldp(int64_t nV, const int *flag, int force, const vec4_
 
 // Contact arrays of the plane finder: pointCollider = p - n * distance.
 template <typename T, int DIM>
-__global__ void k_plane_gather(int K, const int32_t *vert, const vec4_t<T> *x, double n0,
-                               double n1, double n2, double D, double compliance,
-                               double *normal, double *point, double *compl_) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= K) return;
+__device__ __forceinline__ void plane_gather_one(int i, const int32_t *vert, const vec4_t<T> *x,
+                                                 double n0, double n1, double n2, double D,
+                                                 double compliance, double *normal, double *point,
+                                                 double *compl_) {
   const vec4_t<T> p = x[vert[i]];
   const double xs[3] = {(double)p.x, (double)p.y, DIM == 3 ? (double)p.z : 0.0};
   const double nn[3] = {n0, n1, DIM == 3 ? n2 : 0.0};
@@ -1517,6 +1516,25 @@ __global__ void k_plane_gather(int K, const int32_t *vert, const vec4_t<T> *x, d
     point[3 * i + d] = d < DIM ? xs[d] - nn[d] * dist : 0.0;
   }
   compl_[i] = compliance;
+}
+
+template <typename T, int DIM>
+__global__ void k_plane_gather(int K, const int32_t *vert, const vec4_t<T> *x, double n0,
+                               double n1, double n2, double D, double compliance,
+                               double *normal, double *point, double *compl_) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= K) return;
+  plane_gather_one<T, DIM>(i, vert, x, n0, n1, n2, D, compliance, normal, point, compl_);
+}
+
+// The same with the contact count read from the device.
+template <typename T, int DIM>
+__global__ void k_plane_gather_dev(const int32_t *count, const int32_t *vert, const vec4_t<T> *x,
+                                   double n0, double n1, double n2, double D, double compliance,
+                                   double *normal, double *point, double *compl_) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= count[0]) return;
+  plane_gather_one<T, DIM>(i, vert, x, n0, n1, n2, D, compliance, normal, point, compl_);
 }
 
 // The layer maps of integrate(): strain-rate order (device EDiff, host
@@ -2005,6 +2023,7 @@ struct mlx_batch {
   int64_t splitFallbacks = 0;  // split launches redone unsplit (XCD placement differed)
   int useLds = 1;              // scenes that fit keep their positions in LDS (k_batch_plain)
   int ldsVerts = 0;            // vertex limit of the LDS path in the last launch
+  bool tablesReady = false;    // static part of d_tables uploaded
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -2072,6 +2091,16 @@ static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
   return (int)nv;
 }
 
+// First contact of each scene: the number of flagged vertices before the
+// scene's first vertex (pos: exclusive scan of the plane flags); entry ns
+// is the total.
+__global__ void k_scene_contacts(int ns, const int64_t *vertOff, int64_t nV, const int32_t *pos,
+                                 const int32_t *count, int64_t *cont0) {
+  const int sc = blockIdx.x * kBlock + threadIdx.x;
+  if (sc > ns) return;
+  cont0[sc] = (sc == ns || vertOff[sc] >= nV) ? (int64_t)count[0] : (int64_t)pos[vertOff[sc]];
+}
+
 template <typename T>
 static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int frame,
                                 const mlx_plane *plane, mlx_batch_out *out) {
@@ -2099,67 +2128,61 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     hipLaunchKernelGGL(k_any_nan<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, xo, flag);
   hipLaunchKernelGGL((k_reset_oldp<T, 3>), dim3(blocks_for(std::max(nV, nPin))), dim3(kBlock), 0, s,
                      nV, flag, frame == 1, x, xo, nPin, m.d_pinned.as<int32_t>(), m.d_pinnedx.as<T>());
-  // contacts
-  std::vector<int32_t> cv;
-  double nrm[3] = {0, 0, 0}, Dp = 0;
-  if (plane && plane->enabled) {
-    double nn = 0;
-    for (int d = 0; d < 3; ++d) nn += plane->normal[d] * plane->normal[d];
-    nn = std::sqrt(nn);
-    for (int d = 0; d < 3; ++d) {
-      nrm[d] = plane->normal[d] / nn;
-      Dp += plane->normal[d] * plane->position[d];
-    }
-    MLX_TRY(b.d_flags.reserve(sizeof(int32_t) * nV));
-    hipLaunchKernelGGL((k_plane_flags<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
-                       nrm[0], nrm[1], nrm[2], Dp, 0, b.d_flags.as<int32_t>());
-    std::vector<int32_t> hf(nV);
-    MLX_CHECK_HIP(hipMemcpyAsync(hf.data(), b.d_flags.ptr, sizeof(int32_t) * nV,
-                                 hipMemcpyDeviceToHost, s));
-    MLX_CHECK_HIP(hipStreamSynchronize(s));
-    for (int64_t v = 0; v < nV; ++v)
-      if (hf[v]) cv.push_back((int32_t)v);
-  }
-  const int K = (int)cv.size();
-  b.cont0.assign(b.nScenes + 1, 0);
-  {
-    size_t q = 0;
-    for (int sc = 0; sc < b.nScenes; ++sc) {
-      b.cont0[sc] = (int64_t)q;
-      while (q < cv.size() && cv[q] < b.vertOff[sc + 1]) ++q;
-    }
-    b.cont0[b.nScenes] = K;
-  }
-  MLX_TRY(reserve_contacts(m));
-  MLX_CHECK_HIP(hipStreamSynchronize(s));
-  ContactArrays ca = contact_arrays(m, K);
-  if (K > 0) {
-    MLX_CHECK_HIP(hipMemcpy(const_cast<int32_t *>(ca.vert), cv.data(), 4 * (size_t)K,
-                            hipMemcpyHostToDevice));
-    hipLaunchKernelGGL((k_plane_gather<T, 3>), dim3(blocks_for(K)), dim3(kBlock), 0, s, K, ca.vert,
-                       x, nrm[0], nrm[1], nrm[2], Dp, plane->compliance,
-                       const_cast<double *>(ca.normal), const_cast<double *>(ca.point),
-                       const_cast<double *>(ca.compliance));
-    MLX_CHECK_HIP(hipMemsetAsync(ca.lambda, 0, sizeof(double) * K, s));
-  }
-  // per-scene tables: color0 | ncolor | colorOff | colorCnt | slot0 | cont0 | pin0 | vert0
+  // per-scene tables: color0 | ncolor | colorOff | colorCnt | slot0 | cont0 | pin0 | vert0.
+  // All but cont0 are fixed for the batch and uploaded once; cont0 is
+  // written on the device every substep.
   const int ns = b.nScenes, ncT = (int)m.colorCnt.size();
   const size_t o_color0 = 0, o_ncolor = o_color0 + 4 * (size_t)ns,
                o_coff = (o_ncolor + 4 * (size_t)ns + 7) / 8 * 8, o_ccnt = o_coff + 8 * (size_t)ncT,
                o_slot = o_ccnt + 8 * (size_t)ncT, o_cont = o_slot + 8 * (size_t)(ns + 1),
                o_pin = o_cont + 8 * (size_t)(ns + 1), o_vert = o_pin + 8 * (size_t)(ns + 1),
                o_end = o_vert + 8 * (size_t)(ns + 1);
-  std::vector<char> tab(o_end);
-  std::memcpy(tab.data() + o_color0, m.sceneColor0.data(), 4 * (size_t)ns);
-  std::memcpy(tab.data() + o_ncolor, m.sceneNColors.data(), 4 * (size_t)ns);
-  std::memcpy(tab.data() + o_coff, m.colorOff.data(), 8 * (size_t)ncT);
-  std::memcpy(tab.data() + o_ccnt, m.colorCnt.data(), 8 * (size_t)ncT);
-  std::memcpy(tab.data() + o_slot, m.sceneSlot0.data(), 8 * (size_t)(ns + 1));
-  std::memcpy(tab.data() + o_cont, b.cont0.data(), 8 * (size_t)(ns + 1));
-  std::memcpy(tab.data() + o_pin, b.pin0.data(), 8 * (size_t)(ns + 1));
-  std::memcpy(tab.data() + o_vert, b.vertOff.data(), 8 * (size_t)(ns + 1));
-  MLX_TRY(b.d_tables.reserve(o_end));
-  MLX_CHECK_HIP(hipMemcpy(b.d_tables.ptr, tab.data(), o_end, hipMemcpyHostToDevice));
+  if (!b.tablesReady) {
+    std::vector<char> tab(o_end, 0);
+    std::memcpy(tab.data() + o_color0, m.sceneColor0.data(), 4 * (size_t)ns);
+    std::memcpy(tab.data() + o_ncolor, m.sceneNColors.data(), 4 * (size_t)ns);
+    std::memcpy(tab.data() + o_coff, m.colorOff.data(), 8 * (size_t)ncT);
+    std::memcpy(tab.data() + o_ccnt, m.colorCnt.data(), 8 * (size_t)ncT);
+    std::memcpy(tab.data() + o_slot, m.sceneSlot0.data(), 8 * (size_t)(ns + 1));
+    std::memcpy(tab.data() + o_pin, b.pin0.data(), 8 * (size_t)(ns + 1));
+    std::memcpy(tab.data() + o_vert, b.vertOff.data(), 8 * (size_t)(ns + 1));
+    MLX_TRY(b.d_tables.reserve(o_end));
+    MLX_CHECK_HIP(hipMemcpy(b.d_tables.ptr, tab.data(), o_end, hipMemcpyHostToDevice));
+    b.tablesReady = true;
+  }
+  int64_t *cont0Dev = reinterpret_cast<int64_t *>(b.d_tables.as<char>() + o_cont);
+  const int64_t *vertOffDev = reinterpret_cast<const int64_t *>(b.d_tables.as<char>() + o_vert);
+  // contacts at the start-of-step positions: the plane test, the list in
+  // ascending vertex order and each scene's range, all on the device (the
+  // count stays in ca.counts[0]; kernels that need it read it there)
+  MLX_TRY(reserve_contacts(m));
+  ContactArrays ca = contact_arrays(m, 0);
+  int32_t *countDev = const_cast<int32_t *>(ca.counts);
+  const bool planeOn = plane && plane->enabled;
+  if (planeOn) {
+    double nrm[3] = {0, 0, 0}, Dp = 0, nn = 0;
+    for (int d = 0; d < 3; ++d) nn += plane->normal[d] * plane->normal[d];
+    nn = std::sqrt(nn);
+    for (int d = 0; d < 3; ++d) {
+      nrm[d] = plane->normal[d] / nn;
+      Dp += plane->normal[d] * plane->position[d];
+    }
+    MLX_TRY(b.d_flags.reserve(2 * sizeof(int32_t) * std::max<int64_t>(nV, 1)));
+    int32_t *fl = b.d_flags.as<int32_t>(), *pos = fl + nV;
+    hipLaunchKernelGGL((k_plane_flags<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
+                       nrm[0], nrm[1], nrm[2], Dp, 0, fl);
+    MLX_TRY(compact_flags_dev(m, fl, nV, pos, const_cast<int32_t *>(ca.vert), ca.lambda, countDev,
+                              s));
+    hipLaunchKernelGGL((k_plane_gather_dev<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s,
+                       countDev, ca.vert, x, nrm[0], nrm[1], nrm[2], Dp, plane->compliance,
+                       const_cast<double *>(ca.normal), const_cast<double *>(ca.point),
+                       const_cast<double *>(ca.compliance));
+    hipLaunchKernelGGL(k_scene_contacts, dim3(blocks_for(ns + 1)), dim3(kBlock), 0, s, ns,
+                       vertOffDev, nV, pos, countDev, cont0Dev);
+  } else {
+    MLX_CHECK_HIP(hipMemsetAsync(countDev, 0, sizeof(int32_t), s));
+    MLX_CHECK_HIP(hipMemsetAsync(cont0Dev, 0, 8 * (size_t)(ns + 1), s));
+  }
   // shift, prologue, lambda = 0
   MLX_CHECK_HIP(hipMemcpyAsync(xo, x, sizeof(vec4_t<T>) * nV, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_batch_prologue<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV,
@@ -2328,8 +2351,7 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
       MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
       if (b.block == 256) launch_batch<T, 256>(B, ns, ldsBytes, s);
       else if (b.block == 768) launch_batch<T, 768>(B, ns, ldsBytes, s);
-      else if (b.block == 768) launch_batch<T, 768>(B, ns, ldsBytes, s);
-    else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
+      else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
       else launch_batch<T, 512>(B, ns, ldsBytes, s);
       MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
     }
@@ -2343,11 +2365,13 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   }
   hipLaunchKernelGGL((k_velocity<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x, xo,
                      b.d_v.as<T>(), h);
-  if (K > 0)
-    hipLaunchKernelGGL((k_contact_velocity<T, 3>), dim3(blocks_for(K)), dim3(kBlock), 0, s, ca,
+  if (planeOn)
+    hipLaunchKernelGGL((k_contact_velocity_dev<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, ca,
                        b.d_v.as<T>(), h, st->gravity, st->restitution);
   MLX_CHECK_HIP(hipGetLastError());
   MLX_CHECK_HIP(hipStreamSynchronize(s));
+  b.cont0.assign(ns + 1, 0);
+  MLX_CHECK_HIP(hipMemcpy(b.cont0.data(), cont0Dev, 8 * (size_t)(ns + 1), hipMemcpyDeviceToHost));
   if (splitErr) {
     int e = 0;
     MLX_CHECK_HIP(hipMemcpy(&e, splitErr, 4, hipMemcpyDeviceToHost));
