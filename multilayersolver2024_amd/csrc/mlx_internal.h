@@ -134,6 +134,12 @@ struct mlx_mesh {
   mlx::DevBuf d_nh;               // T 4*ld: 2D NeoHookean mu | lambda | lambda_D | lambda_H
   mlx::Ctl *h_ctl = nullptr;      // pinned host mirror
   int32_t *h_counts = nullptr;    // pinned {K, nec, nrc, nUniq} staging for the device counts
+  // pinned staging of one substep's host-to-device copies (and of the
+  // contact list read back), handed out in order, reclaimed once the
+  // stream has drained
+  char *h_stage = nullptr;
+  size_t stageCap = 0, stageUsed = 0;
+  mlx::DevBuf d_pinMask;           // uint8 dim*nV: pinned DOFs (built once)
 
   // replayable graphs of iteration chunks (hipGraph), keyed by everything
   // the captured kernels bake into their parameters
@@ -210,6 +216,8 @@ struct HostMaps {
   bool deviceFlags = false;
   std::vector<int64_t> perColor;  // nSorted x numColors
   std::vector<int64_t> nElastic;  // nSorted
+  // set by layer_maps_dev: the vertex body ids (nSorted x nV) on the device
+  const int32_t *devVbody = nullptr;
 };
 
 // buildLayers + findBoundaryVertices on the device from per-element

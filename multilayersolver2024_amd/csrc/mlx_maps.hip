@@ -399,6 +399,7 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
   std::vector<int32_t> hs(deviceFlags && !setsOut ? 0 : (size_t)nT * ns);
   if (!hs.empty())
     MLX_CHECK_HIP(hipMemcpyAsync(hs.data(), sets, 4 * hs.size(), hipMemcpyDeviceToHost, st));
+  maps.devVbody = vb;
   MLX_CHECK_HIP(hipMemcpyAsync(maps.vbody.data(), vb, 4 * maps.vbody.size(), hipMemcpyDeviceToHost,
                                st));
   MLX_CHECK_HIP(hipMemcpyAsync(maps.boundary.data(), bnd, maps.boundary.size(),

@@ -361,6 +361,45 @@ static void build_csr(const mlx_mesh &m, const HostMaps &maps, int col, const Ho
   }
 }
 
+// Pinned staging for the copies of one substep.  Copies from pinned memory
+// are asynchronous, so enqueueing them never waits for the stream (a copy
+// from pageable memory does).  Regions are handed out in order and reused
+// after stage_begin, which drains the stream first if regions are in use;
+// growing the buffer drains it too, since queued copies may still read the
+// old one.
+static int stage_begin(mlx_mesh &m) {
+  if (m.stageUsed) MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  m.stageUsed = 0;
+  return MLX_OK;
+}
+
+static int stage_take(mlx_mesh &m, size_t bytes, char **out) {
+  const size_t off = (m.stageUsed + 255) & ~(size_t)255;
+  if (off + bytes > m.stageCap) {
+    MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+    const size_t cap = std::max<size_t>(2 * (off + bytes), (size_t)1 << 20);
+    char *p = nullptr;
+    MLX_CHECK_HIP(hipHostMalloc((void **)&p, cap, hipHostMallocDefault));
+    if (m.h_stage) (void)hipHostFree(m.h_stage);
+    m.h_stage = p;
+    m.stageCap = cap;
+    m.stageUsed = 0;
+    return stage_take(m, bytes, out);
+  }
+  m.stageUsed = off + bytes;
+  *out = m.h_stage + off;
+  return MLX_OK;
+}
+
+static int stage_upload(mlx_mesh &m, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return MLX_OK;
+  char *p = nullptr;
+  MLX_TRY(stage_take(m, bytes, &p));
+  std::memcpy(p, src, bytes);
+  MLX_CHECK_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, m.stream));
+  return MLX_OK;
+}
+
 static int read_ctl(mlx_mesh &m, Ctl &out) {
   MLX_CHECK_HIP(hipMemcpyAsync(m.h_ctl, m.d_ctl.ptr, sizeof(Ctl), hipMemcpyDeviceToHost, m.stream));
   MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
@@ -416,16 +455,18 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       oc[c] = m.colorOff[c];
       oc[nc + c] = m.colorCnt[c];
     }
-    MLX_CHECK_HIP(hipMemcpyAsync(m.d_elist.ptr, oc.data(), 16 * nc, hipMemcpyHostToDevice, st));
+    MLX_TRY(stage_upload(m, m.d_elist.ptr, oc.data(), 16 * nc));
   }
   if (!maps.deviceFlags) {
     MLX_TRY(m.d_flags.reserve(flags.size()));
-    MLX_CHECK_HIP(hipMemcpyAsync(m.d_flags.ptr, flags.data(), flags.size(), hipMemcpyHostToDevice,
-                                 st));
+    MLX_TRY(stage_upload(m, m.d_flags.ptr, flags.data(), flags.size()));
   }
   MLX_TRY(m.d_vbody.reserve(sizeof(int32_t) * maps.vbody.size()));
-  MLX_CHECK_HIP(hipMemcpyAsync(m.d_vbody.ptr, maps.vbody.data(), sizeof(int32_t) * maps.vbody.size(),
-                               hipMemcpyHostToDevice, st));
+  if (maps.devVbody)
+    MLX_CHECK_HIP(hipMemcpyAsync(m.d_vbody.ptr, maps.devVbody, sizeof(int32_t) * maps.vbody.size(),
+                                 hipMemcpyDeviceToDevice, st));
+  else
+    MLX_TRY(stage_upload(m, m.d_vbody.ptr, maps.vbody.data(), sizeof(int32_t) * maps.vbody.size()));
   int maxNb = 1;
   for (int l = 0; l < ns; ++l) maxNb = std::max(maxNb, maps.numRigids[l]);
   MLX_TRY(m.d_bodies.reserve(sizeof(Body) * maxNb));
@@ -461,8 +502,8 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   double *d_cnt = d_uniq + uniqCap;
   int32_t *d_lamSlot2d = reinterpret_cast<int32_t *>(d_cnt + uniqCap);
   if (nUniq > 0) {
-    MLX_CHECK_HIP(hipMemcpyAsync(d_uniq, uniq.data(), 8 * nUniq, hipMemcpyHostToDevice, st));
-    MLX_CHECK_HIP(hipMemcpyAsync(d_cnt, cnt.data(), 8 * nUniq, hipMemcpyHostToDevice, st));
+    MLX_TRY(stage_upload(m, d_uniq, uniq.data(), 8 * (size_t)nUniq));
+    MLX_TRY(stage_upload(m, d_cnt, cnt.data(), 8 * (size_t)nUniq));
   }
   std::vector<int32_t> lamSlot;
   if (m.dim == 2 && sp.computeResiduals) {
@@ -477,8 +518,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
         lamSlot[3 * s + r] = (int32_t)(m.slotOf[el] * 3 + comp);
       }
     }
-    MLX_CHECK_HIP(hipMemcpyAsync(d_lamSlot2d, lamSlot.data(), sizeof(int32_t) * lamSlot.size(),
-                                 hipMemcpyHostToDevice, st));
+    MLX_TRY(stage_upload(m, d_lamSlot2d, lamSlot.data(), sizeof(int32_t) * lamSlot.size()));
   }
 
   Ctl c0;
@@ -487,8 +527,8 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   c0.stopLayer = -1;
   c0.runUntil = sp.runUntil;
   c0.lastResidual = sp.residualLast;
-  MLX_CHECK_HIP(hipMemcpyAsync(m.d_ctl.ptr, &c0, sizeof c0, hipMemcpyHostToDevice, st));
-  MLX_CHECK_HIP(hipMemcpyAsync(m.d_resid.ptr, &sp.residualLast, 8, hipMemcpyHostToDevice, st));
+  MLX_TRY(stage_upload(m, m.d_ctl.ptr, &c0, sizeof c0));
+  MLX_TRY(stage_upload(m, m.d_resid.ptr, &sp.residualLast, 8));
   Ctl *ctl = m.d_ctl.as<Ctl>();
   double *partials = m.d_partials.as<double>();
   Body *bodies = m.d_bodies.as<Body>();
@@ -526,8 +566,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     }
     persistMaxCnt = (int)maxCnt;
     MLX_TRY(m.d_colors.reserve(sizeof(int32_t) * ct.size() + 4));
-    MLX_CHECK_HIP(hipMemcpyAsync(m.d_colors.ptr, ct.data(), sizeof(int32_t) * ct.size(),
-                                 hipMemcpyHostToDevice, st));
+    MLX_TRY(stage_upload(m, m.d_colors.ptr, ct.data(), sizeof(int32_t) * ct.size()));
     MLX_TRY(m.d_sync.reserve(256));
     // local: persistG workgroups per XCD are launched on each of the 8 XCDs
     // and one XCD's group runs the loop; otherwise persistG in all
@@ -553,18 +592,31 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   res.layersRun = 0;
   Ctl hc_ctl;
   std::memset(&hc_ctl, 0, sizeof hc_ctl);
+  // every layer's CSR lists, uploaded before the first layer runs, each in
+  // a region of its own at a fixed offset (captured graphs keep their
+  // pointers).  Bodies are vertex-disjoint with at least one vertex each
+  // and a vertex has at most one contact, so a layer needs at most
+  // 3 (nV + 1) + 3 nV + 1 entries whatever the substep's contacts.
+  const size_t csrCap = 6 * (size_t)nV + 8;
+  std::vector<LayerCSR> Ls(sp.nLayers);
+  MLX_TRY(m.d_csr.reserve(sizeof(int32_t) * csrCap * (size_t)sp.nLayers));
+  for (int layer = 0; layer < sp.nLayers; ++layer) {
+    build_csr(m, maps, sp.layersMap0[layer], hc, Ls[layer]);
+    if (Ls[layer].buf.size() > csrCap) {
+      set_error("layer lists exceed their region (%zu > %zu entries)", Ls[layer].buf.size(), csrCap);
+      return MLX_ERR_STATE;
+    }
+    MLX_TRY(stage_upload(m, m.d_csr.as<int32_t>() + csrCap * (size_t)layer, Ls[layer].buf.data(),
+                         sizeof(int32_t) * Ls[layer].buf.size()));
+  }
   for (int layer = 0; layer < sp.nLayers; ++layer) {
     const int col = sp.layersMap0[layer];
     const bool isLast = layer == sp.nLayers - 1;
     const uint8_t *fl = nElastic[col] == nT ? nullptr : m.d_flags.as<uint8_t>() + (size_t)ld * col;
     const int32_t *vb = m.d_vbody.as<int32_t>() + (size_t)nV * col;
-    LayerCSR L;
-    build_csr(m, maps, col, hc, L);
+    const LayerCSR &L = Ls[layer];
     const bool hasRigidEl = nElastic[col] < nT;
-    MLX_TRY(m.d_csr.reserve(sizeof(int32_t) * L.buf.size()));
-    MLX_CHECK_HIP(hipMemcpyAsync(m.d_csr.ptr, L.buf.data(), sizeof(int32_t) * L.buf.size(),
-                                 hipMemcpyHostToDevice, st));
-    const int32_t *cb = m.d_csr.as<int32_t>();
+    const int32_t *cb = m.d_csr.as<int32_t>() + csrCap * (size_t)layer;
     LayerLists LL;
     LL.rig_off = cb + L.o_rig_off;
     LL.rig_vert = cb + L.o_rig_vert;
@@ -611,8 +663,8 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
                          (int)isLast, layer == 0 ? sp.stopAtImprovementPercent : 0.0);
       // 2D layerSolve passes resvnorm = inf to projectGS (xpbdLayer2D.m:735)
       static const double inf = INFINITY;
-      MLX_CHECK_HIP(hipMemcpyAsync(reinterpret_cast<char *>(m.d_ctl.ptr) + offsetof(Ctl, normresv),
-                                   &inf, sizeof(double), hipMemcpyHostToDevice, st));
+      MLX_TRY(stage_upload(m, reinterpret_cast<char *>(m.d_ctl.ptr) + offsetof(Ctl, normresv), &inf,
+                           sizeof(double)));
     }
 
     hc_ctl.iters = 0;  // k_layer_begin cleared the device count
@@ -689,7 +741,8 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
         return u;
       };
       std::vector<uint64_t> sig = {
-          (uint64_t)col, (uint64_t)G, (uint64_t)m.sweepBlock, (uint64_t)L.nb, (uint64_t)L.nbnd,
+          (uint64_t)col, (uint64_t)layer, (uint64_t)csrCap, (uint64_t)G, (uint64_t)m.sweepBlock,
+          (uint64_t)L.nb, (uint64_t)L.nbnd,
           (uint64_t)(L.nec > 0),
           (uint64_t)L.nrc, (uint64_t)(K > 0), (uint64_t)uniqCap, (uint64_t)L.o_rig_vert,
           (uint64_t)L.o_bnd_off, (uint64_t)L.o_bnd_vert, (uint64_t)L.o_rc_off,
@@ -1232,7 +1285,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
                          &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced, &m->d_ecolor,
-                         &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist};
+                         &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist, &m->d_pinMask};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
@@ -1241,6 +1294,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (m->h_ctl) (void)hipHostFree(m->h_ctl);
   if (m->h_counts) (void)hipHostFree(m->h_counts);
+  if (m->h_stage) (void)hipHostFree(m->h_stage);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return MLX_OK;
@@ -1349,6 +1403,7 @@ int mlx_layer_solve(mlx_mesh *m, const mlx_layer_solve_args *a, mlx_layer_solve_
     return MLX_ERR_ARG;
   }
   HostContacts hc;
+  MLX_TRY(stage_begin(*m));
   MLX_TRY(upload_contacts(*m, m->dim, a->nContacts, a->contactVertexID, a->contactNormals,
                           a->pointColliders, a->contactCompliance, hc));
   const bool f64 = m->precision == MLX_FP64;
@@ -1600,10 +1655,17 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
                        s, nV, flag, frame == 1, x, xo, nPin, m.d_pinned.as<int32_t>(),
                        m.d_pinnedx.as<T>());
   }
-  // 2. contacts at the start-of-step positions
+  // 2. contacts at the start-of-step positions: the plane test and the list
+  // in ascending vertex order are built on the device; the list is read
+  // back together with the layer maps, without a wait of its own
+  MLX_TRY(stage_begin(m));
   HostContacts hc;
   double nrm[3] = {0, 0, 0}, Dp = 0;
-  if (plane && plane->enabled) {
+  const bool planeOn = plane && plane->enabled;
+  MLX_TRY(reserve_contacts(m));
+  ContactArrays ca = contact_arrays(m, 0);
+  int32_t *hList = nullptr;  // pinned: the count, then the list
+  if (planeOn) {
     double nn = 0;
     for (int d = 0; d < dim; ++d) nn += plane->normal[d] * plane->normal[d];
     nn = std::sqrt(nn);
@@ -1612,39 +1674,31 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
       Dp += plane->normal[d] * plane->position[d];
     }
     MLX_TRY(S.d_ediff.reserve(sizeof(double) * std::max<int64_t>(nT, nV)));
-    int32_t *fl = S.d_ediff.as<int32_t>();
+    int32_t *fl = S.d_ediff.as<int32_t>(), *pos = fl + nV;
     if (dim == 3)
       hipLaunchKernelGGL((k_plane_flags<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
                          nrm[0], nrm[1], nrm[2], Dp, 0, fl);
     else
       hipLaunchKernelGGL((k_plane_flags<T, 2>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
                          nrm[0], nrm[1], nrm[2], Dp, 1, fl);
-    std::vector<int32_t> hf(nV);
-    MLX_CHECK_HIP(hipMemcpyAsync(hf.data(), fl, sizeof(int32_t) * nV, hipMemcpyDeviceToHost, s));
-    MLX_CHECK_HIP(hipStreamSynchronize(s));
-    for (int64_t v = 0; v < nV; ++v)
-      if (hf[v]) hc.vert.push_back((int32_t)v);
-    hc.K = (int)hc.vert.size();
-    hc.compliance.assign(hc.K, plane->compliance);
-  }
-  const int K = hc.K;
-  MLX_TRY(reserve_contacts(m));
-  ContactArrays ca = contact_arrays(m, K);
-  if (K > 0) {
-    MLX_CHECK_HIP(hipMemcpyAsync(const_cast<int32_t *>(ca.vert), hc.vert.data(), 4 * (size_t)K,
-                                 hipMemcpyHostToDevice, s));
+    int32_t *cnt = const_cast<int32_t *>(ca.counts);
+    MLX_TRY(compact_flags_dev(m, fl, nV, pos, const_cast<int32_t *>(ca.vert), ca.lambda, cnt, s));
     if (dim == 3)
-      hipLaunchKernelGGL((k_plane_gather<T, 3>), dim3(blocks_for(K)), dim3(kBlock), 0, s, K,
+      hipLaunchKernelGGL((k_plane_gather_dev<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, cnt,
                          ca.vert, x, nrm[0], nrm[1], nrm[2], Dp, plane->compliance,
                          const_cast<double *>(ca.normal), const_cast<double *>(ca.point),
                          const_cast<double *>(ca.compliance));
     else
-      hipLaunchKernelGGL((k_plane_gather<T, 2>), dim3(blocks_for(K)), dim3(kBlock), 0, s, K,
+      hipLaunchKernelGGL((k_plane_gather_dev<T, 2>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, cnt,
                          ca.vert, x, nrm[0], nrm[1], nrm[2], Dp, plane->compliance,
                          const_cast<double *>(ca.normal), const_cast<double *>(ca.point),
                          const_cast<double *>(ca.compliance));
+    char *hp = nullptr;
+    MLX_TRY(stage_take(m, 4 * ((size_t)nV + 1), &hp));
+    hList = reinterpret_cast<int32_t *>(hp);
+    MLX_CHECK_HIP(hipMemcpyAsync(hList, cnt, 4, hipMemcpyDeviceToHost, s));
+    MLX_CHECK_HIP(hipMemcpyAsync(hList + 1, ca.vert, 4 * (size_t)nV, hipMemcpyDeviceToHost, s));
   }
-  S.lastContacts = K;
   // 3. layers (buildLayers + findBoundaryVertices) from p and cache.oldp
   double sorted[MLX_MAX_LAYERS];
   int32_t map0[MLX_MAX_LAYERS];
@@ -1660,16 +1714,28 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
       if (sorted[j] == st->layers[i]) map0[i] = j;
   HostMaps maps;
   MLX_TRY(build_layer_maps<T>(S, sorted, ns, h, maps));
+  if (planeOn) {
+    // the maps have usually drained the stream already
+    MLX_CHECK_HIP(hipStreamSynchronize(s));
+    hc.K = hList[0];
+    hc.vert.assign(hList + 1, hList + 1 + hc.K);
+    hc.compliance.assign(hc.K, plane->compliance);
+  }
+  const int K = hc.K;
+  ca = contact_arrays(m, K);
+  S.lastContacts = K;
   const auto t1 = clk::now();
   // 4. shift: cache.oldp = p (3D), prevp = p (2D)
   MLX_CHECK_HIP(hipMemcpyAsync(xo, x, sizeof(vec4_t<T>) * nV, hipMemcpyDeviceToDevice, s));
   // 5. residual velocity (3D) or explicit predictor (2D)
   MLX_TRY(m.d_flags.reserve(std::max<size_t>(m.d_flags.bytes, 1)));
-  std::vector<uint8_t> pinMask((size_t)dim * nV, 0);
-  for (auto d : m.pinnedDOF) pinMask[d] = 1;
-  DevBuf &pm = S.d_ediff;  // reuse: layer maps are on the host now
-  MLX_TRY(pm.reserve(std::max(sizeof(double) * nT, pinMask.size())));
-  MLX_CHECK_HIP(hipMemcpyAsync(pm.ptr, pinMask.data(), pinMask.size(), hipMemcpyHostToDevice, s));
+  if (!m.d_pinMask.ptr) {
+    std::vector<uint8_t> pinMask((size_t)dim * nV, 0);
+    for (auto d : m.pinnedDOF) pinMask[d] = 1;
+    MLX_TRY(m.d_pinMask.reserve(std::max<size_t>(pinMask.size(), 1)));
+    MLX_TRY(stage_upload(m, m.d_pinMask.ptr, pinMask.data(), pinMask.size()));
+  }
+  DevBuf &pm = m.d_pinMask;
   if (dim == 3) {
     hipLaunchKernelGGL(k_residual_velocity3d<T>, dim3(blocks_for(3 * nV)), dim3(kBlock), 0, s, nV,
                        S.d_v.as<T>(), m.d_rv.as<T>(), m.d_mass64.as<double>(), pm.as<uint8_t>(),
