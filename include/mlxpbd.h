@@ -335,7 +335,7 @@ typedef struct {
   double *lastResidual;       /* optional nScenes */
   int32_t *contacts;          /* optional nScenes */
   int64_t elasticProjections; /* all scenes */
-  double kernelMs;            /* duration of the batched solve kernel */
+  double kernelMs;            /* duration of the batched solve kernel (0 for a graph replay) */
 } mlx_batch_out;
 
 /* One integrate() substep of every scene (xpbdLayer3D.m:621-745 per scene),
@@ -356,6 +356,13 @@ int mlx_batch_set_split(mlx_batch *batch, int32_t tetsPerWorkgroup);
  * color to the last residual (enable = 1, default); 0 keeps every scene's
  * positions in HBM.  Same updates in the same order either way. */
 int mlx_batch_set_lds(mlx_batch *batch, int32_t enable);
+/* Substeps after frame 1 replay a captured hipGraph of the whole substep
+ * (contacts, prologue, the solve, velocity update) when the batch is not
+ * split; it is captured again whenever a setting baked into its kernels
+ * changes.  enable = 0 launches every kernel directly (the default is 1).
+ * mlx_batch_out.kernelMs is measured only for directly launched substeps
+ * (0 for a replay).  No counterpart in the reference. */
+int mlx_batch_set_graph(mlx_batch *batch, int32_t enable);
 
 int mlx_batch_integrate(mlx_batch *batch, mlx_settings *settings, double h,
                         int32_t frame, const mlx_plane *plane,

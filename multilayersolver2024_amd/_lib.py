@@ -30,7 +30,7 @@ EXPORTED = (
     "mlx_mesh_create", "mlx_mesh_destroy", "mlx_mesh_info",
     "mlx_stvk_voigt_constraints", "mlx_layer_solve", "mlx_state_create",
     "mlx_state_destroy", "mlx_state_download", "mlx_integrate",
-    "mlx_profile_enable", "mlx_profile_read", "mlx_profile_phases", "mlx_profile_read_persistent", "mlx_layer_maps", "mlx_batch_set_split", "mlx_batch_set_lds", "mlx_state_set_pinned_targets", "mlx_synchronize", "mlx_mesh_set_option",
+    "mlx_profile_enable", "mlx_profile_read", "mlx_profile_phases", "mlx_profile_read_persistent", "mlx_layer_maps", "mlx_batch_set_split", "mlx_batch_set_lds", "mlx_batch_set_graph", "mlx_state_set_pinned_targets", "mlx_synchronize", "mlx_mesh_set_option",
     "mlx_batch_create", "mlx_batch_destroy", "mlx_batch_set_block", "mlx_batch_set_state",
     "mlx_batch_download", "mlx_batch_integrate", "mlx_neohookean_constraints2d",
     "mlx_pcg_solve",
@@ -162,6 +162,7 @@ def lib():
         "mlx_layer_maps": [_p, _d, _d, C.c_int32, _i32, _i32, _u8, _i32],
         "mlx_batch_set_split": [_p, C.c_int32],
         "mlx_batch_set_lds": [_p, C.c_int32],
+        "mlx_batch_set_graph": [_p, C.c_int32],
         "mlx_state_set_pinned_targets": [_p, _d],
         "mlx_synchronize": [],
         "mlx_mesh_set_option": [_p, C.c_int32, C.c_int64],

@@ -95,6 +95,12 @@ class BatchSimulation:
         """Keep positions of scenes that fit in LDS there (default on)."""
         L.check("mlx_batch_set_lds", L.lib().mlx_batch_set_lds(self.handle, int(bool(enable))))
 
+    def set_graph(self, enable):
+        """Replay a captured hipGraph of the whole substep after frame 1
+        (default on); off launches every kernel directly.  The kernel time
+        returned by step() is measured only for direct launches."""
+        L.check("mlx_batch_set_graph", L.lib().mlx_batch_set_graph(self.handle, int(bool(enable))))
+
     def set_block(self, threads):
         L.check("mlx_batch_set_block", L.lib().mlx_batch_set_block(self.handle, int(threads)))
 

@@ -2090,6 +2090,9 @@ struct mlx_batch {
   int useLds = 1;              // scenes that fit keep their positions in LDS (k_batch_plain)
   int ldsVerts = 0;            // vertex limit of the LDS path in the last launch
   bool tablesReady = false;    // static part of d_tables uploaded
+  int useGraph = 1;            // substeps after frame 1 replay a captured graph
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gsig;  // what the captured kernels bake in
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -2157,6 +2160,20 @@ static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
   return (int)nv;
 }
 
+// p[0..n) = v and dst = src, as kernels: a substep that is captured into a
+// graph holds no memset or memcpy nodes.
+template <typename V>
+__global__ void k_fill(int64_t n, V *p, V v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename V>
+__global__ void k_copy(int64_t n, const V *src, V *dst) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 // First contact of each scene: the number of flagged vertices before the
 // scene's first vertex (pos: exclusive scan of the plane flags); entry ns
 // is the total.
@@ -2189,11 +2206,6 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   }
   MLX_TRY(m.d_tmp.reserve(4096));
   int *flag = reinterpret_cast<int *>(m.d_tmp.as<char>() + 4000);
-  MLX_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
-  if (frame != 1)
-    hipLaunchKernelGGL(k_any_nan<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, xo, flag);
-  hipLaunchKernelGGL((k_reset_oldp<T, 3>), dim3(blocks_for(std::max(nV, nPin))), dim3(kBlock), 0, s,
-                     nV, flag, frame == 1, x, xo, nPin, m.d_pinned.as<int32_t>(), m.d_pinnedx.as<T>());
   // per-scene tables: color0 | ncolor | colorOff | colorCnt | slot0 | cont0 | pin0 | vert0.
   // All but cont0 are fixed for the batch and uploaded once; cont0 is
   // written on the device every substep.
@@ -2218,15 +2230,13 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   }
   int64_t *cont0Dev = reinterpret_cast<int64_t *>(b.d_tables.as<char>() + o_cont);
   const int64_t *vertOffDev = reinterpret_cast<const int64_t *>(b.d_tables.as<char>() + o_vert);
-  // contacts at the start-of-step positions: the plane test, the list in
-  // ascending vertex order and each scene's range, all on the device (the
-  // count stays in ca.counts[0]; kernels that need it read it there)
   MLX_TRY(reserve_contacts(m));
   ContactArrays ca = contact_arrays(m, 0);
   int32_t *countDev = const_cast<int32_t *>(ca.counts);
   const bool planeOn = plane && plane->enabled;
+  double nrm[3] = {0, 0, 0}, Dp = 0;
   if (planeOn) {
-    double nrm[3] = {0, 0, 0}, Dp = 0, nn = 0;
+    double nn = 0;
     for (int d = 0; d < 3; ++d) nn += plane->normal[d] * plane->normal[d];
     nn = std::sqrt(nn);
     for (int d = 0; d < 3; ++d) {
@@ -2234,27 +2244,57 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
       Dp += plane->normal[d] * plane->position[d];
     }
     MLX_TRY(b.d_flags.reserve(2 * sizeof(int32_t) * std::max<int64_t>(nV, 1)));
-    int32_t *fl = b.d_flags.as<int32_t>(), *pos = fl + nV;
-    hipLaunchKernelGGL((k_plane_flags<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
-                       nrm[0], nrm[1], nrm[2], Dp, 0, fl);
-    MLX_TRY(compact_flags_dev(m, fl, nV, pos, const_cast<int32_t *>(ca.vert), ca.lambda, countDev,
-                              s));
-    hipLaunchKernelGGL((k_plane_gather_dev<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s,
-                       countDev, ca.vert, x, nrm[0], nrm[1], nrm[2], Dp, plane->compliance,
-                       const_cast<double *>(ca.normal), const_cast<double *>(ca.point),
-                       const_cast<double *>(ca.compliance));
-    hipLaunchKernelGGL(k_scene_contacts, dim3(blocks_for(ns + 1)), dim3(kBlock), 0, s, ns,
-                       vertOffDev, nV, pos, countDev, cont0Dev);
-  } else {
-    MLX_CHECK_HIP(hipMemsetAsync(countDev, 0, sizeof(int32_t), s));
-    MLX_CHECK_HIP(hipMemsetAsync(cont0Dev, 0, 8 * (size_t)(ns + 1), s));
   }
-  // shift, prologue, lambda = 0
-  MLX_CHECK_HIP(hipMemcpyAsync(xo, x, sizeof(vec4_t<T>) * nV, hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(k_batch_prologue<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV,
-                     b.d_v.as<T>(), x, m.d_mass64.as<double>(), b.d_pinMask.as<uint8_t>(), h,
-                     st->gravity, st->airDrag);
-  MLX_CHECK_HIP(hipMemsetAsync(m.d_lam.ptr, 0, sizeof(T) * m.nc * m.ld, s));
+  int32_t *fl = planeOn ? b.d_flags.as<int32_t>() : nullptr;
+  int32_t *pos = planeOn ? fl + nV : nullptr;
+  T *lam = m.d_lam.as<T>();
+  const int64_t nLam = (int64_t)m.nc * m.ld;
+  // Device work before the solve: NaN / frame-1 reset of cache.oldp,
+  // contacts at the start-of-step positions (the plane test, the list in
+  // ascending vertex order and each scene's range; the count stays in
+  // ca.counts[0] for the kernels that need it), the position shift, the
+  // prologue and lambda = 0.  Fill and copy kernels instead of memset and
+  // memcpy nodes, so the sequence can be captured.
+  auto enqueue_pre = [&]() -> int {
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(kBlock), 0, s, (int64_t)1, flag, 0);
+    if (frame != 1)
+      hipLaunchKernelGGL(k_any_nan<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, xo, flag);
+    hipLaunchKernelGGL((k_reset_oldp<T, 3>), dim3(blocks_for(std::max(nV, nPin))), dim3(kBlock), 0,
+                       s, nV, flag, frame == 1, x, xo, nPin, m.d_pinned.as<int32_t>(),
+                       m.d_pinnedx.as<T>());
+    if (planeOn) {
+      hipLaunchKernelGGL((k_plane_flags<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
+                         nrm[0], nrm[1], nrm[2], Dp, 0, fl);
+      MLX_TRY(compact_flags_dev(m, fl, nV, pos, const_cast<int32_t *>(ca.vert), ca.lambda,
+                                countDev, s));
+      hipLaunchKernelGGL((k_plane_gather_dev<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s,
+                         countDev, ca.vert, x, nrm[0], nrm[1], nrm[2], Dp, plane->compliance,
+                         const_cast<double *>(ca.normal), const_cast<double *>(ca.point),
+                         const_cast<double *>(ca.compliance));
+      hipLaunchKernelGGL(k_scene_contacts, dim3(blocks_for(ns + 1)), dim3(kBlock), 0, s, ns,
+                         vertOffDev, nV, pos, countDev, cont0Dev);
+    } else {
+      hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(kBlock), 0, s, (int64_t)1, countDev, 0);
+      hipLaunchKernelGGL(k_fill<int64_t>, dim3(blocks_for(ns + 1)), dim3(kBlock), 0, s,
+                         (int64_t)(ns + 1), cont0Dev, (int64_t)0);
+    }
+    hipLaunchKernelGGL(k_copy<vec4_t<T>>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV,
+                       (const vec4_t<T> *)x, xo);
+    hipLaunchKernelGGL(k_batch_prologue<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV,
+                       b.d_v.as<T>(), x, m.d_mass64.as<double>(), b.d_pinMask.as<uint8_t>(), h,
+                       st->gravity, st->airDrag);
+    hipLaunchKernelGGL(k_fill<T>, dim3(blocks_for(nLam)), dim3(kBlock), 0, s, nLam, lam, T(0));
+    return MLX_OK;
+  };
+  // after the solve: velocity update and contactVelocityUpdate
+  auto enqueue_post = [&]() -> int {
+    hipLaunchKernelGGL((k_velocity<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x, xo,
+                       b.d_v.as<T>(), h);
+    if (planeOn)
+      hipLaunchKernelGGL((k_contact_velocity_dev<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s,
+                         ca, b.d_v.as<T>(), h, st->gravity, st->restitution);
+    return MLX_OK;
+  };
   // the persistent solve
   MLX_TRY(b.d_out.reserve((4 + 8 + 8 + 4) * (size_t)ns + 64));
   char *ob = b.d_out.as<char>();
@@ -2306,8 +2346,16 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     MLX_CHECK_HIP(hipEventCreate(&b.ev0));
     MLX_CHECK_HIP(hipEventCreate(&b.ev1));
   }
+  auto launch_unsplit = [&]() {
+    if (b.block == 256) launch_batch<T, 256>(B, ns, ldsBytes, s);
+    else if (b.block == 768) launch_batch<T, 768>(B, ns, ldsBytes, s);
+    else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
+    else launch_batch<T, 512>(B, ns, ldsBytes, s);
+  };
   int *splitErr = nullptr, *splitMismatch = nullptr;
+  bool replayed = false;
   if (b.split > 0) {
+    MLX_TRY(enqueue_pre());
     // split schedule: scene sc gets G = round(nT_sc / split) workgroups (1..64)
     // on one XCD; scenes go to XCDs largest first, each to the XCD with the
     // fewest tets so far; P = the largest XCD's workgroup count, and the grid
@@ -2421,19 +2469,60 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
       else launch_batch<T, 512>(B, ns, ldsBytes, s);
       MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
     }
+    MLX_TRY(enqueue_post());
+  } else if (b.useGraph && frame != 1) {
+    // the whole substep as one graph, captured again when anything baked
+    // into its kernels changes
+    auto bits = [](double d) {
+      uint64_t u;
+      std::memcpy(&u, &d, 8);
+      return u;
+    };
+    const std::vector<uint64_t> sig = {
+        (uint64_t)ns, (uint64_t)nV, (uint64_t)b.block, (uint64_t)ldsBytes, (uint64_t)B.ldsVerts,
+        (uint64_t)planeOn, bits(nrm[0]), bits(nrm[1]), bits(nrm[2]), bits(Dp),
+        bits(planeOn ? plane->compliance : 0.0), bits(h), bits(st->gravity), bits(st->airDrag),
+        bits(st->restitution), (uint64_t)B.iterations, bits(B.runUntil0), bits(B.stopPct),
+        bits(B.contactResistance), (uint64_t)B.hardCap, (uint64_t)B.mp.computeResiduals,
+        (uint64_t)B.mp.hangingStop, (uint64_t)B.mp.giveUpEnabled,
+        (uint64_t)B.mp.lastLayerGiveUpEnabled, bits(B.mp.giveUpThreshold),
+        (uint64_t)(uintptr_t)m.d_x.ptr, (uint64_t)(uintptr_t)m.d_xold.ptr,
+        (uint64_t)(uintptr_t)m.d_lam.ptr, (uint64_t)(uintptr_t)b.d_v.ptr,
+        (uint64_t)(uintptr_t)b.d_flags.ptr, (uint64_t)(uintptr_t)m.d_scan.ptr,
+        (uint64_t)(uintptr_t)m.d_contacts.ptr, (uint64_t)(uintptr_t)b.d_tables.ptr,
+        (uint64_t)(uintptr_t)b.d_out.ptr, (uint64_t)(uintptr_t)m.d_tmp.ptr,
+        (uint64_t)(uintptr_t)b.d_pinMask.ptr, (uint64_t)(uintptr_t)m.d_pinnedx.ptr};
+    if (!b.gexec || b.gsig != sig) {
+      if (b.gexec) {
+        (void)hipGraphExecDestroy(b.gexec);
+        b.gexec = nullptr;
+      }
+      hipGraph_t graph = nullptr;
+      MLX_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int r = enqueue_pre();
+      if (r == MLX_OK) launch_unsplit();
+      if (r == MLX_OK) r = enqueue_post();
+      const hipError_t ce = hipStreamEndCapture(s, &graph);
+      if (r != MLX_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+      }
+      MLX_CHECK_HIP(ce);
+      const hipError_t ie = hipGraphInstantiate(&b.gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) b.gexec = nullptr;
+      MLX_CHECK_HIP(ie);
+      b.gsig = sig;
+    }
+    MLX_CHECK_HIP(hipGraphLaunch(b.gexec, s));
+    replayed = true;
   } else {
+    MLX_TRY(enqueue_pre());
     MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
-    if (b.block == 256) launch_batch<T, 256>(B, ns, ldsBytes, s);
-    else if (b.block == 768) launch_batch<T, 768>(B, ns, ldsBytes, s);
-    else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
-    else launch_batch<T, 512>(B, ns, ldsBytes, s);
+    launch_unsplit();
     MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
+    MLX_TRY(enqueue_post());
   }
-  hipLaunchKernelGGL((k_velocity<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x, xo,
-                     b.d_v.as<T>(), h);
-  if (planeOn)
-    hipLaunchKernelGGL((k_contact_velocity_dev<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, ca,
-                       b.d_v.as<T>(), h, st->gravity, st->restitution);
   MLX_CHECK_HIP(hipGetLastError());
   MLX_CHECK_HIP(hipStreamSynchronize(s));
   b.cont0.assign(ns + 1, 0);
@@ -2451,7 +2540,7 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   MLX_CHECK_HIP(hipMemcpy(it.data(), B.iters, 4 * (size_t)ns, hipMemcpyDeviceToHost));
   MLX_CHECK_HIP(hipMemcpy(lr.data(), B.lastResidual, 8 * (size_t)ns, hipMemcpyDeviceToHost));
   float kms = 0;
-  MLX_CHECK_HIP(hipEventElapsedTime(&kms, b.ev0, b.ev1));
+  if (!replayed) MLX_CHECK_HIP(hipEventElapsedTime(&kms, b.ev0, b.ev1));
   if (out) {
     int64_t proj = 0;
     for (int sc = 0; sc < ns; ++sc) {
@@ -2533,6 +2622,7 @@ int mlx_batch_destroy(mlx_batch *b) {
   if (!b) return MLX_OK;
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->d_tables.release();
   b->d_v.release();
   b->d_out.release();
@@ -2550,6 +2640,15 @@ int mlx_batch_set_lds(mlx_batch *b, int32_t enable) {
     return MLX_ERR_ARG;
   }
   b->useLds = enable != 0;
+  return MLX_OK;
+}
+
+int mlx_batch_set_graph(mlx_batch *b, int32_t enable) {
+  if (!b) {
+    set_error("mlx_batch_set_graph: null batch");
+    return MLX_ERR_ARG;
+  }
+  b->useGraph = enable != 0;
   return MLX_OK;
 }
 

@@ -134,6 +134,49 @@ def test_batch_lds_positions_equal_hbm_positions(precision, block):
         b.close()
 
 
+@pytest.mark.parametrize("precision,block,ground", [(L.MLX_FP64, 512, True),
+                                                    (L.MLX_FP32, 1024, True),
+                                                    (L.MLX_FP32, 512, False)])
+def test_batch_graph_replays_equal_direct_launches(precision, block, ground):
+    """After frame 1 each substep replays one captured hipGraph of the whole
+    substep (fill and copy kernels, no memset or memcpy nodes).  Against a
+    batch that launches every kernel directly from the same state, every
+    substep gives bit-identical positions, velocities, lambdas, iteration
+    and contact counts.  Between two replays the host downloads both states
+    and the direct batch runs its own substep, so each replay follows other
+    work and a synchronization; the contact sets change as the scenes land.
+    Halfway the block size changes, which captures the graph again."""
+    scs = random_beams(8, seed=7, tets=(600, 6000))
+    merged = merge_meshes([Mesh(sc) for sc in scs])
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0]) if ground else None
+    it = _integ(iters=16, pct=99.99)
+    it.hangingStop = 200
+    a = BatchSimulation(merged, it, 1 / 60, plane, precision=precision, block=block)
+    g = BatchSimulation(merged, copy.deepcopy(it), 1 / 60, plane, precision=precision, block=block)
+    a.set_graph(0)
+    g.set_graph(1)
+    try:
+        for k in range(10):
+            if k == 5:
+                a.set_block(256)
+                g.set_block(256)
+            _, ka = a.step()
+            _, kg = g.step()
+            pa, va, la = a.download()
+            pg, vg, lg = g.download()
+            print("graph", k, list(g.iters), list(g.contacts), ka, kg, flush=True)
+            assert ka > 0
+            assert (kg > 0) if k == 0 else (kg == 0)  # frame 1 runs directly, then replays
+            assert np.array_equal(a.iters, g.iters) and np.array_equal(a.contacts, g.contacts)
+            assert np.array_equal(pa, pg) and np.array_equal(va, vg) and np.array_equal(la, lg)
+            assert np.isfinite(pg).all()
+        if ground:
+            assert np.asarray(g.contacts).sum() > 0
+    finally:
+        a.close()
+        g.close()
+
+
 @pytest.mark.parametrize("layout", ["single_scene", "with_one_cube_scenes"])
 def test_batch_small_scenes_match_oracle(layout, oracle_lib):
     """A batch of one scene, and a batch mixing single-cube scenes (5 tets)
