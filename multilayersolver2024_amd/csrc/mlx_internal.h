@@ -139,6 +139,7 @@ struct mlx_mesh {
   // stream has drained
   char *h_stage = nullptr;
   size_t stageCap = 0, stageUsed = 0;
+  std::vector<char *> stageRetired;  // outgrown staging buffers, freed at stage_begin
   mlx::DevBuf d_pinMask;           // uint8 dim*nV: pinned DOFs (built once)
 
   // replayable graphs of iteration chunks (hipGraph), keyed by everything
@@ -228,6 +229,15 @@ struct HostMaps {
 int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int ns,
                    HostMaps &maps, std::vector<int32_t> *setsOut = nullptr,
                    bool deviceFlags = false);
+
+// Pinned staging of one substep's copies (mlx_solver.hip): stage_begin
+// reclaims every region (draining the stream if regions are in use),
+// stage_take hands out a region valid until the next stage_begin,
+// stage_upload copies host data through a region to the device on the
+// mesh's stream.
+int stage_begin(mlx_mesh &m);
+int stage_take(mlx_mesh &m, size_t bytes, char **out);
+int stage_upload(mlx_mesh &m, void *dst, const void *src, size_t bytes);
 
 // Ordered list of the flagged entries of flag[0..n) (int32 0/1) on stream
 // st: list[k] = the k-th flagged index in ascending order, lambda[k] = 0,

@@ -23,6 +23,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <climits>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -371,9 +372,19 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
     hipLaunchKernelGGL(k_cc_count, dim3(1), dim3(64), 0, st, nT, scan, isRoot, count + l);
   }
   MLX_CHECK_HIP(hipGetLastError());
-  MLX_CHECK_HIP(hipMemcpyAsync(maps.numRigids.data(), count, 4 * (size_t)ns,
-                               hipMemcpyDeviceToHost, st));
+  // everything read back goes through pinned staging, so the copies queue
+  // behind the kernels and one synchronization ends them all
+  auto take32 = [&](size_t n, int32_t **out) -> int {
+    char *p = nullptr;
+    MLX_TRY(stage_take(m, 4 * std::max<size_t>(n, 1), &p));
+    *out = reinterpret_cast<int32_t *>(p);
+    return MLX_OK;
+  };
+  int32_t *hNum = nullptr;
+  MLX_TRY(take32((size_t)ns, &hNum));
+  MLX_CHECK_HIP(hipMemcpyAsync(hNum, count, 4 * (size_t)ns, hipMemcpyDeviceToHost, st));
   std::vector<int32_t> colorCnt;
+  int32_t *hCC = nullptr;
   if (deviceFlags) {
     // 4. elastic flags in slot order straight into m.d_flags
     const int nc = m.numColors;
@@ -393,18 +404,27 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
                        sets, m.d_slotOf.as<int32_t>(), m.d_ecolor.as<int32_t>(),
                        m.d_flags.as<uint8_t>(), cc);
     colorCnt.resize((size_t)ns * nc);
-    MLX_CHECK_HIP(hipMemcpyAsync(colorCnt.data(), cc, 4 * colorCnt.size(), hipMemcpyDeviceToHost,
-                                 st));
+    MLX_TRY(take32(colorCnt.size(), &hCC));
+    MLX_CHECK_HIP(hipMemcpyAsync(hCC, cc, 4 * colorCnt.size(), hipMemcpyDeviceToHost, st));
   }
   std::vector<int32_t> hs(deviceFlags && !setsOut ? 0 : (size_t)nT * ns);
-  if (!hs.empty())
-    MLX_CHECK_HIP(hipMemcpyAsync(hs.data(), sets, 4 * hs.size(), hipMemcpyDeviceToHost, st));
+  int32_t *hSets = nullptr, *hVb = nullptr;
+  if (!hs.empty()) {
+    MLX_TRY(take32(hs.size(), &hSets));
+    MLX_CHECK_HIP(hipMemcpyAsync(hSets, sets, 4 * hs.size(), hipMemcpyDeviceToHost, st));
+  }
   maps.devVbody = vb;
-  MLX_CHECK_HIP(hipMemcpyAsync(maps.vbody.data(), vb, 4 * maps.vbody.size(), hipMemcpyDeviceToHost,
-                               st));
-  MLX_CHECK_HIP(hipMemcpyAsync(maps.boundary.data(), bnd, maps.boundary.size(),
-                               hipMemcpyDeviceToHost, st));
+  MLX_TRY(take32(maps.vbody.size(), &hVb));
+  MLX_CHECK_HIP(hipMemcpyAsync(hVb, vb, 4 * maps.vbody.size(), hipMemcpyDeviceToHost, st));
+  char *hBnd = nullptr;
+  MLX_TRY(stage_take(m, std::max<size_t>(maps.boundary.size(), 1), &hBnd));
+  MLX_CHECK_HIP(hipMemcpyAsync(hBnd, bnd, maps.boundary.size(), hipMemcpyDeviceToHost, st));
   MLX_CHECK_HIP(hipStreamSynchronize(st));
+  std::memcpy(maps.numRigids.data(), hNum, 4 * (size_t)ns);
+  if (hCC) std::memcpy(colorCnt.data(), hCC, 4 * colorCnt.size());
+  if (hSets) std::memcpy(hs.data(), hSets, 4 * hs.size());
+  std::memcpy(maps.vbody.data(), hVb, 4 * maps.vbody.size());
+  std::memcpy(maps.boundary.data(), hBnd, maps.boundary.size());
   if (deviceFlags) {
     const int nc = m.numColors;
     maps.perColor.assign(colorCnt.begin(), colorCnt.end());

@@ -361,37 +361,39 @@ static void build_csr(const mlx_mesh &m, const HostMaps &maps, int col, const Ho
   }
 }
 
-// Pinned staging for the copies of one substep.  Copies from pinned memory
-// are asynchronous, so enqueueing them never waits for the stream (a copy
-// from pageable memory does).  Regions are handed out in order and reused
-// after stage_begin, which drains the stream first if regions are in use;
-// growing the buffer drains it too, since queued copies may still read the
-// old one.
-static int stage_begin(mlx_mesh &m) {
-  if (m.stageUsed) MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+// Pinned staging for the copies of one substep.  Copies from or into
+// pinned memory are asynchronous, so enqueueing them never waits for the
+// stream (a copy from pageable memory does).  Regions are handed out in
+// order and stay valid until the next stage_begin, which drains the stream
+// first if regions are in use; a buffer outgrown meanwhile is kept until
+// then, since queued copies may still use it.
+int stage_begin(mlx_mesh &m) {
+  if (m.stageUsed || !m.stageRetired.empty()) MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  for (char *p : m.stageRetired) (void)hipHostFree(p);
+  m.stageRetired.clear();
   m.stageUsed = 0;
   return MLX_OK;
 }
 
-static int stage_take(mlx_mesh &m, size_t bytes, char **out) {
+int stage_take(mlx_mesh &m, size_t bytes, char **out) {
   const size_t off = (m.stageUsed + 255) & ~(size_t)255;
   if (off + bytes > m.stageCap) {
-    MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
     const size_t cap = std::max<size_t>(2 * (off + bytes), (size_t)1 << 20);
     char *p = nullptr;
     MLX_CHECK_HIP(hipHostMalloc((void **)&p, cap, hipHostMallocDefault));
-    if (m.h_stage) (void)hipHostFree(m.h_stage);
+    if (m.h_stage) m.stageRetired.push_back(m.h_stage);
     m.h_stage = p;
     m.stageCap = cap;
-    m.stageUsed = 0;
-    return stage_take(m, bytes, out);
+    *out = p;
+    m.stageUsed = bytes;
+    return MLX_OK;
   }
   m.stageUsed = off + bytes;
   *out = m.h_stage + off;
   return MLX_OK;
 }
 
-static int stage_upload(mlx_mesh &m, void *dst, const void *src, size_t bytes) {
+int stage_upload(mlx_mesh &m, void *dst, const void *src, size_t bytes) {
   if (bytes == 0) return MLX_OK;
   char *p = nullptr;
   MLX_TRY(stage_take(m, bytes, &p));
@@ -1295,6 +1297,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
   if (m->h_ctl) (void)hipHostFree(m->h_ctl);
   if (m->h_counts) (void)hipHostFree(m->h_counts);
   if (m->h_stage) (void)hipHostFree(m->h_stage);
+  for (char *p : m->stageRetired) (void)hipHostFree(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return MLX_OK;
@@ -1982,6 +1985,7 @@ int mlx_layer_maps(mlx_mesh *m, const double *keys, const double *sortedLayers, 
   MLX_CHECK_HIP(hipMemcpy(dk.ptr, keys, sizeof(double) * m->nT, hipMemcpyHostToDevice));
   HostMaps maps;
   std::vector<int32_t> hs;
+  MLX_TRY(stage_begin(*m));
   const int r = layer_maps_dev(*m, dk.as<double>(), sortedLayers, nSorted, maps, &hs);
   dk.release();
   if (r != MLX_OK) return r;
