@@ -96,7 +96,7 @@ def make_integrator(stop_rule=True, iterations=(2, 4, 14), hanging_stop=1000, ri
     return it
 
 
-def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
+def bench_batched(h, plane, steps, warmup, precision, nscenes=256, graph=True):
     """Batched form: nscenes beams (tet counts log-uniform in [2000, 60000],
     random aspect ratios, per-scene E log-uniform in [1e4, 1e7]) integrated
     in one persistent launch per substep, plain method (one all-elastic
@@ -142,7 +142,7 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
     b.frame = frame0
     # the timed substeps replay the captured graph of the whole substep; the
     # first replayable substep captures it and is not timed
-    b.set_graph(1)
+    b.set_graph(int(graph))
     b.step()
     p1, v1, _ = b.download()
     frame1 = b.frame
@@ -197,7 +197,8 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256):
                      "residual evaluation)", "avg_launch_ms": kms / ksteps,
                      "measured_on": "the timed substeps rerun from the same state with the "
                                     "kernels launched directly"},
-        "graph": "substeps replay one captured hipGraph (contacts, prologue, solve, velocity)",
+        "graph": "substeps replay one captured hipGraph (contacts, prologue, solve, velocity)"
+                 if graph else "substeps launched directly",
     }
 
 
@@ -250,6 +251,8 @@ def main():
                     help="sparse rigid layers sweep all colors in one workgroup (1) or launch "
                          "per color (0)")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
+                    help="timed batched substeps replay a captured graph (1) or launch directly (0, for kernel-trace profiling)")
     ap.add_argument("--batch-warmup", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -382,7 +385,7 @@ def main():
     if not args.no_batched:
         log("batched leg")
         line["batched"] = bench_batched(h, plane, args.batch_steps, args.batch_warmup,
-                                        args.precision)
+                                        args.precision, graph=bool(args.batch_graph))
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(mesh, h, plane)

@@ -167,7 +167,7 @@ struct mlx_mesh {
   mlx::DevBuf d_maps;              // sort keys/values, ranks, parents, labels, outputs
   mlx::DevBuf d_scan;              // rocPRIM temporary storage
   mlx::DevBuf d_sync;              // uint32 barrier counter | int32 error flag
-  mlx::DevBuf d_ecolor;            // int32 nT: 0-based color of each element
+  mlx::DevBuf d_colortab;          // int64 colorOff[numColors] | colorCnt[numColors]
 
   // sweep timing (mlx_profile_enable)
   struct Prof {

@@ -155,6 +155,37 @@ __global__ void k_cc_vmin_new(int64_t nT, int vpe, const int32_t *T, const int32
   for (int k = 0; k < vpe; ++k) atomicMin(&vmin[T[e * vpe + k]], (int32_t)e);
 }
 
+// Each newly inserted element starts under the smallest inserted element
+// sharing one of its vertices (vmin, which already covers the new ones): a
+// smaller id of the same component, so roots stay component minima.  The
+// unions that follow then mostly find both sides in one tree instead of
+// racing to hook the same few roots.
+__global__ void k_cc_hook_new(int64_t nT, int vpe, const int32_t *T, const int32_t *rank,
+                              const uint8_t *forced, int32_t prev, int32_t n, const int32_t *vmin,
+                              int32_t *parent) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int32_t r = rank[e];
+  if (r < prev || r >= n || forced[e]) return;
+  int32_t lo = (int32_t)e;
+  for (int k = 0; k < vpe; ++k) lo = min(lo, vmin[T[e * vpe + k]]);
+  parent[e] = lo;
+}
+
+// One round of pointer jumping over the newly inserted elements: parent
+// becomes the grandparent.  Concurrent rounds of other threads only move
+// pointers to other ancestors, so every pointer stays in its tree.
+__global__ void k_cc_jump_new(int64_t nT, const int32_t *rank, const uint8_t *forced, int32_t prev,
+                              int32_t n, int32_t *parent) {
+  const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int32_t r = rank[e];
+  if (r < prev || r >= n || forced[e]) return;
+  const int32_t p = cc_load(parent + e);
+  const int32_t pp = cc_load(parent + p);
+  if (pp != p) __hip_atomic_store(parent + e, pp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Union of a newly inserted element e with, for each of its vertices v,
 // the smallest earlier element containing v (vminOld: every earlier element
 // containing v is already in that one's tree) and the smallest element of
@@ -178,18 +209,30 @@ __global__ void k_cc_union_new(int64_t nT, int vpe, const int32_t *T, const int3
 
 // Elastic flags of every sorted layer in slot order (flags pre-zeroed, so
 // padding slots stay 0) and elastic counts per (layer, color).
-__global__ void k_slot_flags(int64_t nT, int ns, int64_t ld, int nc, const int32_t *sets,
-                             const int32_t *slotOf, const int32_t *ecolor, uint8_t *flags,
-                             int32_t *counts) {
+__global__ void k_slot_flags(int64_t nT, int ns, int64_t ld, const int32_t *sets,
+                             const int32_t *slotOf, uint8_t *flags) {
   const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
   if (e >= nT) return;
   const int64_t s = slotOf[e];
-  const int c = ecolor[e];
-  for (int l = 0; l < ns; ++l) {
-    const bool f = sets[(int64_t)l * nT + e] == 0;
-    flags[l * ld + s] = f ? 1 : 0;
-    if (f) atomicAdd(&counts[l * nc + c], 1);
+  for (int l = 0; l < ns; ++l) flags[l * ld + s] = sets[(int64_t)l * nT + e] == 0 ? 1 : 0;
+}
+
+// Elastic elements per (layer, color): slots are color-major, so one
+// workgroup per (layer, color) sums the flags of that color's slot range.
+__global__ void k_color_counts(int nc, int64_t ld, const int64_t *colorTab, const uint8_t *flags,
+                               int32_t *counts) {
+  __shared__ int32_t part[kMapBlock];
+  const int l = blockIdx.x / nc, c = blockIdx.x % nc;
+  const int64_t a = colorTab[c], e = a + colorTab[nc + c];
+  int32_t acc = 0;
+  for (int64_t s = a + threadIdx.x; s < e; s += kMapBlock) acc += flags[l * ld + s];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kMapBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) counts[blockIdx.x] = part[0];
 }
 
 // Body count of a layer: last scan entry + last root flag.
@@ -348,6 +391,11 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
       MLX_CHECK_HIP(hipMemcpyAsync(vminOld, vmin, 4 * (size_t)nV, hipMemcpyDeviceToDevice, st));
       hipLaunchKernelGGL(k_cc_vmin_new, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T,
                          rank, forced, prevInsert, nInsert, vmin);
+      hipLaunchKernelGGL(k_cc_hook_new, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T,
+                         rank, forced, prevInsert, nInsert, vmin, parent);
+      for (int round = 0; round < 6; ++round)
+        hipLaunchKernelGGL(k_cc_jump_new, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, rank,
+                           forced, prevInsert, nInsert, parent);
       hipLaunchKernelGGL(k_cc_union_new, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, vpe, T,
                          rank, forced, prevInsert, nInsert, vminOld, vmin, parent);
       prevInsert = nInsert;
@@ -388,21 +436,24 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
   if (deviceFlags) {
     // 4. elastic flags in slot order straight into m.d_flags
     const int nc = m.numColors;
-    if (m.d_ecolor.bytes < 4 * (size_t)nT) {
-      std::vector<int32_t> ec(nT);
-      for (int64_t e = 0; e < nT; ++e) ec[e] = m.color[e] - 1;
-      MLX_TRY(m.d_ecolor.reserve(4 * (size_t)nT));
-      MLX_CHECK_HIP(hipMemcpyAsync(m.d_ecolor.ptr, ec.data(), 4 * (size_t)nT,
-                                   hipMemcpyHostToDevice, st));
-      MLX_CHECK_HIP(hipStreamSynchronize(st));
+    if (m.d_colortab.bytes < 16 * (size_t)nc) {
+      // slot range of each color: colorOff | colorCnt (int64), uploaded once
+      std::vector<int64_t> ct(2 * (size_t)nc);
+      for (int c = 0; c < nc; ++c) {
+        ct[c] = m.colorOff[c];
+        ct[nc + c] = m.colorCnt[c];
+      }
+      MLX_TRY(m.d_colortab.reserve(16 * (size_t)nc));
+      MLX_TRY(stage_upload(m, m.d_colortab.ptr, ct.data(), 16 * (size_t)nc));
     }
     MLX_TRY(m.d_flags.reserve((size_t)ns * m.ld));
     int32_t *cc = at32(oColorCnt);
     MLX_CHECK_HIP(hipMemsetAsync(m.d_flags.ptr, 0, (size_t)ns * m.ld, st));
-    MLX_CHECK_HIP(hipMemsetAsync(cc, 0, 4 * (size_t)ns * nc, st));
-    hipLaunchKernelGGL(k_slot_flags, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, ns, m.ld, nc,
-                       sets, m.d_slotOf.as<int32_t>(), m.d_ecolor.as<int32_t>(),
-                       m.d_flags.as<uint8_t>(), cc);
+    hipLaunchKernelGGL(k_slot_flags, dim3(grid_for(nT)), dim3(kMapBlock), 0, st, nT, ns, m.ld,
+                       sets, m.d_slotOf.as<int32_t>(), m.d_flags.as<uint8_t>());
+    if (ns * nc > 0)
+      hipLaunchKernelGGL(k_color_counts, dim3((unsigned)(ns * nc)), dim3(kMapBlock), 0, st, nc,
+                         m.ld, m.d_colortab.as<int64_t>(), m.d_flags.as<uint8_t>(), cc);
     colorCnt.resize((size_t)ns * nc);
     MLX_TRY(take32(colorCnt.size(), &hCC));
     MLX_CHECK_HIP(hipMemcpyAsync(hCC, cc, 4 * colorCnt.size(), hipMemcpyDeviceToHost, st));

@@ -1286,7 +1286,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_slotOf, &m->d_x, &m->d_xold, &m->d_rv, &m->d_pinnedx,
                          &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
-                         &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced, &m->d_ecolor,
+                         &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced, &m->d_colortab,
                          &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist, &m->d_pinMask};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
