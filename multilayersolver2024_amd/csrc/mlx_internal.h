@@ -131,6 +131,7 @@ struct mlx_mesh {
   mlx::DevBuf d_resid;            // double residual array
   mlx::DevBuf d_tmp;              // misc
   mlx::DevBuf d_count;            // uint32 arrival counter of k_residual_tail (kept at 0)
+  mlx::DevBuf d_bodycnt;          // uint32 per-body arrival counters of the chunked restriction (kept at 0)
   mlx::DevBuf d_nh;               // T 4*ld: 2D NeoHookean mu | lambda | lambda_D | lambda_H
   mlx::Ctl *h_ctl = nullptr;      // pinned host mirror
   int32_t *h_counts = nullptr;    // pinned {K, nec, nrc, nUniq} staging for the device counts

@@ -589,8 +589,12 @@ struct LayerLists {
   const int32_t *bnd_off, *bnd_vert;  // boundary vertices of each body
   const int32_t *rc_off, *rc_idx;     // rigid contacts of each body
   const int32_t *ec_idx;              // elastic contacts
-  int nb, nbnd, nrc, nec;
+  const int32_t *chk_off;             // first restriction chunk of each body (nb + 1), or null
+  int nb, nbnd, nrc, nec, nchunks;
 };
+
+// Rigid vertices per workgroup of the chunked body restriction.
+constexpr int kRestrictChunk = 1024;
 
 // makeRigidsFromConnectivity / matComputeRigidBodyProperties3D
 // (AdaptiveMesh3D.m:32-43, RigidBody3D.m:198-265): mass, COM, COM velocity,
@@ -663,6 +667,145 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LayerLists L, const vec4_t<
     }
     solve3_pivot(s12, s12 + 9, B.omega);
   }
+}
+
+// The same restriction with each body's rigid vertices split in chunks of
+// kRestrictChunk, one workgroup per chunk, for bodies that span most of the
+// mesh.  Two launches: k_restrict_mass (mass, COM, COM velocity) then
+// k_restrict_inertia (r_i, inertia, angular momentum, omega).  Each chunk
+// stores its sums past the caches and counts itself on its body's counter;
+// the body's last chunk adds the chunks' sums in chunk order, finishes the
+// body and re-arms the counter.  Fixed order: deterministic.
+__device__ __forceinline__ int body_of_chunk(const int32_t *chk, int nb, int i) {
+  int lo = 0, hi = nb;  // chk[lo] <= i < chk[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (chk[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int NV>
+__device__ __forceinline__ bool chunk_publish(const double (&v)[NV], double *part, unsigned *cnt,
+                                              int i, int k, int nch, int *sLast) {
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      __hip_atomic_store(part + 12 * (size_t)i + j, v[j], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    const unsigned prev = __hip_atomic_fetch_add(cnt + k, 1u, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    *sLast = prev == (unsigned)(nch - 1);
+  }
+  __syncthreads();
+  return *sLast && threadIdx.x == 0;
+}
+
+template <int NV>
+__device__ __forceinline__ void chunk_total(const double *part, unsigned *cnt, int c0, int nch,
+                                            int k, double (&t)[NV]) {
+#pragma unroll
+  for (int j = 0; j < NV; ++j) t[j] = 0;
+  for (int c = 0; c < nch; ++c)
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      t[j] += __hip_atomic_load(part + 12 * (size_t)(c0 + c) + j, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(cnt + k, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict_mass(LayerLists L, const vec4_t<T> *x,
+                                                          const T *rv, const double *mass,
+                                                          Body *bodies, double *part,
+                                                          unsigned *cnt) {
+  __shared__ double sh[7 * kBlock];
+  __shared__ int sLast;
+  const int i = blockIdx.x;
+  const int k = body_of_chunk(L.chk_off, L.nb, i);
+  const int c0 = L.chk_off[k], nch = L.chk_off[k + 1] - c0;
+  const int off = L.rig_off[k], n = L.rig_off[k + 1] - off;
+  const int q0 = (i - c0) * kRestrictChunk, q1 = min(n, q0 + kRestrictChunk);
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int q = q0 + threadIdx.x; q < q1; q += kBlock) {
+    const int v = L.rig_vert[off + q];
+    const double m = mass[v];
+    const vec4_t<T> p = x[v];
+    a[0] += m;
+    a[1] += (double)p.x * m;
+    a[2] += (double)p.y * m;
+    a[3] += (double)p.z * m;
+    a[4] += (double)rv[3 * v] * m;
+    a[5] += (double)rv[3 * v + 1] * m;
+    a[6] += (double)rv[3 * v + 2] * m;
+  }
+  double s7[7];
+  block_sum<7>(a, sh, s7);
+  if (!chunk_publish<7>(s7, part, cnt, i, k, nch, &sLast)) return;
+  double t[7];
+  chunk_total<7>(part, cnt, c0, nch, k, t);
+  Body &B = bodies[k];
+  B.M = t[0];
+  B.invM = 1.0 / t[0];
+  for (int d = 0; d < 3; ++d) {
+    B.COM[d] = t[1 + d] / t[0];
+    B.COMDOT[d] = t[4 + d] / t[0];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict_inertia(LayerLists L, const vec4_t<T> *x,
+                                                             const T *rv, const double *mass,
+                                                             Body *bodies, double *ri,
+                                                             double *part, unsigned *cnt) {
+  __shared__ double sh[12 * kBlock];
+  __shared__ int sLast;
+  const int i = blockIdx.x;
+  const int k = body_of_chunk(L.chk_off, L.nb, i);
+  const int c0 = L.chk_off[k], nch = L.chk_off[k + 1] - c0;
+  const int off = L.rig_off[k], n = L.rig_off[k + 1] - off;
+  const int q0 = (i - c0) * kRestrictChunk, q1 = min(n, q0 + kRestrictChunk);
+  const Body &Bk = bodies[k];
+  const double com[3] = {Bk.COM[0], Bk.COM[1], Bk.COM[2]};
+  const double cdot[3] = {Bk.COMDOT[0], Bk.COMDOT[1], Bk.COMDOT[2]};
+  double b[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int q = q0 + threadIdx.x; q < q1; q += kBlock) {
+    const int v = L.rig_vert[off + q];
+    const double m = mass[v];
+    const vec4_t<T> p = x[v];
+    const double r[3] = {(double)p.x - com[0], (double)p.y - com[1], (double)p.z - com[2]};
+    const double vd[3] = {(double)rv[3 * v] - cdot[0], (double)rv[3 * v + 1] - cdot[1],
+                          (double)rv[3 * v + 2] - cdot[2]};
+    // -m * (rhat rhat) = m (|r|^2 I - r r^T)
+    const double rh[9] = {0, r[2], -r[1], -r[2], 0, r[0], r[1], -r[0], 0};
+    double rr[9];
+    matmul3(rh, rh, rr);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) b[j] += (-m) * rr[j];
+    double c[3];
+    cross3(r, vd, c);
+    b[9] += m * c[0];
+    b[10] += m * c[1];
+    b[11] += m * c[2];
+    ri[3 * v] = r[0];
+    ri[3 * v + 1] = r[1];
+    ri[3 * v + 2] = r[2];
+  }
+  double s12[12];
+  block_sum<12>(b, sh, s12);
+  if (!chunk_publish<12>(s12, part, cnt, i, k, nch, &sLast)) return;
+  double t[12];
+  chunk_total<12>(part, cnt, c0, nch, k, t);
+  Body &B = bodies[k];
+  for (int j = 0; j < 9; ++j) {
+    B.I0[j] = t[j];
+    B.I[j] = t[j];
+    B.R[j] = (j % 4 == 0) ? 1.0 : 0.0;
+    B.Rstep[j] = B.R[j];
+  }
+  solve3_pivot(t, t + 9, B.omega);
 }
 
 // rigidSymplecticImperativeStep (RigidBody3D.m:174-179).

@@ -289,9 +289,10 @@ static void prof_harvest(mlx_mesh &m, int itersDone) {
 
 // Per-layer CSR lists, built on the host from the layer maps.
 struct LayerCSR {
-  int nb = 0, nbnd = 0, nrc = 0, nec = 0;
-  std::vector<int32_t> buf;  // rig_off | rig_vert | bnd_off | bnd_vert | rc_off | rc_idx | ec_idx
-  size_t o_rig_off, o_rig_vert, o_bnd_off, o_bnd_vert, o_rc_off, o_rc_idx, o_ec;
+  int nb = 0, nbnd = 0, nrc = 0, nec = 0, nchunks = 0;
+  // rig_off | rig_vert | bnd_off | bnd_vert | rc_off | rc_idx | ec_idx | chk_off
+  std::vector<int32_t> buf;
+  size_t o_rig_off, o_rig_vert, o_bnd_off, o_bnd_vert, o_rc_off, o_rc_idx, o_ec, o_chk;
 };
 
 static void build_csr(const mlx_mesh &m, const HostMaps &maps, int col, const HostContacts &hc,
@@ -336,13 +337,19 @@ static void build_csr(const mlx_mesh &m, const HostMaps &maps, int col, const Ho
   L.o_rc_off = L.o_bnd_vert + nbnd;
   L.o_rc_idx = L.o_rc_off + nb + 1;
   L.o_ec = L.o_rc_idx + nrc;
-  L.buf.assign(L.o_ec + nec + 1, 0);
+  L.o_chk = L.o_ec + nec + 1;
+  L.buf.assign(L.o_chk + nb + 1, 0);
   int32_t *B = L.buf.data();
   for (int k = 0; k <= nb; ++k) {
     B[L.o_rig_off + k] = cr[k];
     B[L.o_bnd_off + k] = cb[k];
     B[L.o_rc_off + k] = cc[k];
   }
+  // restriction chunks: kRestrictChunk rigid vertices each, at least one per body
+  for (int k = 0; k < nb; ++k)
+    B[L.o_chk + k + 1] =
+        B[L.o_chk + k] + std::max(1, (cr[k + 1] - cr[k] + kRestrictChunk - 1) / kRestrictChunk);
+  L.nchunks = B[L.o_chk + nb];
   std::vector<int32_t> fr(cr.begin(), cr.end() - 1), fb(cb.begin(), cb.end() - 1),
       fc(cc.begin(), cc.end() - 1);
   for (int64_t v = 0; v < nV; ++v)
@@ -598,8 +605,8 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   // a region of its own at a fixed offset (captured graphs keep their
   // pointers).  Bodies are vertex-disjoint with at least one vertex each
   // and a vertex has at most one contact, so a layer needs at most
-  // 3 (nV + 1) + 3 nV + 1 entries whatever the substep's contacts.
-  const size_t csrCap = 6 * (size_t)nV + 8;
+  // 4 (nV + 1) + 3 nV + 1 entries whatever the substep's contacts.
+  const size_t csrCap = 7 * (size_t)nV + 16;
   std::vector<LayerCSR> Ls(sp.nLayers);
   MLX_TRY(m.d_csr.reserve(sizeof(int32_t) * csrCap * (size_t)sp.nLayers));
   for (int layer = 0; layer < sp.nLayers; ++layer) {
@@ -627,6 +634,8 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     LL.rc_off = cb + L.o_rc_off;
     LL.rc_idx = cb + L.o_rc_idx;
     LL.ec_idx = cb + L.o_ec;
+    LL.chk_off = L.nb > 0 ? cb + L.o_chk : nullptr;
+    LL.nchunks = L.nchunks;
     LL.nb = L.nb;
     LL.nbnd = L.nbnd;
     LL.nrc = L.nrc;
@@ -644,9 +653,23 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
                                  hipMemcpyHostToDevice, st));
 
     if (m.dim == 3) {
-      if (L.nb > 0)
+      if (L.nb > 0 && 12 * (int64_t)L.nchunks <= kMaxPartials) {
+        // bodies split in chunks over many workgroups (the coarse layers'
+        // bodies span most of the mesh); per-body arrival counters start at
+        // zero and are re-armed by each body's last chunk
+        const size_t need = sizeof(unsigned) * ((size_t)L.nb + 1);
+        if (m.d_bodycnt.bytes < need) {
+          MLX_TRY(m.d_bodycnt.reserve(need));
+          MLX_CHECK_HIP(hipMemsetAsync(m.d_bodycnt.ptr, 0, m.d_bodycnt.bytes, st));
+        }
+        hipLaunchKernelGGL(k_restrict_mass<T>, dim3(L.nchunks), dim3(kBlock), 0, st, LL, x, rv,
+                           mass, bodies, partials, m.d_bodycnt.as<unsigned>());
+        hipLaunchKernelGGL(k_restrict_inertia<T>, dim3(L.nchunks), dim3(kBlock), 0, st, LL, x, rv,
+                           mass, bodies, ri, partials, m.d_bodycnt.as<unsigned>());
+      } else if (L.nb > 0) {
         hipLaunchKernelGGL(k_restrict<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, rv, mass,
                            bodies, ri);
+      }
       const int step = hasRigidEl && L.nb > 0;
       if (step)
         hipLaunchKernelGGL(k_body_step, dim3(blocks_for(L.nb)), dim3(kBlock), 0, st, bodies,
@@ -748,7 +771,8 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
           (uint64_t)(L.nec > 0),
           (uint64_t)L.nrc, (uint64_t)(K > 0), (uint64_t)uniqCap, (uint64_t)L.o_rig_vert,
           (uint64_t)L.o_bnd_off, (uint64_t)L.o_bnd_vert, (uint64_t)L.o_rc_off,
-          (uint64_t)L.o_rc_idx, (uint64_t)L.o_ec, (uint64_t)sp.computeResiduals,
+          (uint64_t)L.o_rc_idx, (uint64_t)L.o_ec, (uint64_t)(L.nb > 0 ? L.o_chk : 0),
+          (uint64_t)sp.computeResiduals,
           (uint64_t)sp.useRigidContacts, (uint64_t)(uintptr_t)fl, (uint64_t)(uintptr_t)m.d_csr.ptr,
           (uint64_t)(uintptr_t)m.d_contacts.ptr, (uint64_t)(uintptr_t)m.d_tmp.ptr,
           (uint64_t)(uintptr_t)m.d_lamB.ptr, (uint64_t)(uintptr_t)m.d_bodies.ptr,
@@ -1287,7 +1311,8 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_flags, &m->d_vbody, &m->d_ri, &m->d_bodies, &m->d_csr,
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
                          &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced, &m->d_colortab,
-                         &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist, &m->d_pinMask};
+                         &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist, &m->d_pinMask,
+                         &m->d_bodycnt};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
