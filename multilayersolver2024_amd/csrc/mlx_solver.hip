@@ -232,6 +232,7 @@ static void enqueue_sweeps(mlx_mesh &m, SweepParams<T> P, const uint8_t *flags,
       int64_t elems = 0, launches = 0;
       for (int c = 0; c < m.numColors; ++c) {
         if (m.colorCnt[c] == 0) continue;
+        if (flags && elasticPerColor && elasticPerColor[c] == 0) continue;
         ++launches;
         elems += elasticPerColor ? elasticPerColor[c] : m.colorCnt[c];
       }
@@ -250,6 +251,9 @@ static void enqueue_sweeps(mlx_mesh &m, SweepParams<T> P, const uint8_t *flags,
     P.begin = m.colorOff[c];
     P.end = m.colorOff[c] + m.colorCnt[c];
     if (P.end <= P.begin) continue;
+    // a color with no elastic element in a layer with rigid ones projects
+    // nothing: not launched
+    if (flags && elasticPerColor && elasticPerColor[c] == 0) continue;
     const unsigned grid = (unsigned)((P.end - P.begin + nb - 1) / nb);
     if (m.dim == 3)
       hipLaunchKernelGGL(k_sweep3d<T>, dim3(grid), dim3(nb), 0, m.stream, P);
@@ -784,6 +788,16 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
           (uint64_t)(uintptr_t)m.d_vbody.ptr, bits(sp.h), bits(sp.boundaryCompliance),
           bits(sp.contactResistance), bits(sp.giveUpThreshold), (uint64_t)sp.hangingStop,
           (uint64_t)sp.giveUpEnabled, (uint64_t)sp.lastLayerGiveUpEnabled};
+      if (fl) {
+        // colors skipped for having no elastic element (bit per color)
+        const int64_t *pc = perColor.data() + (size_t)m.numColors * col;
+        for (int c0 = 0; c0 < m.numColors; c0 += 64) {
+          uint64_t word = 0;
+          for (int c = c0; c < std::min(m.numColors, c0 + 64); ++c)
+            if (pc[c] == 0) word |= (uint64_t)1 << (c - c0);
+          sig.push_back(word);
+        }
+      }
       for (auto &g : m.graphs)
         if (g.exec && g.sig == sig) return g.exec;
       if (!capture) return nullptr;
