@@ -910,7 +910,11 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     if (isLast) {
       // while it <= iterations || (~stopSolve && runUntil >= 0 && isLastLayer)
       int64_t guard = 0;
-      int chunk = predict ? 4 : 2;
+      // after a prediction the remaining need is usually one or two
+      // iterations: start small (an iteration enqueued past the stop still
+      // costs its ~45 early-exit launches, more than a read of the control
+      // block) and double
+      int chunk = predict ? 1 : 2;
       while (!hc_ctl.stopSolve && !hc_ctl.endLayer && hc_ctl.runUntil >= 0) {
         MLX_TRY(run_iterations(chunk));
         MLX_TRY(read_ctl(m, hc_ctl));
