@@ -1690,6 +1690,9 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
   const int64_t nPin = (int64_t)m.pinnedDOF.size();
   using clk = std::chrono::steady_clock;
   const auto t0 = clk::now();
+  // staging regions of the previous call are reclaimed while the stream is
+  // still idle (every call ends with the stream drained)
+  MLX_TRY(stage_begin(m));
   MLX_TRY(m.d_tmp.reserve(4096));
   // 1. frame-1 / NaN reset of cache.oldp and pinnedx (3D)
   if (dim == 3) {
@@ -1704,7 +1707,6 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
   // 2. contacts at the start-of-step positions: the plane test and the list
   // in ascending vertex order are built on the device; the list is read
   // back together with the layer maps, without a wait of its own
-  MLX_TRY(stage_begin(m));
   HostContacts hc;
   double nrm[3] = {0, 0, 0}, Dp = 0;
   const bool planeOn = plane && plane->enabled;
