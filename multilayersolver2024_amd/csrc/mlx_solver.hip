@@ -901,9 +901,12 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     // iterations the previous substep needed past the fixed count are
     // enqueued with the fixed ones: kernels return at once after the
     // monitor stops, so a guess that is too long costs only empty launches.
+    // In contact the count repeats from substep to substep, so the guess is
+    // the previous count itself (one read of the control block when it
+    // holds).
     int first = sp.iterations[layer];
     const bool predict = isLast && layer > 0 && hc_ctl.runUntil >= 0 && !persist;
-    if (predict) first += std::max(0, m.prevExtraIters - 1);
+    if (predict) first += m.prevExtraIters;
     MLX_TRY(run_iterations(first));
     MLX_TRY(read_ctl(m, hc_ctl));
     if (m.prof.on) prof_harvest(m, hc_ctl.iters);
