@@ -142,6 +142,7 @@ struct mlx_mesh {
   size_t stageCap = 0, stageUsed = 0;
   std::vector<char *> stageRetired;  // outgrown staging buffers, freed at stage_begin
   mlx::DevBuf d_pinMask;           // uint8 dim*nV: pinned DOFs (built once)
+  mlx::DevBuf d_seam;              // double: the layerSolve seam's arrays in caller layout
 
   // replayable graphs of iteration chunks (hipGraph), keyed by everything
   // the captured kernels bake into their parameters

@@ -184,6 +184,72 @@ static int upload_contacts(mlx_mesh &m, int dim, int K, const int32_t *vid1,
   return MLX_OK;
 }
 
+// p[0..n) = v and dst = src, as kernels: a substep that is captured into a
+// graph holds no memset or memcpy nodes.
+template <typename V>
+__global__ void k_fill(int64_t n, V *p, V v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename V>
+__global__ void k_copy(int64_t n, const V *src, V *dst) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Layout conversions of the layerSolve seam on the device.  The caller's
+// arrays (double, Matlab layouts) travel through pinned staging as they
+// are, and these kernels pack them: positions into vec4 with w = 1/m,
+// lambda (nT x nc column-major, element order) into the slots, scalars to
+// the working precision; the unpack kernels reverse it for the results.
+template <typename T>
+__global__ void k_pack_positions(int64_t nV, int dim, const double *src, const double *minv,
+                                 vec4_t<T> *dst) {
+  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nV) return;
+  vec4_t<T> q;
+  q.x = (T)src[dim * v];
+  q.y = (T)src[dim * v + 1];
+  q.z = dim == 3 ? (T)src[dim * v + 2] : T(0);
+  q.w = (T)minv[v];
+  dst[v] = q;
+}
+
+template <typename T>
+__global__ void k_unpack_positions(int64_t nV, int dim, const vec4_t<T> *src, double *dst) {
+  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nV) return;
+  const vec4_t<T> q = src[v];
+  dst[dim * v] = (double)q.x;
+  dst[dim * v + 1] = (double)q.y;
+  if (dim == 3) dst[dim * v + 2] = (double)q.z;
+}
+
+template <typename T>
+__global__ void k_pack_lambda(int64_t nT, int nc, int64_t ld, const double *src,
+                              const int32_t *slotOf, T *lam) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int64_t s = slotOf[e];
+  for (int i = 0; i < nc; ++i) lam[i * ld + s] = (T)src[e + nT * i];
+}
+
+template <typename T>
+__global__ void k_unpack_lambda(int64_t nT, int nc, int64_t ld, const T *lam,
+                                const int32_t *slotOf, double *dst) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int64_t s = slotOf[e];
+  for (int i = 0; i < nc; ++i) dst[e + nT * i] = (double)lam[i * ld + s];
+}
+
+template <typename S, typename D>
+__global__ void k_cast(int64_t n, const S *src, D *dst) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = (D)src[i];
+}
+
 template <typename T>
 static SweepParams<T> sweep_params(mlx_mesh &m, double h, const Ctl *ctl) {
   SweepParams<T> P;
@@ -1316,6 +1382,79 @@ static int mesh_create_impl(const mlx_mesh_desc *d, int32_t precision, int nScen
   return MLX_OK;
 }
 
+namespace mlx {
+
+// The seam's inputs: the caller's arrays go up through pinned staging
+// unchanged and are packed on the device (xi, oldp, lambdai, the residual
+// velocity in 3D, the pin targets).
+template <typename T>
+static int seam_upload(mlx_mesh &m, const mlx_layer_solve_args *a, const double *pinv,
+                       size_t nPin) {
+  const int64_t nV = m.nV, nT = m.nT, ld = m.ld;
+  const int dim = m.dim, nc = m.nc;
+  hipStream_t st = m.stream;
+  const size_t nPos = (size_t)dim * nV, nLam = (size_t)nc * nT, nRv = dim == 3 ? 3 * (size_t)nV : 0;
+  MLX_TRY(m.d_seam.reserve(sizeof(double) * (2 * nPos + nLam + nRv + nPin + 8)));
+  double *rx = m.d_seam.as<double>(), *ro = rx + nPos, *rl = ro + nPos, *rr = rl + nLam,
+         *rp = rr + nRv;
+  MLX_TRY(stage_upload(m, rx, a->xi, sizeof(double) * nPos));
+  MLX_TRY(stage_upload(m, ro, a->oldp, sizeof(double) * nPos));
+  MLX_TRY(stage_upload(m, rl, a->lambdai, sizeof(double) * nLam));
+  if (nRv) MLX_TRY(stage_upload(m, rr, a->residualv, sizeof(double) * nRv));
+  if (nPin) MLX_TRY(stage_upload(m, rp, pinv, sizeof(double) * nPin));
+  MLX_TRY(m.d_x.reserve(sizeof(vec4_t<T>) * nV));
+  MLX_TRY(m.d_xold.reserve(sizeof(vec4_t<T>) * nV));
+  MLX_TRY(m.d_rv.reserve(sizeof(T) * (nRv > 0 ? nRv : 1)));
+  MLX_TRY(m.d_pinnedx.reserve(sizeof(T) * (nPin > 0 ? nPin : 1)));
+  const double *minv = m.d_minv64.as<double>();
+  hipLaunchKernelGGL(k_pack_positions<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV, dim,
+                     (const double *)rx, minv, m.d_x.as<vec4_t<T>>());
+  hipLaunchKernelGGL(k_pack_positions<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV, dim,
+                     (const double *)ro, minv, m.d_xold.as<vec4_t<T>>());
+  hipLaunchKernelGGL(k_fill<T>, dim3(blocks_for((int64_t)nc * ld)), dim3(kBlock), 0, st,
+                     (int64_t)nc * ld, m.d_lam.as<T>(), T(0));
+  hipLaunchKernelGGL(k_pack_lambda<T>, dim3(blocks_for(nT)), dim3(kBlock), 0, st, nT, nc, ld,
+                     (const double *)rl, m.d_slotOf.as<int32_t>(), m.d_lam.as<T>());
+  if (nRv)
+    hipLaunchKernelGGL((k_cast<double, T>), dim3(blocks_for((int64_t)nRv)), dim3(kBlock), 0, st,
+                       (int64_t)nRv, (const double *)rr, m.d_rv.as<T>());
+  if (nPin)
+    hipLaunchKernelGGL((k_cast<double, T>), dim3(blocks_for((int64_t)nPin)), dim3(kBlock), 0, st,
+                       (int64_t)nPin, (const double *)rp, m.d_pinnedx.as<T>());
+  return MLX_OK;
+}
+
+// The seam's outputs: unpacked on the device, read back through pinned
+// staging with one synchronization, copied into the caller's arrays.
+template <typename T>
+static int seam_download(mlx_mesh &m, const mlx_layer_solve_args *a) {
+  const int64_t nV = m.nV, nT = m.nT, ld = m.ld;
+  const int dim = m.dim, nc = m.nc;
+  hipStream_t st = m.stream;
+  const size_t nPos = (size_t)dim * nV, nLam = (size_t)nc * nT, nRv = dim == 3 ? 3 * (size_t)nV : 0;
+  MLX_TRY(m.d_seam.reserve(sizeof(double) * (nPos + nLam + nRv + 8)));
+  double *dx = m.d_seam.as<double>(), *dl = dx + nPos, *dr = dl + nLam;
+  hipLaunchKernelGGL(k_unpack_positions<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV, dim,
+                     (const vec4_t<T> *)m.d_x.as<vec4_t<T>>(), dx);
+  hipLaunchKernelGGL(k_unpack_lambda<T>, dim3(blocks_for(nT)), dim3(kBlock), 0, st, nT, nc, ld,
+                     (const T *)m.d_lam.as<T>(), m.d_slotOf.as<int32_t>(), dl);
+  if (nRv)
+    hipLaunchKernelGGL((k_cast<T, double>), dim3(blocks_for((int64_t)nRv)), dim3(kBlock), 0, st,
+                       (int64_t)nRv, (const T *)m.d_rv.as<T>(), dr);
+  char *hp = nullptr;
+  MLX_TRY(stage_take(m, sizeof(double) * (nPos + nLam + nRv), &hp));
+  MLX_CHECK_HIP(hipMemcpyAsync(hp, dx, sizeof(double) * (nPos + nLam + nRv),
+                               hipMemcpyDeviceToHost, st));
+  MLX_CHECK_HIP(hipStreamSynchronize(st));
+  const double *h = reinterpret_cast<const double *>(hp);
+  std::memcpy(a->xi, h, sizeof(double) * nPos);
+  std::memcpy(a->lambdai, h + nPos, sizeof(double) * nLam);
+  if (nRv) std::memcpy(a->residualv, h + nPos + nLam, sizeof(double) * nRv);
+  return MLX_OK;
+}
+
+}  // namespace mlx
+
 extern "C" {
 
 int mlx_mesh_create(const mlx_mesh_desc *d, int32_t precision, mlx_mesh **out) {
@@ -1333,7 +1472,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
                          &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced, &m->d_colortab,
                          &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist, &m->d_pinMask,
-                         &m->d_bodycnt};
+                         &m->d_bodycnt, &m->d_seam};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
@@ -1463,30 +1602,15 @@ int mlx_layer_solve(mlx_mesh *m, const mlx_layer_solve_args *a, mlx_layer_solve_
   SolveResult res;
   int r;
   if (f64) {
-    MLX_TRY(upload_positions<double>(*m, m->d_x, a->xi));
-    MLX_TRY(upload_positions<double>(*m, m->d_xold, a->oldp));
-    MLX_TRY(upload_lambda<double>(*m, a->lambdai));
-    MLX_TRY(upload_scalars<double>(m->d_pinnedx, pinv.data(), nPin));
-    if (m->dim == 3) MLX_TRY(upload_scalars<double>(m->d_rv, a->residualv, 3 * nV));
+    MLX_TRY(seam_upload<double>(*m, a, pinv.data(), nPin));
     r = layer_solve_dev<double>(*m, maps, sp, hc, res);
   } else {
-    MLX_TRY(upload_positions<float>(*m, m->d_x, a->xi));
-    MLX_TRY(upload_positions<float>(*m, m->d_xold, a->oldp));
-    MLX_TRY(upload_lambda<float>(*m, a->lambdai));
-    MLX_TRY(upload_scalars<float>(m->d_pinnedx, pinv.data(), nPin));
-    if (m->dim == 3) MLX_TRY(upload_scalars<float>(m->d_rv, a->residualv, 3 * nV));
+    MLX_TRY(seam_upload<float>(*m, a, pinv.data(), nPin));
     r = layer_solve_dev<float>(*m, maps, sp, hc, res);
   }
   if (r != MLX_OK) return r;
-  if (f64) {
-    MLX_TRY(download_positions<double>(*m, m->d_x, a->xi));
-    MLX_TRY(download_lambda<double>(*m, a->lambdai));
-    if (m->dim == 3) MLX_TRY(download_scalars<double>(m->d_rv, a->residualv, 3 * nV));
-  } else {
-    MLX_TRY(download_positions<float>(*m, m->d_x, a->xi));
-    MLX_TRY(download_lambda<float>(*m, a->lambdai));
-    if (m->dim == 3) MLX_TRY(download_scalars<float>(m->d_rv, a->residualv, 3 * nV));
-  }
+  if (f64) MLX_TRY(seam_download<double>(*m, a));
+  else MLX_TRY(seam_download<float>(*m, a));
   out->runUntilResSmallerThan = res.runUntil;
   out->residualCount = res.residualCount;
   if (out->residualArray)
@@ -2210,20 +2334,6 @@ static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
         return 0;
       }
   return (int)nv;
-}
-
-// p[0..n) = v and dst = src, as kernels: a substep that is captured into a
-// graph holds no memset or memcpy nodes.
-template <typename V>
-__global__ void k_fill(int64_t n, V *p, V v) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-template <typename V>
-__global__ void k_copy(int64_t n, const V *src, V *dst) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) dst[i] = src[i];
 }
 
 // First contact of each scene: the number of flagged vertices before the
