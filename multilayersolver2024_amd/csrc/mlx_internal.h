@@ -143,6 +143,7 @@ struct mlx_mesh {
   std::vector<char *> stageRetired;  // outgrown staging buffers, freed at stage_begin
   mlx::DevBuf d_pinMask;           // uint8 dim*nV: pinned DOFs (built once)
   mlx::DevBuf d_seam;              // double: the layerSolve seam's arrays in caller layout
+  mlx::DevBuf d_elastic;           // uint8 nSorted*nT: caller's elastic flags (element order)
 
   // replayable graphs of iteration chunks (hipGraph), keyed by everything
   // the captured kernels bake into their parameters

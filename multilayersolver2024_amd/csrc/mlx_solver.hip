@@ -250,6 +250,16 @@ __global__ void k_cast(int64_t n, const S *src, D *dst) {
   if (i < n) dst[i] = (D)src[i];
 }
 
+// Elastic flags of every layer from element order (0/1 per element and
+// layer) into slot order; padding slots keep the 0 they were filled with.
+__global__ void k_slot_flags_u8(int64_t nT, int ns, int64_t ld, const uint8_t *elastic,
+                                const int32_t *slotOf, uint8_t *flags) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nT) return;
+  const int64_t s = slotOf[e];
+  for (int l = 0; l < ns; ++l) flags[l * ld + s] = elastic[(int64_t)l * nT + e] ? 1 : 0;
+}
+
 template <typename T>
 static SweepParams<T> sweep_params(mlx_mesh &m, double h, const Ctl *ctl) {
   SweepParams<T> P;
@@ -497,22 +507,22 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
   hipStream_t st = m.stream;
   // elastic flags in slot order, one column per sorted layer (already on
   // the device when the maps were built there)
-  std::vector<uint8_t> flags;
   std::vector<int64_t> nElastic(ns, 0), perColor((size_t)ns * m.numColors, 0);
   if (maps.deviceFlags) {
     nElastic = maps.nElastic;
     perColor = maps.perColor;
   } else {
-    flags.assign((size_t)ns * ld, 0);
-    for (int l = 0; l < ns; ++l)
-      for (int c = 0; c < m.numColors; ++c)
-        for (int64_t s = m.colorOff[c]; s < m.colorOff[c] + m.colorCnt[c]; ++s) {
-          const int32_t e = m.perm[s];
-          const uint8_t f = maps.elastic[(size_t)nT * l + e] ? 1 : 0;
-          flags[(size_t)ld * l + s] = f;
-          nElastic[l] += f;
-          perColor[(size_t)m.numColors * l + c] += f;
+    // counts by a sequential pass over the elements; the flags are put in
+    // slot order on the device (k_slot_flags_u8 below)
+    for (int l = 0; l < ns; ++l) {
+      const uint8_t *el = maps.elastic.data() + (size_t)nT * l;
+      int64_t *pc = perColor.data() + (size_t)m.numColors * l;
+      for (int64_t e = 0; e < nT; ++e)
+        if (el[e]) {
+          ++nElastic[l];
+          ++pc[m.color[e] - 1];
         }
+    }
   }
   // layers with rigid elements and at most kSweepWg elastic elements in
   // every color sweep all colors in one workgroup (k_sweep_wg)
@@ -537,8 +547,13 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     MLX_TRY(stage_upload(m, m.d_elist.ptr, oc.data(), 16 * nc));
   }
   if (!maps.deviceFlags) {
-    MLX_TRY(m.d_flags.reserve(flags.size()));
-    MLX_TRY(stage_upload(m, m.d_flags.ptr, flags.data(), flags.size()));
+    MLX_TRY(m.d_flags.reserve((size_t)ns * ld));
+    MLX_TRY(m.d_elastic.reserve(std::max<size_t>(maps.elastic.size(), 1)));
+    MLX_TRY(stage_upload(m, m.d_elastic.ptr, maps.elastic.data(), maps.elastic.size()));
+    MLX_CHECK_HIP(hipMemsetAsync(m.d_flags.ptr, 0, (size_t)ns * ld, st));
+    hipLaunchKernelGGL(k_slot_flags_u8, dim3(blocks_for(nT)), dim3(kBlock), 0, st, nT, ns, ld,
+                       (const uint8_t *)m.d_elastic.as<uint8_t>(), m.d_slotOf.as<int32_t>(),
+                       m.d_flags.as<uint8_t>());
   }
   MLX_TRY(m.d_vbody.reserve(sizeof(int32_t) * maps.vbody.size()));
   if (maps.devVbody)
@@ -1472,7 +1487,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
                          &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced, &m->d_colortab,
                          &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist, &m->d_pinMask,
-                         &m->d_bodycnt, &m->d_seam};
+                         &m->d_bodycnt, &m->d_seam, &m->d_elastic};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
