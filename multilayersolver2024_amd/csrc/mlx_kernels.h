@@ -590,11 +590,14 @@ struct LayerLists {
   const int32_t *rc_off, *rc_idx;     // rigid contacts of each body
   const int32_t *ec_idx;              // elastic contacts
   const int32_t *chk_off;             // first restriction chunk of each body (nb + 1), or null
-  int nb, nbnd, nrc, nec, nchunks;
+  const int32_t *bchk_off;            // first boundary chunk of each body (nb + 1), or null
+  int nb, nbnd, nrc, nec, nchunks, nbchunks;
 };
 
 // Rigid vertices per workgroup of the chunked body restriction.
 constexpr int kRestrictChunk = 1024;
+// Boundary vertices per workgroup of the chunked boundary accumulation.
+constexpr int kBoundaryChunk = 1024;
 
 // makeRigidsFromConnectivity / matComputeRigidBodyProperties3D
 // (AdaptiveMesh3D.m:32-43, RigidBody3D.m:198-265): mass, COM, COM velocity,
@@ -921,6 +924,52 @@ __device__ __forceinline__ void body_apply(Body &B, const double *dCOM, const do
   inertia_update(B.R, B.I0, B.I);
 }
 
+// rigidBoundaryConstraint.project (rigidBoundaryConstraint.m:4-35) for
+// boundary entry q (vertex v) of body B: updates its multiplier and adds the
+// impulse's dCOM and dOmega terms to acc.
+template <typename T>
+__device__ __forceinline__ void boundary_project(const Body &B, int v, const vec4_t<T> *x,
+                                                 const double *minv, const double *ri,
+                                                 double *lamq, double compliance, double h,
+                                                 double (&acc)[6]) {
+  double rx[3], r[3], dir[3], nn[3];
+  rigid_pos(B.R, ri + 3 * v, B.COM, rx);
+  const vec4_t<T> p = x[v];
+  const double xe[3] = {(double)p.x, (double)p.y, (double)p.z};
+  for (int d = 0; d < 3; ++d) {
+    r[d] = rx[d] - B.COM[d];
+    dir[d] = rx[d] - xe[d];
+  }
+  const double C = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  for (int d = 0; d < 3; ++d) {
+    nn[d] = dir[d] / C;
+    if (nn[d] != nn[d]) nn[d] = 0;
+  }
+  const double w = gen_inv_mass(B.invM, r, B.I, nn);
+  const double dl = (-C - compliance * *lamq) / ((minv[v] + w) + compliance);
+  *lamq += dl;
+  const double pp[3] = {dl * nn[0], dl * nn[1], dl * nn[2]};
+  const double ph[3] = {pp[0] / h, pp[1] / h, pp[2] / h};
+  double dom[3];
+  cross3(r, ph, dom);
+  for (int d = 0; d < 3; ++d) {
+    acc[d] += pp[d] * B.invM;
+    acc[3 + d] += dom[d];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void boundary_snap(const double *R, const double *COM, int v,
+                                              const double *ri, vec4_t<T> *x) {
+  double xn[3];
+  rigid_pos(R, ri + 3 * v, COM, xn);
+  vec4_t<T> p = x[v];
+  p.x = (T)xn[0];
+  p.y = (T)xn[1];
+  p.z = (T)xn[2];
+  x[v] = p;
+}
+
 // rigidElasticBoundaryAccumulationConstraints (xpbdLayer3D.m:381-420) with
 // rigidBoundaryConstraint.project (rigidBoundaryConstraint.m:4-35).  One
 // workgroup per body: project its boundary vertices, reduce, apply, snap.
@@ -936,33 +985,8 @@ __global__ __launch_bounds__(kBlock) void k_boundary(LayerLists L, vec4_t<T> *x,
   Body &B = bodies[k];
   const int off = L.bnd_off[k], n = L.bnd_off[k + 1] - off;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int q = threadIdx.x; q < n; q += kBlock) {
-    const int v = L.bnd_vert[off + q];
-    double rx[3], r[3], dir[3], nn[3];
-    rigid_pos(B.R, ri + 3 * v, B.COM, rx);
-    const vec4_t<T> p = x[v];
-    const double xe[3] = {(double)p.x, (double)p.y, (double)p.z};
-    for (int d = 0; d < 3; ++d) {
-      r[d] = rx[d] - B.COM[d];
-      dir[d] = rx[d] - xe[d];
-    }
-    const double C = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-    for (int d = 0; d < 3; ++d) {
-      nn[d] = dir[d] / C;
-      if (nn[d] != nn[d]) nn[d] = 0;
-    }
-    const double w = gen_inv_mass(B.invM, r, B.I, nn);
-    const double dl = (-C - compliance * lamB[off + q]) / ((minv[v] + w) + compliance);
-    lamB[off + q] += dl;
-    const double pp[3] = {dl * nn[0], dl * nn[1], dl * nn[2]};
-    const double ph[3] = {pp[0] / h, pp[1] / h, pp[2] / h};
-    double dom[3];
-    cross3(r, ph, dom);
-    for (int d = 0; d < 3; ++d) {
-      acc[d] += pp[d] * B.invM;
-      acc[3 + d] += dom[d];
-    }
-  }
+  for (int q = threadIdx.x; q < n; q += kBlock)
+    boundary_project<T>(B, L.bnd_vert[off + q], x, minv, ri, lamB + off + q, compliance, h, acc);
   double s6[6];
   block_sum<6>(acc, sh, s6);
   __shared__ double sR[9], sC[3];
@@ -972,16 +996,54 @@ __global__ __launch_bounds__(kBlock) void k_boundary(LayerLists L, vec4_t<T> *x,
     for (int i = 0; i < 3; ++i) sC[i] = B.COM[i];
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < n; q += kBlock) {
-    const int v = L.bnd_vert[off + q];
-    double xn[3];
-    rigid_pos(sR, ri + 3 * v, sC, xn);
-    vec4_t<T> p = x[v];
-    p.x = (T)xn[0];
-    p.y = (T)xn[1];
-    p.z = (T)xn[2];
-    x[v] = p;
-  }
+  for (int q = threadIdx.x; q < n; q += kBlock)
+    boundary_snap<T>(sR, sC, L.bnd_vert[off + q], ri, x);
+}
+
+// The same accumulation with each body's boundary vertices split in chunks
+// of kBoundaryChunk, one workgroup per chunk (every body has at least one
+// chunk, so a body without boundary vertices is still applied, with zero
+// sums, as above).  Each chunk projects its vertices against the body's
+// pose at the start and publishes its sums; the body's last chunk adds them
+// in chunk order and applies them.  k_boundary_snap then moves every
+// boundary vertex to its body's new pose.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_boundary_chunk(LayerLists L, const vec4_t<T> *x,
+                                                           const double *minv, Body *bodies,
+                                                           const double *ri, double *lamB,
+                                                           double compliance, double h,
+                                                           const Ctl *ctl, double *part,
+                                                           unsigned *cnt) {
+  if (iteration_stopped(ctl)) return;
+  __shared__ double sh[6 * kBlock];
+  __shared__ int sLast;
+  const int i = blockIdx.x;
+  const int k = body_of_chunk(L.bchk_off, L.nb, i);
+  const int c0 = L.bchk_off[k], nch = L.bchk_off[k + 1] - c0;
+  const int off = L.bnd_off[k], n = L.bnd_off[k + 1] - off;
+  const int q0 = (i - c0) * kBoundaryChunk, q1 = min(n, q0 + kBoundaryChunk);
+  const Body &Bk = bodies[k];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = q0 + threadIdx.x; q < q1; q += kBlock)
+    boundary_project<T>(Bk, L.bnd_vert[off + q], x, minv, ri, lamB + off + q, compliance, h, acc);
+  double s6[6];
+  block_sum<6>(acc, sh, s6);
+  if (!chunk_publish<6>(s6, part, cnt, i, k, nch, &sLast)) return;
+  double t[6];
+  chunk_total<6>(part, cnt, c0, nch, k, t);
+  body_apply(bodies[k], t, t + 3, h);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_boundary_snap(LayerLists L, vec4_t<T> *x,
+                                                          const Body *bodies, const double *ri,
+                                                          const Ctl *ctl) {
+  if (iteration_stopped(ctl)) return;
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= L.nbnd) return;
+  const int k = body_of_chunk(L.bnd_off, L.nb, q);
+  const Body &B = bodies[k];
+  boundary_snap<T>(B.R, B.COM, L.bnd_vert[q], ri, x);
 }
 
 // ---------------------------------------------------------------------------

@@ -369,10 +369,10 @@ static void prof_harvest(mlx_mesh &m, int itersDone) {
 
 // Per-layer CSR lists, built on the host from the layer maps.
 struct LayerCSR {
-  int nb = 0, nbnd = 0, nrc = 0, nec = 0, nchunks = 0;
-  // rig_off | rig_vert | bnd_off | bnd_vert | rc_off | rc_idx | ec_idx | chk_off
+  int nb = 0, nbnd = 0, nrc = 0, nec = 0, nchunks = 0, nbchunks = 0;
+  // rig_off | rig_vert | bnd_off | bnd_vert | rc_off | rc_idx | ec_idx | chk_off | bchk_off
   std::vector<int32_t> buf;
-  size_t o_rig_off, o_rig_vert, o_bnd_off, o_bnd_vert, o_rc_off, o_rc_idx, o_ec, o_chk;
+  size_t o_rig_off, o_rig_vert, o_bnd_off, o_bnd_vert, o_rc_off, o_rc_idx, o_ec, o_chk, o_bchk;
 };
 
 static void build_csr(const mlx_mesh &m, const HostMaps &maps, int col, const HostContacts &hc,
@@ -418,7 +418,8 @@ static void build_csr(const mlx_mesh &m, const HostMaps &maps, int col, const Ho
   L.o_rc_idx = L.o_rc_off + nb + 1;
   L.o_ec = L.o_rc_idx + nrc;
   L.o_chk = L.o_ec + nec + 1;
-  L.buf.assign(L.o_chk + nb + 1, 0);
+  L.o_bchk = L.o_chk + nb + 1;
+  L.buf.assign(L.o_bchk + nb + 1, 0);
   int32_t *B = L.buf.data();
   for (int k = 0; k <= nb; ++k) {
     B[L.o_rig_off + k] = cr[k];
@@ -430,6 +431,11 @@ static void build_csr(const mlx_mesh &m, const HostMaps &maps, int col, const Ho
     B[L.o_chk + k + 1] =
         B[L.o_chk + k] + std::max(1, (cr[k + 1] - cr[k] + kRestrictChunk - 1) / kRestrictChunk);
   L.nchunks = B[L.o_chk + nb];
+  // boundary chunks: kBoundaryChunk boundary vertices each, at least one per body
+  for (int k = 0; k < nb; ++k)
+    B[L.o_bchk + k + 1] =
+        B[L.o_bchk + k] + std::max(1, (cb[k + 1] - cb[k] + kBoundaryChunk - 1) / kBoundaryChunk);
+  L.nbchunks = B[L.o_bchk + nb];
   std::vector<int32_t> fr(cr.begin(), cr.end() - 1), fb(cb.begin(), cb.end() - 1),
       fc(cc.begin(), cc.end() - 1);
   for (int64_t v = 0; v < nV; ++v)
@@ -720,7 +726,9 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     LL.rc_idx = cb + L.o_rc_idx;
     LL.ec_idx = cb + L.o_ec;
     LL.chk_off = L.nb > 0 ? cb + L.o_chk : nullptr;
+    LL.bchk_off = L.nb > 0 ? cb + L.o_bchk : nullptr;
     LL.nchunks = L.nchunks;
+    LL.nbchunks = L.nbchunks;
     LL.nb = L.nb;
     LL.nbnd = L.nbnd;
     LL.nrc = L.nrc;
@@ -738,15 +746,18 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
                                  hipMemcpyHostToDevice, st));
 
     if (m.dim == 3) {
-      if (L.nb > 0 && 12 * (int64_t)L.nchunks <= kMaxPartials) {
-        // bodies split in chunks over many workgroups (the coarse layers'
-        // bodies span most of the mesh); per-body arrival counters start at
-        // zero and are re-armed by each body's last chunk
+      if (L.nb > 0) {
+        // per-body arrival counters of the chunked restriction and boundary
+        // kernels: they start at zero and each body's last chunk re-arms its own
         const size_t need = sizeof(unsigned) * ((size_t)L.nb + 1);
         if (m.d_bodycnt.bytes < need) {
           MLX_TRY(m.d_bodycnt.reserve(need));
           MLX_CHECK_HIP(hipMemsetAsync(m.d_bodycnt.ptr, 0, m.d_bodycnt.bytes, st));
         }
+      }
+      if (L.nb > 0 && 12 * (int64_t)L.nchunks <= kMaxPartials) {
+        // bodies split in chunks over many workgroups (the coarse layers'
+        // bodies span most of the mesh)
         hipLaunchKernelGGL(k_restrict_mass<T>, dim3(L.nchunks), dim3(kBlock), 0, st, LL, x, rv,
                            mass, bodies, partials, m.d_bodycnt.as<unsigned>());
         hipLaunchKernelGGL(k_restrict_inertia<T>, dim3(L.nchunks), dim3(kBlock), 0, st, LL, x, rv,
@@ -794,9 +805,19 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
         enqueue_sweep_wg<T>(m, P);
       else
         enqueue_sweeps<T>(m, P, fl, perColor.data() + (size_t)m.numColors * col, iterEnqueued++);
-      if (m.dim == 3 && L.nbnd > 0)
+      if (m.dim == 3 && L.nbnd > 0 && 12 * (int64_t)L.nbchunks <= kMaxPartials) {
+        // the partials are free here: the restriction and the layer's norm
+        // consumed theirs before the iterations, the residual tail writes its
+        // own after this
+        hipLaunchKernelGGL(k_boundary_chunk<T>, dim3(L.nbchunks), dim3(kBlock), 0, st, LL, x,
+                           minv, bodies, ri, m.d_lamB.as<double>(), sp.boundaryCompliance, sp.h,
+                           ctl, partials, m.d_bodycnt.as<unsigned>());
+        hipLaunchKernelGGL(k_boundary_snap<T>, dim3(blocks_for(L.nbnd)), dim3(kBlock), 0, st, LL,
+                           x, bodies, ri, ctl);
+      } else if (m.dim == 3 && L.nbnd > 0) {
         hipLaunchKernelGGL(k_boundary<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, minv, bodies,
                            ri, m.d_lamB.as<double>(), sp.boundaryCompliance, sp.h, ctl);
+      }
       if (m.dim == 2 && L.nbnd > 0)
         hipLaunchKernelGGL(k_boundary2d<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, minv, bodies,
                            ri, m.d_lamB.as<double>(), sp.boundaryCompliance / (sp.h * sp.h),
@@ -856,7 +877,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
           (uint64_t)(L.nec > 0),
           (uint64_t)L.nrc, (uint64_t)(K > 0), (uint64_t)uniqCap, (uint64_t)L.o_rig_vert,
           (uint64_t)L.o_bnd_off, (uint64_t)L.o_bnd_vert, (uint64_t)L.o_rc_off,
-          (uint64_t)L.o_rc_idx, (uint64_t)L.o_ec, (uint64_t)(L.nb > 0 ? L.o_chk : 0),
+          (uint64_t)L.o_rc_idx, (uint64_t)L.o_ec, (uint64_t)(L.nb > 0 ? L.o_chk : 0), (uint64_t)L.nbchunks,
           (uint64_t)sp.computeResiduals,
           (uint64_t)sp.useRigidContacts, (uint64_t)(uintptr_t)fl, (uint64_t)(uintptr_t)m.d_csr.ptr,
           (uint64_t)(uintptr_t)m.d_contacts.ptr, (uint64_t)(uintptr_t)m.d_tmp.ptr,
