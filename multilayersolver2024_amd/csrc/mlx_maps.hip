@@ -192,19 +192,48 @@ __global__ void k_cc_jump_new(int64_t nT, const int32_t *rank, const uint8_t *fo
 // the layer containing v (vmin: joins the new elements sharing v).  With
 // the earlier layer's forest this gives the components of the whole
 // inserted set: two earlier elements sharing a vertex were joined before.
+// The roots of e and of all its partners are walked together, one level of
+// every walk per round of loads, so a thread waits for the deepest walk
+// instead of the sum of them; only partners found in another tree go
+// through cc_union_pair.  A root found equal to e's is in e's component
+// (trees only ever merge), and a root found different is an ancestor of
+// its element, so cc_union_pair may start from it.
 __global__ void k_cc_union_new(int64_t nT, int vpe, const int32_t *T, const int32_t *rank,
                                const uint8_t *forced, int32_t prev, int32_t n,
                                const int32_t *vminOld, const int32_t *vmin, int32_t *parent) {
+  constexpr int kMaxWalk = 1 + 2 * 4;  // e, then (vminOld, vmin) per vertex
   const int64_t e = (int64_t)blockIdx.x * kMapBlock + threadIdx.x;
   if (e >= nT) return;
   const int32_t r = rank[e];
   if (r < prev || r >= n || forced[e]) return;
-  for (int k = 0; k < vpe; ++k) {
-    const int32_t v = T[e * vpe + k];
-    const int32_t o = vminOld[v];
-    if (o != INT_MAX) cc_union_pair(parent, (int32_t)e, o);
-    cc_union_pair(parent, (int32_t)e, vmin[v]);
+  // fixed slots (the array stays in registers); unused slots walk e again
+  int32_t cur[kMaxWalk];
+#pragma unroll
+  for (int j = 0; j < kMaxWalk; ++j) cur[j] = (int32_t)e;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < vpe) {
+      const int32_t v = T[e * vpe + k];
+      const int32_t o = vminOld[v];
+      if (o != INT_MAX) cur[1 + 2 * k] = o;
+      cur[2 + 2 * k] = vmin[v];
+    }
+  unsigned live = (1u << kMaxWalk) - 1;
+  while (live) {
+    int32_t nxt[kMaxWalk];
+#pragma unroll
+    for (int j = 0; j < kMaxWalk; ++j)
+      if (live >> j & 1) nxt[j] = cc_load(parent + cur[j]);
+#pragma unroll
+    for (int j = 0; j < kMaxWalk; ++j)
+      if (live >> j & 1) {
+        if (nxt[j] == cur[j]) live &= ~(1u << j);
+        else cur[j] = nxt[j];
+      }
   }
+#pragma unroll
+  for (int j = 1; j < kMaxWalk; ++j)
+    if (cur[j] != cur[0]) cc_union_pair(parent, cur[0], cur[j]);
 }
 
 // Elastic flags of every sorted layer in slot order (flags pre-zeroed, so
@@ -317,6 +346,10 @@ int layer_maps_dev(mlx_mesh &m, const double *d_keys, const double *sorted, int 
   maps.vbody.assign((size_t)ns * nV, 0);
   maps.boundary.assign((size_t)ns * nV, 0);
   if (nT == 0) return MLX_OK;
+  if (vpe > 4) {
+    set_error("layer maps: %d vertices per element (tetrahedra and triangles only)", vpe);
+    return MLX_ERR_ARG;
+  }
   if (nT > INT_MAX - 1) {
     set_error("layer maps: more than 2^31 elements");
     return MLX_ERR_UNSUPPORTED;
