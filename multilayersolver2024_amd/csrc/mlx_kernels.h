@@ -594,10 +594,10 @@ struct LayerLists {
   int nb, nbnd, nrc, nec, nchunks, nbchunks;
 };
 
-// Rigid vertices per workgroup of the chunked body restriction.
-constexpr int kRestrictChunk = 1024;
+// Rigid vertices per workgroup of the chunked body restriction (one per thread).
+constexpr int kRestrictChunk = 256;
 // Boundary vertices per workgroup of the chunked boundary accumulation.
-constexpr int kBoundaryChunk = 1024;
+constexpr int kBoundaryChunk = 256;
 
 // makeRigidsFromConnectivity / matComputeRigidBodyProperties3D
 // (AdaptiveMesh3D.m:32-43, RigidBody3D.m:198-265): mass, COM, COM velocity,
@@ -677,8 +677,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LayerLists L, const vec4_t<
 // mesh.  Two launches: k_restrict_mass (mass, COM, COM velocity) then
 // k_restrict_inertia (r_i, inertia, angular momentum, omega).  Each chunk
 // stores its sums past the caches and counts itself on its body's counter;
-// the body's last chunk adds the chunks' sums in chunk order, finishes the
-// body and re-arms the counter.  Fixed order: deterministic.
+// the first wave of the body's last chunk adds the chunks' sums in a fixed
+// order (chunk_total), finishes the body and re-arms the counter: the result
+// does not depend on which chunk arrives last.
 __device__ __forceinline__ int body_of_chunk(const int32_t *chk, int nb, int i) {
   int lo = 0, hi = nb;  // chk[lo] <= i < chk[hi]
   while (hi - lo > 1) {
@@ -689,6 +690,8 @@ __device__ __forceinline__ int body_of_chunk(const int32_t *chk, int nb, int i) 
   return lo;
 }
 
+// Returns, in every thread of the workgroup, whether this chunk arrived
+// last on its body's counter.
 template <int NV>
 __device__ __forceinline__ bool chunk_publish(const double (&v)[NV], double *part, unsigned *cnt,
                                               int i, int k, int nch, int *sLast) {
@@ -703,20 +706,29 @@ __device__ __forceinline__ bool chunk_publish(const double (&v)[NV], double *par
     *sLast = prev == (unsigned)(nch - 1);
   }
   __syncthreads();
-  return *sLast && threadIdx.x == 0;
+  return *sLast != 0;
 }
 
+// Sum of the body's chunk sums by the first wave of its last chunk (call
+// with threadIdx.x < 64): lane l adds chunks l, l + 64, ... in order, then a
+// fixed butterfly over the lanes; every lane ends with the same totals.
+// Lane 0 re-arms the body's counter.
 template <int NV>
 __device__ __forceinline__ void chunk_total(const double *part, unsigned *cnt, int c0, int nch,
                                             int k, double (&t)[NV]) {
+  const int lane = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < NV; ++j) t[j] = 0;
-  for (int c = 0; c < nch; ++c)
+  for (int c = lane; c < nch; c += 64)
 #pragma unroll
     for (int j = 0; j < NV; ++j)
       t[j] += __hip_atomic_load(part + 12 * (size_t)(c0 + c) + j, __ATOMIC_RELAXED,
                                 __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(cnt + k, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) t[j] += __shfl_xor(t[j], o, 64);
+  if (lane == 0) __hip_atomic_store(cnt + k, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
@@ -746,9 +758,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict_mass(LayerLists L, const ve
   }
   double s7[7];
   block_sum<7>(a, sh, s7);
-  if (!chunk_publish<7>(s7, part, cnt, i, k, nch, &sLast)) return;
+  if (!chunk_publish<7>(s7, part, cnt, i, k, nch, &sLast) || threadIdx.x >= 64) return;
   double t[7];
   chunk_total<7>(part, cnt, c0, nch, k, t);
+  if (threadIdx.x != 0) return;
   Body &B = bodies[k];
   B.M = t[0];
   B.invM = 1.0 / t[0];
@@ -798,9 +811,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict_inertia(LayerLists L, const
   }
   double s12[12];
   block_sum<12>(b, sh, s12);
-  if (!chunk_publish<12>(s12, part, cnt, i, k, nch, &sLast)) return;
+  if (!chunk_publish<12>(s12, part, cnt, i, k, nch, &sLast) || threadIdx.x >= 64) return;
   double t[12];
   chunk_total<12>(part, cnt, c0, nch, k, t);
+  if (threadIdx.x != 0) return;
   Body &B = bodies[k];
   for (int j = 0; j < 9; ++j) {
     B.I0[j] = t[j];
@@ -1028,9 +1042,10 @@ __global__ __launch_bounds__(kBlock) void k_boundary_chunk(LayerLists L, const v
     boundary_project<T>(Bk, L.bnd_vert[off + q], x, minv, ri, lamB + off + q, compliance, h, acc);
   double s6[6];
   block_sum<6>(acc, sh, s6);
-  if (!chunk_publish<6>(s6, part, cnt, i, k, nch, &sLast)) return;
+  if (!chunk_publish<6>(s6, part, cnt, i, k, nch, &sLast) || threadIdx.x >= 64) return;
   double t[6];
   chunk_total<6>(part, cnt, c0, nch, k, t);
+  if (threadIdx.x != 0) return;
   body_apply(bodies[k], t, t + 3, h);
 }
 
