@@ -398,13 +398,14 @@ def _maps_from_sets(m, sets, nr):
                 isRigidVertex=isR.reshape(ns, m.nV).T, numRigids=L.i32(nr))
 
 
-def _seam_case():
+def _seam_case(**beam):
     """Frame 1 of the beam sunk 1 cm into the ground plane (45 contacts on
-    rigid, boundary, pinned and elastic vertices), with the layer sets the
-    oracle built for that substep and the inputs integrate() hands to
-    layerSolve at frame 1 (xi = oldp = p, residual velocity h f / m)."""
+    rigid, boundary, pinned and elastic vertices for the default beam), with
+    the layer sets the oracle built for that substep and the inputs
+    integrate() hands to layerSolve at frame 1 (xi = oldp = p, residual
+    velocity h f / m)."""
     import copy
-    m, integ, plane = beam_case(z0=-0.01)
+    m, integ, plane = beam_case(z0=-0.01, **beam)
     h = 1 / 60
     osim = O.OracleSim(m, copy.deepcopy(integ), h, plane)
     c = O.plane_contacts(3, m.p, [0, 0, 1], [0, 0, 0], 1e-4, strict=False)
@@ -499,3 +500,33 @@ def test_pcg_matches_oracle_and_dense_solve(oracle_lib):
         assert rel(xg, xo) < 1e-9
         if n <= 40:
             assert rel(xg, np.linalg.solve(A.toarray(), b)) < 1e-9
+
+
+@pytest.mark.gpu
+def test_layer_solve_seam_bodies_over_many_chunks_match_oracle(oracle_lib):
+    """A 10,240-tet beam with a 17 x 17-vertex cross-section and layers
+    [90, 40, 0]: each coarse layer is one body of 940 and 2,096 rigid
+    vertices with 598 and 544 boundary vertices, so the body restriction
+    and the boundary accumulation run over several 256-vertex chunks per
+    body (the small beams above fit one chunk).  FP64 to 1e-9 against the
+    oracle's substep on the oracle's layer maps."""
+    from multilayersolver2024_amd.integrator import layer_solve
+    from multilayersolver2024_amd.mesh import DeviceMesh
+    m, integ, h, osim, maps, rv, contacts = _seam_case(nx=8, ny=16, nz=16, layers=(90, 40, 0))
+    vb, bd = maps["vertexRigidBody"], maps["isBoundaryVertex"]
+    for col in (1, 2):
+        assert int(maps["numRigids"][col]) == 1
+        assert np.count_nonzero(vb[:, col] == 1) > 2 * 256
+        assert np.count_nonzero((vb[:, col] == 1) & (bd[:, col] != 0)) > 2 * 256
+    dm = DeviceMesh(m, L.MLX_FP64)
+    try:
+        xi, lam, rv2, out, res, cl = layer_solve(dm, integ, h, maps, m.p, np.zeros((m.nT, 6)), m.p,
+                                                 residualv=rv, pinnedx=m.p[m.pinnedDOFs - 1],
+                                                 contacts=contacts)
+        e, el = rel(xi, osim.p), rel(lam, osim.lambdas())
+        _say("seam chunks", m.nT, list(out.itersPerLayer[:out.layersRun]), e, el)
+        assert list(out.itersPerLayer[:out.layersRun]) == [2, 4, 14]
+        assert e < 1e-9
+        assert el < 1e-9
+    finally:
+        dm.close()
