@@ -700,8 +700,9 @@ __device__ __forceinline__ bool chunk_publish(const double (&v)[NV], double *par
     for (int j = 0; j < NV; ++j)
       __hip_atomic_store(part + 12 * (size_t)i + j, v[j], __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_s_waitcnt(0);
-    const unsigned prev = __hip_atomic_fetch_add(cnt + k, 1u, __ATOMIC_RELAXED,
+    // release: the sums above are visible before the count; acquire: the
+    // last chunk sees every other chunk's sums.
+    const unsigned prev = __hip_atomic_fetch_add(cnt + k, 1u, __ATOMIC_ACQ_REL,
                                                  __HIP_MEMORY_SCOPE_AGENT);
     *sLast = prev == (unsigned)(nch - 1);
   }
@@ -717,6 +718,7 @@ template <int NV>
 __device__ __forceinline__ void chunk_total(const double *part, unsigned *cnt, int c0, int nch,
                                             int k, double (&t)[NV]) {
   const int lane = threadIdx.x;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
   for (int j = 0; j < NV; ++j) t[j] = 0;
   for (int c = lane; c < nch; c += 64)
@@ -1673,13 +1675,13 @@ __global__ __launch_bounds__(kBlock) void k_residual_tail(
   block_sum<1>(v1, sh, o);
   if (t == 0) {
     __hip_atomic_store(partials + blockIdx.x, o[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_s_waitcnt(0);
     const unsigned prev =
-        __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     sLast = prev == gridDim.x - 1;
   }
   __syncthreads();
   if (!sLast) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   double a2 = 0;
   for (int q = t; q < (int)gridDim.x; q += kBlock)
     a2 += __hip_atomic_load(partials + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -887,7 +887,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
           (uint64_t)(uintptr_t)m.d_partials.ptr, (uint64_t)(uintptr_t)m.d_resid.ptr,
           (uint64_t)(uintptr_t)m.d_pinnedx.ptr, (uint64_t)(uintptr_t)m.d_ctl.ptr,
           (uint64_t)(uintptr_t)m.d_count.ptr, (uint64_t)sweepWg, (uint64_t)(uintptr_t)m.d_elist.ptr,
-          (uint64_t)(uintptr_t)m.d_vbody.ptr, bits(sp.h), bits(sp.boundaryCompliance),
+          (uint64_t)(uintptr_t)m.d_vbody.ptr, (uint64_t)(uintptr_t)m.d_bodycnt.ptr, bits(sp.h), bits(sp.boundaryCompliance),
           bits(sp.contactResistance), bits(sp.giveUpThreshold), (uint64_t)sp.hangingStop,
           (uint64_t)sp.giveUpEnabled, (uint64_t)sp.lastLayerGiveUpEnabled};
       if (fl) {
@@ -977,18 +977,19 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     };
     // n iterations as replays of captured chunks of 16, 8, 4 and 2
     // iterations (at most MLX_OPT_GRAPH_ITERS), the rest launched directly.
-    // Graphs are captured only for layers without bodies: their launch
-    // parameters do not change from substep to substep (contact counts are
-    // read on the device), so one capture serves the whole run; layers with
-    // bodies reuse a graph only if one matches.
+    // Graphs are captured and replayed only for layers without bodies: their
+    // launch parameters do not change from substep to substep (contact counts
+    // are read on the device), so one capture serves the whole run.  Layers
+    // with bodies always launch directly (their lists, body counters and
+    // chunk tables change with every substep's layer maps).
     auto run_iterations = [&](int n) -> int {
       if (persist) return run_persistent(n);
       int left = n;
-      if (m.useGraphs && !m.prof.on) {
+      if (m.useGraphs && !m.prof.on && L.nb == 0) {
         for (int G = 16; G >= 2; G /= 2) {
           if (G > m.graphIters) continue;
           while (left >= G) {
-            hipGraphExec_t ex = graph_for(G, L.nb == 0);
+            hipGraphExec_t ex = graph_for(G, true);
             if (!ex) break;
             MLX_CHECK_HIP(hipGraphLaunch(ex, st));
             left -= G;

@@ -5,6 +5,7 @@ computed by the native library and kept in the reference's Matlab layouts
 gateway would.
 """
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -136,6 +137,11 @@ class DeviceMesh:
             self.handle = None
 
     def __del__(self):
+        # A handle still open while the interpreter shuts down is left to
+        # process exit: no HIP call is made from finalization, whose order
+        # against the runtime's own teardown is not defined.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
