@@ -307,6 +307,28 @@ int mlx_pcg_solve(int64_t n, const int32_t *colptr, const int32_t *rowind,
                   const double *val, const double *b, const double *pinv,
                   int32_t iterations, double *x, double *resvec);
 
+/* Allocation-free variant of mlx_pcg_solve.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Every array is caller-owned DEVICE memory on the current HIP device, with
+ * the meaning it has in mlx_pcg_solve (colptr n+1 and rowind nnz 0-based,
+ * val nnz, b, pinv, x n; resvecDev iterations or NULL).  The work vectors
+ * live in a static device array of the library (4 x 65,536 doubles, 2 MiB
+ * per device, held until the process ends): n > 65,536 returns MLX_ERR_ARG.
+ * The call only enqueues one kernel on `stream` (a hipStream_t; NULL = the
+ * null stream) and returns; results are ready when the stream reaches it.
+ * It calls no hipMalloc/hipFree, no memset and no synchronization.  The
+ * compressed structure is checked on the device before the solve: an
+ * invalid one leaves x untouched and sets *statusDev = 1 (0 after a solve;
+ * statusDev may be NULL).  All calls on one device share its scratch array:
+ * calls that overlap in time on one device (other streams, other threads)
+ * race and must be serialized by the caller. */
+int mlx_pcg_solve_noalloc(int64_t n, int64_t nnz, const int32_t *colptrDev,
+                          const int32_t *rowindDev, const double *valDev,
+                          const double *bDev, const double *pinvDev,
+                          int32_t iterations, double *xDev, double *resvecDev,
+                          int32_t *statusDev, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Many independent scenes in one launch                                */
 /* ------------------------------------------------------------------ */

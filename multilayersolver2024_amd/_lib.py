@@ -33,7 +33,7 @@ EXPORTED = (
     "mlx_profile_enable", "mlx_profile_read", "mlx_profile_phases", "mlx_profile_read_persistent", "mlx_layer_maps", "mlx_batch_set_split", "mlx_batch_set_lds", "mlx_batch_set_graph", "mlx_state_set_pinned_targets", "mlx_synchronize", "mlx_mesh_set_option",
     "mlx_batch_create", "mlx_batch_destroy", "mlx_batch_set_block", "mlx_batch_set_state",
     "mlx_batch_download", "mlx_batch_integrate", "mlx_neohookean_constraints2d",
-    "mlx_pcg_solve",
+    "mlx_pcg_solve", "mlx_pcg_solve_noalloc",
 )
 MLX_OPT_GRAPHS = 1
 MLX_OPT_GRAPH_ITERS = 2
@@ -168,6 +168,8 @@ def lib():
         "mlx_mesh_set_option": [_p, C.c_int32, C.c_int64],
         "mlx_neohookean_constraints2d": [_p, _u8, C.c_double, _d, _d, _d, _d],
         "mlx_pcg_solve": [C.c_int64, _i32, _i32, _d, _d, _d, C.c_int32, _d, _d],
+        "mlx_pcg_solve_noalloc": [C.c_int64, C.c_int64, _p, _p, _p, _p, _p, C.c_int32, _p, _p,
+                                  _p, _p],
         "mlx_batch_create": [C.POINTER(MeshDesc), C.c_int32, _i64, _i64, C.c_int32, C.POINTER(_p)],
         "mlx_batch_destroy": [_p],
         "mlx_batch_set_block": [_p, C.c_int32],

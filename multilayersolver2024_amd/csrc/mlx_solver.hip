@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "mlx_host.h"
@@ -19,6 +20,12 @@
 using namespace mlx;
 
 namespace mlx {
+
+// Work vectors (r, z, d, Ap) of mlx_pcg_solve_noalloc: one copy per device,
+// held until the process ends (2 MiB).  Host code never passes this name to
+// a kernel: it passes the device address resolved by pcg_scratch_resolve.
+static constexpr int64_t kPcgScratchMaxN = 65536;
+static __device__ __align__(256) double scratch[4 * kPcgScratchMaxN];
 
 int DevBuf::reserve(size_t n) {
   if (n <= bytes && ptr) return MLX_OK;
@@ -62,6 +69,36 @@ static int check_device(int dev) {
     set_error("device %d is %s; this library is built for gfx950", dev, prop.gcnArchName);
     return MLX_ERR_NODEVICE;
   }
+  return MLX_OK;
+}
+
+// Device address of the no-allocation solve's scratch array, per device.
+// Resolved once per device with that device current (mlx_init, or the
+// entry point's first call on a device); host code uses only this address.
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices] = {};
+
+static int pcg_scratch_resolve(int dev, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device %d out of range for the scratch table", dev);
+    return MLX_ERR_NODEVICE;
+  }
+  if (!g_scratch_dev[dev]) {
+    int cur = -1;
+    MLX_CHECK_HIP(hipGetDevice(&cur));
+    if (cur != dev) MLX_CHECK_HIP(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(mlx::scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return MLX_ERR_HIP;
+    }
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[dev];
   return MLX_OK;
 }
 
@@ -1173,6 +1210,8 @@ int mlx_init(int32_t device) {
   MLX_TRY(check_device(device));
   g_device = device;
   MLX_CHECK_HIP(hipSetDevice(device));
+  double *scr = nullptr;
+  MLX_TRY(pcg_scratch_resolve(device, &scr));
   return MLX_OK;
 }
 
@@ -3011,12 +3050,11 @@ __device__ __forceinline__ double wg_sum1024(double v, double *sh) {
 
 // solvePCG.m:1-57 in one workgroup (coarse systems are small): SpMV,
 // FP64 dot products with a fixed reduction tree, diagonal preconditioner.
-__global__ __launch_bounds__(1024) void k_pcg(int n, const int32_t *rp, const int32_t *ci,
-                                              const double *v, const double *b,
-                                              const double *pinv, int iters, double *x,
-                                              double *r, double *z, double *d, double *Ap,
-                                              double *resvec) {
-  __shared__ double sh[1024];
+// r, z, d, Ap: n-vectors of scratch, every entry written before it is read.
+__device__ __forceinline__ void pcg_run(int n, const int32_t *rp, const int32_t *ci,
+                                        const double *v, const double *b, const double *pinv,
+                                        int iters, double *x, double *r, double *z, double *d,
+                                        double *Ap, double *resvec, double *sh) {
   const int t = threadIdx.x;
   double acc = 0;
   for (int i = t; i < n; i += 1024) {
@@ -3061,6 +3099,44 @@ __global__ __launch_bounds__(1024) void k_pcg(int n, const int32_t *rp, const in
       if (t == 0) resvec[it] = sqrt(ss);
     }
   }
+}
+
+__global__ __launch_bounds__(1024) void k_pcg(int n, const int32_t *rp, const int32_t *ci,
+                                              const double *v, const double *b,
+                                              const double *pinv, int iters, double *x,
+                                              double *r, double *z, double *d, double *Ap,
+                                              double *resvec) {
+  __shared__ double sh[1024];
+  pcg_run(n, rp, ci, v, b, pinv, iters, x, r, z, d, Ap, resvec, sh);
+}
+
+// The solve of k_pcg on caller-owned device arrays, with the work vectors
+// in `work` (4n doubles).  The caller's compressed structure is device
+// memory the host cannot inspect, so it is checked here first: colptr starts
+// at 0, never decreases and ends at nnz, row indices lie in [0, n).  An
+// invalid structure leaves x untouched and sets status[0] = 1 (0 after a
+// solve); status may be null.
+__global__ __launch_bounds__(1024) void k_pcg_dev(int n, int nnz, const int32_t *rp,
+                                                  const int32_t *ci, const double *v,
+                                                  const double *b, const double *pinv, int iters,
+                                                  double *x, double *work, double *resvec,
+                                                  int32_t *status) {
+  __shared__ double sh[1024];
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = (rp[0] != 0 || rp[n] != nnz);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 1024)
+    if (rp[i + 1] < rp[i]) bad = 1;
+  __syncthreads();
+  if (!bad)
+    for (int k = threadIdx.x; k < nnz; k += 1024)
+      if (ci[k] < 0 || ci[k] >= n) bad = 1;
+  __syncthreads();
+  const int invalid = bad;
+  if (threadIdx.x == 0 && status) status[0] = invalid;
+  if (invalid) return;
+  pcg_run(n, rp, ci, v, b, pinv, iters, x, work, work + n, work + 2 * n, work + 3 * n, resvec,
+          sh);
 }
 
 }  // namespace mlx
@@ -3115,5 +3191,32 @@ extern "C" int mlx_pcg_solve(int64_t n, const int32_t *colptr, const int32_t *ro
     set_error("mlx_pcg_solve: %s", hipGetErrorString(e));
     return MLX_ERR_HIP;
   }
+  return MLX_OK;
+}
+
+// Allocation-free variant of mlx_pcg_solve on caller-owned device arrays.
+extern "C" int mlx_pcg_solve_noalloc(int64_t n, int64_t nnz, const int32_t *colptrDev,
+                                     const int32_t *rowindDev, const double *valDev,
+                                     const double *bDev, const double *pinvDev,
+                                     int32_t iterations, double *xDev, double *resvecDev,
+                                     int32_t *statusDev, void *stream) {
+  if (n <= 0 || n > kPcgScratchMaxN) {
+    set_error("mlx_pcg_solve_noalloc: n = %lld outside 1..%lld (the size of its static "
+              "scratch array)", (long long)n, (long long)kPcgScratchMaxN);
+    return MLX_ERR_ARG;
+  }
+  if (nnz < 0 || nnz > INT32_MAX || !colptrDev || (nnz > 0 && (!rowindDev || !valDev)) ||
+      !bDev || !pinvDev || !xDev || iterations < 0) {
+    set_error("mlx_pcg_solve_noalloc: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  int dev = -1;
+  MLX_CHECK_HIP(hipGetDevice(&dev));
+  double *work = nullptr;
+  MLX_TRY(pcg_scratch_resolve(dev, &work));
+  hipLaunchKernelGGL(k_pcg_dev, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream),
+                     (int)n, (int)nnz, colptrDev, rowindDev, valDev, bDev, pinvDev, iterations,
+                     xDev, work, resvecDev, statusDev);
+  MLX_CHECK_HIP(hipGetLastError());
   return MLX_OK;
 }
