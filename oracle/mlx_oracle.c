@@ -1325,6 +1325,7 @@ int ora_integrate3d(const ora_mesh *m, ora_settings *s, double h, int frame,
     memcpy(oldp, p, sizeof(double) * N3);
     for (int q = 0; q < m->nPinnedDOFs; ++q) pinnedx[q] = p[m->pinnedDOFs[q]];
   }
+  if (out) out->runUntilAtEntry = s->runUntil;
   /* forces: gravity on z (:638-643) */
   double *f = (double *)calloc(N3, sizeof(double));
   for (int i = 0; i < nV; ++i) f[3 * i + 2] = f[3 * i + 2] + m->mass[3 * i + 2] * s->gravity;
@@ -1414,6 +1415,7 @@ int ora_integrate3d(const ora_mesh *m, ora_settings *s, double h, int frame,
     double normresv = 0;
     for (int i = 0; i < N3; ++i) normresv = normresv + rv[i] * rv[i];
     normresv = sqrt(normresv);
+    if (out) out->normresv[layer] = normresv;
     /* projectGS (:243-345) */
     int nbv = 0;
     for (int i = 0; i < nV; ++i) if (isB[i]) bverts[nbv++] = i;

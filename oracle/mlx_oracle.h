@@ -81,6 +81,9 @@ typedef struct {
   int64_t elasticProjections; /* out: elastic element updates summed */
   int32_t *setsPerLayer;   /* out optional: nT*nSorted column-major */
   int32_t *numRigids;      /* out optional: nSorted */
+  double normresv[ORA_MAX_LAYERS]; /* out: norm of the residual velocity each
+                                      layer's monitor compares (3D) */
+  double runUntilAtEntry;  /* out: runUntilResSmallerThan as layerSolve received it */
 } ora_step_out;
 
 /* ---- precompute (Mesh3D / Mesh constructors) ---- */

@@ -51,6 +51,7 @@ class OraStepOut(C.Structure):
         ("residualCount", C.c_int), ("itersPerLayer", C.c_int * MAX_LAYERS),
         ("layersRun", C.c_int), ("elasticProjections", C.c_int64),
         ("setsPerLayer", _i32), ("numRigids", _i32),
+        ("normresv", C.c_double * MAX_LAYERS), ("runUntilAtEntry", C.c_double),
     ]
 
 
@@ -221,7 +222,9 @@ class OracleSim:
         n = min(out.residualCount, len(self.resid))
         return dict(residuals=self.resid[:n].copy(), iters=list(out.itersPerLayer[:out.layersRun]),
                     projections=int(out.elasticProjections), runUntil=self.s.runUntil,
-                    sets=sets.reshape(ns, self.om.m.nT).T.copy(), numRigids=nrig)
+                    sets=sets.reshape(ns, self.om.m.nT).T.copy(), numRigids=nrig,
+                    normresv=list(out.normresv[:out.layersRun]),
+                    runUntilAtEntry=float(out.runUntilAtEntry), residualCount=int(out.residualCount))
 
     def lambdas(self):
         m = self.om.m
