@@ -65,44 +65,68 @@ int graph_coloring(const int32_t *T0, int64_t nT, int vpe, int64_t nV,
   return MLX_OK;
 }
 
-// 3x3 inverse by cofactors, column-major.
-static void inv3(const double *A, double *R, double *detOut) {
-  double a = A[0], b = A[3], c = A[6], d = A[1], e = A[4], f = A[7], g = A[2],
-         h = A[5], i = A[8];
-  double c00 = e * i - f * h, c01 = -(d * i - f * g), c02 = d * h - e * g;
-  double det = a * c00 + b * c01 + c * c02;
-  double id = 1.0 / det;
-  R[0] = c00 * id;
-  R[1] = c01 * id;
-  R[2] = c02 * id;
-  R[3] = -(b * i - c * h) * id;
-  R[4] = (a * i - c * g) * id;
-  R[5] = -(a * h - b * g) * id;
-  R[6] = (b * f - c * e) * id;
-  R[7] = -(a * f - c * d) * id;
-  R[8] = (a * e - b * d) * id;
-  if (detOut) *detOut = det;
+// Inverse of an n x n column-major matrix (n <= 6) as Matlab's inv /
+// pageinv forms it: an LU factorisation with partial pivoting (LAPACK
+// getrf), then one forward/back substitution per column of the identity.
+// Returns false if a pivot is exactly zero.  The strain-rate keys of the
+// layer maps (B*p at rest) are rounding noise, so the inverse's last bits
+// decide the layer sets: every implementation that must reproduce them
+// uses this elimination order.
+static bool inv_lu(int n, const double *A, double *R) {
+#pragma clang fp contract(off)
+  double lu[36];
+  int perm[6];
+  bool ok = true;
+  std::memcpy(lu, A, sizeof(double) * n * n);
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(lu[i + n * k]) > std::fabs(lu[piv + n * k])) piv = i;
+    if (piv != k) {
+      for (int j = 0; j < n; ++j) std::swap(lu[k + n * j], lu[piv + n * j]);
+      std::swap(perm[k], perm[piv]);
+    }
+    const double d = lu[k + n * k];
+    if (d == 0.0) ok = false;
+    for (int i = k + 1; i < n; ++i) {
+      const double l = lu[i + n * k] / d;
+      lu[i + n * k] = l;
+      for (int j = k + 1; j < n; ++j) lu[i + n * j] = lu[i + n * j] - l * lu[k + n * j];
+    }
+  }
+  for (int c = 0; c < n; ++c) {
+    // right-hand side: column c of the identity, rows permuted like the LU
+    double y[6], x[6];
+    for (int i = 0; i < n; ++i) y[i] = perm[i] == c ? 1.0 : 0.0;
+    for (int i = 1; i < n; ++i)
+      for (int j = 0; j < i; ++j) y[i] = y[i] - lu[i + n * j] * y[j];
+    for (int i = n - 1; i >= 0; --i) {
+      double t = y[i];
+      for (int j = i + 1; j < n; ++j) t = t - lu[i + n * j] * x[j];
+      x[i] = t / lu[i + n * i];
+    }
+    for (int i = 0; i < n; ++i) R[i + n * c] = x[i];
+  }
+  return ok;
 }
 
-// Inverse of the isotropic Voigt stiffness K = [[a b b 0 0 0]..; shear s]:
-// the normal block [[a b b],[b a b],[b b a]] has inverse
-// p I + q (11^T - I) with p = (a+b)/((a-b)(a+2b)), q = -b/((a-b)(a+2b)).
-static void inv_voigt(int n, double a, double b, double s, double *alpha) {
-  std::memset(alpha, 0, sizeof(double) * n * n);
-  int nn = (n == 6) ? 3 : 2;
-  double p, q;
-  if (nn == 3) {
-    double den = (a - b) * (a + 2 * b);
-    p = (a + b) / den;
-    q = -b / den;
-  } else {
-    double den = a * a - b * b;
-    p = a / den;
-    q = -b / den;
-  }
+// det of a 3x3 column-major matrix, cofactor expansion along the first row.
+static double det3(const double *A) {
+#pragma clang fp contract(off)
+  return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) +
+         A[6] * (A[1] * A[5] - A[4] * A[2]);
+}
+
+// The isotropic Voigt stiffness K of updateMaterials (3D: normal block
+// diag lambda + 2 (2 mu), off-diagonal lambda, shear 2 mu; 2D: diag
+// lambda + 2 mu, off-diagonal lambda, shear 2 mu), column-major n x n.
+static void voigt_k(int n, double diag, double off, double shear, double *K) {
+  std::memset(K, 0, sizeof(double) * n * n);
+  const int nn = (n == 6) ? 3 : 2;
   for (int i = 0; i < nn; ++i)
-    for (int j = 0; j < nn; ++j) alpha[i + n * j] = (i == j) ? p : q;
-  for (int i = nn; i < n; ++i) alpha[i + n * i] = 1.0 / s;
+    for (int j = 0; j < nn; ++j) K[i + n * j] = (i == j) ? diag : off;
+  for (int i = nn; i < n; ++i) K[i + n * i] = shear;
 }
 
 int precompute3d(const double *p, int64_t nV, const int32_t *T0, int64_t nT,
@@ -115,9 +139,8 @@ int precompute3d(const double *p, int64_t nV, const int32_t *T0, int64_t nT,
     double Dm[9];
     for (int a = 0; a < 3; ++a)
       for (int d = 0; d < 3; ++d) Dm[d + 3 * a] = p[3 * t[a + 1] + d] - p[3 * t[0] + d];
-    double det;
-    inv3(Dm, DmInv + 9 * e, &det);
-    if (!(det != 0.0) || !std::isfinite(det)) {
+    const double det = det3(Dm);
+    if (!(det != 0.0) || !std::isfinite(det) || !inv_lu(3, Dm, DmInv + 9 * e)) {
       set_error("element %lld is degenerate (det(Dm) = %g)", (long long)e, det);
       return MLX_ERR_ARG;
     }
@@ -130,12 +153,13 @@ int precompute3d(const double *p, int64_t nV, const int32_t *T0, int64_t nT,
   }
   for (int64_t e = 0; e < nT; ++e) {
     double l = lam[e], dm = 2 * mu[e], dv = l + 2 * dm;
-    double *A = alpha + 36 * e, *B = beta + 36 * e;
-    inv_voigt(6, dv, l, dm, A);
-    std::memset(B, 0, sizeof(double) * 36);
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) B[i + 6 * j] = ((i == j) ? dv : l) * alpha1[e];
-    for (int i = 3; i < 6; ++i) B[i + 6 * i] = dm * alpha1[e];
+    double K[36];
+    voigt_k(6, dv, l, dm, K);
+    if (!inv_lu(6, K, alpha + 36 * e)) {
+      set_error("element %lld: singular stiffness", (long long)e);
+      return MLX_ERR_ARG;
+    }
+    for (int k = 0; k < 36; ++k) beta[36 * e + k] = K[k] * alpha1[e];
   }
   return MLX_OK;
 }
@@ -154,11 +178,8 @@ int precompute2d(const double *p, int64_t nV, const int32_t *T0, int64_t nT,
       set_error("triangle %lld is degenerate", (long long)e);
       return MLX_ERR_ARG;
     }
-    double *M = DmInv + 4 * e;
-    M[0] = d / det;
-    M[1] = -c / det;
-    M[2] = -b / det;
-    M[3] = a / det;
+    const double Dm[4] = {a, c, b, d};
+    inv_lu(2, Dm, DmInv + 4 * e);
     double u0 = pj[0] - pi[0], u1 = pj[1] - pi[1], w0 = pk[0] - pi[0], w1 = pk[1] - pi[1];
     area[e] = 0.5 * std::fabs(u0 * w1 - w0 * u1);
   }
@@ -171,10 +192,13 @@ int precompute2d(const double *p, int64_t nV, const int32_t *T0, int64_t nT,
     }
   for (int64_t e = 0; e < nT; ++e) {
     double l = lam[e], dv = l + 2 * mu[e], s = 2 * mu[e];
-    inv_voigt(3, dv, l, s, alpha + 9 * e);
-    double *B = beta + 9 * e;
-    double K[9] = {dv, l, 0, l, dv, 0, 0, 0, s};
-    for (int k = 0; k < 9; ++k) B[k] = K[k] * alpha1[e];
+    double K[9];
+    voigt_k(3, dv, l, s, K);
+    if (!inv_lu(3, K, alpha + 9 * e)) {
+      set_error("triangle %lld: singular stiffness", (long long)e);
+      return MLX_ERR_ARG;
+    }
+    for (int k = 0; k < 9; ++k) beta[9 * e + k] = K[k] * alpha1[e];
   }
   return MLX_OK;
 }

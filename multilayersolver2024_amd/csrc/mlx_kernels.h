@@ -12,6 +12,13 @@
 
 #include "mlx_internal.h"
 
+// Multiply-adds are fused only within one source expression (never across
+// statements), so a device function gives the same bits wherever it is
+// inlined: the batched kernel's LDS and HBM position paths, the persistent
+// loop and the per-color launches all share the projection code and must
+// agree bit for bit.
+#pragma clang fp contract(on)
+
 namespace mlx {
 
 constexpr int kBlock = 256;
@@ -245,12 +252,109 @@ __device__ __forceinline__ void tet_fetch(const SweepParams<T> &P, const int64_t
 #pragma unroll
   for (int k = 0; k < 9; ++k) in.D[k] = P.dminv[k * ld + s];
   in.vol = P.vol[s];
-  in.tad = P.mat[s] * P.ih2;
+  const T m0 = P.mat[s], m2 = P.mat[2 * ld + s];
+  in.tad = m0 * P.ih2;
   in.tao = P.mat[ld + s] * P.ih2;
-  in.tas = P.mat[2 * ld + s] * P.ih2;
+  in.tas = m0 < T(0) ? m2 : m2 * P.ih2;  // NeoHookean slot: gamma as stored
   in.g = P.mat[3 * ld + s] * P.ih;
 #pragma unroll
   for (int i = 0; i < 6; ++i) in.lam[i] = P.lam[i * ld + s];
+}
+
+// Material scalars of a slot.  StVK-Voigt: alpha_d = alpha(1,1),
+// alpha_o = alpha(1,2), alpha_s = alpha(4,4), alpha1 (all > 0 but alpha_o).
+// NeoHookean tetrahedron (the deviatoric / volumetric pair of
+// mexNeoHookeanDConstraint.cpp:46-138 and mexNeoHookeanHConstraint.cpp:47-138
+// lifted to three dimensions): -1/mu, 1/lambda, gamma = 1 + mu/lambda, 0.
+// The negative first scalar tags the slot (alpha(1,1) of an isotropic
+// stiffness is positive), so one slot layout serves both materials.
+
+// F of a tetrahedron from its current vertices and DmInv (column-major).
+template <typename T>
+__device__ __forceinline__ void tet_F(const vec4_t<T> (&X)[4], const T (&D)[9], T (&F)[3][3]) {
+  const T e[3][3] = {{X[1].x - X[0].x, X[1].y - X[0].y, X[1].z - X[0].z},
+                     {X[2].x - X[0].x, X[2].y - X[0].y, X[2].z - X[0].z},
+                     {X[3].x - X[0].x, X[3].y - X[0].y, X[3].z - X[0].z}};
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      F[r][c] = e[0][r] * D[3 * c] + e[1][r] * D[1 + 3 * c] + e[2][r] * D[2 + 3 * c];
+}
+
+// ||F||_F and cof(F) (the gradient of det F with respect to F); det F.
+template <typename T>
+__device__ __forceinline__ void nh_terms(const T (&F)[3][3], T &n, T (&PH)[3][3], T &J) {
+  T n2 = T(0);
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) n2 += F[r][c] * F[r][c];
+  n = sqrt(n2);
+  PH[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+  PH[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+  PH[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+  PH[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+  PH[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+  PH[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+  PH[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+  PH[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+  PH[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+  J = F[0][0] * PH[0][0] + F[0][1] * PH[0][1] + F[0][2] * PH[0][2];
+}
+
+// The NeoHookean pair on one tetrahedron: C_D = ||F||_F with compliance
+// 1/(vol h^2 mu), C_H = det F - gamma with compliance 1/(vol h^2 lambda),
+// both evaluated at the positions the projection starts from; each is a
+// scalar XPBD step dl = -(C + a~ l) / (sum_a w_a |g_a|^2 + a~) with vertex
+// gradients g_a = (dC/dF) grad_a, grad_a the rows of [-1^T DmInv; DmInv];
+// the displacements are summed, D first.  lambda_D, lambda_H are the slot's
+// first two multipliers.
+template <typename T, typename Store>
+__device__ __forceinline__ void tet_solve_nh_st(const SweepParams<T> &P, const int64_t s,
+                                                const TetIn<T> &in, const vec4_t<T> (&X)[4],
+                                                Store &&st) {
+  const int64_t ld = P.ld;
+  T D[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) D[k] = in.D[k];
+  T F[3][3], PH[3][3], n, J;
+  tet_F<T>(X, D, F);
+  nh_terms<T>(F, n, PH, J);
+  T G[4][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    G[1][c] = D[3 * c];
+    G[2][c] = D[1 + 3 * c];
+    G[3][c] = D[2 + 3 * c];
+    G[0][c] = -(G[1][c] + G[2][c] + G[3][c]);
+  }
+  const T w[4] = {X[0].w, X[1].w, X[2].w, X[3].w};
+  const T iv = T(1) / in.vol;
+  const T aD = iv * (-in.tad), aH = iv * in.tao, gam = in.tas;
+  T gD[4][3], gH[4][3];
+  T sD = aD, sH = aH;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      gD[a][r] = (F[r][0] * G[a][0] + F[r][1] * G[a][1] + F[r][2] * G[a][2]) / n;
+      gH[a][r] = PH[r][0] * G[a][0] + PH[r][1] * G[a][1] + PH[r][2] * G[a][2];
+      sD += w[a] * (gD[a][r] * gD[a][r]);
+      sH += w[a] * (gH[a][r] * gH[a][r]);
+    }
+  const T dlD = -(n + aD * in.lam[0]) / sD;
+  const T dlH = -((J - gam) + aH * in.lam[1]) / sH;
+  P.lam[s] = in.lam[0] + dlD;
+  P.lam[ld + s] = in.lam[1] + dlH;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    vec4_t<T> o = X[a];
+    o.x = (o.x + w[a] * gD[a][0] * dlD) + w[a] * gH[a][0] * dlH;
+    o.y = (o.y + w[a] * gD[a][1] * dlD) + w[a] * gH[a][1] * dlH;
+    o.z = (o.z + w[a] * gD[a][2] * dlD) + w[a] * gH[a][2] * dlH;
+    st(a, o);
+  }
 }
 
 // The projection proper, on fetched slot data and the current positions X
@@ -259,6 +363,10 @@ template <typename T, typename Store>
 __device__ __forceinline__ void tet_solve_st(const SweepParams<T> &P, const int64_t s,
                                              const TetIn<T> &in, const vec4_t<T> (&X)[4],
                                              Store &&st) {
+  if (in.tad < T(0)) {
+    tet_solve_nh_st<T>(P, s, in, X, st);
+    return;
+  }
   const int64_t ld = P.ld;
   const vec4_t<T> X0 = X[0], X1 = X[1], X2 = X[2], X3 = X[3];
   const vec4_t<T> O0 = in.O[0], O1 = in.O[1], O2 = in.O[2], O3 = in.O[3];
@@ -1424,9 +1532,10 @@ template <typename T, bool COH>
 __device__ __forceinline__ void res_fetch(const SweepParams<T> &P, const int64_t s,
                                           ResIn<T> &in) {
   const int64_t ld = P.ld;
-  in.tad = P.mat[s] * P.ih2;
+  const T m0 = P.mat[s], m2 = P.mat[2 * ld + s];
+  in.tad = m0 * P.ih2;
   in.tao = P.mat[ld + s] * P.ih2;
-  in.tas = P.mat[2 * ld + s] * P.ih2;
+  in.tas = m0 < T(0) ? m2 : m2 * P.ih2;  // NeoHookean slot: gamma as stored
   in.vol = P.vol[s];
 #pragma unroll
   for (int a = 0; a < 4; ++a) in.X[a] = ldc<COH>(P.x + P.vid[a * ld + s]);
@@ -1436,8 +1545,32 @@ __device__ __forceinline__ void res_fetch(const SweepParams<T> &P, const int64_t
   for (int i = 0; i < 6; ++i) in.lam[i] = ldc<COH>(P.lam + i * ld + s);
 }
 
+// Residual terms of a NeoHookean tetrahedron: (|C| - min(|C|, |a~ l|))^2
+// for C_D = ||F|| and C_H = det F - gamma, in double.
+template <typename T>
+__device__ __forceinline__ double res_compute_nh(const ResIn<T> &in) {
+  T D[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) D[k] = in.D[k];
+  T F[3][3], PH[3][3], n, J;
+  tet_F<T>(in.X, D, F);
+  nh_terms<T>(F, n, PH, J);
+  const T iv = T(1) / in.vol;
+  const double C[2] = {(double)n, (double)(J - in.tas)};
+  const double al[2] = {(double)(iv * (-in.tad) * in.lam[0]), (double)(iv * in.tao * in.lam[1])};
+  double acc = 0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double a = fabs(C[i]), cr = fabs(al[i]);
+    const double t = a - (a < cr ? a : cr);
+    acc += t * t;
+  }
+  return acc;
+}
+
 template <typename T>
 __device__ __forceinline__ double res_compute(const ResIn<T> &in) {
+  if (in.tad < T(0)) return res_compute_nh<T>(in);
   const T tad = in.tad, tao = in.tao, tas = in.tas;
   const T vol = in.vol;
   const vec4_t<T> X0 = in.X[0], X1 = in.X[1], X2 = in.X[2], X3 = in.X[3];
@@ -1490,9 +1623,10 @@ template <typename T, typename Pos>
 __device__ __forceinline__ void res_fetch_pos(const SweepParams<T> &P, const int64_t s,
                                               ResIn<T> &in, Pos &&pos) {
   const int64_t ld = P.ld;
-  in.tad = P.mat[s] * P.ih2;
+  const T m0 = P.mat[s], m2 = P.mat[2 * ld + s];
+  in.tad = m0 * P.ih2;
   in.tao = P.mat[ld + s] * P.ih2;
-  in.tas = P.mat[2 * ld + s] * P.ih2;
+  in.tas = m0 < T(0) ? m2 : m2 * P.ih2;  // NeoHookean slot: gamma as stored
   in.vol = P.vol[s];
 #pragma unroll
   for (int a = 0; a < 4; ++a) in.X[a] = pos(P.vid[a * ld + s]);

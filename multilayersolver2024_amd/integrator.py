@@ -132,16 +132,19 @@ class Simulation:
     simulate frame loop restricted to the XPBD path)."""
 
     def __init__(self, mesh, integrator, h, contact=None, precision=L.MLX_FP64, device=0,
-                 v0=None, residual_cap=4096):
+                 v0=None, residual_cap=4096, p0=None):
         self.mesh = mesh
         self.integrator = integrator
         self.h = float(h)
         self.contact = contact
         self.dmesh = DeviceMesh(mesh, precision, device)
         v0 = np.zeros_like(mesh.p) if v0 is None else L.f64(v0)
+        p0 = mesh.p if p0 is None else L.f64(p0)
+        if p0.shape != mesh.p.shape or v0.shape != mesh.p.shape:
+            raise ValueError("p0 and v0 need dim * nV values")
         st = C.c_void_p()
         L.check("mlx_state_create",
-                L.lib().mlx_state_create(self.dmesh.handle, L.ptr(mesh.p), L.ptr(v0), C.byref(st)))
+                L.lib().mlx_state_create(self.dmesh.handle, L.ptr(p0), L.ptr(v0), C.byref(st)))
         self.state = st
         self.frame = 0
         self._settings = integrator.settings()

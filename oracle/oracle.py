@@ -79,35 +79,105 @@ def P(a, t=C.c_double):
 
 
 class OracleMesh:
-    """ora_mesh built from a product Mesh (Matlab layout -> C layout)."""
+    """ora_mesh for a product Mesh, with the oracle's OWN precompute: DmInv,
+    volumes, masses, alpha/beta (ora_precompute3d/2d), colors
+    (ora_naive_graph_coloring) and pinned DOFs / pinned elements are rebuilt
+    here from the scene (rest positions, connectivity, materials, pinned
+    vertices), never copied from the product.  The product's arrays are then
+    checked against them bit for bit -- colors, pinned DOFs, pinned elements
+    and the constants, after a 1e-12 relative check that names a gross
+    mismatch (ProductPrecomputeMismatch otherwise) -- so every run through
+    the oracle also checks the product's precompute and coloring."""
 
-    def __init__(self, mesh):
+    def __init__(self, mesh, check=True):
         self.m = mesh
         dim, nT, nV = mesh.dim, mesh.nT, mesh.nV
-        self.T = np.ascontiguousarray(mesh.T0, dtype=np.int32)
-        self.DmInv = np.ascontiguousarray(mesh.DmInv)
-        self.alpha = np.ascontiguousarray(mesh.alpha)
-        self.beta = np.ascontiguousarray(mesh.beta)
-        self.vol = np.ascontiguousarray(mesh.elV)
-        self.mass = np.ascontiguousarray(mesh.mass)
-        self.color = np.ascontiguousarray(mesh.elementColor, dtype=np.int32)
-        self.pinned = np.ascontiguousarray(mesh.pinnedDOFs - 1, dtype=np.int32)
-        self.forced = np.ascontiguousarray(mesh.forcedElastic, dtype=np.uint8)
+        vpe, nc = dim + 1, (6 if dim == 3 else 3)
+        sc = mesh.scene
+        self.T = np.ascontiguousarray(np.asarray(sc["t"], dtype=np.int32).reshape(nT, vpe))
+        P0 = np.ascontiguousarray(np.asarray(sc["p"], dtype=np.float64).reshape(-1))
+        mat = [np.ascontiguousarray(np.broadcast_to(np.asarray(sc[k], dtype=np.float64), (nT,)))
+               for k in ("rho", "mu", "lam", "alpha1")]
+        self.DmInv = np.zeros(dim * dim * nT)
+        self.vol = np.zeros(nT)
+        self.mass = np.zeros(dim * nV)
+        self.alpha = np.zeros(nc * nc * nT)
+        self.beta = np.zeros(nc * nc * nT)
+        fn = lib().ora_precompute3d if dim == 3 else lib().ora_precompute2d
+        if fn(nV, nT, P(P0), P(self.T, C.c_int32), *(P(a) for a in mat), P(self.DmInv),
+              P(self.vol), P(self.mass), P(self.alpha), P(self.beta)) != 0:
+            raise RuntimeError("oracle precompute: singular element")
+        self.color = np.zeros(nT, dtype=np.int32)
+        if lib().ora_naive_graph_coloring(P(self.T, C.c_int32), nT, vpe, nV,
+                                          P(self.color, C.c_int32)) != 0:
+            raise RuntimeError("oracle coloring failed")
+        self.numColors = int(self.color.max()) if nT else 0
+        pv = np.asarray(mesh.pinnedInds, dtype=np.int64)
+        if dim == 3:
+            dofs = np.sort(np.concatenate([3 * pv, 3 * pv + 1, 3 * pv + 2]))
+        else:
+            dofs = np.concatenate([2 * pv, 2 * pv + 1])
+        self.pinned = np.ascontiguousarray(dofs, dtype=np.int32)
+        mark = np.zeros(nV, dtype=np.uint8)
+        mark[pv] = 1
+        self.forced = np.zeros(nT, dtype=np.uint8)
+        lib().ora_pinned_elements(P(self.T, C.c_int32), nT, vpe, nV, P(mark, C.c_uint8),
+                                  P(self.forced, C.c_uint8))
+        if check:
+            self.check_product(mesh)
         s = OraMesh()
-        s.dim, s.nV, s.nT, s.vpe = dim, nV, nT, dim + 1
+        s.dim, s.nV, s.nT, s.vpe = dim, nV, nT, vpe
         s.T, s.DmInv, s.alpha, s.beta = P(self.T, C.c_int32), P(self.DmInv), P(self.alpha), P(self.beta)
         s.vol, s.mass = P(self.vol), P(self.mass)
-        s.color, s.numColors = P(self.color, C.c_int32), int(mesh.numColors)
+        s.color, s.numColors = P(self.color, C.c_int32), self.numColors
         s.pinnedDOFs, s.nPinnedDOFs = P(self.pinned, C.c_int32), len(self.pinned)
         s.forcedElastic = P(self.forced, C.c_uint8)
         nh = getattr(mesh, "neohookean", None)
         if dim == 3 and nh is not None and nh.any():
             self.nh = np.ascontiguousarray(nh, dtype=np.uint8)
-            self.nhMu = np.ascontiguousarray(mesh.elMu, dtype=np.float64)
-            self.nhLam = np.ascontiguousarray(mesh.elLambda, dtype=np.float64)
+            self.nhMu = mat[1].copy()
+            self.nhLam = mat[2].copy()
             s.nhModel = P(self.nh, C.c_uint8)
             s.nhMu, s.nhLam = P(self.nhMu), P(self.nhLam)
         self.s = s
+
+    def check_product(self, mesh, tol=1e-12):
+        """Raise ProductPrecomputeMismatch unless the product Mesh's
+        precompute and coloring equal the oracle's."""
+        def exact(name, a, b):
+            if not np.array_equal(np.asarray(a), np.asarray(b)):
+                raise ProductPrecomputeMismatch(f"{name} differs from the oracle's")
+
+        def close(name, a, b):
+            # entry by entry to tol relative, with a floor of 1e-15 of the
+            # array's largest entry for entries that are zero up to rounding
+            a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+            if a.shape != b.shape:
+                raise ProductPrecomputeMismatch(f"{name} has shape {a.shape}, oracle {b.shape}")
+            if not a.size:
+                return
+            scale = float(np.max(np.abs(b)))
+            bound = tol * np.maximum(np.abs(a), np.abs(b)) + 1e-15 * scale
+            if not np.all(np.abs(a - b) <= bound):
+                raise ProductPrecomputeMismatch(f"{name} differs from the oracle's by more than {tol}")
+
+        exact("elementColor", mesh.elementColor, self.color)
+        exact("numColors", int(mesh.numColors), self.numColors)
+        exact("pinnedDOFs", np.asarray(mesh.pinnedDOFs) - 1, self.pinned)
+        exact("pinnedTets", np.asarray(mesh.forcedElastic, dtype=np.uint8), self.forced)
+        exact("t", mesh.T0, self.T)
+        # the constants follow the same elimination order on both sides, so
+        # they agree bit for bit (the layer maps at rest depend on the last
+        # bits of DmInv through the strain-rate keys)
+        for name, a, b in (("DmInv", mesh.DmInv, self.DmInv), ("elV", mesh.elV, self.vol),
+                           ("mass", mesh.mass, self.mass), ("alpha", mesh.alpha, self.alpha),
+                           ("beta", mesh.beta, self.beta)):
+            close(name, a, b)
+            exact(name, a, b)
+
+
+class ProductPrecomputeMismatch(AssertionError):
+    pass
 
 
 def settings_from(integ):
@@ -154,13 +224,13 @@ class OracleSim:
     """The oracle's integrate() loop for one mesh (simulate3D / simulate step
     order, contacts at the start of each step)."""
 
-    def __init__(self, mesh, integ, h, contact=None, v0=None, residual_cap=4096):
+    def __init__(self, mesh, integ, h, contact=None, v0=None, residual_cap=4096, p0=None):
         self.om = OracleMesh(mesh)
         self.integ = integ
         self.h = h
         self.contact = contact
         self.dim = mesh.dim
-        self.p = mesh.p.copy()
+        self.p = mesh.p.copy() if p0 is None else np.array(p0, dtype=np.float64)
         self.v = np.zeros_like(self.p) if v0 is None else np.array(v0, dtype=np.float64)
         self.oldp = self.p.copy()       # cache.oldp (3D) / prevPrevp source (2D)
         self.oldOldp = self.p.copy()

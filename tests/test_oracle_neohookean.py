@@ -75,14 +75,14 @@ def test_neohookean_sweep_uses_color_start_positions():
     e = 0
     T = m.T0[e]
     xe = xi.reshape(-1, 2)[T].reshape(-1).copy()
-    mass = m.mass.reshape(-1, 2)[T].reshape(-1).copy()
-    D = m.DmInv[4 * e:4 * e + 4].copy()
+    mass = om.mass.reshape(-1, 2)[T].reshape(-1).copy()
+    D = om.DmInv[4 * e:4 * e + 4].copy()
     dxD, dxH, dlD, dlH = np.zeros(6), np.zeros(6), C.c_double(), C.c_double()
     O.lib().ora_neohookean_d2d(C.c_double(0.01), P(xe), P(D), C.c_double(sc["mu"][e]),
-                               C.c_double(m.elV[e]), C.c_double(0.0), P(mass), P(dxD), C.byref(dlD), None)
+                               C.c_double(om.vol[e]), C.c_double(0.0), P(mass), P(dxD), C.byref(dlD), None)
     O.lib().ora_neohookean_h2d(C.c_double(0.01), P(xe), P(D), C.c_double(sc["lam"][e]),
-                               C.c_double(m.elV[e]), C.c_double(0.0), P(mass),
+                               C.c_double(om.vol[e]), C.c_double(0.0), P(mass),
                                C.c_double(1 + sc["mu"][e] / sc["lam"][e]), P(dxH), C.byref(dlH), None)
-    assert m.elementColor[e] == 1
+    assert om.color[e] == 1
     np.testing.assert_array_equal(x1.reshape(-1, 2)[T].reshape(-1), (xe + dxD) + dxH)
     assert lam[0] == dlD.value and lam[1] == dlH.value

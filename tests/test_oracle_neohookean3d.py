@@ -148,14 +148,14 @@ def test_sweep_sums_both_displacements_from_the_same_positions(oracle_lib):
     isEl = np.zeros(m.nT, dtype=np.uint8)
     isEl[first] = 1
     x = np.ascontiguousarray(xi.reshape(-1, 3)[m.T0[first]].reshape(-1))
-    ms = np.ascontiguousarray(m.mass.reshape(-1, 3)[m.T0[first]].reshape(-1))
-    D = np.ascontiguousarray(m.DmInv[9 * first:9 * first + 9])
+    ms = np.ascontiguousarray(om.mass.reshape(-1, 3)[m.T0[first]].reshape(-1))
+    D = np.ascontiguousarray(om.DmInv[9 * first:9 * first + 9])
     h = 1 / 60
     mu, la = m.elMu[first], m.elLambda[first]
     dxD, dxH, dlD, dlH = np.zeros(12), np.zeros(12), np.zeros(1), np.zeros(1)
-    L.ora_neohookean_d3d(C.c_double(h), P(x), P(D), C.c_double(mu), C.c_double(m.elV[first]),
+    L.ora_neohookean_d3d(C.c_double(h), P(x), P(D), C.c_double(mu), C.c_double(om.vol[first]),
                          C.c_double(0.0), P(ms), P(dxD), P(dlD), None)
-    L.ora_neohookean_h3d(C.c_double(h), P(x), P(D), C.c_double(la), C.c_double(m.elV[first]),
+    L.ora_neohookean_h3d(C.c_double(h), P(x), P(D), C.c_double(la), C.c_double(om.vol[first]),
                          C.c_double(0.0), P(ms), C.c_double(1 + mu / la), P(dxH), P(dlH), None)
     before = xi.copy()
     L.ora_neohookean_constraints3d(C.byref(om.s), P(isEl, C.c_uint8), C.c_double(h), P(lam), P(xi))
