@@ -141,6 +141,21 @@ typedef struct {
   const int32_t *pinnedDOFs;   /* 1-based, ascending */
   int64_t nPinnedDOFs;
   const uint8_t *forcedElastic;/* nT: pinned elements (pinnedTets) */
+  /* Optional (3D): elements projected with the NeoHookean pair instead of
+   * StVK-Voigt -- the deviatoric C_D = ||F||_F (compliance 1/(vol h^2 mu))
+   * and volumetric C_H = det F - (1 + mu/lambda) (compliance
+   * 1/(vol h^2 lambda)) constraints of mexNeoHookeanDConstraint.cpp:46-138 /
+   * mexNeoHookeanHConstraint.cpp:47-138 (xpbdLayer2D.neoHookeanConstraints,
+   * 2d/integrator/xpbdLayer2D.m:260-294) lifted to tetrahedra, both
+   * evaluated at the positions the projection starts from, displacements
+   * summed.  No counterpart in the reference's 3D integrator, which projects
+   * StVK-Voigt only (3d/integrator/xpbdLayer3D.m:318,347-379).  Their
+   * multipliers are lambdai(:,1:2); alpha/beta of such elements are not
+   * read.  neohookean: nT logical or NULL (all StVK-Voigt); mu, lambda: nT
+   * Lame parameters (mesh.elMu / elLambda), read for the flagged elements. */
+  const uint8_t *neohookean;
+  const double *mu;
+  const double *lambda;
 } mlx_mesh_desc;
 
 int mlx_mesh_create(const mlx_mesh_desc *desc, int32_t precision,
@@ -153,7 +168,8 @@ int mlx_mesh_info(const mlx_mesh *mesh, int32_t *numColors,
 /* Replaces xpbdLayer3D.stvkVoigtConstraints(numColors, elementDOFs, elementColor,
  * DmInv, alpha/h^2, mass, volume, lambdai, xi, elasticElements, beta*h^2, oldp, h)
  * (3d/integrator/xpbdLayer3D.m:347-379) and its 2D twin
- * (2d/integrator/xpbdLayer2D.m:61-89): one Gauss-Seidel sweep over all colors.
+ * (2d/integrator/xpbdLayer2D.m:61-89): one Gauss-Seidel sweep over all colors
+ * (elements flagged neohookean in the mesh take the NeoHookean pair).
  * isElasticElement: nT logical.  lambdai: nT x nc column-major, in/out.
  * xi: dim*nV in/out.  oldp: dim*nV. */
 int mlx_stvk_voigt_constraints(mlx_mesh *mesh, const uint8_t *isElasticElement,
@@ -341,7 +357,9 @@ typedef struct mlx_batch mlx_batch;
  * ascending).  sceneVertOff / sceneElemOff: nScenes+1 offsets (0-based).
  * Each scene is integrated independently, with its own residual and stop
  * rule; no extension of the reference (whose merged mesh shares one
- * residual) beyond that.  Plain method only (every layer 0 %% rigid). */
+ * residual) beyond that.  Plain method only (every layer 0 %% rigid).  A
+ * scene is all StVK-Voigt or all NeoHookean (mlx_mesh_desc.neohookean);
+ * a scene that mixes them is rejected with MLX_ERR_UNSUPPORTED. */
 int mlx_batch_create(const mlx_mesh_desc *merged, int32_t nScenes,
                      const int64_t *sceneVertOff, const int64_t *sceneElemOff,
                      int32_t precision, mlx_batch **out);

@@ -57,6 +57,7 @@ class MeshDesc(C.Structure):
         ("t", _i32), ("DmInv", _d), ("alpha", _d), ("beta", _d),
         ("volume", _d), ("mass", _d), ("elementColor", _i32),
         ("pinnedDOFs", _i32), ("nPinnedDOFs", C.c_int64), ("forcedElastic", _u8),
+        ("neohookean", _u8), ("mu", _d), ("lambda_", _d),
     ]
 
 

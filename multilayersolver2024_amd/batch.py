@@ -15,15 +15,13 @@ import numpy as np
 
 from . import _lib as L
 from . import scenes
-from .mesh import Mesh, _reject_neohookean
+from .mesh import Mesh
 
 
 class MergedMesh:
     """Concatenation of 3D Mesh objects (duck-types Mesh.desc())."""
 
     def __init__(self, meshes):
-        for m in meshes:
-            _reject_neohookean(m)
         self.meshes = meshes
         self.dim = 3
         self.nc = 6
@@ -46,6 +44,9 @@ class MergedMesh:
         self.pinnedDOFs = L.i32(np.concatenate(
             [m.pinnedDOFs + 3 * int(self.vertOff[i]) for i, m in enumerate(meshes)]))
         self.forcedElastic = L.u8(np.concatenate([m.forcedElastic for m in meshes]))
+        self.neohookean = L.u8(np.concatenate([m.neohookean for m in meshes]))
+        self.elMu = L.f64(np.concatenate([m.elMu for m in meshes]))
+        self.elLambda = L.f64(np.concatenate([m.elLambda for m in meshes]))
 
     def desc(self):
         return Mesh.desc(self)

@@ -91,6 +91,7 @@ struct mlx_mesh {
   std::vector<double> massV;        // per vertex
   std::vector<int32_t> pinnedDOF;   // 0-based
   std::vector<uint8_t> forcedElastic;
+  bool hasNH = false;               // some element is a NeoHookean tetrahedron
   std::vector<int32_t> color;       // 1-based
   std::vector<int64_t> colorOff;    // numColors+1 padded slot offsets
   std::vector<int64_t> colorCnt;    // elements per color

@@ -12,12 +12,6 @@
 
 #include "mlx_internal.h"
 
-// Multiply-adds are fused only within one source expression (never across
-// statements), so a device function gives the same bits wherever it is
-// inlined: the batched kernel's LDS and HBM position paths, the persistent
-// loop and the per-color launches all share the projection code and must
-// agree bit for bit.
-#pragma clang fp contract(on)
 
 namespace mlx {
 
@@ -216,6 +210,7 @@ struct SweepParams {
   const Ctl *ctl;
   int64_t ld, begin, end;
   T ih2, ih;  // 1/h^2, 1/h
+  int nh;     // some slot holds a NeoHookean tetrahedron (uniform per launch)
 };
 
 // StVK-Voigt tetrahedron (matStvkVoigtConstraint3D.m:1-45).  With F the
@@ -242,7 +237,23 @@ struct TetIn {
   vec4_t<T> O[4];
 };
 
-template <typename T>
+// Material mode of a specialised projection / residual: StVK-Voigt slots
+// only, NeoHookean slots only, or either (chosen per slot from the sign of
+// the first material scalar).  The batched kernel runs a scene with the
+// single-material body, the per-color launches of a mesh with NeoHookean
+// elements the mixed one.
+constexpr int kMatStVK = 0, kMatMixed = 1, kMatNH = 2;
+
+// Third material scalar as the projection uses it: alpha_s / h^2
+// (StVK-Voigt) or gamma as stored (NeoHookean).
+template <typename T, int MAT>
+__device__ __forceinline__ T mat_third(T m0, T m2, T ih2) {
+  if constexpr (MAT == kMatStVK) return m2 * ih2;
+  else if constexpr (MAT == kMatNH) return m2;
+  else return m0 < T(0) ? m2 : m2 * ih2;
+}
+
+template <typename T, int MAT = 0>
 __device__ __forceinline__ void tet_fetch(const SweepParams<T> &P, const int64_t s, TetIn<T> &in) {
   const int64_t ld = P.ld;
 #pragma unroll
@@ -255,7 +266,7 @@ __device__ __forceinline__ void tet_fetch(const SweepParams<T> &P, const int64_t
   const T m0 = P.mat[s], m2 = P.mat[2 * ld + s];
   in.tad = m0 * P.ih2;
   in.tao = P.mat[ld + s] * P.ih2;
-  in.tas = m0 < T(0) ? m2 : m2 * P.ih2;  // NeoHookean slot: gamma as stored
+  in.tas = mat_third<T, MAT>(m0, m2, P.ih2);
   in.g = P.mat[3 * ld + s] * P.ih;
 #pragma unroll
   for (int i = 0; i < 6; ++i) in.lam[i] = P.lam[i * ld + s];
@@ -332,40 +343,56 @@ __device__ __forceinline__ void tet_solve_nh_st(const SweepParams<T> &P, const i
   const T w[4] = {X[0].w, X[1].w, X[2].w, X[3].w};
   const T iv = T(1) / in.vol;
   const T aD = iv * (-in.tad), aH = iv * in.tao, gam = in.tas;
-  T gD[4][3], gH[4][3];
-  T sD = aD, sH = aH;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
+  // vertex gradients: gD_a = F g_a / n, gH_a = cof(F) g_a; formed twice (for
+  // the denominators, then for the displacements) rather than kept live
+  auto grads = [&](int a, T (&gD)[3], T (&gH)[3]) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      gD[a][r] = (F[r][0] * G[a][0] + F[r][1] * G[a][1] + F[r][2] * G[a][2]) / n;
-      gH[a][r] = PH[r][0] * G[a][0] + PH[r][1] * G[a][1] + PH[r][2] * G[a][2];
-      sD += w[a] * (gD[a][r] * gD[a][r]);
-      sH += w[a] * (gH[a][r] * gH[a][r]);
+      gD[r] = (F[r][0] * G[a][0] + F[r][1] * G[a][1] + F[r][2] * G[a][2]) / n;
+      gH[r] = PH[r][0] * G[a][0] + PH[r][1] * G[a][1] + PH[r][2] * G[a][2];
     }
+  };
+  T sD = aD, sH = aH;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    T gD[3], gH[3];
+    grads(a, gD, gH);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      sD += w[a] * (gD[r] * gD[r]);
+      sH += w[a] * (gH[r] * gH[r]);
+    }
+  }
   const T dlD = -(n + aD * in.lam[0]) / sD;
   const T dlH = -((J - gam) + aH * in.lam[1]) / sH;
   P.lam[s] = in.lam[0] + dlD;
   P.lam[ld + s] = in.lam[1] + dlH;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
+    T gD[3], gH[3];
+    grads(a, gD, gH);
     vec4_t<T> o = X[a];
-    o.x = (o.x + w[a] * gD[a][0] * dlD) + w[a] * gH[a][0] * dlH;
-    o.y = (o.y + w[a] * gD[a][1] * dlD) + w[a] * gH[a][1] * dlH;
-    o.z = (o.z + w[a] * gD[a][2] * dlD) + w[a] * gH[a][2] * dlH;
+    o.x = (o.x + w[a] * gD[0] * dlD) + w[a] * gH[0] * dlH;
+    o.y = (o.y + w[a] * gD[1] * dlD) + w[a] * gH[1] * dlH;
+    o.z = (o.z + w[a] * gD[2] * dlD) + w[a] * gH[2] * dlH;
     st(a, o);
   }
 }
 
 // The projection proper, on fetched slot data and the current positions X
 // of the tet's four vertices; the moved vertex a is handed to st(a, x).
-template <typename T, typename Store>
+template <typename T, int MAT = 0, typename Store>
 __device__ __forceinline__ void tet_solve_st(const SweepParams<T> &P, const int64_t s,
                                              const TetIn<T> &in, const vec4_t<T> (&X)[4],
                                              Store &&st) {
-  if (in.tad < T(0)) {
+  if constexpr (MAT == kMatNH) {
     tet_solve_nh_st<T>(P, s, in, X, st);
     return;
+  } else if constexpr (MAT == kMatMixed) {
+    if (in.tad < T(0)) {
+      tet_solve_nh_st<T>(P, s, in, X, st);
+      return;
+    }
   }
   const int64_t ld = P.ld;
   const vec4_t<T> X0 = X[0], X1 = X[1], X2 = X[2], X3 = X[3];
@@ -471,10 +498,10 @@ __device__ __forceinline__ void tet_solve_st(const SweepParams<T> &P, const int6
   }
 }
 
-template <typename T>
+template <typename T, int MAT = 0>
 __device__ __forceinline__ void tet_solve_x(const SweepParams<T> &P, const int64_t s,
                                             const TetIn<T> &in, const vec4_t<T> (&X)[4]) {
-  tet_solve_st<T>(P, s, in, X, [&](int a, const vec4_t<T> &o) { P.x[in.v[a]] = o; });
+  tet_solve_st<T, MAT>(P, s, in, X, [&](int a, const vec4_t<T> &o) { P.x[in.v[a]] = o; });
 }
 
 // Current positions of a fetched tet's vertices (ldc<COH>).
@@ -485,19 +512,19 @@ __device__ __forceinline__ void tet_gather(const SweepParams<T> &P, const TetIn<
   for (int a = 0; a < 4; ++a) X[a] = ldc<COH>(P.x + in.v[a]);
 }
 
-template <typename T, bool COH>
+template <typename T, bool COH, int MAT = 0>
 __device__ __forceinline__ void tet_solve(const SweepParams<T> &P, const int64_t s,
                                           const TetIn<T> &in) {
   vec4_t<T> X[4];
   tet_gather<T, COH>(P, in, X);
-  tet_solve_x<T>(P, s, in, X);
+  tet_solve_x<T, MAT>(P, s, in, X);
 }
 
-template <typename T>
+template <typename T, int MAT = 0>
 __device__ __forceinline__ void project_tet(const SweepParams<T> &P, const int64_t s) {
   TetIn<T> in;
-  tet_fetch<T>(P, s, in);
-  tet_solve<T, false>(P, s, in);
+  tet_fetch<T, MAT>(P, s, in);
+  tet_solve<T, false, MAT>(P, s, in);
 }
 
 // The slots s, s + step, ... < e of one color for one thread, two at a time:
@@ -531,7 +558,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep3d(SweepParams<T> P) {
   const int64_t s = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.end) return;
   if (P.flags && !P.flags[s]) return;
-  project_tet<T>(P, s);
+  if (P.nh)
+    project_tet<T, kMatMixed>(P, s);
+  else
+    project_tet<T>(P, s);
 }
 
 // Elastic slots of each color of a layer, compacted in ascending slot order
@@ -580,7 +610,12 @@ __global__ __launch_bounds__(NT) void k_sweep_wg(SweepParams<T> P, const int64_t
   for (int c = 0; c < numColors; ++c) {
     const int n = cnt[c];
     const int32_t *l = list + colorOff[c];
-    for (int i = threadIdx.x; i < n; i += NT) project_tet<T>(P, l[i]);
+    for (int i = threadIdx.x; i < n; i += NT) {
+      if (P.nh)
+        project_tet<T, kMatMixed>(P, l[i]);
+      else
+        project_tet<T>(P, l[i]);
+    }
     __syncthreads();
   }
 }
@@ -1528,14 +1563,14 @@ struct ResIn {
   T lam[6];
 };
 
-template <typename T, bool COH>
+template <typename T, bool COH, int MAT = 0>
 __device__ __forceinline__ void res_fetch(const SweepParams<T> &P, const int64_t s,
                                           ResIn<T> &in) {
   const int64_t ld = P.ld;
   const T m0 = P.mat[s], m2 = P.mat[2 * ld + s];
   in.tad = m0 * P.ih2;
   in.tao = P.mat[ld + s] * P.ih2;
-  in.tas = m0 < T(0) ? m2 : m2 * P.ih2;  // NeoHookean slot: gamma as stored
+  in.tas = mat_third<T, MAT>(m0, m2, P.ih2);
   in.vol = P.vol[s];
 #pragma unroll
   for (int a = 0; a < 4; ++a) in.X[a] = ldc<COH>(P.x + P.vid[a * ld + s]);
@@ -1568,9 +1603,11 @@ __device__ __forceinline__ double res_compute_nh(const ResIn<T> &in) {
   return acc;
 }
 
-template <typename T>
+template <typename T, int MAT = 0>
 __device__ __forceinline__ double res_compute(const ResIn<T> &in) {
-  if (in.tad < T(0)) return res_compute_nh<T>(in);
+  if constexpr (MAT == kMatNH) return res_compute_nh<T>(in);
+  if constexpr (MAT == kMatMixed)
+    if (in.tad < T(0)) return res_compute_nh<T>(in);
   const T tad = in.tad, tao = in.tao, tas = in.tas;
   const T vol = in.vol;
   const vec4_t<T> X0 = in.X[0], X1 = in.X[1], X2 = in.X[2], X3 = in.X[3];
@@ -1619,14 +1656,14 @@ __device__ __forceinline__ double res_compute(const ResIn<T> &in) {
 }
 
 // res_fetch with the positions from pos(vertex id) instead of P.x.
-template <typename T, typename Pos>
+template <typename T, int MAT = 0, typename Pos>
 __device__ __forceinline__ void res_fetch_pos(const SweepParams<T> &P, const int64_t s,
                                               ResIn<T> &in, Pos &&pos) {
   const int64_t ld = P.ld;
   const T m0 = P.mat[s], m2 = P.mat[2 * ld + s];
   in.tad = m0 * P.ih2;
   in.tao = P.mat[ld + s] * P.ih2;
-  in.tas = m0 < T(0) ? m2 : m2 * P.ih2;  // NeoHookean slot: gamma as stored
+  in.tas = mat_third<T, MAT>(m0, m2, P.ih2);
   in.vol = P.vol[s];
 #pragma unroll
   for (int a = 0; a < 4; ++a) in.X[a] = pos(P.vid[a * ld + s]);
@@ -1636,11 +1673,11 @@ __device__ __forceinline__ void res_fetch_pos(const SweepParams<T> &P, const int
   for (int i = 0; i < 6; ++i) in.lam[i] = P.lam[i * ld + s];
 }
 
-template <typename T, bool COH = false>
+template <typename T, bool COH = false, int MAT = 0>
 __device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const int64_t s) {
   ResIn<T> in;
-  res_fetch<T, COH>(P, s, in);
-  return res_compute<T>(in);
+  res_fetch<T, COH, MAT>(P, s, in);
+  return res_compute<T, MAT>(in);
 }
 
 // Sum of residual_tet over the real slots s, s + step, ... < e of one
@@ -1673,7 +1710,10 @@ __device__ __forceinline__ double residual_slots(const SweepParams<T> &P, int64_
 template <typename T, int DIM, bool COH = false>
 __device__ __forceinline__ double residual_elem(const SweepParams<T> &P, int64_t s,
                                                 const int32_t *lamSlot2d) {
-  if (DIM == 3) return residual_tet<T, COH>(P, s);
+  if (DIM == 3) {
+    if (P.nh) return residual_tet<T, COH, kMatMixed>(P, s);
+    return residual_tet<T, COH>(P, s);
+  }
   static_assert(DIM == 3 || !COH, "coherent loads are for 3D only");
   const int64_t ld = P.ld;
   const T tad = P.mat[s] * P.ih2, tao = P.mat[ld + s] * P.ih2, tas = P.mat[2 * ld + s] * P.ih2;
@@ -2119,6 +2159,7 @@ struct BatchParams {
   const int64_t *pin0;           // nScenes+1 pinned-DOF range per scene
   const int64_t *vert0;          // nScenes+1 vertex range per scene
   int ldsVerts;                  // scenes with at most this many vertices keep positions in LDS
+  const uint8_t *sceneNH;        // per scene: holds NeoHookean tetrahedra (null: none)
   const int32_t *pinDof;         // 0-based DOF
   const T *pinVal;
   ContactArrays c;
@@ -2156,7 +2197,7 @@ __device__ __forceinline__ double wg_sum(double v, double *shw) {
 // from the first color to the last residual, and go back to HBM at the end.
 // The inverse mass (w) comes from x_old, a copy of x taken at the start of
 // the substep.  Same updates in the same order as the HBM path.
-template <typename T, int NT>
+template <typename T, int NT, int MAT>
 __device__ __forceinline__ void batch_scene_lds(const BatchParams<T> &B, const int sc, T *pos,
                                                 double *sh) {
   __shared__ int sStop;
@@ -2214,7 +2255,7 @@ __device__ __forceinline__ void batch_scene_lds(const BatchParams<T> &B, const i
       const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
       for (int64_t s = a + tid; s < e; s += NT) {
         TetIn<T> in;
-        tet_fetch<T>(B.P, s, in);
+        tet_fetch<T, MAT>(B.P, s, in);
         vec4_t<T> X[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -2224,7 +2265,8 @@ __device__ __forceinline__ void batch_scene_lds(const BatchParams<T> &B, const i
           X[q].z = pos[2 * nv + l];
           X[q].w = in.O[q].w;
         }
-        tet_solve_st<T>(B.P, s, in, X, [&](int q, const vec4_t<T> &o) { st_pos(in.v[q], o); });
+        tet_solve_st<T, MAT>(B.P, s, in, X,
+                            [&](int q, const vec4_t<T> &o) { st_pos(in.v[q], o); });
       }
       __syncthreads();
     }
@@ -2257,8 +2299,8 @@ __device__ __forceinline__ void batch_scene_lds(const BatchParams<T> &B, const i
       for (int64_t s = s0 + tid; s < s1; s += NT)
         if (B.P.vid[s] >= 0) {
           ResIn<T> in;
-          res_fetch_pos<T>(B.P, s, in, ld_pos);
-          acc += res_compute<T>(in);
+          res_fetch_pos<T, MAT>(B.P, s, in, ld_pos);
+          acc += res_compute<T, MAT>(in);
         }
       const double Ks = (double)(k1 - k0);
       for (int64_t i = k0 + tid; i < k1; i += NT) {
@@ -2306,19 +2348,17 @@ __device__ __forceinline__ void batch_scene_lds(const BatchParams<T> &B, const i
 // projectGS loop of xpbdLayer3D.m:315-341 with colors separated by workgroup
 // barriers, then contacts, pins and the scene's own residual/stop rule.
 // Every loop is bounded (hardCap) so the kernel always returns.  Scenes
-// with at most ldsVerts vertices run batch_scene_lds on the dynamic LDS.
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
-  __shared__ double sh[NT / 64];
+// with at most ldsVerts vertices keep their positions in the dynamic LDS
+// (batch_scene_lds), the others in HBM (batch_scene_hbm).  Each scene runs
+// the body specialised for its materials (StVK-Voigt only, or with
+// NeoHookean tetrahedra), chosen once per workgroup, so the StVK body
+// carries no material branch.
+template <typename T, int NT, int MAT>
+__device__ __forceinline__ void batch_scene_hbm(const BatchParams<T> &B, const int sc,
+                                                double *sh) {
   __shared__ int sStop;
   __shared__ double sRunUntil;
-  extern __shared__ __align__(16) unsigned char mlx_dyn_lds[];
-  const int sc = blockIdx.x;
   const int tid = threadIdx.x;
-  if (B.vert0[sc + 1] - B.vert0[sc] <= B.ldsVerts) {
-    batch_scene_lds<T, NT>(B, sc, reinterpret_cast<T *>(mlx_dyn_lds), sh);
-    return;
-  }
   Ctl c;
   c.stopSolve = 0;
   c.endLayer = 0;
@@ -2346,7 +2386,7 @@ __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
   while (it < B.hardCap) {
     for (int k = 0; k < nc; ++k) {
       const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
-      for (int64_t s = a + tid; s < e; s += NT) project_tet<T>(B.P, s);
+      for (int64_t s = a + tid; s < e; s += NT) project_tet<T, MAT>(B.P, s);
       __syncthreads();
     }
     for (int64_t i = k0 + tid; i < k1; i += NT) {
@@ -2377,7 +2417,7 @@ __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
     if (B.mp.computeResiduals) {
       double acc = 0;
       for (int64_t s = s0 + tid; s < s1; s += NT)
-        if (B.P.vid[s] >= 0) acc += residual_tet<T>(B.P, s);
+        if (B.P.vid[s] >= 0) acc += residual_tet<T, false, MAT>(B.P, s);
       const double Ks = (double)(k1 - k0);
       for (int64_t i = k0 + tid; i < k1; i += NT) {
         const int v = B.c.vert[i];
@@ -2411,6 +2451,26 @@ __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
     B.runUntilOut[sc] = c.runUntil;
     B.residualCount[sc] = c.residualCount;
   }
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
+  __shared__ double sh[NT / 64];
+  extern __shared__ __align__(16) unsigned char mlx_dyn_lds[];
+  const int sc = blockIdx.x;
+  const bool nh = B.sceneNH && B.sceneNH[sc];  // the scene is all NeoHookean
+  if (B.vert0[sc + 1] - B.vert0[sc] <= B.ldsVerts) {
+    T *pos = reinterpret_cast<T *>(mlx_dyn_lds);
+    if (nh)
+      batch_scene_lds<T, NT, kMatNH>(B, sc, pos, sh);
+    else
+      batch_scene_lds<T, NT, kMatStVK>(B, sc, pos, sh);
+    return;
+  }
+  if (nh)
+    batch_scene_hbm<T, NT, kMatNH>(B, sc, sh);
+  else
+    batch_scene_hbm<T, NT, kMatStVK>(B, sc, sh);
 }
 
 }  // namespace mlx
@@ -2947,7 +3007,7 @@ __global__ __launch_bounds__(NT) void k_batch_split(BatchParams<T> B, SplitParam
   if (G > 1)
     batch_scene<T, NT, true>(B, S, sc, r, G, sh);
   else if (B.vert0[sc + 1] - B.vert0[sc] <= B.ldsVerts)
-    batch_scene_lds<T, NT>(B, sc, reinterpret_cast<T *>(mlx_dyn_lds), sh);
+    batch_scene_lds<T, NT, kMatStVK>(B, sc, reinterpret_cast<T *>(mlx_dyn_lds), sh);
   else
     batch_scene<T, NT, false>(B, S, sc, 0, 1, sh);
 }

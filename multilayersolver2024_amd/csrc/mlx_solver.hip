@@ -314,6 +314,7 @@ static SweepParams<T> sweep_params(mlx_mesh &m, double h, const Ctl *ctl) {
   P.end = 0;
   P.ih2 = (T)(1.0 / (h * h));
   P.ih = (T)(1.0 / h);
+  P.nh = m.hasNH ? 1 : 0;
   return P;
 }
 
@@ -956,7 +957,8 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     };
     // Layers without rigid elements: the iterations run inside one
     // cooperative launch (k_persist) per call.
-    const bool persist = m.usePersistent && !hasRigidEl && L.nb == 0;
+    // (StVK-Voigt meshes only: the persistent loop has no NeoHookean body)
+    const bool persist = m.usePersistent && !hasRigidEl && L.nb == 0 && !m.hasNH;
     auto run_persistent = [&](int n) -> int {
       if (n <= 0) return MLX_OK;
       MLX_CHECK_HIP(hipMemsetAsync(m.d_sync.ptr, 0, 256, st));
@@ -1272,6 +1274,25 @@ static int mesh_create_impl(const mlx_mesh_desc *d, int32_t precision, int nScen
   m->mat.resize(4 * (size_t)nT);
   const int nn = dim == 3 ? 3 : 2;
   for (int64_t e = 0; e < nT; ++e) {
+    if (d->neohookean && d->neohookean[e]) {
+      // NeoHookean tetrahedron: -1/mu, 1/lambda, gamma = 1 + mu/lambda, 0
+      // (the negative first scalar tags the slot for the kernels)
+      if (dim != 3 || !d->mu || !d->lambda) {
+        set_error("NeoHookean elements need a 3D mesh and the mu/lambda arrays");
+        return fail(MLX_ERR_UNSUPPORTED);
+      }
+      const double mu = d->mu[e], la = d->lambda[e];
+      if (!(mu > 0) || !(la > 0) || !std::isfinite(mu) || !std::isfinite(la)) {
+        set_error("element %lld: NeoHookean mu and lambda must be positive", (long long)e + 1);
+        return fail(MLX_ERR_ARG);
+      }
+      m->mat[4 * e] = -1.0 / mu;
+      m->mat[4 * e + 1] = 1.0 / la;
+      m->mat[4 * e + 2] = 1.0 + mu / la;
+      m->mat[4 * e + 3] = 0.0;
+      m->hasNH = true;
+      continue;
+    }
     const double *A = d->alpha + (size_t)nc * nc * e, *B = d->beta + (size_t)nc * nc * e;
     const double ad = A[0], ao = A[nc], as = A[nn + nc * nn];
     double a1 = 0;
@@ -2334,6 +2355,7 @@ struct mlx_batch {
   DevBuf d_v;        // velocity T 3nV
   DevBuf d_out;      // per-scene outputs
   DevBuf d_pinMask;  // uint8 3nV
+  DevBuf d_sceneNH;  // uint8 per scene: holds NeoHookean tetrahedra (allocated only if any)
   DevBuf d_flags;    // int32 nV (plane test)
   DevBuf d_split;    // split schedule, per-scene barriers and flags, partials
   int split = 0;     // target tets per workgroup (0: one workgroup per scene)
@@ -2556,6 +2578,7 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
                : b.block == 1024 ? batch_lds_verts<T, 1024>(maxSceneVerts, b.useLds, &ldsBytes)
                                  : batch_lds_verts<T, 512>(maxSceneVerts, b.useLds, &ldsBytes);
   b.ldsVerts = B.ldsVerts;
+  B.sceneNH = m.hasNH ? b.d_sceneNH.as<uint8_t>() : nullptr;
   B.pinDof = m.d_pinned.as<int32_t>();
   B.pinVal = m.d_pinnedx.as<T>();
   B.c = ca;
@@ -2592,7 +2615,8 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   };
   int *splitErr = nullptr, *splitMismatch = nullptr;
   bool replayed = false;
-  if (b.split > 0) {
+  // the split schedule has a StVK-Voigt body only
+  if (b.split > 0 && !m.hasNH) {
     MLX_TRY(enqueue_pre());
     // split schedule: scene sc gets G = round(nT_sc / split) workgroups (1..64)
     // on one XCD; scenes go to XCDs largest first, each to the XCD with the
@@ -2845,6 +2869,26 @@ int mlx_batch_create(const mlx_mesh_desc *d, int32_t nScenes, const int64_t *ver
     }
     b->pin0[nScenes] = (int64_t)pd.size();
   }
+  if (b->m->hasNH) {
+    // a scene is all StVK-Voigt or all NeoHookean: the batched kernel runs
+    // each scene with the body specialised for its one material
+    std::vector<uint8_t> snh(nScenes, 0);
+    for (int sc = 0; sc < nScenes; ++sc) {
+      int64_t n = 0;
+      for (int64_t e = elemOff[sc]; e < elemOff[sc + 1]; ++e) n += d->neohookean[e] != 0;
+      if (n > 0 && n < elemOff[sc + 1] - elemOff[sc]) {
+        mlx_batch_destroy(b);
+        set_error("mlx_batch_create: scene %d mixes StVK-Voigt and NeoHookean elements", sc);
+        return MLX_ERR_UNSUPPORTED;
+      }
+      snh[sc] = n > 0;
+    }
+    if ((r = b->d_sceneNH.reserve(snh.size())) != MLX_OK ||
+        hipMemcpy(b->d_sceneNH.ptr, snh.data(), snh.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      mlx_batch_destroy(b);
+      return r != MLX_OK ? r : MLX_ERR_HIP;
+    }
+  }
   std::vector<uint8_t> mask(3 * (size_t)d->nV, 0);
   for (auto q : b->m->pinnedDOF) mask[q] = 1;
   if ((r = b->d_pinMask.reserve(mask.size())) != MLX_OK ||
@@ -2865,6 +2909,7 @@ int mlx_batch_destroy(mlx_batch *b) {
   b->d_v.release();
   b->d_out.release();
   b->d_pinMask.release();
+  b->d_sceneNH.release();
   b->d_flags.release();
   b->d_split.release();
   mlx_mesh_destroy(b->m);

@@ -91,18 +91,10 @@ class Mesh:
         d.pinnedDOFs = L.ptr(self.pinnedDOFs, C.c_int32)
         d.nPinnedDOFs = len(self.pinnedDOFs)
         d.forcedElastic = L.ptr(self.forcedElastic, C.c_uint8)
+        if self.neohookean.any():
+            d.neohookean = L.ptr(self.neohookean, C.c_uint8)
+            d.mu, d.lambda_ = L.ptr(self.elMu), L.ptr(self.elLambda)
         return d
-
-
-def _reject_neohookean(mesh):
-    """The GPU substep projects StVK-Voigt tetrahedra only.  The NeoHookean
-    pair exists in the CPU oracle, where it needs substeps of about 1/6000 s
-    to converge with lambda restarted every substep; a mesh that asks for it
-    is refused rather than run with another material."""
-    nh = getattr(mesh, "neohookean", None)
-    if nh is not None and np.any(nh):
-        raise NotImplementedError("NeoHookean elements are not projected by the GPU substep "
-                                  "(StVK-Voigt only); clear the scene's neohookean mask")
 
 
 class DeviceMesh:
@@ -110,7 +102,6 @@ class DeviceMesh:
     MLX_FP32."""
 
     def __init__(self, mesh, precision=L.MLX_FP64, device=0):
-        _reject_neohookean(mesh)
         lib = L.lib()
         L.check("mlx_init", lib.mlx_init(device))
         self.mesh = mesh

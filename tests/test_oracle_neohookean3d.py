@@ -189,13 +189,18 @@ def test_pair_converges_only_with_small_steps(oracle_lib):
     assert r[400] > 1e-2 * r[1]
 
 
-def test_gpu_mesh_refuses_neohookean_elements():
-    """The product path has no NeoHookean projection: a mesh or batch that
-    asks for one fails before anything reaches the device."""
+def test_neohookean_mask_reaches_the_mesh_descriptor():
+    """The NeoHookean mask and the Lame parameters travel to mlx_mesh_create
+    (single meshes and merged batches); StVK-only meshes pass NULL."""
+    import ctypes as C
     from multilayersolver2024_amd.batch import MergedMesh
-    from multilayersolver2024_amd.mesh import DeviceMesh
-    m = Mesh(scenes.beam3d(2, 1, 1, seed=0, neohookean=True))
-    with pytest.raises(NotImplementedError):
-        DeviceMesh(m)
-    with pytest.raises(NotImplementedError):
-        MergedMesh([m])
+    a = Mesh(scenes.beam3d(2, 1, 1, seed=0, neohookean=True))
+    b = Mesh(scenes.beam3d(2, 1, 1, seed=1))
+    d = a.desc()
+    assert bool(d.neohookean) and bool(d.mu) and bool(d.lambda_)
+    assert d.neohookean[0] == 1 and d.mu[3] == a.elMu[3] and d.lambda_[3] == a.elLambda[3]
+    assert not bool(b.desc().neohookean)
+    mm = MergedMesh([b, a])
+    dm = mm.desc()
+    assert dm.neohookean[0] == 0 and dm.neohookean[b.nT] == 1
+    assert dm.mu[b.nT + 2] == a.elMu[2]
