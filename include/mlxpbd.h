@@ -493,6 +493,73 @@ int mlx_profile_read_persistent(mlx_mesh *mesh, double *ms, int64_t *launches,
 #define MLX_OPT_SWEEP_WG 8
 int mlx_mesh_set_option(mlx_mesh *mesh, int32_t option, int64_t value);
 
+/* ---- learned warm start (3D) ----
+ * No counterpart in the reference: its substep restarts every multiplier at
+ * zero (xpbdLayer3D.m:666) and orders the layers by the strain rate of the
+ * last substep (xpbdLayer3D.m:601-619).  A per-element network, 21 inputs
+ * -> 64 -> 64 -> 2 outputs with SiLU activations (BF16 weights, FP32
+ * accumulation on the matrix cores), reads at the start of each substep
+ *   F (9, row-major) of the current positions, (F - F_prev)/h (9) with
+ *   F_prev from the previous substep's, log10(|lambda_prev| / (vol h^2 E)
+ *   + 1e-12), log10 E and nu (E, nu from the element's material)
+ * and gives kappa and c.  MLX_LEARN_WARMSTART: at the start of the last
+ * layer (when it has no rigid body), every elastic element whose
+ * multipliers are still zero starts from lambda_0 = clamp(kappa, 0,
+ * kappaMax) lambda_prev, its vertices displaced by (1/m_a) grad_a C^T
+ * lambda_0 (pinned vertices stay); the warm start is kept only if the
+ * residual of the equations the projection solves (per constraint of the
+ * layer's elastic elements C + alpha~ lambda + gamma grad C (x - x_old),
+ * per contact r phi) does not rise over the cold start's, else positions
+ * and multipliers are restored exactly.  (The monitor's measure,
+ * xpbdLayer3D.m:546-586, cannot judge it: it vanishes wherever
+ * |alpha~ lambda| >= |C|, however far lambda overshoots.)  MLX_LEARN_KEYS: each strain-rate key is multiplied by
+ * 10^clamp(c, -3, 3) before the layers are built (the rate predicted for
+ * the coming substep).  Neither changes the stop rule. */
+typedef struct mlx_mlp mlx_mlp;
+#define MLX_LEARN_WARMSTART 1
+#define MLX_LEARN_KEYS 2
+
+/* A network on the current device: zero weights and biases, identity
+ * normalisation and output map. */
+int mlx_mlp_create(mlx_mlp **out);
+int mlx_mlp_destroy(mlx_mlp *model);
+/* Parameters: W1 64x21, b1 64, W2 64x64, b2 64, W3 2x64, b3 2 (row-major,
+ * out x in); features are normalised as (x - featMean) * featInvStd (21
+ * each), outputs mapped as y = raw * outScale + outShift (2 each).  Resets
+ * the optimiser state.  Non-finite values are rejected. */
+int mlx_mlp_set(mlx_mlp *model, const float *W1, const float *b1, const float *W2,
+                const float *b2, const float *W3, const float *b3, const float *featMean,
+                const float *featInvStd, const float *outScale, const float *outShift);
+int mlx_mlp_get(mlx_mlp *model, float *W1, float *b1, float *W2, float *b2, float *W3,
+                float *b3);
+/* Outputs Y (n x 2) of raw features X (n x 21), host arrays. */
+int mlx_mlp_infer(mlx_mlp *model, const float *X, int64_t n, float *Y);
+/* Gradient of the weighted mean squared error sum_s w_s |y_s - t_s|^2 /
+ * sum_s w_s over X (n x 21), targets T (n x 2), weights W (n, or null for
+ * ones), in the layout dW1 64x21 | db1 | dW2 | db2 | dW3 2x64 | db3 (7,426
+ * floats); *loss the error (optional). */
+int mlx_mlp_grad(mlx_mlp *model, const float *X, const float *T, const float *W, int64_t n,
+                 float *grad, double *loss);
+/* Adam (0.9, 0.999, 1e-8) over shuffled minibatches, the shuffle from
+ * seed; epochLoss (optional, epochs) the mean error of each epoch. */
+int mlx_mlp_train(mlx_mlp *model, const float *X, const float *T, const float *W, int64_t n,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epochLoss);
+
+/* Attach a network to a 3D single-scene mesh (null detaches); flags
+ * MLX_LEARN_*; kappaMax bounds kappa (> 0).  The model must live on the
+ * mesh's device and outlive the attachment. */
+int mlx_mesh_set_learned(mlx_mesh *mesh, mlx_mlp *model, int32_t flags, float kappaMax);
+/* Warm starts kept and rejected since the last call (then cleared). */
+int mlx_learn_stats(mlx_mesh *mesh, int64_t *kept, int64_t *rejected);
+/* The last substep's network outputs in element order: lambda_0 (nT x 6)
+ * and the key factors 10^c (nT); either may be null. */
+int mlx_learn_outputs(mlx_mesh *mesh, double *lambda0, double *keyScale);
+/* The network's 21 raw features of every element (nT x 21, element order)
+ * from the state as the next mlx_integrate call will see it (positions,
+ * the previous substep's positions and multipliers), step h.  Needs a
+ * mesh with a network attached. */
+int mlx_learn_features(mlx_state *state, double h, float *features);
+
 /* hipDeviceSynchronize on the library's device. */
 int mlx_synchronize(void);
 

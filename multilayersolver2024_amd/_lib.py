@@ -34,7 +34,12 @@ EXPORTED = (
     "mlx_batch_create", "mlx_batch_destroy", "mlx_batch_set_block", "mlx_batch_set_state",
     "mlx_batch_download", "mlx_batch_integrate", "mlx_neohookean_constraints2d",
     "mlx_pcg_solve", "mlx_pcg_solve_noalloc",
+    "mlx_mlp_create", "mlx_mlp_destroy", "mlx_mlp_set", "mlx_mlp_get", "mlx_mlp_infer",
+    "mlx_mlp_grad", "mlx_mlp_train", "mlx_mesh_set_learned", "mlx_learn_stats",
+    "mlx_learn_outputs", "mlx_learn_features",
 )
+MLX_LEARN_WARMSTART = 1
+MLX_LEARN_KEYS = 2
 MLX_OPT_GRAPHS = 1
 MLX_OPT_GRAPH_ITERS = 2
 MLX_OPT_PERSISTENT = 3
@@ -49,6 +54,7 @@ _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
 _u8 = C.POINTER(C.c_uint8)
 _i64 = C.POINTER(C.c_int64)
+_f = C.POINTER(C.c_float)
 
 
 class MeshDesc(C.Structure):
@@ -178,6 +184,18 @@ def lib():
         "mlx_batch_download": [_p, _d, _d, _d],
         "mlx_batch_integrate": [_p, C.POINTER(Settings), C.c_double, C.c_int32, C.POINTER(Plane),
                                 C.POINTER(BatchOut)],
+        "mlx_mlp_create": [C.POINTER(_p)],
+        "mlx_mlp_destroy": [_p],
+        "mlx_mlp_set": [_p, _f, _f, _f, _f, _f, _f, _f, _f, _f, _f],
+        "mlx_mlp_get": [_p, _f, _f, _f, _f, _f, _f],
+        "mlx_mlp_infer": [_p, _f, C.c_int64, _f],
+        "mlx_mlp_grad": [_p, _f, _f, _f, C.c_int64, _f, _d],
+        "mlx_mlp_train": [_p, _f, _f, _f, C.c_int64, C.c_int32, C.c_int32, C.c_float,
+                          C.c_uint64, _f],
+        "mlx_mesh_set_learned": [_p, _p, C.c_int32, C.c_float],
+        "mlx_learn_stats": [_p, _i64, _i64],
+        "mlx_learn_outputs": [_p, _d, _d],
+        "mlx_learn_features": [_p, C.c_double, _f],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -199,6 +217,10 @@ def ptr(a, ctype=C.c_double):
         return None
     assert a.flags["C_CONTIGUOUS"] or a.flags["F_CONTIGUOUS"], "array must be contiguous"
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
 
 
 def f64(a):

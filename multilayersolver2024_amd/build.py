@@ -20,7 +20,7 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["mlx_host.cpp", "mlx_solver.hip", "mlx_maps.hip"]
+SOURCES = ["mlx_host.cpp", "mlx_solver.hip", "mlx_maps.hip", "mlx_learn.hip"]
 
 
 def build(verbose=False):

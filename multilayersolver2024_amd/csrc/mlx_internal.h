@@ -9,9 +9,16 @@
 
 #include "mlxpbd.h"
 
+struct mlx_mlp;
+
 namespace mlx {
 
 void set_error(const char *fmt, ...);
+
+// Device view of a network (mlx_learn.hip) and the device it lives on.
+struct MlpDev;
+MlpDev mlp_device_view(const mlx_mlp *M);
+int mlp_device_of(const mlx_mlp *M);
 
 #define MLX_CHECK_HIP(call)                                                   \
   do {                                                                        \
@@ -193,6 +200,22 @@ struct mlx_mesh {
     double layersMs = 0, solveMs = 0, totalMs = 0;
     int64_t substeps = 0;
   } phase;
+
+  // learned warm start / key correction (mlx_mesh_set_learned, 3D)
+  struct Learn {
+    mlx_mlp *model = nullptr;
+    int flags = 0;
+    float kappaMax = 2.0f;
+    bool ready = false;     // inference ran for the current substep
+    mlx::DevBuf d_lam0;     // T nc*ld: warm-start multipliers (slot-major like d_lam)
+    mlx::DevBuf d_keyScale; // double ld: strain-rate key factors 10^c
+    mlx::DevBuf d_save;     // vec4<T> nV positions | T nc*ld multipliers (fallback copy)
+    mlx::DevBuf d_dx;       // T 12*ld per (vertex, slot) displacements
+    mlx::DevBuf d_inc;      // int32 nV+1 offsets | 4*nT (a*ld + slot) incidences
+    mlx::DevBuf d_res;      // double 2 (cold, warm) | partials
+    mlx::DevBuf d_counts;   // uint64 2: warm starts kept, rejected
+    mlx::DevBuf d_feat;     // float 21*ld features (mlx_learn_features)
+  } learn;
 };
 
 struct mlx_state {

@@ -16,6 +16,7 @@
 #include "mlx_host.h"
 #include "mlx_internal.h"
 #include "mlx_kernels.h"
+#include "mlx_learn_kernels.h"
 
 using namespace mlx;
 
@@ -543,6 +544,63 @@ static int read_ctl(mlx_mesh &m, Ctl &out) {
 // layerSolve on the device (xpbdLayer3D.m:44-232, xpbdLayer2D.m:606-738).
 // Expects d_x (xi), d_xold (oldp), d_lam, d_rv (3D), d_pinnedx and the
 // contact arrays to be on the device already.
+// ---------------------------------------------------------------------------
+// learned warm start and key correction (3D)
+// ---------------------------------------------------------------------------
+
+// The network on every slot from the state at the start of a substep
+// (positions, the previous substep's positions and multipliers): lambda_0
+// and the key factors, and the raw features when feat is given.
+template <typename T>
+static int learn_infer(mlx_mesh &m, double h, float *feat) {
+  mlx_mesh::Learn &L = m.learn;
+  SweepParams<T> P = sweep_params<T>(m, h, nullptr);
+  const unsigned nb = (unsigned)((m.ld + kLearnBlock - 1) / kLearnBlock);
+  hipLaunchKernelGGL(k_learn_infer<T>, dim3(nb), dim3(kLearnBlock), 0, m.stream, P,
+                     mlp_device_view(L.model), (const vec4_t<T> *)m.d_xold.as<vec4_t<T>>(),
+                     (float)(1.0 / h), (float)(1.0 / (h * h)), L.d_lam0.as<T>(),
+                     L.d_keyScale.as<double>(), L.kappaMax, feat);
+  MLX_CHECK_HIP(hipGetLastError());
+  return MLX_OK;
+}
+
+// Warm start of the last layer with its fallback: residual of the cold
+// start, copies of positions and multipliers, the displacement of every
+// fresh elastic slot gathered per vertex, residual of the warm start, and
+// the choice (restoring the copies when the warm start is worse).
+template <typename T>
+static int learn_warm_start(mlx_mesh &m, SweepParams<T> P, const uint8_t *fl,
+                            const ContactArrays &ca, int nE, int nC, const SolveParams &sp) {
+  mlx_mesh::Learn &L = m.learn;
+  hipStream_t st = m.stream;
+  const int64_t ld = m.ld, nV = m.nV, nLam = (int64_t)m.nc * ld;
+  double *res = L.d_res.as<double>(), *part = res + 2;
+  P.flags = fl;
+  auto residual = [&](double *out) {
+    hipLaunchKernelGGL(k_learn_res_part<T>, dim3(nE + nC), dim3(kBlock), 0, st, P, nE, ca,
+                       sp.contactResistance, part);
+    hipLaunchKernelGGL(k_learn_res_final, dim3(1), dim3(kBlock), 0, st, (const double *)part,
+                       nE + nC, out);
+  };
+  residual(res);
+  vec4_t<T> *xs = L.d_save.as<vec4_t<T>>();
+  T *lams = reinterpret_cast<T *>(xs + nV);
+  MLX_CHECK_HIP(hipMemcpyAsync(xs, P.x, sizeof(vec4_t<T>) * nV, hipMemcpyDeviceToDevice, st));
+  MLX_CHECK_HIP(hipMemcpyAsync(lams, P.lam, sizeof(T) * nLam, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_learn_disp<T>, dim3(blocks_for(ld)), dim3(kBlock), 0, st, P,
+                     (const T *)L.d_lam0.as<T>(), L.d_dx.as<T>());
+  const int32_t *inc = L.d_inc.as<int32_t>();
+  hipLaunchKernelGGL(k_learn_gather<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV, ld, inc,
+                     inc + nV + 1, (const T *)L.d_dx.as<T>(),
+                     (const uint8_t *)m.d_pinMask.as<uint8_t>(), P.x);
+  residual(res + 1);
+  hipLaunchKernelGGL(k_learn_select<T>, dim3(blocks_for(std::max(nV, nLam))), dim3(kBlock), 0, st,
+                     nV, nLam, (const double *)res, P.x, (const vec4_t<T> *)xs, P.lam,
+                     (const T *)lams, L.d_counts.as<unsigned long long>());
+  MLX_CHECK_HIP(hipGetLastError());
+  return MLX_OK;
+}
+
 template <typename T>
 static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams &sp,
                            const HostContacts &hc, SolveResult &res) {
@@ -813,6 +871,9 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       hipLaunchKernelGGL(k_sumsq<T>, dim3(256), dim3(kBlock), 0, st, rv, 3 * nV, partials);
       hipLaunchKernelGGL(k_layer_begin, dim3(1), dim3(kBlock), 0, st, ctl, partials, 256, layer,
                          (int)isLast, layer == 0 ? sp.stopAtImprovementPercent : 0.0);
+      if (isLast && L.nb == 0 && !hasRigidEl && m.learn.model && m.learn.ready &&
+          (m.learn.flags & MLX_LEARN_WARMSTART))
+        MLX_TRY(learn_warm_start<T>(m, P, fl, ca, nResElemBlocks, nResContactBlocks, sp));
     } else {
       // bodies from the current positions, not stepped (xpbdLayer2D.m:676-681)
       if (L.nb > 0)
@@ -1569,7 +1630,10 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_lamB, &m->d_contacts, &m->d_partials, &m->d_ctl, &m->d_resid,
                          &m->d_tmp, &m->d_nh, &m->d_colors, &m->d_sync, &m->d_forced, &m->d_colortab,
                          &m->d_maps, &m->d_scan, &m->d_count, &m->d_elist, &m->d_pinMask,
-                         &m->d_bodycnt, &m->d_seam, &m->d_elastic};
+                         &m->d_bodycnt, &m->d_seam, &m->d_elastic, &m->learn.d_lam0,
+                         &m->learn.d_keyScale, &m->learn.d_save, &m->learn.d_dx,
+                         &m->learn.d_inc, &m->learn.d_res, &m->learn.d_counts,
+                         &m->learn.d_feat};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
@@ -1877,11 +1941,15 @@ static int build_layer_maps(mlx_state &S, const double *sorted, int ns, double h
   for (int l = 0; l < ns; ++l) any |= std::floor((sorted[l] / 100.0) * (double)nT) > 0;
   if (!any) return MLX_OK;
   MLX_TRY(S.d_ediff.reserve(sizeof(double) * nT));
-  if (m.dim == 3)
+  if (m.dim == 3) {
     hipLaunchKernelGGL(k_ediff3d<T>, dim3(blocks_for(nT)), dim3(kBlock), 0, m.stream, nT,
                        m.d_T.as<int32_t>(), m.d_dminv64.as<double>(), m.d_x.as<vec4_t<T>>(),
                        m.d_xold.as<vec4_t<T>>(), h, S.d_ediff.as<double>());
-  else
+    if (m.learn.model && m.learn.ready && (m.learn.flags & MLX_LEARN_KEYS))
+      hipLaunchKernelGGL(k_learn_keys, dim3(blocks_for(nT)), dim3(kBlock), 0, m.stream, nT,
+                         (const int32_t *)m.d_slotOf.as<int32_t>(),
+                         (const double *)m.learn.d_keyScale.as<double>(), S.d_ediff.as<double>());
+  } else
     hipLaunchKernelGGL(k_ediff2d<T>, dim3(blocks_for(nT)), dim3(kBlock), 0, m.stream, nT,
                        m.d_T.as<int32_t>(), m.d_dminv64.as<double>(), m.d_x.as<vec4_t<T>>(),
                        m.d_xold.as<vec4_t<T>>(), h, S.d_ediff.as<double>());
@@ -1927,6 +1995,12 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
     hipLaunchKernelGGL((k_reset_oldp<T, 3>), dim3(blocks_for(std::max(nV, nPin))), dim3(kBlock), 0,
                        s, nV, flag, frame == 1, x, xo, nPin, m.d_pinned.as<int32_t>(),
                        m.d_pinnedx.as<T>());
+    // the network reads p, cache.oldp and the previous substep's multipliers
+    m.learn.ready = false;
+    if (m.learn.model) {
+      MLX_TRY(learn_infer<T>(m, h, nullptr));
+      m.learn.ready = true;
+    }
   }
   // 2. contacts at the start-of-step positions: the plane test and the list
   // in ascending vertex order are built on the device; the list is read
@@ -2068,6 +2142,7 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
   }
   MLX_CHECK_HIP(hipGetLastError());
   MLX_CHECK_HIP(hipStreamSynchronize(s));
+  m.learn.ready = false;
   {
     auto ms = [](clk::time_point a, clk::time_point b) {
       return std::chrono::duration<double, std::milli>(b - a).count();
@@ -2332,6 +2407,149 @@ int mlx_mesh_set_option(mlx_mesh *m, int32_t option, int64_t value) {
   }
   set_error("mlx_mesh_set_option: bad option %d or value %lld", option, (long long)value);
   return MLX_ERR_ARG;
+}
+
+int mlx_mesh_set_learned(mlx_mesh *m, mlx_mlp *model, int32_t flags, float kappaMax) {
+  if (!m) {
+    set_error("mlx_mesh_set_learned: null mesh");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh::Learn &L = m->learn;
+  if (!model) {
+    L.model = nullptr;
+    L.flags = 0;
+    L.ready = false;
+    return MLX_OK;
+  }
+  if (m->dim != 3 || m->nScenes != 1) {
+    set_error("mlx_mesh_set_learned: needs a 3D single-scene mesh");
+    return MLX_ERR_UNSUPPORTED;
+  }
+  if ((flags & ~(MLX_LEARN_WARMSTART | MLX_LEARN_KEYS)) != 0 || !(kappaMax > 0) ||
+      !std::isfinite(kappaMax)) {
+    set_error("mlx_mesh_set_learned: bad flags %d or kappaMax %g", flags, (double)kappaMax);
+    return MLX_ERR_ARG;
+  }
+  if (mlp_device_of(model) != m->device) {
+    set_error("mlx_mesh_set_learned: model on device %d, mesh on device %d",
+              mlp_device_of(model), m->device);
+    return MLX_ERR_ARG;
+  }
+  MLX_CHECK_HIP(hipSetDevice(m->device));
+  const int64_t ld = m->ld, nV = m->nV, nT = m->nT;
+  const size_t tb = m->precision == MLX_FP64 ? sizeof(double) : sizeof(float);
+  const size_t nLam = (size_t)m->nc * ld;
+  MLX_TRY(L.d_lam0.reserve(tb * nLam));
+  MLX_TRY(L.d_keyScale.reserve(sizeof(double) * ld));
+  MLX_TRY(L.d_save.reserve(4 * tb * nV + tb * nLam));
+  MLX_TRY(L.d_dx.reserve(tb * 12 * ld));
+  MLX_TRY(L.d_res.reserve(sizeof(double) * (2 + kMaxPartials)));
+  if (!L.d_counts.ptr) {
+    MLX_TRY(L.d_counts.reserve(16));
+    MLX_CHECK_HIP(hipMemsetAsync(L.d_counts.ptr, 0, 16, m->stream));
+  }
+  // incidences (vertex a of slot s, as a ld + s) grouped by vertex, each
+  // vertex's in ascending slot order
+  std::vector<int32_t> off(nV + 1, 0), inc(4 * (size_t)nT);
+  for (int64_t s = 0; s < ld; ++s) {
+    const int32_t e = m->perm[s];
+    if (e < 0) continue;
+    for (int a = 0; a < 4; ++a) off[m->T[4 * (size_t)e + a] + 1] += 1;
+  }
+  for (int64_t v = 0; v < nV; ++v) off[v + 1] += off[v];
+  std::vector<int32_t> fill(off.begin(), off.end() - 1);
+  for (int64_t s = 0; s < ld; ++s) {
+    const int32_t e = m->perm[s];
+    if (e < 0) continue;
+    for (int a = 0; a < 4; ++a) inc[fill[m->T[4 * (size_t)e + a]]++] = (int32_t)(a * ld + s);
+  }
+  if (4 * ld > INT32_MAX) {
+    set_error("mlx_mesh_set_learned: slot count too large");
+    return MLX_ERR_UNSUPPORTED;
+  }
+  MLX_TRY(L.d_inc.reserve(sizeof(int32_t) * (off.size() + inc.size())));
+  MLX_CHECK_HIP(hipMemcpyAsync(L.d_inc.ptr, off.data(), sizeof(int32_t) * off.size(),
+                               hipMemcpyHostToDevice, m->stream));
+  MLX_CHECK_HIP(hipMemcpyAsync(L.d_inc.as<int32_t>() + off.size(), inc.data(),
+                               sizeof(int32_t) * inc.size(), hipMemcpyHostToDevice, m->stream));
+  MLX_CHECK_HIP(hipStreamSynchronize(m->stream));
+  L.model = model;
+  L.flags = flags;
+  L.kappaMax = kappaMax;
+  L.ready = false;
+  return MLX_OK;
+}
+
+int mlx_learn_stats(mlx_mesh *m, int64_t *kept, int64_t *rejected) {
+  if (!m || !m->learn.d_counts.ptr) {
+    set_error("mlx_learn_stats: no network attached");
+    return MLX_ERR_STATE;
+  }
+  MLX_CHECK_HIP(hipSetDevice(m->device));
+  unsigned long long c[2];
+  MLX_CHECK_HIP(hipMemcpyAsync(c, m->learn.d_counts.ptr, 16, hipMemcpyDeviceToHost, m->stream));
+  MLX_CHECK_HIP(hipMemsetAsync(m->learn.d_counts.ptr, 0, 16, m->stream));
+  MLX_CHECK_HIP(hipStreamSynchronize(m->stream));
+  if (kept) *kept = (int64_t)c[0];
+  if (rejected) *rejected = (int64_t)c[1];
+  return MLX_OK;
+}
+
+int mlx_learn_outputs(mlx_mesh *m, double *lambda0, double *keyScale) {
+  if (!m || !m->learn.d_lam0.ptr) {
+    set_error("mlx_learn_outputs: no network attached");
+    return MLX_ERR_STATE;
+  }
+  MLX_CHECK_HIP(hipSetDevice(m->device));
+  const int64_t ld = m->ld, nT = m->nT;
+  if (lambda0) {
+    std::vector<double> lam(6 * (size_t)ld);
+    if (m->precision == MLX_FP64) {
+      MLX_CHECK_HIP(hipMemcpy(lam.data(), m->learn.d_lam0.ptr, sizeof(double) * 6 * ld,
+                              hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> lf(6 * (size_t)ld);
+      MLX_CHECK_HIP(hipMemcpy(lf.data(), m->learn.d_lam0.ptr, sizeof(float) * 6 * ld,
+                              hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < lf.size(); ++i) lam[i] = lf[i];
+    }
+    for (int64_t e = 0; e < nT; ++e)
+      for (int i = 0; i < 6; ++i) lambda0[6 * e + i] = lam[(size_t)i * ld + m->slotOf[e]];
+  }
+  if (keyScale) {
+    std::vector<double> ks(ld);
+    MLX_CHECK_HIP(hipMemcpy(ks.data(), m->learn.d_keyScale.ptr, sizeof(double) * ld,
+                            hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < nT; ++e) keyScale[e] = ks[m->slotOf[e]];
+  }
+  return MLX_OK;
+}
+
+int mlx_learn_features(mlx_state *S, double h, float *features) {
+  if (!S || !S->mesh || !features || !(h > 0)) {
+    set_error("mlx_learn_features: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *S->mesh;
+  if (!m.learn.model) {
+    set_error("mlx_learn_features: no network attached");
+    return MLX_ERR_STATE;
+  }
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  const int64_t ld = m.ld, nT = m.nT;
+  MLX_TRY(m.learn.d_feat.reserve(sizeof(float) * kMlpIn * ld));
+  if (m.precision == MLX_FP64)
+    MLX_TRY(learn_infer<double>(m, h, m.learn.d_feat.as<float>()));
+  else
+    MLX_TRY(learn_infer<float>(m, h, m.learn.d_feat.as<float>()));
+  std::vector<float> f((size_t)kMlpIn * ld);
+  MLX_CHECK_HIP(hipMemcpyAsync(f.data(), m.learn.d_feat.ptr, sizeof(float) * f.size(),
+                               hipMemcpyDeviceToHost, m.stream));
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  for (int64_t e = 0; e < nT; ++e)
+    std::memcpy(features + (size_t)kMlpIn * e, f.data() + (size_t)kMlpIn * m.slotOf[e],
+                sizeof(float) * kMlpIn);
+  return MLX_OK;
 }
 
 int mlx_synchronize(void) {

@@ -1275,7 +1275,7 @@ static void layers_free(layers_t *L) {
 /* buildLayers (xpbdLayer3D.m:601-619; 2D xpbdLayer2D.m:740-780 ordering type
  * 0) followed by findBoundaryVertices. */
 static void build_layers(const ora_mesh *m, const ora_settings *s, const double *pa,
-                         const double *pb, double h, layers_t *L) {
+                         const double *pb, double h, const double *keyScale, layers_t *L) {
   int nT = m->nT, nV = m->nV;
   L->nSorted = ora_sorted_layers(s->layers, s->nLayers, L->sorted, L->map);
   int ns = L->nSorted;
@@ -1287,6 +1287,8 @@ static void build_layers(const ora_mesh *m, const ora_settings *s, const double 
     ora_b_times_p3d(m->T, nT, m->DmInv, pa, Fa);
     ora_b_times_p3d(m->T, nT, m->DmInv, pb, Fb);
     ora_ediff_norm3d(Fa, Fb, nT, h, ed);
+    if (keyScale)
+      for (int e = 0; e < nT; ++e) ed[e] = ed[e] * keyScale[e];
   } else {
     ora_b_times_p2d(m->T, nT, m->DmInv, pa, Fa);
     ora_b_times_p2d(m->T, nT, m->DmInv, pb, Fb);
@@ -1309,6 +1311,149 @@ static void build_layers(const ora_mesh *m, const ora_settings *s, const double 
 /* 3D substep                                                           */
 /* ------------------------------------------------------------------ */
 
+/* ------------------------------------------------------------------ */
+/* learned warm start (no reference counterpart)                        */
+/* ------------------------------------------------------------------ */
+
+/* Residual of the equations the projection solves, over the elastic
+ * elements (per constraint the numerator of its update: C + alpha~ lambda +
+ * gamma gradC (x - x_old) for StVK-Voigt as matStvkVoigtConstraint3D.m:4-21
+ * forms it, the normal rows with the 3x3 block of alpha~, the shear rows
+ * with alpha~(4,4); C + alpha~ lambda for the NeoHookean pair) and the
+ * contacts (r phi, their multipliers being zero when a layer starts).
+ * Pins: zero at a projected fixed point and the closed forms of the tests. */
+double ora_learn_eq_residual3d(const ora_mesh *m, double h, const uint8_t *isEl,
+                               const double *lambda, const double *xi, const double *oldp,
+                               const ora_contacts *c, double resistance) {
+  double acc = 0;
+  for (int e = 0; e < m->nT; ++e) {
+    if (!isEl[e]) continue;
+    double x[12], xo[12];
+    for (int a = 0; a < 4; ++a)
+      for (int d = 0; d < 3; ++d) {
+        int dof = 3 * m->T[4 * e + a] + d;
+        x[3 * a + d] = xi[dof];
+        xo[3 * a + d] = oldp[dof];
+      }
+    const double *lam = lambda + 6 * e;
+    if (m->nhModel && m->nhModel[e]) {
+      double n, J;
+      ora_neohookean_grad3d(m->DmInv + 9 * e, x, NULL, NULL, &n, &J);
+      double iv = (1.0 / m->vol[e]) * (1.0 / (h * h));
+      double rD = n + iv * (1.0 / m->nhMu[e]) * lam[0];
+      double rH = (J - (1 + m->nhMu[e] / m->nhLam[e])) + iv * (1.0 / m->nhLam[e]) * lam[1];
+      acc = acc + rD * rD + rH * rH;
+      continue;
+    }
+    double g[72], C[6], aT[36], bT[36], gam[36];
+    ora_stvk_grad3d(m->DmInv + 9 * e, x, g, C);
+    for (int r = 0; r < 6; ++r) C[r] = C[r] * m->vol[e];
+    for (int k = 0; k < 72; ++k) g[k] = g[k] * m->vol[e];
+    for (int k = 0; k < 36; ++k) {
+      aT[k] = m->alpha[36 * e + k] / (h * h);
+      bT[k] = m->beta[36 * e + k] * (h * h);
+    }
+    matmul_n(6, aT, bT, gam);
+    for (int k = 0; k < 36; ++k) gam[k] = gam[k] / h;
+    double gdx[6];
+    for (int i = 0; i < 6; ++i) {
+      double sgd = 0;
+      for (int q = 0; q < 12; ++q) sgd = sgd + g[i + 6 * q] * (x[q] - xo[q]);
+      gdx[i] = sgd;
+    }
+    for (int i = 0; i < 6; ++i) {
+      double ng = 0;
+      for (int k = 0; k < 6; ++k) ng = ng + gam[i + 6 * k] * gdx[k];
+      double al;
+      if (i < 3) {
+        al = aT[i] * lam[0];
+        for (int k = 1; k < 3; ++k) al = al + aT[i + 6 * k] * lam[k];
+      } else {
+        al = aT[3 + 6 * 3] * lam[i];
+      }
+      double r = C[i] + al + ng;
+      acc = acc + r * r;
+    }
+  }
+  for (int i = 0; c && i < c->K; ++i) {
+    const double *n = c->normal + 3 * i, *pc = c->point + 3 * i;
+    const double *xv = xi + 3 * c->vertex[i];
+    double phi = n[0] * (xv[0] - pc[0]) + n[1] * (xv[1] - pc[1]) + n[2] * (xv[2] - pc[2]);
+    if (phi > 0) phi = 0;
+    acc = acc + (resistance * phi) * (resistance * phi);
+  }
+  return sqrt(acc);
+}
+
+/* Warm start of the elastic elements whose multipliers are still zero:
+ * lambda = lambda0 and x += M^-1 gradC(x)^T lambda0, gradients at the
+ * positions before the move, pinned DOFs left in place.  Pins: the
+ * displacement against central differences of lambda0 . C (tests). */
+void ora_learn_displace3d(const ora_mesh *m, const uint8_t *isEl, const double *lam0,
+                          double *lambda, double *xi, const uint8_t *pinnedDof) {
+  int nV = m->nV, nT = m->nT;
+  double *dx = (double *)calloc(3 * (nV > 0 ? nV : 1), sizeof(double));
+  for (int e = 0; e < nT; ++e) {
+    if (!isEl[e]) continue;
+    int fresh = 1;
+    for (int k = 0; k < 6; ++k) fresh = fresh && lambda[6 * e + k] == 0;
+    if (!fresh) continue;
+    double x[12];
+    for (int a = 0; a < 4; ++a)
+      for (int d = 0; d < 3; ++d) x[3 * a + d] = xi[3 * m->T[4 * e + a] + d];
+    const double *l0 = lam0 + 6 * e;
+    double step[12];
+    if (m->nhModel && m->nhModel[e]) {
+      double gD[12], gH[12];
+      ora_neohookean_grad3d(m->DmInv + 9 * e, x, gD, gH, NULL, NULL);
+      for (int q = 0; q < 12; ++q) step[q] = gD[q] * l0[0] + gH[q] * l0[1];
+    } else {
+      double g[72], C[6];
+      ora_stvk_grad3d(m->DmInv + 9 * e, x, g, C);
+      for (int q = 0; q < 12; ++q) {
+        double sq_ = 0;
+        for (int i = 0; i < 6; ++i) sq_ = sq_ + g[i + 6 * q] * m->vol[e] * l0[i];
+        step[q] = sq_;
+      }
+    }
+    for (int a = 0; a < 4; ++a)
+      for (int d = 0; d < 3; ++d) {
+        int dof = 3 * m->T[4 * e + a] + d;
+        dx[dof] = dx[dof] + step[3 * a + d] / m->mass[dof];
+      }
+    for (int k = 0; k < 6; ++k) lambda[6 * e + k] = l0[k];
+  }
+  for (int q = 0; q < 3 * nV; ++q)
+    if (!pinnedDof[q]) xi[q] = xi[q] + dx[q];
+  free(dx);
+}
+
+/* The warm start with its judge: kept only if the equation residual does
+ * not rise, else x and lambda are restored.  Returns 1 kept, 0 rejected;
+ * res gets the residuals before and after. */
+int ora_learn_warm_start3d(const ora_mesh *m, double h, const uint8_t *isEl,
+                           const double *lam0, double *lambda, double *xi, const double *oldp,
+                           const uint8_t *pinnedDof, const ora_contacts *c, double resistance,
+                           double *res) {
+  int nV = m->nV, nT = m->nT;
+  double cold = ora_learn_eq_residual3d(m, h, isEl, lambda, xi, oldp, c, resistance);
+  double *xs = (double *)malloc(sizeof(double) * 3 * (nV > 0 ? nV : 1));
+  double *ls = (double *)malloc(sizeof(double) * 6 * (nT > 0 ? nT : 1));
+  memcpy(xs, xi, sizeof(double) * 3 * nV);
+  memcpy(ls, lambda, sizeof(double) * 6 * nT);
+  ora_learn_displace3d(m, isEl, lam0, lambda, xi, pinnedDof);
+  double warm = ora_learn_eq_residual3d(m, h, isEl, lambda, xi, oldp, c, resistance);
+  res[0] = cold;
+  res[1] = warm;
+  int keep = warm <= cold;
+  if (!keep) {
+    memcpy(xi, xs, sizeof(double) * 3 * nV);
+    memcpy(lambda, ls, sizeof(double) * 6 * nT);
+  }
+  free(xs); free(ls);
+  return keep;
+}
+
 /* xpbdLayer3D.integrate (xpbdLayer3D.m:621-745) with layerSolve (:44-232)
  * and projectGS (:243-345).  Pins: the pieces above, plus an independent
  * numpy restatement on small scenes (tests). */
@@ -1330,7 +1475,8 @@ int ora_integrate3d(const ora_mesh *m, ora_settings *s, double h, int frame,
   double *f = (double *)calloc(N3, sizeof(double));
   for (int i = 0; i < nV; ++i) f[3 * i + 2] = f[3 * i + 2] + m->mass[3 * i + 2] * s->gravity;
   layers_t L;
-  build_layers(m, s, p, oldp, h, &L);
+  build_layers(m, s, p, oldp, h, out ? out->learnKeyScale : NULL, &L);
+  if (out) out->learnKept = -1;
   if (out && out->setsPerLayer)
     memcpy(out->setsPerLayer, L.sets, sizeof(int32_t) * (size_t)nT * L.nSorted);
   if (out && out->numRigids) memcpy(out->numRigids, L.numRigids, sizeof(int32_t) * L.nSorted);
@@ -1416,6 +1562,9 @@ int ora_integrate3d(const ora_mesh *m, ora_settings *s, double h, int frame,
     for (int i = 0; i < N3; ++i) normresv = normresv + rv[i] * rv[i];
     normresv = sqrt(normresv);
     if (out) out->normresv[layer] = normresv;
+    if (isLast && out && out->learnLam0 && nRigidEl == 0 && b.nb == 0)
+      out->learnKept = ora_learn_warm_start3d(m, h, isEl, out->learnLam0, lambda, xi, oldp,
+                                              pinnedDof, c, s->contactResistance, out->learnRes);
     /* projectGS (:243-345) */
     int nbv = 0;
     for (int i = 0; i < nV; ++i) if (isB[i]) bverts[nbv++] = i;
@@ -1716,7 +1865,7 @@ int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h, double *p,
   double *f = (double *)calloc(N2, sizeof(double));
   for (int i = 0; i < nV; ++i) f[2 * i + 1] = f[2 * i + 1] + m->mass[2 * i + 1] * s->gravity;
   layers_t L;
-  build_layers(m, s, p, prevPrevp, h, &L);
+  build_layers(m, s, p, prevPrevp, h, NULL, &L);
   if (out && out->setsPerLayer)
     memcpy(out->setsPerLayer, L.sets, sizeof(int32_t) * (size_t)nT * L.nSorted);
   if (out && out->numRigids) memcpy(out->numRigids, L.numRigids, sizeof(int32_t) * L.nSorted);

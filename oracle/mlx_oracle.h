@@ -84,6 +84,14 @@ typedef struct {
   double normresv[ORA_MAX_LAYERS]; /* out: norm of the residual velocity each
                                       layer's monitor compares (3D) */
   double runUntilAtEntry;  /* out: runUntilResSmallerThan as layerSolve received it */
+  /* in, optional (3D): the learned warm start of the last layer and the
+   * strain-rate key factors, applied as the product's learned mode applies
+   * them (no reference counterpart); learnKept out: 1 kept, 0 rejected,
+   * -1 not applied */
+  const double *learnLam0;     /* nT*6 row-major */
+  const double *learnKeyScale; /* nT */
+  int learnKept;
+  double learnRes[2];          /* out: equation residual before / after */
 } ora_step_out;
 
 /* ---- precompute (Mesh3D / Mesh constructors) ---- */
@@ -230,6 +238,20 @@ int ora_integrate3d(const ora_mesh *m, ora_settings *s, double h, int frame,
                     double *p, double *v, double *oldp, double *oldOldp,
                     double *pinnedx, const ora_contacts *c,
                     ora_step_out *out);
+/* Learned warm start of the last layer (no reference counterpart; the
+ * product's learned mode): the residual of the projection's equations over
+ * the elastic elements and contacts, the displacement of the fresh elastic
+ * elements for multipliers lam0 (nT*6), and both with the keep / restore
+ * judge (res: residual before, after). */
+double ora_learn_eq_residual3d(const ora_mesh *m, double h, const uint8_t *isEl,
+                               const double *lambda, const double *xi, const double *oldp,
+                               const ora_contacts *c, double resistance);
+void ora_learn_displace3d(const ora_mesh *m, const uint8_t *isEl, const double *lam0,
+                          double *lambda, double *xi, const uint8_t *pinnedDof);
+int ora_learn_warm_start3d(const ora_mesh *m, double h, const uint8_t *isEl,
+                           const double *lam0, double *lambda, double *xi, const double *oldp,
+                           const uint8_t *pinnedDof, const ora_contacts *c, double resistance,
+                           double *res);
 /* 2D rigid-layer steps on explicit body state (test hooks): bodies from
  * connectivity, boundary accumulation and contacts (xpbdLayer2D.m:296-444). */
 void ora_rigid_properties2d(int nV, int nb, const int32_t *vb, const double *p,

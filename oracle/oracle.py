@@ -52,6 +52,8 @@ class OraStepOut(C.Structure):
         ("layersRun", C.c_int), ("elasticProjections", C.c_int64),
         ("setsPerLayer", _i32), ("numRigids", _i32),
         ("normresv", C.c_double * MAX_LAYERS), ("runUntilAtEntry", C.c_double),
+        ("learnLam0", _d), ("learnKeyScale", _d), ("learnKept", C.c_int),
+        ("learnRes", C.c_double * 2),
     ]
 
 
@@ -71,6 +73,7 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         _lib.ora_residual3d.restype = C.c_double
         _lib.ora_residual2d.restype = C.c_double
+        _lib.ora_learn_eq_residual3d.restype = C.c_double
     return _lib
 
 
@@ -262,10 +265,16 @@ class OracleSim:
         oc.compliance = P(c["compliance"])
         return oc, keep
 
-    def step(self):
+    def step(self, lam0=None, key_scale=None):
+        """One substep; lam0 (nT x 6) and key_scale (nT) apply the learned
+        warm start and key factors as the product's learned mode does."""
         self.frame += 1
         oc, keep = self._contacts()
         out = OraStepOut()
+        l0 = None if lam0 is None else np.ascontiguousarray(lam0, dtype=np.float64).reshape(-1)
+        ks = None if key_scale is None else np.ascontiguousarray(key_scale, dtype=np.float64)
+        out.learnLam0 = P(l0)
+        out.learnKeyScale = P(ks)
         out.lambda_ = P(self.lam)
         out.residualArray = P(self.resid)
         out.residualCap = len(self.resid)
@@ -294,7 +303,8 @@ class OracleSim:
                     projections=int(out.elasticProjections), runUntil=self.s.runUntil,
                     sets=sets.reshape(ns, self.om.m.nT).T.copy(), numRigids=nrig,
                     normresv=list(out.normresv[:out.layersRun]),
-                    runUntilAtEntry=float(out.runUntilAtEntry), residualCount=int(out.residualCount))
+                    runUntilAtEntry=float(out.runUntilAtEntry), residualCount=int(out.residualCount),
+                    learnKept=int(out.learnKept), learnRes=list(out.learnRes))
 
     def lambdas(self):
         m = self.om.m
