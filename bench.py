@@ -16,6 +16,12 @@ is under the 1e-8 floor); the timed ones start at touchdown.
 A step is one integrate() substep: layer maps, layerSolve to the stop rule,
 velocity update.  value = tet projections / s over the timed steps (every
 elastic element update in every iteration); ms_per_step = wall ms/substep.
+After the warmup the window runs once untimed (capturing the iteration
+graphs it needs) and is then timed from the same state; the other legs
+restart from that state too: `monitor_excluded` (the window with each
+substep's iteration counts fixed to the stop rule's and the residual monitor
+off, the time the reference reports), `fp64` (the same window in FP64) with
+`fp32_error_vs_fp64`, and `learned` (the committed network attached).
 The roofline entry is for the element-sweep kernel: over --profile-steps
 substeps that follow the timed ones, one HIP event pair (no system fence)
 brackets the back-to-back color launches of every iteration on the mesh's
@@ -23,7 +29,9 @@ stream (direct launches; the timed substeps replay the same kernels from
 captured hipGraphs), and the average launch duration is that time over the
 number of launches.  `phases` splits the timed substeps' host wall time into
 contacts + layer maps and layerSolve.  cpu_baseline is the FP64 CPU oracle
-(a restatement of the reference path, one core) on a bounded sample.
+(a C restatement of the reference path) on one substep from the window-start
+state with the same stop rule, median of 5 runs, on one core and with OpenMP
+over each color's elements on every core the process may use.
 """
 import argparse
 import ctypes as C
@@ -62,6 +70,12 @@ TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 197.73, "fetch_scale": 2.0,
                                     "write": 96.19, "write_scale": 1.0},
                "k_batch_plain<float,1024>": {"fetch": 15336905.12, "fetch_scale": 2.0,
                                              "write": 3388201.03, "write_scale": 1.0}}
+
+
+# Where the constants above come from.
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench at commit 2fb7b75, "
+                  "scaled by the same counters of a 1 GiB device copy (FETCH x2.0, WRITE x1.0); "
+                  "summaries in profiles/hbm_traffic_*.md")
 
 
 def traffic_bytes(kernel):
@@ -202,25 +216,158 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256, graph=True):
     }
 
 
-def cpu_baseline(mesh, h, plane, budget_s=12.0, max_steps=4):
-    """The oracle (FP64, one core) on the same scene with the iteration counts
-    fixed at [2, 4, 14] (the stop rule off), substeps 1.. until budget_s."""
+def host_info():
+    """CPU model and the cores this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, cores
+
+
+def oracle_from_snapshot(O, mesh, h, plane, snap, integ):
+    """The oracle's integrator positioned at a GPU snapshot: positions,
+    velocities, cache.oldp, pinnedx (the pinned DOFs' positions) and
+    runUntil, so its next substep is the one the GPU runs from there."""
+    osim = O.OracleSim(mesh, integ, h, plane)
+    osim.p = snap["p"].copy()
+    osim.v = snap["v"].copy()
+    osim.oldp = snap["oldp"].copy()
+    osim.pinnedx = snap["p"][np.asarray(mesh.pinnedDOFs) - 1].copy()
+    osim.frame = int(snap["frame"])
+    osim.s.runUntil = float(snap["runUntil"])
+    return osim
+
+
+def cpu_baseline(mesh, h, plane, snap, runs=5, threads=None, label="one substep"):
+    """The oracle (FP64 C restatement of the reference path) timed on the
+    host from the GPU's timed-window start state: one substep with the same
+    integrator and stop rule, run `runs` times from that state after one
+    untimed run, median reported; serially and with OpenMP over the elements
+    of each color on every core this process may use.  Returns the baseline
+    and the oracle's one-substep positions from that state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     O.build()
-    osim = O.OracleSim(mesh, make_integrator(stop_rule=False), h, plane)
-    proj, t, steps = 0, 0.0, 0
-    while steps < max_steps and (t < budget_s or steps == 0):
-        t0 = time.perf_counter()
-        r = osim.step()
-        t += time.perf_counter() - t0
-        proj += r["projections"]
-        steps += 1
-    return dict(value=proj / t, unit="tet-proj/s", cores=1, kind="port",
-                sample=f"{steps} substeps of the same beam from rest, iterations fixed at "
-                       f"[2,4,14] (no stop rule), residual monitor on; "
-                       f"{proj} tet projections in {t:.2f} s; {1e3 * t / steps:.1f} ms/substep")
+    model, ncores = host_info()
+    threads = threads or ncores
+    legs, p_one = {}, None
+    for nt in (1, threads):
+        O.lib().ora_set_threads(nt)
+        try:
+            times, proj, iters = [], None, None
+            for k in range(runs + 1):
+                osim = oracle_from_snapshot(O, mesh, h, plane, snap, make_integrator())
+                t0 = time.perf_counter()
+                r = osim.step()
+                dt = time.perf_counter() - t0
+                if k > 0:
+                    times.append(dt)
+                proj, iters = r["projections"], sum(r["iters"])
+                if p_one is None:
+                    p_one = osim.p.copy()
+            med = float(np.median(times))
+            legs[nt] = dict(value=proj / med, ms_per_substep=1e3 * med, tet_projections=proj,
+                            iterations=iters, runs_ms=[1e3 * t for t in times])
+            log(f"cpu baseline, {nt} thread(s): {1e3 * med:.0f} ms/substep median, "
+                f"{iters} iterations")
+        finally:
+            O.lib().ora_set_threads(1)
+    one, many = legs[1], legs[threads]
+    return dict(
+        value=many["value"], unit="tet-proj/s", cores=threads, kind="port",
+        sample=(f"{label}, run from the GPU's state before it (contacts {snap['contacts']}, "
+                f"same layers, iterations and rel. residual 1e-4 stop: "
+                f"{many['iterations']} iterations, {many['tet_projections']} tet projections), "
+                f"median of {runs} runs after 1 untimed run; OpenMP over the elements of each "
+                f"color on {threads} threads, the rest of the substep serial; FP64"),
+        ms_per_substep=many["ms_per_substep"], runs_ms=many["runs_ms"],
+        single_thread=dict(value=one["value"], ms_per_substep=one["ms_per_substep"],
+                           runs_ms=one["runs_ms"], cores=1),
+        nproc=ncores, cpu_model=model,
+        what="CPU restatement of the reference path (oracle/mlx_oracle.c), not Matlab"), p_one
+
+
+def run_window(sim, steps, settings_per_step=None):
+    """`steps` substeps, wall-timed between device synchronisations; with
+    settings_per_step, a callback that adjusts the settings before each."""
+    lib = L.lib()
+    ph = [C.c_double() for _ in range(3)]
+    nsub = C.c_int64()
+    L.check("mlx_synchronize", lib.mlx_synchronize())
+    L.check("mlx_profile_phases", lib.mlx_profile_phases(sim.dmesh.handle, *[C.byref(x) for x in ph],
+                                                         C.byref(nsub)))
+    t0 = time.perf_counter()
+    proj, iters, per_layer = 0, [], []
+    for k in range(steps):
+        if settings_per_step is not None:
+            settings_per_step(k)
+        r = sim.step()
+        proj += r.elasticProjections
+        iters.append(sum(r.itersPerLayer))
+        per_layer.append(list(r.itersPerLayer))
+    L.check("mlx_synchronize", lib.mlx_synchronize())
+    wall = time.perf_counter() - t0
+    L.check("mlx_profile_phases", lib.mlx_profile_phases(sim.dmesh.handle, *[C.byref(x) for x in ph],
+                                                         C.byref(nsub)))
+    n = max(nsub.value, 1)
+    return dict(wall=wall, proj=proj, iters=iters, per_layer=per_layer,
+                value=proj / wall, ms_per_step=1e3 * wall / steps,
+                mean_iterations=float(np.mean(iters)),
+                phases={"layer_maps_ms_per_step": ph[0].value / n,
+                        "layer_solve_ms_per_step": ph[1].value / n,
+                        "integrate_ms_per_step": ph[2].value / n})
+
+
+def timed_from(sim, snap, steps, settings_per_step=None, record=None):
+    """The window from `snap`: one untimed pass (it captures the iteration
+    graphs the window needs; with `record`, a list, the state before each of
+    its substeps is appended), then the timed pass from the same state."""
+    sim.restore(snap)
+    if record is None:
+        run_window(sim, steps, settings_per_step)
+    else:
+        for k in range(steps):
+            if settings_per_step is not None:
+                settings_per_step(k)
+            s_ = sim.snapshot()
+            s_["contacts"] = sim.download()[3]
+            record.append(s_)
+            sim.step()
+    sim.restore(snap)
+    return run_window(sim, steps, settings_per_step)
+
+
+def one_substep(sim, snap):
+    sim.restore(snap)
+    sim.step()
+    return sim.download()[0]
+
+
+def set_options(sim, args):
+    sim.dmesh.set_option(L.MLX_OPT_GRAPHS, args.graphs)
+    sim.dmesh.set_option(L.MLX_OPT_PERSISTENT, args.persistent)
+    sim.dmesh.set_option(L.MLX_OPT_PERSIST_WG, args.persist_wg)
+    sim.dmesh.set_option(L.MLX_OPT_PERSIST_LOCAL, args.persist_local)
+    sim.dmesh.set_option(L.MLX_OPT_SWEEP_BLOCK, args.sweep_block)
+    sim.dmesh.set_option(L.MLX_OPT_SWEEP_WG, args.sweep_wg)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+LEARNED_MODEL = os.path.join(ROOT, "multilayersolver2024_amd", "learned_warmstart.npz")
 
 
 def main():
@@ -234,6 +381,9 @@ def main():
     ap.add_argument("--nz", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-extra-legs", action="store_true",
+                    help="skip the FP64, monitor-excluded and learned legs")
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--hanging-stop", type=int, default=1000,
                     help="iteration cap of the stop rule (reference default 1000)")
@@ -269,39 +419,31 @@ def main():
     sim = Simulation(mesh, make_integrator(hanging_stop=args.hanging_stop), h, plane,
                      precision=prec)
     lib = L.lib()
-    sim.dmesh.set_option(L.MLX_OPT_GRAPHS, args.graphs)
-    sim.dmesh.set_option(L.MLX_OPT_PERSISTENT, args.persistent)
-    sim.dmesh.set_option(L.MLX_OPT_PERSIST_WG, args.persist_wg)
-    sim.dmesh.set_option(L.MLX_OPT_PERSIST_LOCAL, args.persist_local)
-    sim.dmesh.set_option(L.MLX_OPT_SWEEP_BLOCK, args.sweep_block)
-    sim.dmesh.set_option(L.MLX_OPT_SWEEP_WG, args.sweep_wg)
+    set_options(sim, args)
     ncol, _ = sim.dmesh.info()
     for _ in range(args.warmup):
         sim.step()
-    log(f"warmup done ({args.warmup} substeps)")
-    L.check("mlx_synchronize", lib.mlx_synchronize())
-    ph = [C.c_double() for _ in range(3)]
-    nsub = C.c_int64()
-    L.check("mlx_profile_phases", lib.mlx_profile_phases(sim.dmesh.handle, *[C.byref(x) for x in ph],
-                                                         C.byref(nsub)))
-    t0 = time.perf_counter()
-    proj, iters, contacts = 0, [], 0
-    for _ in range(args.steps):
-        r = sim.step()
-        proj += r.elasticProjections
-        iters.append(sum(r.itersPerLayer))
-    L.check("mlx_synchronize", lib.mlx_synchronize())
-    t1 = time.perf_counter()
-    L.check("mlx_profile_phases", lib.mlx_profile_phases(sim.dmesh.handle, *[C.byref(x) for x in ph],
-                                                         C.byref(nsub)))
-    phases = {"layer_maps_ms_per_step": ph[0].value / max(nsub.value, 1),
-              "layer_solve_ms_per_step": ph[1].value / max(nsub.value, 1),
-              "integrate_ms_per_step": ph[2].value / max(nsub.value, 1)}
+    snap0 = sim.snapshot()
+    snap0["contacts"] = sim.download()[3]
+    log(f"warmup done ({args.warmup} substeps, {snap0['contacts']} contacts)")
+    window_states = []
+    w = timed_from(sim, snap0, args.steps, record=window_states)
+    snap_end = sim.snapshot()
+    snap_end["contacts"] = sim.download()[3]
+    # the window's representative substep (all layers run, iteration count
+    # nearest the window's median): the CPU baseline times that one
+    med = float(np.median(w["iters"]))
+    cand = [k for k in range(args.steps) if len(w["per_layer"][k]) == 3] or list(range(args.steps))
+    k_rep = min(cand, key=lambda k: (abs(w["iters"][k] - med), k))
+    snap_rep = window_states[k_rep]
+    snap_rep["contacts"] = (window_states[k_rep + 1]["contacts"] if k_rep + 1 < args.steps
+                            else snap_end["contacts"])
+    log(f"timed: {args.steps} substeps, {w['ms_per_step']:.3f} ms/substep, "
+        f"{w['mean_iterations']:.2f} iterations")
     # Kernel timing window: the next substeps.  The persistent loop (layers
     # without rigid elements) is timed with one HIP event pair per launch;
     # the per-color sweeps of the rigid layers with one pair around each
     # iteration's back-to-back color launches.
-    log(f"timed: {args.steps} substeps in {t1 - t0:.3f} s")
     L.check("mlx_profile_enable", lib.mlx_profile_enable(sim.dmesh.handle, 1))
     for _ in range(args.profile_steps):
         sim.step()
@@ -315,8 +457,88 @@ def main():
     L.check("mlx_profile_enable", lib.mlx_profile_enable(sim.dmesh.handle, 0))
     pfin, vfin, _, contacts = sim.download()
     finite = bool(np.isfinite(pfin).all() and np.isfinite(vfin).all())
+
+    extra = {}
+    p32_one = p32_rep = None
+    if not args.no_extra_legs:
+        # monitor-excluded time (the reference's sumNonMonitoredTime,
+        # xpbdLayer3D.m:742): the same window with each substep's iteration
+        # counts fixed to the ones the stop rule chose and the residual
+        # monitor off -- the same projections in the same order
+        st = sim._settings
+        if all(len(pl) == len(w["per_layer"][0]) for pl in w["per_layer"]) and \
+                len(w["per_layer"][0]) == st.nLayers:
+            keep = (st.computeResiduals, st.stopAtImprovementPercent)
+
+            def fixed(k):
+                st.computeResiduals = 0
+                st.stopAtImprovementPercent = 0.0
+                st.runUntilResSmallerThan = -1.0
+                for l, n in enumerate(w["per_layer"][k]):
+                    st.iterations[l] = n
+
+            nm = timed_from(sim, snap0, args.steps, fixed)
+            same = bool(np.array_equal(sim.download()[0], snap_end["p"]))
+            st.computeResiduals, st.stopAtImprovementPercent = keep
+            for l, n in enumerate((2, 4, 14)):
+                st.iterations[l] = n
+            extra["monitor_excluded"] = dict(
+                ms_per_step=nm["ms_per_step"], value=nm["value"], unit="tet-proj/s",
+                same_positions_as_timed=same,
+                what="the timed window again with each substep's iteration counts fixed to the "
+                     "stop rule's and the residual monitor off (the reference times the solve "
+                     "without its monitor, xpbdLayer3D.m:742)")
+            log(f"monitor excluded: {nm['ms_per_step']:.3f} ms/substep (same positions: {same})")
+        p32_one = one_substep(sim, snap0)
+        p32_rep = one_substep(sim, snap_rep)
+        # FP64 plain line and the FP32 error against it
+        sim64 = Simulation(mesh, make_integrator(hanging_stop=args.hanging_stop), h, plane,
+                           precision=L.MLX_FP64)
+        set_options(sim64, args)
+        w64 = timed_from(sim64, snap0, args.steps)
+        p64_end = sim64.download()[0]
+        p64_one = one_substep(sim64, snap0)
+        sim64.close()
+        extra["fp64"] = dict(value=w64["value"], unit="tet-proj/s", ms_per_step=w64["ms_per_step"],
+                             mean_iterations_per_substep=w64["mean_iterations"],
+                             phases=w64["phases"], dtype="f64")
+        extra["fp32_error_vs_fp64"] = dict(
+            one_substep=rel(p32_one, p64_one), after_window=rel(snap_end["p"], p64_end),
+            what="relative 2-norm error of the FP32 positions against the FP64 GPU path (itself "
+                 "equal to the FP64 oracle to 1e-9 in the parity tests): one substep from the "
+                 "window-start state, and after the timed window")
+        log(f"fp64: {w64['ms_per_step']:.3f} ms/substep; fp32 error one substep "
+            f"{extra['fp32_error_vs_fp64']['one_substep']:.2e}, after window "
+            f"{extra['fp32_error_vs_fp64']['after_window']:.2e}")
+        # learned form: the committed network attached, same window
+        if os.path.exists(LEARNED_MODEL) and args.precision == "fp32":
+            from multilayersolver2024_amd import learn
+            net = learn.Mlp.load(LEARNED_MODEL)
+            simL = Simulation(mesh, make_integrator(hanging_stop=args.hanging_stop), h, plane,
+                              precision=prec)
+            set_options(simL, args)
+            learn.attach(simL, net)
+            simL.restore(snap0)
+            run_window(simL, args.steps)
+            simL.restore(snap0)
+            learn.stats(simL)
+            wl = run_window(simL, args.steps)
+            kept, rej = learn.stats(simL)
+            pl_end = simL.download()[0]
+            simL.close()
+            net.close()
+            extra["learned"] = dict(
+                value=wl["value"], unit="tet-proj/s", ms_per_step=wl["ms_per_step"],
+                mean_iterations_per_substep=wl["mean_iterations"],
+                warm_starts_kept=kept, warm_starts_rejected=rej,
+                position_difference_vs_plain=rel(pl_end, snap_end["p"]),
+                model=os.path.relpath(LEARNED_MODEL, ROOT),
+                what="the same window with the learned warm start and key factors "
+                     "(mlx_mesh_set_learned, MLX_LEARN_WARMSTART | MLX_LEARN_KEYS)")
+            log(f"learned: {wl['ms_per_step']:.3f} ms/substep, {wl['mean_iterations']:.2f} "
+                f"iterations, kept {kept} rejected {rej}")
     sim.close()
-    wall = t1 - t0
+    wall = w["wall"]
     bpt = BYTES_PER_TET[args.precision]
     ftype = "float" if args.precision == "fp32" else "double"
     sweep_s = ms.value / 1e3
@@ -344,12 +566,12 @@ def main():
                and args.sweep_block == 64 else None)
     line = {
         "metric": "ms/substep & tet-proj/s to rel. residual 1e-4: plain vs learned+batched, 1 GPU",
-        "value": proj / wall,
+        "value": w["value"],
         "unit": "tet-proj/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": w["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -366,12 +588,15 @@ def main():
             "persistent": bool(args.persistent),
             "persist_local": bool(args.persist_local) if args.persistent else None,
             "colors": ncol,
-            "mean_iterations_per_substep": float(np.mean(iters)),
-            "tet_projections": int(proj),
+            "mean_iterations_per_substep": w["mean_iterations"],
+            "tet_projections": int(w["proj"]),
+            "contacts_at_window_start": int(snap0["contacts"]),
             "contacts_last_substep": int(contacts),
             "finite": finite,
             "rigid_contacts": False,
-            "phases": phases,
+            "phases": w["phases"],
+            "timing": "after the warmup substeps, one untimed pass over the window (iteration "
+                      "graphs captured), then the window timed again from the same state",
         },
         "roofline": dict({
             "bound": "hbm",
@@ -380,15 +605,24 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": TRAFFIC_SOURCE if traffic is not None else None,
         }, **roof),
     }
+    line.update(extra)
     if not args.no_batched:
         log("batched leg")
         line["batched"] = bench_batched(h, plane, args.batch_steps, args.batch_warmup,
                                         args.precision, graph=bool(args.batch_graph))
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(mesh, h, plane)
+        cb, p_oracle_one = cpu_baseline(mesh, h, plane, snap_rep, runs=args.cpu_runs,
+                                        label=f"substep {k_rep + 1} of the timed window "
+                                              f"({w['iters'][k_rep]} GPU iterations; window "
+                                              f"median {med:g})")
+        line["cpu_baseline"] = cb
+        if p32_rep is not None:
+            line["fp32_error_vs_fp64"]["representative_substep_vs_oracle"] = rel(p32_rep,
+                                                                                 p_oracle_one)
     print(json.dumps(line), flush=True)
 
 

@@ -290,6 +290,17 @@ int mlx_state_destroy(mlx_state *state);
  * last substep; contacts: number of active contacts of the last substep. */
 int mlx_state_download(mlx_state *state, double *p, double *v,
                        double *lambdai, int32_t *nContacts);
+/* Overwrites the state between substeps; any pointer may be NULL (kept).
+ * p, v, oldp: dim*nV; oldp is the integrator's position history
+ * (xpbdLayer3D's cache.oldp, the start-of-substep positions of the last
+ * substep, xpbdLayer3D.m:629-635 / :660); lambdai: nT x nc multipliers of
+ * the last substep.  With mlx_state_get_oldp it restarts a run mid-way on
+ * another mesh object (another precision, a learned mode) from the same
+ * state; continuing with frame >= 2 keeps oldp and pinnedx (set the
+ * latter with mlx_state_set_pinned_targets). */
+int mlx_state_set(mlx_state *state, const double *p, const double *v, const double *oldp,
+                  const double *lambdai);
+int mlx_state_get_oldp(mlx_state *state, double *oldp);
 
 /* One xpbdLayer3D.integrate (3d/integrator/xpbdLayer3D.m:621-745) or
  * xpbdLayer2D.integrate (2d/integrator/xpbdLayer2D.m:793-917) substep, with

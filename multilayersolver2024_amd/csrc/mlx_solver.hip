@@ -2252,6 +2252,44 @@ int mlx_state_download(mlx_state *S, double *p, double *v, double *lambdai, int3
   return MLX_OK;
 }
 
+int mlx_state_set(mlx_state *S, const double *p, const double *v, const double *oldp,
+                  const double *lambdai) {
+  if (!S) {
+    set_error("mlx_state_set: null state");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *S->mesh;
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  const size_t n = (size_t)m.dim * m.nV;
+  if (m.precision == MLX_FP64) {
+    if (p) MLX_TRY(upload_positions<double>(m, m.d_x, p));
+    if (oldp) MLX_TRY(upload_positions<double>(m, m.d_xold, oldp));
+    if (v) MLX_TRY(upload_scalars<double>(S->d_v, v, n));
+    if (lambdai) MLX_TRY(upload_lambda<double>(m, lambdai));
+  } else {
+    if (p) MLX_TRY(upload_positions<float>(m, m.d_x, p));
+    if (oldp) MLX_TRY(upload_positions<float>(m, m.d_xold, oldp));
+    if (v) MLX_TRY(upload_scalars<float>(S->d_v, v, n));
+    if (lambdai) MLX_TRY(upload_lambda<float>(m, lambdai));
+  }
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  return MLX_OK;
+}
+
+int mlx_state_get_oldp(mlx_state *S, double *oldp) {
+  if (!S || !oldp) {
+    set_error("mlx_state_get_oldp: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *S->mesh;
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  if (m.precision == MLX_FP64) MLX_TRY(download_positions<double>(m, m.d_xold, oldp));
+  else MLX_TRY(download_positions<float>(m, m.d_xold, oldp));
+  return MLX_OK;
+}
+
 int mlx_state_set_pinned_targets(mlx_state *S, const double *pinnedx) {
   if (!S || !pinnedx) {
     set_error("mlx_state_set_pinned_targets: bad arguments");

@@ -174,6 +174,30 @@ class Simulation:
         L.check("mlx_state_set_pinned_targets",
                 L.lib().mlx_state_set_pinned_targets(self.state, L.ptr(x)))
 
+    def snapshot(self):
+        """(p, v, oldp, lambda nT x nc, frame, runUntil): everything the
+        next substep reads from the state."""
+        p, v, lam, _ = self.download()
+        oldp = np.zeros_like(p)
+        L.check("mlx_state_get_oldp", L.lib().mlx_state_get_oldp(self.state, L.ptr(oldp)))
+        return dict(p=p, v=v, oldp=oldp, lam=lam, frame=self.frame,
+                    runUntil=self._settings.runUntilResSmallerThan)
+
+    def restore(self, snap):
+        """Continue from a snapshot (of this or another Simulation of the same
+        mesh): state, frame count, runUntil and, in 3D, the pinned targets
+        (the pinned DOFs' values in the snapshot's positions)."""
+        m = self.mesh
+        lam = L.fortran(np.asarray(snap["lam"], dtype=np.float64), np.float64)
+        L.check("mlx_state_set", L.lib().mlx_state_set(
+            self.state, L.ptr(L.f64(snap["p"])), L.ptr(L.f64(snap["v"])),
+            L.ptr(L.f64(snap["oldp"])), L.ptr(lam)))
+        self.frame = int(snap["frame"])
+        self._settings.runUntilResSmallerThan = float(snap["runUntil"])
+        self.integrator.runUntilResSmallerThan = float(snap["runUntil"])
+        if m.dim == 3 and len(m.pinnedDOFs) and self.frame >= 1:
+            self.set_pinned_targets(L.f64(snap["p"])[np.asarray(m.pinnedDOFs) - 1])
+
     def download(self):
         m = self.mesh
         p = np.zeros(m.dim * m.nV)

@@ -15,9 +15,10 @@ matrix cores) predicts per element
 * ``c``: the log10 factor from the last substep's strain-rate key to the
   coming substep's (target: log10 of the ratio of consecutive keys).
 
-The library keeps a warm start only when it does not raise the reference
-residual (simpleEvaluateAllConstraintResiduals, xpbdLayer3D.m:546-586) of
-the cold start, so a poor model costs time, never accuracy.
+The library keeps a warm start only when it does not raise the residual of
+the equations the projection solves (C + alpha~ lambda + damping per
+constraint, r phi per contact) over the cold start's, so a poor model costs
+time, never accuracy; the stop rule is the reference's, unchanged.
 
 Run ``python -m multilayersolver2024_amd.learn --out FILE`` on a GPU to
 generate the data, train and save a model.
@@ -188,20 +189,22 @@ def features(sim):
 # training data
 # ---------------------------------------------------------------------------
 
-def strain_rate_keys(P0, T, pa, pb, h):
-    """Per-element ||E(pa) - E(pb)||_F^2 / h^2 with E = (F^T F - I) / 2, F =
-    Ds Dm^-1 from rest positions P0 (the strain-rate measure the layers are
-    sorted by, mexEdiffNorm3D.cpp:25-61)."""
-    def F_of(X):
-        X = X.reshape(-1, 3)
-        Ds = np.stack([X[T[:, k]] - X[T[:, 0]] for k in (1, 2, 3)], axis=2)
-        return Ds @ Dinv
+def rest_inverse(P0, T):
+    """Dm^-1 per element from the rest positions."""
     R = P0.reshape(-1, 3)
-    Dm = np.stack([R[T[:, k]] - R[T[:, 0]] for k in (1, 2, 3)], axis=2)
-    Dinv = np.linalg.inv(Dm)
-    Fa, Fb = F_of(pa), F_of(pb)
-    Ea = 0.5 * (np.swapaxes(Fa, 1, 2) @ Fa)
-    Eb = 0.5 * (np.swapaxes(Fb, 1, 2) @ Fb)
+    return np.linalg.inv(np.stack([R[T[:, k]] - R[T[:, 0]] for k in (1, 2, 3)], axis=2))
+
+
+def green_strain(Dinv, T, x):
+    """E = (F^T F - I) / 2 per element, F = Ds Dm^-1."""
+    X = x.reshape(-1, 3)
+    F = np.stack([X[T[:, k]] - X[T[:, 0]] for k in (1, 2, 3)], axis=2) @ Dinv
+    return 0.5 * (np.swapaxes(F, 1, 2) @ F)
+
+
+def strain_rate_keys(Ea, Eb, h):
+    """Per-element ||E_a - E_b||_F^2 / h^2: the strain-rate measure the
+    layers are sorted by (mexEdiffNorm3D.cpp:25-61)."""
     return np.sum((Ea - Eb) ** 2, axis=(1, 2)) / (h * h)
 
 
@@ -258,27 +261,28 @@ def collect(seeds, substeps, per_substep, precision=L.MLX_FP32, log=print, base_
                          precision=precision)
         attach(sim, probe, flags=0)
         T = mesh.T0
-        P0 = mesh.p
-        p_prev = None
+        Dinv = rest_inverse(mesh.p, T)
         t0 = time.time()
         n_before = sum(len(x) for x in Xs)
+        p_now, _, lam_prev, _ = sim.download()
+        E_prev = E_now = green_strain(Dinv, T, p_now)
         for n in range(substeps):
-            p_now, _, lam_prev, _ = sim.download()
             X = features(sim) if n >= 1 else None
             sim.step()
             p_next, _, lam_new, _ = sim.download()
-            if n >= 1 and p_prev is not None:
+            E_next = green_strain(Dinv, T, p_next)
+            if n >= 1:
                 kap, w = kappa_target(lam_new, lam_prev)
-                ed_now = strain_rate_keys(P0, T, p_now, p_prev, h)
-                ed_next = strain_rate_keys(P0, T, p_next, p_now, h)
-                c = key_target(ed_next, ed_now)
+                c = key_target(strain_rate_keys(E_next, E_now, h),
+                               strain_rate_keys(E_now, E_prev, h))
                 idx = rng.choice(mesh.nT, size=min(per_substep, mesh.nT), replace=False)
                 ok = np.isfinite(X[idx]).all(axis=1) & np.isfinite(kap[idx]) & np.isfinite(c[idx])
                 idx = idx[ok]
                 Xs.append(X[idx])
                 Ys.append(np.stack([kap[idx], c[idx]], axis=1).astype(np.float32))
                 Ws.append(w[idx])
-            p_prev = p_now
+            lam_prev = lam_new
+            E_prev, E_now = E_now, E_next
         log(f"seed {seed}: {sc['name']} z0={sc['p'][:, 2].min():.3f}, {substeps} substeps, "
             f"{sum(len(x) for x in Xs) - n_before} samples, {time.time() - t0:.1f} s")
         sim.close()
@@ -305,6 +309,56 @@ def fit(X, Y, W, epochs=20, batch=4096, lr=2e-3, seed=0, log=print):
     return net, losses
 
 
+def evaluate(model, nx=64, ny=16, nz=16, seed=0, substeps=60, warmup=30,
+             flags=L.MLX_LEARN_WARMSTART | L.MLX_LEARN_KEYS, precision=L.MLX_FP32, log=print):
+    """Plain against learned on one beam (the 81,920-tet workload by
+    default, seed 0, which the training data never uses): both runs take
+    `warmup` plain substeps from rest, then `substeps` substeps timed, the
+    learned run with the model attached.  Returns per-run mean iterations
+    and ms per substep, and the warm starts kept / rejected."""
+    from . import scenes
+    from .integrator import PlaneContactFinder, Simulation, XpbdLayer3D
+    from .mesh import Mesh
+
+    h = 1.0 / 60.0
+    mesh = Mesh(scenes.beam3d(nx, ny, nz, seed=seed))
+    out = {}
+    for name in ("plain", "learned"):
+        it = XpbdLayer3D()
+        it.Gravity = -9.81
+        it.setLayers([98.4375, 87.5, 0])
+        it.iterations = [2, 4, 14]
+        it.computeResiduals = True
+        it.StopAtImprovementPercent = 99.99
+        it.hangingStop = 1000
+        it.useRigidContacts = False
+        sim = Simulation(mesh, it, h, contact=PlaneContactFinder([0, 0, 1], [0, 0, 0]),
+                         precision=precision)
+        try:
+            for _ in range(warmup):
+                sim.step()
+            if name == "learned":
+                attach(sim, model, flags)
+                stats(sim)
+            iters, ms, finals = [], [], []
+            for _ in range(substeps):
+                t0 = time.perf_counter()
+                r = sim.step()
+                ms.append(1e3 * (time.perf_counter() - t0))
+                iters.append(sum(r.itersPerLayer))
+                finals.append(float(r.residualArray[-1]))
+            kept, rej = stats(sim) if name == "learned" else (0, 0)
+            out[name] = dict(mean_iterations=float(np.mean(iters)),
+                             median_ms=float(np.median(ms)), mean_ms=float(np.mean(ms)),
+                             mean_final_residual=float(np.mean(finals)), kept=kept,
+                             rejected=rej, iterations=iters)
+            log(f"{name}: {np.mean(iters):.2f} iterations/substep, "
+                f"{np.median(ms):.2f} ms median, kept {kept} rejected {rej}")
+        finally:
+            sim.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", required=True)
@@ -314,6 +368,9 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--eval", type=int, default=0,
+                    help="then compare plain and learned over this many substeps of the "
+                         "81,920-tet beam")
     args = ap.parse_args(argv)
     lg = lambda *a: print("[learn]", *a, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.time()
@@ -329,6 +386,9 @@ def main(argv=None):
                 samples=int(len(X)), epochs=args.epochs, batch=args.batch, lr=args.lr,
                 final_loss=float(losses[-1]), mean_predictor_loss=base,
                 train_seconds=time.time() - t1, data_seconds=t1 - t0)
+    if args.eval:
+        ev = evaluate(net, substeps=args.eval, log=lg)
+        meta["eval"] = {k: {a: b for a, b in v.items() if a != "iterations"} for k, v in ev.items()}
     net.save(args.out, meta)
     lg(json.dumps(meta))
     net.close()

@@ -29,6 +29,11 @@
  * for a square non-triangular page (LAPACK getrf/getrs).  A and b are
  * copied.  Returns 0, or 1 for an exactly singular pivot (the result is then
  * Inf/NaN as in Matlab). */
+/* Threads of the element loops (ora_set_threads; 1 = serial). */
+static int g_threads = 1;
+
+void ora_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
 static int solve_n(int n, const double *A_in, const double *b_in, double *x) {
   double A[36], b[6];
   memcpy(A, A_in, sizeof(double) * n * n);
@@ -734,14 +739,17 @@ static double nh_residual_sq3d(const ora_mesh *m, int e, double h, const double 
 void ora_stvk_voigt_constraints3d(const ora_mesh *m, const uint8_t *isElastic,
                                   double h, double *lambda, double *xi,
                                   const double *oldp) {
-  double aT[36], bT[36];
-  for (int c = 1; c <= m->numColors; ++c)
+  for (int c = 1; c <= m->numColors; ++c) {
+    /* the elements of one color share no vertex: with threads they run in
+     * any order and still give the serial result bit for bit */
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int e = 0; e < m->nT; ++e) {
       if (m->color[e] != c || !isElastic[e]) continue;
       if (m->nhModel && m->nhModel[e]) {
         nh_project3d(m, e, h, lambda, xi);
         continue;
       }
+      double aT[36], bT[36];
       double x[12], xo[12], mi[12], dx[12], dl[6];
       for (int a = 0; a < 4; ++a)
         for (int d = 0; d < 3; ++d) {
@@ -763,6 +771,7 @@ void ora_stvk_voigt_constraints3d(const ora_mesh *m, const uint8_t *isElastic,
           xi[dof] = x[3 * a + d] + dx[3 * a + d];
         }
     }
+  }
 }
 
 /* xpbdLayer2D.stvkVoigtConstraints (xpbdLayer2D.m:61-89), call site :178. */
@@ -828,29 +837,40 @@ static double contact_residual_sq3d(const ora_contacts *c, const double *xi,
  * element, elastic or rigid.  The sum of squares is accumulated in element
  * order; the reference's norm() scales to avoid overflow, which changes
  * only the last bits. */
+static double elem_residual_sq3d(const ora_mesh *m, int e, double h, const double *lambda,
+                                 const double *xi) {
+  if (m->nhModel && m->nhModel[e]) return nh_residual_sq3d(m, e, h, lambda, xi);
+  double h2 = h * h, acc = 0.0;
+  double x[12], g[72], C[6];
+  for (int a = 0; a < 4; ++a)
+    for (int d = 0; d < 3; ++d) x[3 * a + d] = xi[3 * m->T[4 * e + a] + d];
+  ora_stvk_grad3d(m->DmInv + 9 * e, x, g, C);
+  for (int i = 0; i < 6; ++i) {
+    double Ci = C[i] * m->vol[e];
+    double al = (m->alpha[36 * e + i] / h2) * lambda[6 * e];
+    for (int k = 1; k < 6; ++k) al = al + (m->alpha[36 * e + i + 6 * k] / h2) * lambda[6 * e + k];
+    double aC = fabs(Ci), corr = fabs(al);
+    double t = aC - (aC < corr ? aC : corr);
+    acc = acc + t * t;
+  }
+  return acc;
+}
+
 double ora_residual3d(const ora_mesh *m, double h, const double *lambda,
                       const double *xi, const double *oldp,
                       const ora_contacts *c, const double *cl,
                       double contactResistance) {
   (void)oldp;
-  double acc = 0.0, h2 = h * h;
-  for (int e = 0; e < m->nT; ++e) {
-    if (m->nhModel && m->nhModel[e]) {
-      acc = acc + nh_residual_sq3d(m, e, h, lambda, xi);
-      continue;
-    }
-    double x[12], g[72], C[6];
-    for (int a = 0; a < 4; ++a)
-      for (int d = 0; d < 3; ++d) x[3 * a + d] = xi[3 * m->T[4 * e + a] + d];
-    ora_stvk_grad3d(m->DmInv + 9 * e, x, g, C);
-    for (int i = 0; i < 6; ++i) {
-      double Ci = C[i] * m->vol[e];
-      double al = (m->alpha[36 * e + i] / h2) * lambda[6 * e];
-      for (int k = 1; k < 6; ++k) al = al + (m->alpha[36 * e + i + 6 * k] / h2) * lambda[6 * e + k];
-      double aC = fabs(Ci), corr = fabs(al);
-      double t = aC - (aC < corr ? aC : corr);
-      acc = acc + t * t;
-    }
+  double acc = 0.0;
+  if (g_threads > 1 && m->nT > 0) {
+    /* element terms in parallel, summed in element order as below */
+    double *terms = (double *)malloc(sizeof(double) * m->nT);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int e = 0; e < m->nT; ++e) terms[e] = elem_residual_sq3d(m, e, h, lambda, xi);
+    for (int e = 0; e < m->nT; ++e) acc = acc + terms[e];
+    free(terms);
+  } else {
+    for (int e = 0; e < m->nT; ++e) acc = acc + elem_residual_sq3d(m, e, h, lambda, xi);
   }
   if (c && c->K > 0) acc = acc + contact_residual_sq3d(c, xi, cl, contactResistance, h);
   return sqrt(acc);

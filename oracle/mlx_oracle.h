@@ -94,6 +94,12 @@ typedef struct {
   double learnRes[2];          /* out: equation residual before / after */
 } ora_step_out;
 
+/* Threads for the per-color element loop of the 3D sweep and the 3D
+ * residual's element terms (OpenMP; default 1).  Results are the serial
+ * ones bit for bit: elements of one color share no vertex, and the residual
+ * terms are summed in element order. */
+void ora_set_threads(int n);
+
 /* ---- precompute (Mesh3D / Mesh constructors) ---- */
 int ora_naive_graph_coloring(const int32_t *T, int nT, int vpe, int nV,
                              int32_t *color);
