@@ -27,7 +27,8 @@ substeps that follow the timed ones, one HIP event pair (no system fence)
 brackets the back-to-back color launches of every iteration on the mesh's
 stream (direct launches; the timed substeps replay the same kernels from
 captured hipGraphs), and the average launch duration is that time over the
-number of launches.  `phases` splits the timed substeps' host wall time into
+number of launches: kernel time plus the dispatch gap to the next launch
+(a pair around every launch adds more than the gap).  `phases` splits the timed substeps' host wall time into
 contacts + layer maps and layerSolve.  cpu_baseline is the FP64 CPU oracle
 (a C restatement of the reference path) on one substep from the window-start
 state with the same stop rule, median of 5 runs, on one core and with OpenMP
@@ -60,27 +61,30 @@ BYTES_PER_TET = {"fp32": 280, "fp64": 544}
 # ids 16 + x read 48 + DmInv 36 + vol 4 + (alpha_d, alpha_o, alpha_s) 12 +
 # lambda read 24 (FP64: 16 + 96 + 72 + 8 + 24 + 48).
 RESIDUAL_BYTES_PER_TET = {"fp32": 140, "fp64": 264}
-# HBM-side bytes per dispatch on the default workloads, from the rocprofv3
-# FETCH_SIZE / WRITE_SIZE runs (KiB per dispatch; k_sweep3d averaged over
-# 8,200 dispatches of the 81,920-tet beam with 64-thread workgroups,
-# k_batch_plain over the 256-beam substeps at 1,024 threads), scaled by what
-# the same counters read for a 1 GiB device copy (FETCH_SIZE reads half the
-# bytes fetched, WRITE_SIZE all).
-TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 197.73, "fetch_scale": 2.0,
-                                    "write": 96.19, "write_scale": 1.0},
-               "k_batch_plain<float,1024>": {"fetch": 15336905.12, "fetch_scale": 2.0,
-                                             "write": 3388201.03, "write_scale": 1.0}}
-
-
+# HBM-side traffic from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (KiB),
+# scaled by what the same counters read for a 1 GiB device copy (FETCH_SIZE
+# reads half the bytes fetched, WRITE_SIZE all).  k_sweep3d: per launched
+# thread slot (one slot per slot of the color, elastic or not), averaged over
+# the 12,872 sweep dispatches of the 81,920-tet beam run (64-thread
+# workgroups); a launch's traffic is that times its slots.  k_batch_plain:
+# per dispatch (one per substep of the 256 beams, 1,024 threads).
+TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 114.569 / 1024, "fetch_scale": 2.0,
+                                    "write": 55.865 / 1024, "write_scale": 1.0,
+                                    "per": "thread slot"},
+               "k_batch_plain<float,1024>": {"fetch": 15272309.61, "fetch_scale": 2.0,
+                                             "write": 3361058.59, "write_scale": 1.0,
+                                             "per": "dispatch"}}
 # Where the constants above come from.
-TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench at commit 2fb7b75, "
-                  "scaled by the same counters of a 1 GiB device copy (FETCH x2.0, WRITE x1.0); "
-                  "summaries in profiles/hbm_traffic_*.md")
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this bench (--graphs 0) "
+                  "at commit 3c06c62, scaled by the same counters of a 1 GiB device copy "
+                  "(FETCH x2.0, WRITE x1.0); summaries in profiles/hbm_traffic_*.md")
 
 
-def traffic_bytes(kernel):
+def traffic_bytes(kernel, units=1.0):
+    """HBM bytes of one launch of `kernel` covering `units` (thread slots
+    for the sweep, 1 for a batched dispatch)."""
     tk = TRAFFIC_KIB.get(kernel)
-    return (1024 * (tk["fetch"] * tk["fetch_scale"] + tk["write"] * tk["write_scale"])
+    return (1024 * units * (tk["fetch"] * tk["fetch_scale"] + tk["write"] * tk["write_scale"])
             if tk else None)
 
 
@@ -420,7 +424,7 @@ def main():
                      precision=prec)
     lib = L.lib()
     set_options(sim, args)
-    ncol, _ = sim.dmesh.info()
+    ncol, sim_buckets = sim.dmesh.info()
     for _ in range(args.warmup):
         sim.step()
     snap0 = sim.snapshot()
@@ -442,8 +446,8 @@ def main():
         f"{w['mean_iterations']:.2f} iterations")
     # Kernel timing window: the next substeps.  The persistent loop (layers
     # without rigid elements) is timed with one HIP event pair per launch;
-    # the per-color sweeps of the rigid layers with one pair around each
-    # iteration's back-to-back color launches.
+    # the per-color sweeps with one pair around each iteration's
+    # back-to-back color launches.
     L.check("mlx_profile_enable", lib.mlx_profile_enable(sim.dmesh.handle, 1))
     for _ in range(args.profile_steps):
         sim.step()
@@ -562,8 +566,14 @@ def main():
         roof = {"kernel": sweeps["kernel"], "bytes_per_unit": bpt,
                 "avg_launch_us": sweeps["avg_launch_us"], "launches": launches.value,
                 "elements_per_launch": elems.value / max(launches.value, 1)}
-    traffic = (traffic_bytes(roof["kernel"]) if (args.nx, args.ny, args.nz) == (64, 16, 16)
+    # thread slots per sweep launch: the colors' slot counts rounded up to
+    # the workgroup, averaged over the colors (the all-elastic layer, which
+    # makes most of the launches, launches every color)
+    bucket = np.asarray(sim_buckets, dtype=np.int64)
+    slots = float(np.mean(-(-bucket // args.sweep_block) * args.sweep_block))
+    traffic = (traffic_bytes(roof["kernel"], slots) if roof["kernel"] == "k_sweep3d<float>"
                and args.sweep_block == 64 else None)
+    roof["thread_slots_per_launch"] = slots
     line = {
         "metric": "ms/substep & tet-proj/s to rel. residual 1e-4: plain vs learned+batched, 1 GPU",
         "value": w["value"],
