@@ -377,6 +377,20 @@ int mlx_batch_create(const mlx_mesh_desc *merged, int32_t nScenes,
 int mlx_batch_destroy(mlx_batch *batch);
 /* Threads per scene workgroup: 256, 512 (default), 768 or 1024. */
 int mlx_batch_set_block(mlx_batch *batch, int32_t threads);
+/* Rigid clusters: isRigidElement (nT, merged element order) marks the
+ * elements that are rigid in the coarse layer (pinned elements stay
+ * elastic); elements sharing a vertex form one body, numbered by ascending
+ * smallest element as mexUnionFindByVertice numbers them
+ * (2d/mex/mexUnionFindByVertice.cpp:117-178), at most 8 bodies per scene.
+ * Every substep then runs, per scene, the two-layer layerSolve
+ * (xpbdLayer3D.m:109-226): the coarse layer with these bodies (restriction,
+ * rigid step, boundary accumulation, prolongation) for iterations[0], then
+ * the all-elastic layer for iterations[1] and the stop rule; settings must
+ * have layers = {x, 0} with x != 0 (the coarse layer's rigid set comes from
+ * here, not from the strain-rate percentage) and useRigidContacts off.
+ * NULL returns to the plain method.  The layered launch keeps positions in
+ * HBM and runs one workgroup per scene. */
+int mlx_batch_set_rigid(mlx_batch *batch, const uint8_t *isRigidElement);
 /* (Re)starts all scenes from p, v (3nV of the merged mesh). */
 int mlx_batch_set_state(mlx_batch *batch, const double *p, const double *v);
 int mlx_batch_download(mlx_batch *batch, double *p, double *v, double *lambdai);

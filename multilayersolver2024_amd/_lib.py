@@ -37,6 +37,7 @@ EXPORTED = (
     "mlx_mlp_create", "mlx_mlp_destroy", "mlx_mlp_set", "mlx_mlp_get", "mlx_mlp_infer",
     "mlx_mlp_grad", "mlx_mlp_train", "mlx_mesh_set_learned", "mlx_learn_stats",
     "mlx_learn_outputs", "mlx_learn_features", "mlx_state_set", "mlx_state_get_oldp",
+    "mlx_batch_set_rigid",
 )
 MLX_LEARN_WARMSTART = 1
 MLX_LEARN_KEYS = 2
@@ -198,6 +199,7 @@ def lib():
         "mlx_learn_features": [_p, C.c_double, _f],
         "mlx_state_set": [_p, _d, _d, _d, _d],
         "mlx_state_get_oldp": [_p, _d],
+        "mlx_batch_set_rigid": [_p, _u8],
     }
     for name, args in sig.items():
         f = getattr(L, name)

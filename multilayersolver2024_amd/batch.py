@@ -105,6 +105,14 @@ class BatchSimulation:
     def set_block(self, threads):
         L.check("mlx_batch_set_block", L.lib().mlx_batch_set_block(self.handle, int(threads)))
 
+    def set_rigid(self, rigid):
+        """Rigid clusters of the coarse layer (merged element order; None
+        returns to the plain method).  The integrator then needs two layers,
+        [x, 0] with x != 0, and iterations for both."""
+        r = None if rigid is None else L.u8(np.asarray(rigid, dtype=bool))
+        L.check("mlx_batch_set_rigid", L.lib().mlx_batch_set_rigid(
+            self.handle, L.ptr(r, C.c_uint8) if r is not None else None))
+
     def step(self):
         self.frame += 1
         out = L.BatchOut()
@@ -131,11 +139,65 @@ class BatchSimulation:
             self.handle = None
 
 
+def rigid_components(mesh, rigid):
+    """Body id per element (0 elastic, 1-based) of the rigid elements less
+    the pinned ones: elements sharing a vertex form a body, bodies numbered
+    by ascending smallest element (breadth-first search; the numbering of
+    mexUnionFindByVertice)."""
+    T = np.asarray(mesh.T0)
+    rig = np.asarray(rigid, dtype=bool) & ~np.asarray(mesh.forcedElastic, dtype=bool)
+    by_vert = {}
+    for e in np.nonzero(rig)[0]:
+        for v in T[e]:
+            by_vert.setdefault(int(v), []).append(int(e))
+    body = np.zeros(len(T), dtype=np.int32)
+    nb = 0
+    for e in np.nonzero(rig)[0]:
+        if body[e]:
+            continue
+        nb += 1
+        body[e] = nb
+        queue = [int(e)]
+        while queue:
+            f = queue.pop()
+            for v in T[f]:
+                for g in by_vert[int(v)]:
+                    if not body[g]:
+                        body[g] = nb
+                        queue.append(g)
+    return body
+
+
+def cluster_mask(scene, count):
+    """Elements of `count` (1..3) rigid slabs across a beam from beam3d: the
+    beam's cells along x (after the pinned first cell) are cut into `count`
+    segments and each segment's interior cells (a one-cell elastic gap on
+    both sides, so slabs share no vertex) are rigid, all five tets of each."""
+    a = int(round(np.ptp(np.asarray(scene["p"])[:, 0]) / 0.02))
+    nT = len(scene["t"])
+    per_slab = nT // a  # tets per x cell (5 b c)
+    mask = np.zeros(nT, dtype=bool)
+    free = a - 1
+    if free < 1:
+        return mask
+    count = max(1, min(count, (free + 1) // 2))
+    seg = free / count
+    for j in range(count):
+        lo = 1 + int(round(j * seg))
+        hi = 1 + int(round((j + 1) * seg))
+        lo_i, hi_i = (lo + 1, hi - 1) if hi - lo >= 3 else (lo, lo + 1)
+        for i in range(lo_i, max(hi_i, lo_i + 1)):
+            if i < a:
+                mask[i * per_slab:(i + 1) * per_slab] = True
+    return mask
+
+
 def random_beams(n=256, seed=0, tets=(2000, 60000), cell=0.02, z0=0.05,
-                 E_range=(1e4, 1e7), nu_range=(0.30, 0.45)):
+                 E_range=(1e4, 1e7), nu_range=(0.30, 0.45), neohookean_odd=False):
     """n beam scenes: tet count log-uniform in `tets` (snapped to 5 a b c
     cubes with aspect ratios drawn from the seed), per-scene E log-uniform,
-    nu uniform; each pinned at x = 0 over the ground plane z = 0."""
+    nu uniform; each pinned at x = 0 over the ground plane z = 0.  With
+    neohookean_odd, odd-numbered scenes are NeoHookean."""
     rng = scenes.Pcg32(seed, stream=77)
     out = []
     lo, hi = math.log10(tets[0]), math.log10(tets[1])
@@ -150,5 +212,16 @@ def random_beams(n=256, seed=0, tets=(2000, 60000), cell=0.02, z0=0.05,
         E = 10 ** (math.log10(E_range[0]) + (math.log10(E_range[1]) - math.log10(E_range[0]))
                    * rng.next_double())
         nu = nu_range[0] + (nu_range[1] - nu_range[0]) * rng.next_double()
-        out.append(scenes.beam3d(a, b, c, cell=cell, z0=z0, E=E, nu=nu))
+        out.append(scenes.beam3d(a, b, c, cell=cell, z0=z0, E=E, nu=nu,
+                                 neohookean=bool(neohookean_odd and i % 2 == 1)))
     return out
+
+
+def cluster_workload(n=256, seed=0, tets=(2000, 60000)):
+    """The batched workload with rigid clusters: random_beams with odd
+    scenes NeoHookean, and 1..3 rigid slabs per scene (count from the seed).
+    Returns (scenes, per-scene masks)."""
+    scs = random_beams(n, seed=seed, tets=tets, neohookean_odd=True)
+    rng = scenes.Pcg32(seed, stream=91)
+    masks = [cluster_mask(sc, 1 + int(3 * rng.next_double()) % 3) for sc in scs]
+    return scs, masks

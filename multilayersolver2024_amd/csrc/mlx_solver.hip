@@ -17,6 +17,7 @@
 #include "mlx_internal.h"
 #include "mlx_kernels.h"
 #include "mlx_learn_kernels.h"
+#include "mlx_batch_layered.h"
 
 using namespace mlx;
 
@@ -2624,6 +2625,15 @@ struct mlx_batch {
   hipGraphExec_t gexec = nullptr;
   std::vector<uint64_t> gsig;  // what the captured kernels bake in
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // coarse layer with rigid clusters (mlx_batch_set_rigid)
+  bool layered = false;
+  int nBodies = 0;
+  DevBuf d_layer;    // body0 | rigOff | rigVert | bndOff | bndVert | vbody | flags0
+  size_t o_body0 = 0, o_rigOff = 0, o_rigVert = 0, o_bndOff = 0, o_bndVert = 0, o_vbody = 0,
+         o_flags0 = 0;
+  DevBuf d_bodies, d_ri, d_rv, d_lamB;
+  DevBuf d_citers;                   // int32 per scene: iterations of the coarse layer
+  std::vector<int64_t> elastic0;     // per scene: elastic elements of the coarse layer
 };
 
 namespace mlx {
@@ -2653,6 +2663,12 @@ __global__ void k_batch_prologue(int64_t nV, const T *v, vec4_t<T> *x, const dou
 template <typename T, int NT>
 static void launch_batch(const BatchParams<T> &B, int n, size_t ldsBytes, hipStream_t s) {
   hipLaunchKernelGGL((k_batch_plain<T, NT>), dim3(n), dim3(NT), ldsBytes, s, B);
+}
+
+template <typename T, int NT>
+static void launch_layered(const BatchParams<T> &B, const LayeredParams<T> &Lp, int n,
+                           hipStream_t s) {
+  hipLaunchKernelGGL((k_batch_layered<T, NT>), dim3(n), dim3(NT), 0, s, B, Lp);
 }
 
 // Scenes with at most this many vertices keep their positions in LDS
@@ -2709,11 +2725,25 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   vec4_t<T> *x = m.d_x.as<vec4_t<T>>();
   vec4_t<T> *xo = m.d_xold.as<vec4_t<T>>();
   const int64_t nPin = (int64_t)m.pinnedDOF.size();
-  for (int i = 0; i < st->nLayers; ++i)
-    if (st->layers[i] != 0) {
-      set_error("the batched path runs the plain method: every layer must be 0 %% rigid");
+  if (b.layered) {
+    // the coarse layer's rigid elements are the clusters of
+    // mlx_batch_set_rigid, then the all-elastic layer
+    if (st->nLayers != 2 || st->layers[1] != 0 || st->layers[0] == 0) {
+      set_error("a batch with rigid clusters runs two layers: layers = {coarse, 0}");
       return MLX_ERR_UNSUPPORTED;
     }
+    if (st->useRigidContacts) {
+      set_error("the batched path projects elastic contacts only (useRigidContacts = false)");
+      return MLX_ERR_UNSUPPORTED;
+    }
+  } else {
+    for (int i = 0; i < st->nLayers; ++i)
+      if (st->layers[i] != 0) {
+        set_error("the batched path runs the plain method: every layer must be 0 %% rigid "
+                  "(rigid clusters: mlx_batch_set_rigid)");
+        return MLX_ERR_UNSUPPORTED;
+      }
+  }
   if (st->runUntilResSmallerThan >= 0 || st->stopAtImprovementPercent > 0) {
     if (!st->computeResiduals || st->hangingStop < 0) {
       set_error("a residual stop in the batched path needs computeResiduals and hangingStop >= 0");
@@ -2796,9 +2826,15 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     }
     hipLaunchKernelGGL(k_copy<vec4_t<T>>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV,
                        (const vec4_t<T> *)x, xo);
-    hipLaunchKernelGGL(k_batch_prologue<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV,
-                       b.d_v.as<T>(), x, m.d_mass64.as<double>(), b.d_pinMask.as<uint8_t>(), h,
-                       st->gravity, st->airDrag);
+    if (b.layered)
+      hipLaunchKernelGGL(k_residual_velocity3d<T>, dim3(blocks_for(3 * nV)), dim3(kBlock), 0, s,
+                         nV, (const T *)b.d_v.as<T>(), b.d_rv.as<T>(), m.d_mass64.as<double>(),
+                         (const uint8_t *)b.d_pinMask.as<uint8_t>(), h, st->gravity,
+                         st->airDrag);
+    else
+      hipLaunchKernelGGL(k_batch_prologue<T>, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV,
+                         b.d_v.as<T>(), x, m.d_mass64.as<double>(), b.d_pinMask.as<uint8_t>(), h,
+                         st->gravity, st->airDrag);
     hipLaunchKernelGGL(k_fill<T>, dim3(blocks_for(nLam)), dim3(kBlock), 0, s, nLam, lam, T(0));
     return MLX_OK;
   };
@@ -2863,7 +2899,34 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     MLX_CHECK_HIP(hipEventCreate(&b.ev0));
     MLX_CHECK_HIP(hipEventCreate(&b.ev1));
   }
+  LayeredParams<T> Lp;
+  if (b.layered) {
+    const char *lb = b.d_layer.as<char>();
+    Lp.body0 = reinterpret_cast<const int32_t *>(lb + b.o_body0);
+    Lp.rigOff = reinterpret_cast<const int32_t *>(lb + b.o_rigOff);
+    Lp.rigVert = reinterpret_cast<const int32_t *>(lb + b.o_rigVert);
+    Lp.bndOff = reinterpret_cast<const int32_t *>(lb + b.o_bndOff);
+    Lp.bndVert = reinterpret_cast<const int32_t *>(lb + b.o_bndVert);
+    Lp.vbody = reinterpret_cast<const int32_t *>(lb + b.o_vbody);
+    Lp.flags0 = reinterpret_cast<const uint8_t *>(lb + b.o_flags0);
+    Lp.bodies = b.d_bodies.as<Body>();
+    Lp.ri = b.d_ri.as<double>();
+    Lp.rv = b.d_rv.as<T>();
+    Lp.lamB = b.d_lamB.as<double>();
+    Lp.mass = m.d_mass64.as<double>();
+    Lp.coarseIters = b.d_citers.as<int32_t>();
+    Lp.it0 = st->iterations[0];
+    Lp.it1 = st->iterations[1];
+    Lp.boundaryCompliance = st->boundaryCompliance;
+  }
   auto launch_unsplit = [&]() {
+    if (b.layered) {
+      if (b.block == 256) launch_layered<T, 256>(B, Lp, ns, s);
+      else if (b.block == 768) launch_layered<T, 768>(B, Lp, ns, s);
+      else if (b.block == 1024) launch_layered<T, 1024>(B, Lp, ns, s);
+      else launch_layered<T, 512>(B, Lp, ns, s);
+      return;
+    }
     if (b.block == 256) launch_batch<T, 256>(B, ns, ldsBytes, s);
     else if (b.block == 768) launch_batch<T, 768>(B, ns, ldsBytes, s);
     else if (b.block == 1024) launch_batch<T, 1024>(B, ns, ldsBytes, s);
@@ -2871,8 +2934,8 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   };
   int *splitErr = nullptr, *splitMismatch = nullptr;
   bool replayed = false;
-  // the split schedule has a StVK-Voigt body only
-  if (b.split > 0 && !m.hasNH) {
+  // the split schedule has a StVK-Voigt plain body only
+  if (b.split > 0 && !m.hasNH && !b.layered) {
     MLX_TRY(enqueue_pre());
     // split schedule: scene sc gets G = round(nT_sc / split) workgroups (1..64)
     // on one XCD; scenes go to XCDs largest first, each to the XCD with the
@@ -3009,7 +3072,12 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
         (uint64_t)(uintptr_t)b.d_flags.ptr, (uint64_t)(uintptr_t)m.d_scan.ptr,
         (uint64_t)(uintptr_t)m.d_contacts.ptr, (uint64_t)(uintptr_t)b.d_tables.ptr,
         (uint64_t)(uintptr_t)b.d_out.ptr, (uint64_t)(uintptr_t)m.d_tmp.ptr,
-        (uint64_t)(uintptr_t)b.d_pinMask.ptr, (uint64_t)(uintptr_t)m.d_pinnedx.ptr};
+        (uint64_t)(uintptr_t)b.d_pinMask.ptr, (uint64_t)(uintptr_t)m.d_pinnedx.ptr,
+        (uint64_t)b.layered, (uint64_t)(uintptr_t)b.d_layer.ptr,
+        (uint64_t)(uintptr_t)b.d_bodies.ptr, (uint64_t)(uintptr_t)b.d_ri.ptr,
+        (uint64_t)(uintptr_t)b.d_rv.ptr, (uint64_t)(uintptr_t)b.d_lamB.ptr,
+        (uint64_t)(b.layered ? Lp.it0 : 0), (uint64_t)(b.layered ? Lp.it1 : 0),
+        bits(b.layered ? Lp.boundaryCompliance : 0.0)};
     if (!b.gexec || b.gsig != sig) {
       if (b.gexec) {
         (void)hipGraphExecDestroy(b.gexec);
@@ -3059,10 +3127,15 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   MLX_CHECK_HIP(hipMemcpy(lr.data(), B.lastResidual, 8 * (size_t)ns, hipMemcpyDeviceToHost));
   float kms = 0;
   if (!replayed) MLX_CHECK_HIP(hipEventElapsedTime(&kms, b.ev0, b.ev1));
+  std::vector<int32_t> cit(ns, 0);
+  if (b.layered)
+    MLX_CHECK_HIP(hipMemcpy(cit.data(), b.d_citers.ptr, 4 * (size_t)ns, hipMemcpyDeviceToHost));
   if (out) {
     int64_t proj = 0;
     for (int sc = 0; sc < ns; ++sc) {
-      proj += (int64_t)it[sc] * (b.elemOff[sc + 1] - b.elemOff[sc]);
+      // the coarse layer projects its elastic elements only
+      proj += (int64_t)(it[sc] - cit[sc]) * (b.elemOff[sc + 1] - b.elemOff[sc]) +
+              (b.layered ? (int64_t)cit[sc] * b.elastic0[sc] : 0);
       if (out->iterations) out->iterations[sc] = it[sc];
       if (out->lastResidual) out->lastResidual[sc] = lr[sc];
       if (out->contacts) out->contacts[sc] = (int32_t)(b.cont0[sc + 1] - b.cont0[sc]);
@@ -3168,6 +3241,12 @@ int mlx_batch_destroy(mlx_batch *b) {
   b->d_sceneNH.release();
   b->d_flags.release();
   b->d_split.release();
+  b->d_layer.release();
+  b->d_bodies.release();
+  b->d_ri.release();
+  b->d_rv.release();
+  b->d_lamB.release();
+  b->d_citers.release();
   mlx_mesh_destroy(b->m);
   delete b;
   return MLX_OK;
@@ -3197,6 +3276,142 @@ int mlx_batch_set_split(mlx_batch *b, int32_t tetsPerWorkgroup) {
     return MLX_ERR_ARG;
   }
   b->split = tetsPerWorkgroup;
+  return MLX_OK;
+}
+
+int mlx_batch_set_rigid(mlx_batch *b, const uint8_t *isRigidElement) {
+  if (!b) {
+    set_error("mlx_batch_set_rigid: null batch");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *b->m;
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  b->gsig.clear();
+  if (!isRigidElement) {
+    b->layered = false;
+    return MLX_OK;
+  }
+  const int64_t nT = m.nT, nV = m.nV, ld = m.ld;
+  const int ns = b->nScenes;
+  // rigid elements: the mask less the forced-elastic (pinned) ones
+  std::vector<uint8_t> rigid(nT);
+  for (int64_t e = 0; e < nT; ++e) rigid[e] = isRigidElement[e] && !m.forcedElastic[e];
+  // bodies: rigid elements sharing a vertex, numbered by ascending
+  // smallest element (the numbering of mexUnionFindByVertice)
+  std::vector<int32_t> parent(nT), owner(nV, -1);
+  for (int64_t e = 0; e < nT; ++e) parent[e] = (int32_t)e;
+  auto find = [&](int32_t a) {
+    while (parent[a] != a) {
+      parent[a] = parent[parent[a]];
+      a = parent[a];
+    }
+    return a;
+  };
+  for (int64_t e = 0; e < nT; ++e) {
+    if (!rigid[e]) continue;
+    for (int a = 0; a < 4; ++a) {
+      const int32_t v = m.T[4 * (size_t)e + a];
+      if (owner[v] < 0) {
+        owner[v] = (int32_t)e;
+        continue;
+      }
+      int32_t r1 = find((int32_t)e), r2 = find(owner[v]);
+      if (r1 != r2) parent[std::max(r1, r2)] = std::min(r1, r2);
+    }
+  }
+  std::vector<int32_t> sets(nT, 0), bodyOfRoot(nT, 0);
+  int nb = 0;
+  for (int64_t e = 0; e < nT; ++e) {
+    if (!rigid[e]) continue;
+    const int32_t r = find((int32_t)e);
+    if (!bodyOfRoot[r]) bodyOfRoot[r] = ++nb;
+    sets[e] = bodyOfRoot[r];
+  }
+  std::vector<uint8_t> isB(nV), isRV(nV);
+  std::vector<int32_t> vbody(nV);
+  MLX_TRY(find_boundary_vertices(m.T.data(), nT, 4, nV, 1, sets.data(), isB.data(), isRV.data(),
+                                 vbody.data()));
+  // bodies per scene: a body lies in one scene (scenes share no vertex) and
+  // bodies are numbered in ascending smallest element, so each scene's
+  // bodies are consecutive
+  std::vector<int32_t> body0(ns + 1, 0);
+  b->elastic0.assign(ns, 0);
+  for (int sc = 0; sc < ns; ++sc) {
+    int32_t lo = INT32_MAX, hi = 0;
+    for (int64_t e = b->elemOff[sc]; e < b->elemOff[sc + 1]; ++e) {
+      if (sets[e]) {
+        lo = std::min(lo, sets[e]);
+        hi = std::max(hi, sets[e]);
+      } else {
+        b->elastic0[sc] += 1;
+      }
+    }
+    const int32_t cnt = hi ? hi - lo + 1 : 0;
+    if (cnt > kMaxSceneBodies) {
+      set_error("mlx_batch_set_rigid: scene %d has %d rigid bodies (at most %d)", sc, cnt,
+                kMaxSceneBodies);
+      return MLX_ERR_UNSUPPORTED;
+    }
+    if (cnt && lo - 1 != body0[sc]) {
+      set_error("mlx_batch_set_rigid: bodies of scene %d are not consecutive", sc);
+      return MLX_ERR_STATE;
+    }
+    body0[sc + 1] = body0[sc] + cnt;
+  }
+  // rigid and boundary vertices of each body, ascending
+  std::vector<int32_t> rigOff(nb + 1, 0), bndOff(nb + 1, 0);
+  for (int64_t v = 0; v < nV; ++v)
+    if (vbody[v]) {
+      rigOff[vbody[v]] += 1;
+      if (isB[v]) bndOff[vbody[v]] += 1;
+    }
+  for (int k = 0; k < nb; ++k) {
+    rigOff[k + 1] += rigOff[k];
+    bndOff[k + 1] += bndOff[k];
+  }
+  std::vector<int32_t> rigVert(std::max(rigOff[nb], 1)), bndVert(std::max(bndOff[nb], 1));
+  {
+    std::vector<int32_t> fr(rigOff.begin(), rigOff.end() - 1), fb(bndOff.begin(), bndOff.end() - 1);
+    for (int64_t v = 0; v < nV; ++v)
+      if (vbody[v]) {
+        rigVert[fr[vbody[v] - 1]++] = (int32_t)v;
+        if (isB[v]) bndVert[fb[vbody[v] - 1]++] = (int32_t)v;
+      }
+  }
+  std::vector<uint8_t> flags0(ld, 1);
+  for (int64_t e = 0; e < nT; ++e) flags0[m.slotOf[e]] = sets[e] == 0;
+  auto al = [](size_t o) { return (o + 255) / 256 * 256; };
+  b->o_body0 = 0;
+  b->o_rigOff = al(b->o_body0 + 4 * body0.size());
+  b->o_rigVert = al(b->o_rigOff + 4 * rigOff.size());
+  b->o_bndOff = al(b->o_rigVert + 4 * rigVert.size());
+  b->o_bndVert = al(b->o_bndOff + 4 * bndOff.size());
+  b->o_vbody = al(b->o_bndVert + 4 * bndVert.size());
+  b->o_flags0 = al(b->o_vbody + 4 * (size_t)nV);
+  const size_t total = al(b->o_flags0 + (size_t)ld);
+  std::vector<char> buf(total, 0);
+  std::memcpy(buf.data() + b->o_body0, body0.data(), 4 * body0.size());
+  std::memcpy(buf.data() + b->o_rigOff, rigOff.data(), 4 * rigOff.size());
+  std::memcpy(buf.data() + b->o_rigVert, rigVert.data(), 4 * rigVert.size());
+  std::memcpy(buf.data() + b->o_bndOff, bndOff.data(), 4 * bndOff.size());
+  std::memcpy(buf.data() + b->o_bndVert, bndVert.data(), 4 * bndVert.size());
+  std::memcpy(buf.data() + b->o_vbody, vbody.data(), 4 * (size_t)nV);
+  std::memcpy(buf.data() + b->o_flags0, flags0.data(), (size_t)ld);
+  MLX_TRY(b->d_layer.reserve(total));
+  MLX_CHECK_HIP(hipMemcpy(b->d_layer.ptr, buf.data(), total, hipMemcpyHostToDevice));
+  const size_t tb = m.precision == MLX_FP64 ? 8 : 4;
+  MLX_TRY(b->d_bodies.reserve(sizeof(Body) * std::max(nb, 1)));
+  MLX_TRY(b->d_ri.reserve(sizeof(double) * 3 * (size_t)nV));
+  MLX_TRY(b->d_rv.reserve(tb * 3 * (size_t)nV));
+  MLX_TRY(b->d_lamB.reserve(sizeof(double) * std::max<size_t>(bndOff[nb], 1)));
+  MLX_TRY(b->d_citers.reserve(4 * (size_t)ns));
+  MLX_CHECK_HIP(hipMemset(b->d_citers.ptr, 0, 4 * (size_t)ns));
+  b->nBodies = nb;
+  b->layered = true;
   return MLX_OK;
 }
 

@@ -1295,10 +1295,27 @@ static void layers_free(layers_t *L) {
 /* buildLayers (xpbdLayer3D.m:601-619; 2D xpbdLayer2D.m:740-780 ordering type
  * 0) followed by findBoundaryVertices. */
 static void build_layers(const ora_mesh *m, const ora_settings *s, const double *pa,
-                         const double *pb, double h, const double *keyScale, layers_t *L) {
+                         const double *pb, double h, const double *keyScale,
+                         const int32_t *givenSets, layers_t *L) {
   int nT = m->nT, nV = m->nV;
   L->nSorted = ora_sorted_layers(s->layers, s->nLayers, L->sorted, L->map);
   int ns = L->nSorted;
+  if (givenSets) {
+    L->sets = (int32_t *)malloc(sizeof(int32_t) * (size_t)nT * ns + 4);
+    L->numRigids = (int32_t *)malloc(sizeof(int32_t) * (ns + 1));
+    memcpy(L->sets, givenSets, sizeof(int32_t) * (size_t)nT * ns);
+    for (int l = 0; l < ns; ++l) {
+      int mx = 0;
+      for (int e = 0; e < nT; ++e) mx = L->sets[(size_t)nT * l + e] > mx ? L->sets[(size_t)nT * l + e] : mx;
+      L->numRigids[l] = mx;
+    }
+    L->vertexRigidBody = (int32_t *)malloc(sizeof(int32_t) * (size_t)nV * ns + 4);
+    L->isBoundary = (uint8_t *)malloc((size_t)nV * ns + 4);
+    L->isRigidVert = (uint8_t *)malloc((size_t)nV * ns + 4);
+    ora_find_boundary_vertices(nV, nT, m->vpe, m->T, ns, L->sets, L->isBoundary,
+                               L->isRigidVert, L->vertexRigidBody);
+    return;
+  }
   double *Fa = (double *)malloc(sizeof(double) * 9 * (nT > 0 ? nT : 1));
   double *Fb = (double *)malloc(sizeof(double) * 9 * (nT > 0 ? nT : 1));
   double *ed = (double *)malloc(sizeof(double) * (nT > 0 ? nT : 1));
@@ -1495,7 +1512,8 @@ int ora_integrate3d(const ora_mesh *m, ora_settings *s, double h, int frame,
   double *f = (double *)calloc(N3, sizeof(double));
   for (int i = 0; i < nV; ++i) f[3 * i + 2] = f[3 * i + 2] + m->mass[3 * i + 2] * s->gravity;
   layers_t L;
-  build_layers(m, s, p, oldp, h, out ? out->learnKeyScale : NULL, &L);
+  build_layers(m, s, p, oldp, h, out ? out->learnKeyScale : NULL, out ? out->givenSets : NULL,
+               &L);
   if (out) out->learnKept = -1;
   if (out && out->setsPerLayer)
     memcpy(out->setsPerLayer, L.sets, sizeof(int32_t) * (size_t)nT * L.nSorted);
@@ -1885,7 +1903,7 @@ int ora_integrate2d(const ora_mesh *m, ora_settings *s, double h, double *p,
   double *f = (double *)calloc(N2, sizeof(double));
   for (int i = 0; i < nV; ++i) f[2 * i + 1] = f[2 * i + 1] + m->mass[2 * i + 1] * s->gravity;
   layers_t L;
-  build_layers(m, s, p, prevPrevp, h, NULL, &L);
+  build_layers(m, s, p, prevPrevp, h, NULL, NULL, &L);
   if (out && out->setsPerLayer)
     memcpy(out->setsPerLayer, L.sets, sizeof(int32_t) * (size_t)nT * L.nSorted);
   if (out && out->numRigids) memcpy(out->numRigids, L.numRigids, sizeof(int32_t) * L.nSorted);

@@ -92,6 +92,11 @@ typedef struct {
   const double *learnKeyScale; /* nT */
   int learnKept;
   double learnRes[2];          /* out: equation residual before / after */
+  /* in, optional (3D): the layers' rigid body ids (nT*nSorted, column l for
+   * sorted layer l, 0 = elastic, bodies 1-based) used instead of the
+   * strain-rate layers: layerSolve with maps handed in, as its signature
+   * allows (xpbdLayer3D.m:44-93) */
+  const int32_t *givenSets;
 } ora_step_out;
 
 /* Threads for the per-color element loop of the 3D sweep and the 3D

@@ -53,7 +53,7 @@ class OraStepOut(C.Structure):
         ("setsPerLayer", _i32), ("numRigids", _i32),
         ("normresv", C.c_double * MAX_LAYERS), ("runUntilAtEntry", C.c_double),
         ("learnLam0", _d), ("learnKeyScale", _d), ("learnKept", C.c_int),
-        ("learnRes", C.c_double * 2),
+        ("learnRes", C.c_double * 2), ("givenSets", _i32),
     ]
 
 
@@ -265,12 +265,15 @@ class OracleSim:
         oc.compliance = P(c["compliance"])
         return oc, keep
 
-    def step(self, lam0=None, key_scale=None):
+    def step(self, lam0=None, key_scale=None, sets=None):
         """One substep; lam0 (nT x 6) and key_scale (nT) apply the learned
-        warm start and key factors as the product's learned mode does."""
+        warm start and key factors as the product's learned mode does; sets
+        (nT x nSorted body ids, 0 elastic) replaces the strain-rate layers."""
         self.frame += 1
         oc, keep = self._contacts()
         out = OraStepOut()
+        gs = None if sets is None else np.ascontiguousarray(np.asarray(sets, dtype=np.int32).T)
+        out.givenSets = P(gs, C.c_int32)
         l0 = None if lam0 is None else np.ascontiguousarray(lam0, dtype=np.float64).reshape(-1)
         ks = None if key_scale is None else np.ascontiguousarray(key_scale, dtype=np.float64)
         out.learnLam0 = P(l0)

@@ -1,0 +1,136 @@
+"""Batched scenes with rigid clusters (mlx_batch_set_rigid: per scene a
+coarse layer whose clusters move as rigid bodies, then the all-elastic
+layer, in one launch per substep) against the oracle run scene by scene
+with the same layer maps (ora_step_out.givenSets); StVK-Voigt and
+NeoHookean scenes alternate as in the batched workload."""
+import copy
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel
+from multilayersolver2024_amd import _lib as L
+from multilayersolver2024_amd.batch import (BatchSimulation, cluster_workload, merge_meshes,
+                                            rigid_components)
+from multilayersolver2024_amd.integrator import PlaneContactFinder, XpbdLayer3D
+from multilayersolver2024_amd.mesh import Mesh
+
+pytestmark = pytest.mark.gpu
+
+H = 1 / 60
+
+
+def _say(*a):
+    print("[batch-layered]", *a, flush=True)
+
+
+def _integ(iterations=(3, 10), pct=99.99, hanging=60):
+    it = XpbdLayer3D()
+    it.Gravity = -9.81
+    it.setLayers([50, 0])  # the coarse layer's rigid set comes from the clusters
+    it.iterations = list(iterations)
+    it.computeResiduals = True
+    it.StopAtImprovementPercent = pct
+    it.hangingStop = hanging
+    it.useRigidContacts = False
+    return it
+
+
+def _sets(mesh, mask):
+    """Oracle layer maps: sorted layers [0, 50] -> column 0 all elastic,
+    column 1 the clusters' bodies."""
+    body = rigid_components(mesh, mask)
+    return np.stack([np.zeros_like(body), body], axis=1), int(body.max())
+
+
+def _run_against_oracle(scs, masks, it, substeps, block, precision=L.MLX_FP64, check=None,
+                        tol_p=1e-9, tol_l=1e-8, h=H):
+    meshes = [Mesh(s) for s in scs]
+    merged = merge_meshes(meshes)
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    b = BatchSimulation(merged, it, h, plane, precision=precision, block=block)
+    b.set_rigid(np.concatenate(masks))
+    check = range(len(meshes)) if check is None else check
+    osims = {s: O.OracleSim(meshes[s], copy.deepcopy(it), h, plane) for s in check}
+    sets = {s: _sets(meshes[s], masks[s]) for s in check}
+    bodies = [sets[s][1] for s in check]
+    try:
+        for k in range(substeps):
+            b.step()
+            p, v, lam = b.download()
+            assert np.isfinite(p).all() and np.isfinite(v).all()
+            for s in check:
+                r = osims[s].step(sets=sets[s][0])
+                v0, v1 = merged.vertOff[s], merged.vertOff[s + 1]
+                e0, e1 = merged.elemOff[s], merged.elemOff[s + 1]
+                ep, el = rel(p[3 * v0:3 * v1], osims[s].p), rel(lam[e0:e1], osims[s].lambdas())
+                _say(f"substep {k + 1} scene {s} ({meshes[s].nT} tets, {sets[s][1]} bodies, "
+                     f"nh {bool(meshes[s].neohookean.any())}): iters {b.iters[s]}/{r['iters']} "
+                     f"contacts {b.contacts[s]}/{osims[s].last_contacts} pos {ep:.2e} lam {el:.2e}")
+                assert b.iters[s] == sum(r["iters"])
+                assert b.contacts[s] == osims[s].last_contacts
+                assert ep < tol_p and el < tol_l
+    finally:
+        b.close()
+    return bodies
+
+
+@pytest.mark.parametrize("block,h", [(256, 1 / 60), (512, 1 / 60), (256, 1 / 6000)])
+def test_layered_batch_fp64_matches_oracle_per_scene(block, h, oracle_lib):
+    """Six scenes (1..3 bodies each, every other one NeoHookean) in one
+    batch.  At the workload's step (1/60 s) the NeoHookean pair does not
+    converge: its iterations run to hangingStop and blow rounding
+    differences up to O(1) within about eight of them, in the plain batch and
+    the single-scene path alike, so there only the StVK-Voigt scenes are
+    compared; at h = 1/6000 the pair converges and every scene is."""
+    scs, masks = cluster_workload(6, seed=0, tets=(400, 2000))
+    for sc in scs:
+        sc["p"] = sc["p"] - np.array([0, 0, 0.045])  # touch down within the run
+    check = [0, 2, 4] if h > 1e-3 else None
+    substeps = 8 if h > 1e-3 else 4
+    bodies = _run_against_oracle(scs, masks, _integ(hanging=60), substeps, block, h=h,
+                                 check=check)
+    assert set(bodies) >= ({1, 2} if check else {1, 2, 3})
+
+
+def test_layered_batch_large_scenes_fp64(oracle_lib):
+    """The batched workload's scene shapes at full size, three scenes: the
+    smallest, the median and the largest StVK-Voigt scenes of the 256 (by
+    tet count), one launch, FP64 against the oracle over the first
+    substeps (free fall and touchdown)."""
+    scs, masks = cluster_workload(256, seed=0)
+    stvk = [i for i in range(256) if i % 2 == 0]
+    by_size = sorted(stvk, key=lambda i: len(scs[i]["t"]))
+    pick = [by_size[0], by_size[len(by_size) // 2], by_size[-1]]
+    sel_s = [scs[i] for i in pick]
+    sel_m = [masks[i] for i in pick]
+    for sc in sel_s:
+        sc["p"] = sc["p"] - np.array([0, 0, 0.04])
+    _say("scenes", pick, [len(s["t"]) for s in sel_s])
+    _run_against_oracle(sel_s, sel_m, _integ(iterations=(2, 18), hanging=100), 4, 256)
+
+
+def test_layered_batch_fp32_full_workload_runs():
+    """The 256-scene workload with clusters and the NeoHookean half, FP32,
+    a few substeps: every scene finite, the coarse layer ran with bodies,
+    NeoHookean scenes stop at hangingStop (their pair does not converge at
+    this step) and StVK-Voigt scenes by the relative stop."""
+    scs, masks = cluster_workload(256, seed=0)
+    merged = merge_meshes([Mesh(s) for s in scs])
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    it = _integ(iterations=(2, 18), hanging=100)
+    b = BatchSimulation(merged, it, H, plane, precision=L.MLX_FP32, block=256)
+    b.set_rigid(np.concatenate(masks))
+    try:
+        for k in range(3):
+            proj, kms = b.step()
+            p, v, _ = b.download()
+            assert np.isfinite(p).all() and np.isfinite(v).all()
+            nh_it = b.iters[1::2]
+            st_it = b.iters[0::2]
+            _say(f"substep {k + 1}: {proj} projections, NH iterations {nh_it.min()}..{nh_it.max()}, "
+                 f"StVK {st_it.min()}..{st_it.max()}")
+            assert proj > 0
+    finally:
+        b.close()
