@@ -391,6 +391,18 @@ int mlx_batch_set_block(mlx_batch *batch, int32_t threads);
  * NULL returns to the plain method.  The layered launch keeps positions in
  * HBM and runs one workgroup per scene. */
 int mlx_batch_set_rigid(mlx_batch *batch, const uint8_t *isRigidElement);
+/* Block-size autotuner per color bucket: scenes are grouped by their mean
+ * elements per color (< 256, < 1024, more), and each group is launched with
+ * its own block size on its own stream (forked from and joined to the
+ * batch's stream, captured with the rest of the substep).  Every candidate
+ * (256, 512, 768, 1024 threads; 256 and 512 with rigid clusters) runs one
+ * substep of its group alone from the current state, timed with HIP events;
+ * the fastest is kept per group and the state restored.  Outputs (3 each,
+ * optional): chosen threads (0: empty group), its kernel ms, scenes per
+ * group.  mlx_batch_set_block returns to one launch at one block size. */
+int mlx_batch_autotune(mlx_batch *batch, mlx_settings *settings, double h, int32_t frame,
+                       const mlx_plane *plane, int32_t *bucketBlocks, float *bucketMs,
+                       int32_t *bucketScenes);
 /* (Re)starts all scenes from p, v (3nV of the merged mesh). */
 int mlx_batch_set_state(mlx_batch *batch, const double *p, const double *v);
 int mlx_batch_download(mlx_batch *batch, double *p, double *v, double *lambdai);
@@ -400,7 +412,7 @@ typedef struct {
   double *lastResidual;       /* optional nScenes */
   int32_t *contacts;          /* optional nScenes */
   int64_t elasticProjections; /* all scenes */
-  double kernelMs;            /* duration of the batched solve kernel (0 for a graph replay) */
+  double kernelMs;            /* duration of the batched solve (a replay: of the whole graph) */
 } mlx_batch_out;
 
 /* One integrate() substep of every scene (xpbdLayer3D.m:621-745 per scene),
@@ -425,8 +437,9 @@ int mlx_batch_set_lds(mlx_batch *batch, int32_t enable);
  * (contacts, prologue, the solve, velocity update) when the batch is not
  * split; it is captured again whenever a setting baked into its kernels
  * changes.  enable = 0 launches every kernel directly (the default is 1).
- * mlx_batch_out.kernelMs is measured only for directly launched substeps
- * (0 for a replay).  No counterpart in the reference. */
+ * mlx_batch_out.kernelMs is the solve kernel's time for directly launched
+ * substeps and the whole graph's for a replay.  No counterpart in the
+ * reference. */
 int mlx_batch_set_graph(mlx_batch *batch, int32_t enable);
 
 int mlx_batch_integrate(mlx_batch *batch, mlx_settings *settings, double h,

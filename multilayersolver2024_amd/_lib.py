@@ -37,7 +37,7 @@ EXPORTED = (
     "mlx_mlp_create", "mlx_mlp_destroy", "mlx_mlp_set", "mlx_mlp_get", "mlx_mlp_infer",
     "mlx_mlp_grad", "mlx_mlp_train", "mlx_mesh_set_learned", "mlx_learn_stats",
     "mlx_learn_outputs", "mlx_learn_features", "mlx_state_set", "mlx_state_get_oldp",
-    "mlx_batch_set_rigid",
+    "mlx_batch_set_rigid", "mlx_batch_autotune",
 )
 MLX_LEARN_WARMSTART = 1
 MLX_LEARN_KEYS = 2
@@ -200,6 +200,8 @@ def lib():
         "mlx_state_set": [_p, _d, _d, _d, _d],
         "mlx_state_get_oldp": [_p, _d],
         "mlx_batch_set_rigid": [_p, _u8],
+        "mlx_batch_autotune": [_p, C.POINTER(Settings), C.c_double, C.c_int32, C.POINTER(Plane),
+                               _i32, _f, _i32],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -105,6 +105,19 @@ class BatchSimulation:
     def set_block(self, threads):
         L.check("mlx_batch_set_block", L.lib().mlx_batch_set_block(self.handle, int(threads)))
 
+    def autotune(self):
+        """Per-bucket block sizes from one timed substep per candidate (the
+        next substep's state, restored afterwards).  Returns a list of
+        (threads, kernel ms, scenes) per bucket."""
+        blk = np.zeros(3, np.int32)
+        ms = np.zeros(3, np.float32)
+        cnt = np.zeros(3, np.int32)
+        pl = C.byref(self._plane) if self._plane is not None else None
+        L.check("mlx_batch_autotune", L.lib().mlx_batch_autotune(
+            self.handle, C.byref(self._settings), self.h, self.frame + 1, pl,
+            L.ptr(blk, C.c_int32), L.ptr(ms, C.c_float), L.ptr(cnt, C.c_int32)))
+        return [(int(a), float(b_), int(c)) for a, b_, c in zip(blk, ms, cnt)]
+
     def set_rigid(self, rigid):
         """Rigid clusters of the coarse layer (merged element order; None
         returns to the plain method).  The integrator then needs two layers,

@@ -377,7 +377,7 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_batch_layered(BatchParams<T> B, LayeredParams<T> Lp) {
   __shared__ double shw[12 * (NT / 64)];
-  const int sc = blockIdx.x;
+  const int sc = B.sceneList ? B.sceneList[blockIdx.x] : (int)blockIdx.x;
   if (B.sceneNH && B.sceneNH[sc])
     batch_scene_layered<T, NT, kMatNH>(B, Lp, sc, shw);
   else

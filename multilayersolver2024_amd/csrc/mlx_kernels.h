@@ -2173,6 +2173,7 @@ struct BatchParams {
   double *lastResidual;
   double *runUntilOut;
   int32_t *residualCount;
+  const int32_t *sceneList;      // scene of each workgroup (null: workgroup i runs scene i)
 };
 
 // Workgroup sum of one double per thread in a fixed order: a butterfly
@@ -2457,7 +2458,7 @@ template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_batch_plain(BatchParams<T> B) {
   __shared__ double sh[NT / 64];
   extern __shared__ __align__(16) unsigned char mlx_dyn_lds[];
-  const int sc = blockIdx.x;
+  const int sc = B.sceneList ? B.sceneList[blockIdx.x] : (int)blockIdx.x;
   const bool nh = B.sceneNH && B.sceneNH[sc];  // the scene is all NeoHookean
   if (B.vert0[sc + 1] - B.vert0[sc] <= B.ldsVerts) {
     T *pos = reinterpret_cast<T *>(mlx_dyn_lds);

@@ -2634,6 +2634,18 @@ struct mlx_batch {
   DevBuf d_bodies, d_ri, d_rv, d_lamB;
   DevBuf d_citers;                   // int32 per scene: iterations of the coarse layer
   std::vector<int64_t> elastic0;     // per scene: elastic elements of the coarse layer
+  // per-bucket launches (mlx_batch_autotune): scenes grouped by their mean
+  // elements per color, each group launched with its own block size on its
+  // own stream, the streams forked from and joined to the batch's stream
+  static constexpr int kBuckets = 3;
+  bool useBuckets = false;
+  int bucketBlock[kBuckets] = {512, 512, 512};
+  int bucketOff[kBuckets + 1] = {0, 0, 0, 0};
+  int64_t bucketMaxVerts[kBuckets] = {0, 0, 0};
+  DevBuf d_bucket;                   // int32 scene lists, bucket after bucket
+  hipStream_t bstream[kBuckets] = {nullptr, nullptr, nullptr};
+  hipEvent_t bfork = nullptr, bjoin[kBuckets] = {nullptr, nullptr, nullptr};
+  int tuneOnly = -1;                 // while tuning: launch this bucket only
 };
 
 namespace mlx {
@@ -2895,6 +2907,7 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   B.lastResidual = reinterpret_cast<double *>(ob + (4 * (size_t)ns + 7) / 8 * 8);
   B.runUntilOut = B.lastResidual + ns;
   B.residualCount = reinterpret_cast<int32_t *>(B.runUntilOut + ns);
+  B.sceneList = nullptr;
   if (!b.ev0) {
     MLX_CHECK_HIP(hipEventCreate(&b.ev0));
     MLX_CHECK_HIP(hipEventCreate(&b.ev1));
@@ -2919,7 +2932,39 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     Lp.it1 = st->iterations[1];
     Lp.boundaryCompliance = st->boundaryCompliance;
   }
+  // one bucket's launch: its scene list, block size and LDS share
+  auto launch_bucket = [&](int i, hipStream_t bs) {
+    BatchParams<T> Bi = B;
+    Bi.sceneList = b.d_bucket.as<int32_t>() + b.bucketOff[i];
+    const int n = b.bucketOff[i + 1] - b.bucketOff[i];
+    const int blk = b.bucketBlock[i];
+    if (b.layered) {
+      if (blk == 256) launch_layered<T, 256>(Bi, Lp, n, bs);
+      else launch_layered<T, 512>(Bi, Lp, n, bs);
+      return;
+    }
+    size_t lb = 0;
+    Bi.ldsVerts = blk == 256    ? batch_lds_verts<T, 256>(b.bucketMaxVerts[i], b.useLds, &lb)
+                  : blk == 768  ? batch_lds_verts<T, 768>(b.bucketMaxVerts[i], b.useLds, &lb)
+                  : blk == 1024 ? batch_lds_verts<T, 1024>(b.bucketMaxVerts[i], b.useLds, &lb)
+                                : batch_lds_verts<T, 512>(b.bucketMaxVerts[i], b.useLds, &lb);
+    if (blk == 256) launch_batch<T, 256>(Bi, n, lb, bs);
+    else if (blk == 768) launch_batch<T, 768>(Bi, n, lb, bs);
+    else if (blk == 1024) launch_batch<T, 1024>(Bi, n, lb, bs);
+    else launch_batch<T, 512>(Bi, n, lb, bs);
+  };
   auto launch_unsplit = [&]() {
+    if (b.useBuckets) {
+      (void)hipEventRecord(b.bfork, s);
+      for (int i = 0; i < mlx_batch::kBuckets; ++i) {
+        if (b.bucketOff[i + 1] == b.bucketOff[i] || (b.tuneOnly >= 0 && b.tuneOnly != i)) continue;
+        (void)hipStreamWaitEvent(b.bstream[i], b.bfork, 0);
+        launch_bucket(i, b.bstream[i]);
+        (void)hipEventRecord(b.bjoin[i], b.bstream[i]);
+        (void)hipStreamWaitEvent(s, b.bjoin[i], 0);
+      }
+      return;
+    }
     if (b.layered) {
       if (b.block == 256) launch_layered<T, 256>(B, Lp, ns, s);
       else if (b.block == 768) launch_layered<T, 768>(B, Lp, ns, s);
@@ -2935,7 +2980,7 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   int *splitErr = nullptr, *splitMismatch = nullptr;
   bool replayed = false;
   // the split schedule has a StVK-Voigt plain body only
-  if (b.split > 0 && !m.hasNH && !b.layered) {
+  if (b.split > 0 && !m.hasNH && !b.layered && !b.useBuckets) {
     MLX_TRY(enqueue_pre());
     // split schedule: scene sc gets G = round(nT_sc / split) workgroups (1..64)
     // on one XCD; scenes go to XCDs largest first, each to the XCD with the
@@ -3077,7 +3122,9 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
         (uint64_t)(uintptr_t)b.d_bodies.ptr, (uint64_t)(uintptr_t)b.d_ri.ptr,
         (uint64_t)(uintptr_t)b.d_rv.ptr, (uint64_t)(uintptr_t)b.d_lamB.ptr,
         (uint64_t)(b.layered ? Lp.it0 : 0), (uint64_t)(b.layered ? Lp.it1 : 0),
-        bits(b.layered ? Lp.boundaryCompliance : 0.0)};
+        bits(b.layered ? Lp.boundaryCompliance : 0.0), (uint64_t)b.useBuckets,
+        (uint64_t)b.bucketBlock[0], (uint64_t)b.bucketBlock[1], (uint64_t)b.bucketBlock[2],
+        (uint64_t)(uintptr_t)b.d_bucket.ptr, (uint64_t)b.useLds};
     if (!b.gexec || b.gsig != sig) {
       if (b.gexec) {
         (void)hipGraphExecDestroy(b.gexec);
@@ -3100,7 +3147,9 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
       MLX_CHECK_HIP(ie);
       b.gsig = sig;
     }
+    MLX_CHECK_HIP(hipEventRecord(b.ev0, s));
     MLX_CHECK_HIP(hipGraphLaunch(b.gexec, s));
+    MLX_CHECK_HIP(hipEventRecord(b.ev1, s));
     replayed = true;
   } else {
     MLX_TRY(enqueue_pre());
@@ -3126,7 +3175,10 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
   MLX_CHECK_HIP(hipMemcpy(it.data(), B.iters, 4 * (size_t)ns, hipMemcpyDeviceToHost));
   MLX_CHECK_HIP(hipMemcpy(lr.data(), B.lastResidual, 8 * (size_t)ns, hipMemcpyDeviceToHost));
   float kms = 0;
-  if (!replayed) MLX_CHECK_HIP(hipEventElapsedTime(&kms, b.ev0, b.ev1));
+  // direct launches: the solve kernel(s); a replay: the whole graph (its
+  // small pre- and post-solve kernels included)
+  MLX_CHECK_HIP(hipEventElapsedTime(&kms, b.ev0, b.ev1));
+  (void)replayed;
   std::vector<int32_t> cit(ns, 0);
   if (b.layered)
     MLX_CHECK_HIP(hipMemcpy(cit.data(), b.d_citers.ptr, 4 * (size_t)ns, hipMemcpyDeviceToHost));
@@ -3247,6 +3299,12 @@ int mlx_batch_destroy(mlx_batch *b) {
   b->d_rv.release();
   b->d_lamB.release();
   b->d_citers.release();
+  b->d_bucket.release();
+  for (int i = 0; i < mlx_batch::kBuckets; ++i) {
+    if (b->bjoin[i]) (void)hipEventDestroy(b->bjoin[i]);
+    if (b->bstream[i]) (void)hipStreamDestroy(b->bstream[i]);
+  }
+  if (b->bfork) (void)hipEventDestroy(b->bfork);
   mlx_mesh_destroy(b->m);
   delete b;
   return MLX_OK;
@@ -3421,6 +3479,7 @@ int mlx_batch_set_block(mlx_batch *b, int32_t threads) {
     return MLX_ERR_ARG;
   }
   b->block = threads;
+  b->useBuckets = false;
   return MLX_OK;
 }
 
@@ -3464,6 +3523,103 @@ int mlx_batch_download(mlx_batch *b, double *p, double *v, double *lambdai) {
     if (lambdai) MLX_TRY(download_lambda<float>(m, lambdai));
   }
   return MLX_OK;
+}
+
+int mlx_batch_autotune(mlx_batch *b, mlx_settings *st, double h, int32_t frame,
+                       const mlx_plane *plane, int32_t *bucketBlocks, float *bucketMs,
+                       int32_t *bucketScenes) {
+  if (!b || !st || !(h > 0) || frame < 1) {
+    set_error("mlx_batch_autotune: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *b->m;
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  if (!b->d_v.ptr) {
+    set_error("mlx_batch_autotune: call mlx_batch_set_state first");
+    return MLX_ERR_STATE;
+  }
+  const int ns = b->nScenes, K = mlx_batch::kBuckets;
+  // buckets by mean elements per color: < 256, < 1024, the rest
+  std::vector<int32_t> lists[K];
+  for (int i = 0; i < K; ++i) b->bucketMaxVerts[i] = 0;
+  for (int sc = 0; sc < ns; ++sc) {
+    const double per = (double)(b->elemOff[sc + 1] - b->elemOff[sc]) /
+                       std::max(1, m.sceneNColors[sc]);
+    const int i = per < 256 ? 0 : per < 1024 ? 1 : 2;
+    lists[i].push_back(sc);
+    b->bucketMaxVerts[i] = std::max<int64_t>(b->bucketMaxVerts[i],
+                                             b->vertOff[sc + 1] - b->vertOff[sc]);
+  }
+  std::vector<int32_t> all;
+  for (int i = 0; i < K; ++i) {
+    b->bucketOff[i] = (int)all.size();
+    all.insert(all.end(), lists[i].begin(), lists[i].end());
+  }
+  b->bucketOff[K] = (int)all.size();
+  MLX_TRY(b->d_bucket.reserve(sizeof(int32_t) * all.size()));
+  MLX_CHECK_HIP(hipMemcpy(b->d_bucket.ptr, all.data(), sizeof(int32_t) * all.size(),
+                          hipMemcpyHostToDevice));
+  for (int i = 0; i < K; ++i) {
+    if (!b->bstream[i]) MLX_CHECK_HIP(hipStreamCreateWithFlags(&b->bstream[i], hipStreamNonBlocking));
+    if (!b->bjoin[i]) MLX_CHECK_HIP(hipEventCreateWithFlags(&b->bjoin[i], hipEventDisableTiming));
+  }
+  if (!b->bfork) MLX_CHECK_HIP(hipEventCreateWithFlags(&b->bfork, hipEventDisableTiming));
+  // the state every candidate starts from
+  const size_t tb = m.precision == MLX_FP64 ? 8 : 4;
+  DevBuf sx, sv;
+  MLX_TRY(sx.reserve(4 * tb * (size_t)m.nV));
+  MLX_TRY(sv.reserve(3 * tb * (size_t)m.nV));
+  MLX_CHECK_HIP(hipMemcpy(sx.ptr, m.d_x.ptr, 4 * tb * (size_t)m.nV, hipMemcpyDeviceToDevice));
+  MLX_CHECK_HIP(hipMemcpy(sv.ptr, b->d_v.ptr, 3 * tb * (size_t)m.nV, hipMemcpyDeviceToDevice));
+  auto restore = [&]() -> int {
+    MLX_CHECK_HIP(hipMemcpy(m.d_x.ptr, sx.ptr, 4 * tb * (size_t)m.nV, hipMemcpyDeviceToDevice));
+    MLX_CHECK_HIP(hipMemcpy(b->d_v.ptr, sv.ptr, 3 * tb * (size_t)m.nV, hipMemcpyDeviceToDevice));
+    return MLX_OK;
+  };
+  const int useGraph = b->useGraph;
+  b->useGraph = 0;  // candidates launch directly: their kernel time is measured
+  b->useBuckets = true;
+  const std::vector<int> cands = b->layered ? std::vector<int>{256, 512}
+                                            : std::vector<int>{256, 512, 768, 1024};
+  mlx_settings st0 = *st;
+  int r = MLX_OK;
+  for (int i = 0; i < K && r == MLX_OK; ++i) {
+    if (bucketScenes) bucketScenes[i] = b->bucketOff[i + 1] - b->bucketOff[i];
+    if (bucketBlocks) bucketBlocks[i] = 0;
+    if (bucketMs) bucketMs[i] = 0;
+    if (b->bucketOff[i + 1] == b->bucketOff[i]) continue;
+    int best = cands[0];
+    float bestMs = 0;
+    for (int c : cands) {
+      if ((r = restore()) != MLX_OK) break;
+      b->bucketBlock[i] = c;
+      b->tuneOnly = i;
+      mlx_settings sti = st0;
+      mlx_batch_out o;
+      std::memset(&o, 0, sizeof o);
+      r = m.precision == MLX_FP64 ? batch_integrate_impl<double>(*b, &sti, h, frame, plane, &o)
+                                  : batch_integrate_impl<float>(*b, &sti, h, frame, plane, &o);
+      if (r != MLX_OK) break;
+      if (bestMs == 0 || o.kernelMs < bestMs) {
+        bestMs = o.kernelMs;
+        best = c;
+      }
+    }
+    b->bucketBlock[i] = best;
+    if (bucketBlocks) bucketBlocks[i] = best;
+    if (bucketMs) bucketMs[i] = bestMs;
+  }
+  b->tuneOnly = -1;
+  b->useGraph = useGraph;
+  const int r2 = restore();
+  sx.release();
+  sv.release();
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  b->gsig.clear();
+  return r != MLX_OK ? r : r2;
 }
 
 int mlx_batch_integrate(mlx_batch *b, mlx_settings *st, double h, int32_t frame,

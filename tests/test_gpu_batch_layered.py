@@ -45,15 +45,21 @@ def _sets(mesh, mask):
 
 
 def _run_against_oracle(scs, masks, it, substeps, block, precision=L.MLX_FP64, check=None,
-                        tol_p=1e-9, tol_l=1e-8, h=H):
+                        tol_p=1e-9, tol_l=1e-8, h=H, autotune=False):
     meshes = [Mesh(s) for s in scs]
     merged = merge_meshes(meshes)
     plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
     b = BatchSimulation(merged, it, h, plane, precision=precision, block=block)
-    b.set_rigid(np.concatenate(masks))
+    if masks is not None:
+        b.set_rigid(np.concatenate(masks))
+    if autotune:
+        tuned = b.autotune()
+        _say("autotune", tuned)
+        assert [t[2] for t in tuned] == [1, 1, 1] and all(t[0] > 0 for t in tuned)
     check = range(len(meshes)) if check is None else check
     osims = {s: O.OracleSim(meshes[s], copy.deepcopy(it), h, plane) for s in check}
-    sets = {s: _sets(meshes[s], masks[s]) for s in check}
+    sets = {s: _sets(meshes[s], masks[s] if masks is not None else np.zeros(meshes[s].nT, bool))
+            for s in check}
     bodies = [sets[s][1] for s in check]
     try:
         for k in range(substeps):
@@ -61,7 +67,7 @@ def _run_against_oracle(scs, masks, it, substeps, block, precision=L.MLX_FP64, c
             p, v, lam = b.download()
             assert np.isfinite(p).all() and np.isfinite(v).all()
             for s in check:
-                r = osims[s].step(sets=sets[s][0])
+                r = osims[s].step(sets=sets[s][0] if masks is not None else None)
                 v0, v1 = merged.vertOff[s], merged.vertOff[s + 1]
                 e0, e1 = merged.elemOff[s], merged.elemOff[s + 1]
                 ep, el = rel(p[3 * v0:3 * v1], osims[s].p), rel(lam[e0:e1], osims[s].lambdas())
@@ -94,21 +100,28 @@ def test_layered_batch_fp64_matches_oracle_per_scene(block, h, oracle_lib):
     assert set(bodies) >= ({1, 2} if check else {1, 2, 3})
 
 
-def test_layered_batch_large_scenes_fp64(oracle_lib):
+@pytest.mark.parametrize("clusters", [True, False])
+def test_large_scenes_fp64_autotuned_buckets(clusters, oracle_lib):
     """The batched workload's scene shapes at full size, three scenes: the
     smallest, the median and the largest StVK-Voigt scenes of the 256 (by
-    tet count), one launch, FP64 against the oracle over the first
-    substeps (free fall and touchdown)."""
+    tet count), one per color bucket, with the per-bucket autotuner's block
+    sizes (three launches on three streams per substep), FP64 against the
+    oracle over the first substeps (free fall and touchdown); with the rigid
+    clusters (two layers) and without (plain method)."""
     scs, masks = cluster_workload(256, seed=0)
     stvk = [i for i in range(256) if i % 2 == 0]
     by_size = sorted(stvk, key=lambda i: len(scs[i]["t"]))
     pick = [by_size[0], by_size[len(by_size) // 2], by_size[-1]]
     sel_s = [scs[i] for i in pick]
-    sel_m = [masks[i] for i in pick]
+    sel_m = [masks[i] for i in pick] if clusters else None
     for sc in sel_s:
         sc["p"] = sc["p"] - np.array([0, 0, 0.04])
     _say("scenes", pick, [len(s["t"]) for s in sel_s])
-    _run_against_oracle(sel_s, sel_m, _integ(iterations=(2, 18), hanging=100), 4, 256)
+    it = _integ(iterations=(2, 18), hanging=100)
+    if not clusters:
+        it.setLayers([0])
+        it.iterations = [20]
+    _run_against_oracle(sel_s, sel_m, it, 4, 256, autotune=True)
 
 
 def test_layered_batch_fp32_full_workload_runs():
