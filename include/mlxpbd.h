@@ -282,6 +282,38 @@ typedef struct {
   double compliance;              /* ContactFinder.xpbdContactCompliance (1e-4) */
 } mlx_plane;
 
+/* Contact finders of a 3D substep (mlx_integrate_finders), evaluated on the
+ * device at the start-of-step positions; their contacts are concatenated
+ * in finder order, each finder's in ascending vertex order.  A vertex
+ * claimed twice (by two finders) is refused with MLX_ERR_UNSUPPORTED.
+ *   MLX_CONTACT_PLANE: PlaneContactFinder3D (3d/contact/PlaneContactFinder3D.m:34-75),
+ *     normal and position as in mlx_plane.
+ *   MLX_CONTACT_BOX: PlatformContactDetector (3d/contact/PlatformContactDetector.m:65-120):
+ *     center, axes (3 unit rows: the rows of the box's rotation) and
+ *     halfExtent (width, depth, height halves); a vertex inside the box
+ *     gets phi = -(distance to the nearest face) and that face's outward
+ *     normal (the first axis on ties).
+ *   MLX_CONTACT_SDF: a collider's signed distance sampled on a regular grid
+ *     (values x fastest, dims[0] x dims[1] x dims[2], sample (i,j,k) at
+ *     origin + spacing (i,j,k)), the samples in place of gptoolbox's
+ *     signed_distance query (3d/contact/MeshSCD3D.m:139): trilinear value
+ *     and gradient in the vertex's cell, a contact where the value is
+ *     negative, normal the unit gradient, none outside the grid.
+ * The collider point is x - normal phi (contactInfo3D.m:34-38). */
+#define MLX_CONTACT_PLANE 0
+#define MLX_CONTACT_BOX 1
+#define MLX_CONTACT_SDF 2
+#define MLX_MAX_FINDERS 8
+typedef struct {
+  int32_t kind;
+  double compliance;              /* xpbdContactCompliance */
+  double normal[3], position[3];  /* plane */
+  double center[3], axes[9], halfExtent[3];  /* box */
+  const double *sdf;              /* sdf: host array of dims[0]*dims[1]*dims[2] */
+  int32_t dims[3];
+  double origin[3], spacing;
+} mlx_contact_finder;
+
 /* p, v: dim*nV initial state (mesh.p, mesh.v). */
 int mlx_state_create(mlx_mesh *mesh, const double *p, const double *v,
                      mlx_state **out);
@@ -310,6 +342,19 @@ int mlx_state_get_oldp(mlx_state *state, double *oldp);
 int mlx_integrate(mlx_state *state, mlx_settings *settings, double h,
                   int32_t frame, const mlx_plane *plane,
                   mlx_layer_solve_out *out);
+
+/* mlx_integrate with the contact finders above (3D; nFinders 0..8) in
+ * place of the single plane. */
+int mlx_integrate_finders(mlx_state *state, mlx_settings *settings, double h, int32_t frame,
+                          const mlx_contact_finder *finders, int32_t nFinders,
+                          mlx_layer_solve_out *out);
+/* The finders' contacts at the state's current positions, as the next
+ * substep would find them: K, then per contact (cap at most) the 0-based
+ * vertex, normal (3), collider point (3) and compliance.  Any output but K
+ * may be NULL. */
+int mlx_find_contacts(mlx_state *state, const mlx_contact_finder *finders, int32_t nFinders,
+                      int32_t *K, int32_t cap, int32_t *vertex, double *normals, double *points,
+                      double *compliance);
 
 /* Prescribed values of the pinned DOFs (nPinnedDOFs, in pinnedDOFs order)
  * for the following substeps of frame >= 2 (3D).  No counterpart in the

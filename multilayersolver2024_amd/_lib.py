@@ -37,7 +37,7 @@ EXPORTED = (
     "mlx_mlp_create", "mlx_mlp_destroy", "mlx_mlp_set", "mlx_mlp_get", "mlx_mlp_infer",
     "mlx_mlp_grad", "mlx_mlp_train", "mlx_mesh_set_learned", "mlx_learn_stats",
     "mlx_learn_outputs", "mlx_learn_features", "mlx_state_set", "mlx_state_get_oldp",
-    "mlx_batch_set_rigid", "mlx_batch_autotune",
+    "mlx_batch_set_rigid", "mlx_batch_autotune", "mlx_integrate_finders", "mlx_find_contacts",
 )
 MLX_LEARN_WARMSTART = 1
 MLX_LEARN_KEYS = 2
@@ -111,6 +111,20 @@ class Plane(C.Structure):
     _fields_ = [
         ("enabled", C.c_int32), ("normal", C.c_double * 3),
         ("position", C.c_double * 3), ("compliance", C.c_double),
+    ]
+
+
+MLX_CONTACT_PLANE = 0
+MLX_CONTACT_BOX = 1
+MLX_CONTACT_SDF = 2
+
+
+class ContactFinder(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("compliance", C.c_double), ("normal", C.c_double * 3),
+        ("position", C.c_double * 3), ("center", C.c_double * 3), ("axes", C.c_double * 9),
+        ("halfExtent", C.c_double * 3), ("sdf", C.POINTER(C.c_double)),
+        ("dims", C.c_int32 * 3), ("origin", C.c_double * 3), ("spacing", C.c_double),
     ]
 
 
@@ -202,6 +216,10 @@ def lib():
         "mlx_batch_set_rigid": [_p, _u8],
         "mlx_batch_autotune": [_p, C.POINTER(Settings), C.c_double, C.c_int32, C.POINTER(Plane),
                                _i32, _f, _i32],
+        "mlx_integrate_finders": [_p, C.POINTER(Settings), C.c_double, C.c_int32,
+                                  C.POINTER(ContactFinder), C.c_int32, C.POINTER(LayerSolveOut)],
+        "mlx_find_contacts": [_p, C.POINTER(ContactFinder), C.c_int32, _i32, C.c_int32, _i32, _d,
+                              _d, _d],
     }
     for name, args in sig.items():
         f = getattr(L, name)

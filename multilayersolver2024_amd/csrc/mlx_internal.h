@@ -134,6 +134,7 @@ struct mlx_mesh {
   mlx::DevBuf d_csr;              // int32 per-layer CSR lists
   mlx::DevBuf d_lamB;             // double boundary lambdas
   mlx::DevBuf d_contacts;         // contact arrays
+  mlx::DevBuf d_finder;           // contact finders: parameters | sdf samples | flags | scan | codes
   mlx::DevBuf d_partials;         // double reduction partials
   mlx::DevBuf d_ctl;              // Ctl
   mlx::DevBuf d_resid;            // double residual array

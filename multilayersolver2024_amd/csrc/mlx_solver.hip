@@ -18,6 +18,7 @@
 #include "mlx_kernels.h"
 #include "mlx_learn_kernels.h"
 #include "mlx_batch_layered.h"
+#include "mlx_contacts.h"
 
 using namespace mlx;
 
@@ -1634,7 +1635,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_bodycnt, &m->d_seam, &m->d_elastic, &m->learn.d_lam0,
                          &m->learn.d_keyScale, &m->learn.d_save, &m->learn.d_dx,
                          &m->learn.d_inc, &m->learn.d_res, &m->learn.d_counts,
-                         &m->learn.d_feat};
+                         &m->learn.d_feat, &m->d_finder};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
@@ -1927,6 +1928,131 @@ __global__ void k_plane_gather_dev(const int32_t *count, const int32_t *vert, co
   plane_gather_one<T, DIM>(i, vert, x, n0, n1, n2, D, compliance, normal, point, compl_);
 }
 
+// Contact finders on the device (mlx_contacts.h): parameters and grid
+// samples staged to the device, flags of every finder over every vertex,
+// one compaction (finder-major codes f nV + v), the contact arrays from the
+// codes; the count and the codes are copied to pinned host memory (hList:
+// count, then up to nV codes) without a wait of their own.
+template <typename T>
+static int enqueue_finders(mlx_mesh &m, const mlx_contact_finder *fi, int nf, int32_t **hListOut) {
+  const int64_t nV = m.nV;
+  hipStream_t s = m.stream;
+  std::vector<FinderDev> fd(nf);
+  std::vector<size_t> sdfOff(nf, 0);
+  size_t sdfTotal = 0;
+  for (int f = 0; f < nf; ++f) {
+    const mlx_contact_finder &c = fi[f];
+    FinderDev &d = fd[f];
+    std::memset(&d, 0, sizeof d);
+    d.kind = c.kind;
+    d.compliance = c.compliance;
+    if (c.kind == MLX_CONTACT_PLANE) {
+      double nn = 0;
+      for (int a = 0; a < 3; ++a) nn += c.normal[a] * c.normal[a];
+      nn = std::sqrt(nn);
+      if (!(nn > 0)) {
+        set_error("contact finder %d: zero plane normal", f);
+        return MLX_ERR_ARG;
+      }
+      double D = 0;
+      for (int a = 0; a < 3; ++a) {
+        d.n[a] = c.normal[a] / nn;
+        D += c.normal[a] * c.position[a];
+      }
+      d.D = D;
+    } else if (c.kind == MLX_CONTACT_BOX) {
+      for (int a = 0; a < 3; ++a) {
+        d.c[a] = c.center[a];
+        d.he[a] = c.halfExtent[a];
+      }
+      for (int k = 0; k < 9; ++k) d.ax[k] = c.axes[k];
+    } else if (c.kind == MLX_CONTACT_SDF) {
+      if (!c.sdf || c.dims[0] < 2 || c.dims[1] < 2 || c.dims[2] < 2 || !(c.spacing > 0)) {
+        set_error("contact finder %d: an SDF grid needs samples, dims >= 2 and spacing > 0", f);
+        return MLX_ERR_ARG;
+      }
+      for (int a = 0; a < 3; ++a) {
+        d.dims[a] = c.dims[a];
+        d.origin[a] = c.origin[a];
+      }
+      d.spacing = c.spacing;
+      sdfOff[f] = sdfTotal;
+      sdfTotal += (size_t)c.dims[0] * c.dims[1] * c.dims[2];
+    } else {
+      set_error("contact finder %d: unknown kind %d", f, c.kind);
+      return MLX_ERR_ARG;
+    }
+  }
+  const size_t n = (size_t)nf * nV;
+  auto al = [](size_t o) { return (o + 255) / 256 * 256; };
+  const size_t oPar = 0, oSdf = al(sizeof(FinderDev) * nf), oFlag = al(oSdf + 8 * sdfTotal),
+               oPos = al(oFlag + 4 * n), oCode = al(oPos + 4 * n), oLam = al(oCode + 4 * n),
+               oEnd = al(oLam + 8 * n);
+  MLX_TRY(m.d_finder.reserve(oEnd));
+  char *base = m.d_finder.as<char>();
+  const double *sdfDev = reinterpret_cast<const double *>(base + oSdf);
+  for (int f = 0; f < nf; ++f)
+    if (fd[f].kind == MLX_CONTACT_SDF) {
+      fd[f].sdf = sdfDev + sdfOff[f];
+      MLX_TRY(stage_upload(m, base + oSdf + 8 * sdfOff[f], fi[f].sdf,
+                           8 * (size_t)fi[f].dims[0] * fi[f].dims[1] * fi[f].dims[2]));
+    }
+  MLX_TRY(stage_upload(m, base + oPar, fd.data(), sizeof(FinderDev) * nf));
+  int32_t *flag = reinterpret_cast<int32_t *>(base + oFlag);
+  int32_t *pos = reinterpret_cast<int32_t *>(base + oPos);
+  int32_t *code = reinterpret_cast<int32_t *>(base + oCode);
+  double *lamTmp = reinterpret_cast<double *>(base + oLam);
+  const FinderDev *fdev = reinterpret_cast<const FinderDev *>(base + oPar);
+  vec4_t<T> *x = m.d_x.as<vec4_t<T>>();
+  ContactArrays ca = contact_arrays(m, 0);
+  int32_t *cnt = const_cast<int32_t *>(ca.counts);
+  hipLaunchKernelGGL(k_finder_flags<T>, dim3(blocks_for((int64_t)n)), dim3(kBlock), 0, s, nV,
+                     (const vec4_t<T> *)x, fdev, nf, flag);
+  MLX_TRY(compact_flags_dev(m, flag, (int64_t)n, pos, code, lamTmp, cnt, s));
+  const int cap = (int)contact_cap(m);
+  hipLaunchKernelGGL(k_finder_gather<T>, dim3(blocks_for(cap)), dim3(kBlock), 0, s,
+                     (const int32_t *)cnt, cap, nV, (const int32_t *)code, (const vec4_t<T> *)x,
+                     fdev, const_cast<int32_t *>(ca.vert), const_cast<double *>(ca.normal),
+                     const_cast<double *>(ca.point), const_cast<double *>(ca.compliance),
+                     ca.lambda);
+  char *hp = nullptr;
+  MLX_TRY(stage_take(m, 4 * ((size_t)nV + 1), &hp));
+  int32_t *hList = reinterpret_cast<int32_t *>(hp);
+  MLX_CHECK_HIP(hipMemcpyAsync(hList, cnt, 4, hipMemcpyDeviceToHost, s));
+  MLX_CHECK_HIP(hipMemcpyAsync(hList + 1, code, 4 * (size_t)nV, hipMemcpyDeviceToHost, s));
+  *hListOut = hList;
+  return MLX_OK;
+}
+
+// Host view of the finders' contacts after the stream has drained: vertex
+// list and per-contact compliance from the codes; a vertex claimed twice is
+// refused.
+static int finder_contacts_host(const mlx_mesh &m, const mlx_contact_finder *fi, int nf,
+                                const int32_t *hList, HostContacts &hc) {
+  const int64_t nV = m.nV;
+  const int K = hList[0];
+  if (K > (int)contact_cap(m)) {
+    set_error("contact finders: %d contacts for %lld vertices (a vertex claimed twice)", K,
+              (long long)nV);
+    return MLX_ERR_UNSUPPORTED;
+  }
+  hc.K = K;
+  hc.vert.resize(K);
+  hc.compliance.resize(K);
+  std::vector<uint8_t> seen(nV, 0);
+  for (int i = 0; i < K; ++i) {
+    const int32_t f = (int32_t)(hList[1 + i] / nV), v = (int32_t)(hList[1 + i] % nV);
+    if (seen[v]) {
+      set_error("contact finders: vertex %d is claimed by two finders", v + 1);
+      return MLX_ERR_UNSUPPORTED;
+    }
+    seen[v] = 1;
+    hc.vert[i] = v;
+    hc.compliance[i] = fi[f].compliance;
+  }
+  return MLX_OK;
+}
+
 // The layer maps of integrate(): strain-rate order (device EDiff, host
 // stable sort), union-find over shared vertices, boundary vertices.
 template <typename T>
@@ -1973,7 +2099,8 @@ static int build_layer_maps(mlx_state &S, const double *sorted, int ns, double h
 
 template <typename T>
 static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
-                          const mlx_plane *plane, mlx_layer_solve_out *out) {
+                          const mlx_plane *plane, mlx_layer_solve_out *out,
+                          const mlx_contact_finder *finders = nullptr, int nFinders = 0) {
   mlx_mesh &m = *S.mesh;
   const int64_t nV = m.nV, nT = m.nT;
   const int dim = m.dim;
@@ -2045,6 +2172,9 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
     hList = reinterpret_cast<int32_t *>(hp);
     MLX_CHECK_HIP(hipMemcpyAsync(hList, cnt, 4, hipMemcpyDeviceToHost, s));
     MLX_CHECK_HIP(hipMemcpyAsync(hList + 1, ca.vert, 4 * (size_t)nV, hipMemcpyDeviceToHost, s));
+  } else if (nFinders > 0) {
+    MLX_TRY(reserve_contacts(m));
+    MLX_TRY(enqueue_finders<T>(m, finders, nFinders, &hList));
   }
   // 3. layers (buildLayers + findBoundaryVertices) from p and cache.oldp
   double sorted[MLX_MAX_LAYERS];
@@ -2067,6 +2197,9 @@ static int integrate_impl(mlx_state &S, mlx_settings *st, double h, int frame,
     hc.K = hList[0];
     hc.vert.assign(hList + 1, hList + 1 + hc.K);
     hc.compliance.assign(hc.K, plane->compliance);
+  } else if (nFinders > 0) {
+    MLX_CHECK_HIP(hipStreamSynchronize(s));
+    MLX_TRY(finder_contacts_host(m, finders, nFinders, hList, hc));
   }
   const int K = hc.K;
   ca = contact_arrays(m, K);
@@ -2324,6 +2457,60 @@ int mlx_integrate(mlx_state *S, mlx_settings *st, double h, int32_t frame, const
 }  // extern "C"
 
 extern "C" {
+
+int mlx_integrate_finders(mlx_state *S, mlx_settings *st, double h, int32_t frame,
+                          const mlx_contact_finder *finders, int32_t nFinders,
+                          mlx_layer_solve_out *out) {
+  if (!S || !st || !(h > 0) || frame < 1 || st->nLayers < 1 || st->nLayers > MLX_MAX_LAYERS ||
+      nFinders < 0 || nFinders > MLX_MAX_FINDERS || (nFinders > 0 && !finders)) {
+    set_error("mlx_integrate_finders: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  if (S->mesh->dim != 3 || S->mesh->nScenes != 1) {
+    set_error("mlx_integrate_finders: 3D single-scene meshes");
+    return MLX_ERR_UNSUPPORTED;
+  }
+  MLX_CHECK_HIP(hipSetDevice(S->mesh->device));
+  if (S->mesh->precision == MLX_FP64)
+    return integrate_impl<double>(*S, st, h, frame, nullptr, out, finders, nFinders);
+  return integrate_impl<float>(*S, st, h, frame, nullptr, out, finders, nFinders);
+}
+
+int mlx_find_contacts(mlx_state *S, const mlx_contact_finder *finders, int32_t nFinders,
+                      int32_t *K, int32_t cap, int32_t *vertex, double *normals, double *points,
+                      double *compliance) {
+  if (!S || !K || nFinders < 1 || nFinders > MLX_MAX_FINDERS || !finders || cap < 0) {
+    set_error("mlx_find_contacts: bad arguments");
+    return MLX_ERR_ARG;
+  }
+  mlx_mesh &m = *S->mesh;
+  if (m.dim != 3) {
+    set_error("mlx_find_contacts: 3D meshes");
+    return MLX_ERR_UNSUPPORTED;
+  }
+  MLX_CHECK_HIP(hipSetDevice(m.device));
+  MLX_TRY(stage_begin(m));
+  MLX_TRY(reserve_contacts(m));
+  int32_t *hList = nullptr;
+  const int r = m.precision == MLX_FP64 ? enqueue_finders<double>(m, finders, nFinders, &hList)
+                                        : enqueue_finders<float>(m, finders, nFinders, &hList);
+  if (r != MLX_OK) return r;
+  MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
+  HostContacts hc;
+  MLX_TRY(finder_contacts_host(m, finders, nFinders, hList, hc));
+  *K = hc.K;
+  const int n = std::min(hc.K, cap);
+  ContactArrays ca = contact_arrays(m, hc.K);
+  if (vertex)
+    for (int i = 0; i < n; ++i) vertex[i] = hc.vert[i];
+  if (normals && n > 0)
+    MLX_CHECK_HIP(hipMemcpy(normals, ca.normal, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  if (points && n > 0)
+    MLX_CHECK_HIP(hipMemcpy(points, ca.point, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  if (compliance)
+    for (int i = 0; i < n; ++i) compliance[i] = hc.compliance[i];
+  return MLX_OK;
+}
 
 int mlx_profile_enable(mlx_mesh *m, int32_t on) {
   if (!m) {

@@ -117,6 +117,82 @@ class PlaneContactFinder:
         pl.compliance = self.compliance
         return pl
 
+    def finder(self):
+        f = L.ContactFinder()
+        f.kind = L.MLX_CONTACT_PLANE
+        f.compliance = self.compliance
+        for d in range(len(self.normal)):
+            f.normal[d] = self.normal[d]
+            f.position[d] = self.position[d]
+        return f
+
+
+def eul2rotm_xyz(radians):
+    """Rotation of the Euler angles (x, y, z) in the 'XYZ' sequence:
+    Rx(a) Ry(b) Rz(c) (what RotationMatrixDegrees builds with eul2rotm)."""
+    a, b, c = radians
+    Rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    Ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+    Rz = np.array([[np.cos(c), -np.sin(c), 0], [np.sin(c), np.cos(c), 0], [0, 0, 1]])
+    return Rx @ Ry @ Rz
+
+
+class PlatformContactDetector:
+    """PlatformContactDetector(degrees, scale, position)
+    (3d/contact/PlatformContactDetector.m:25-64): a box of size `scale`
+    (width, depth, height) centred at `position`, rotated by the Euler angles
+    `degrees` ('XYZ'); its axes are the rows of the rotation, normalised."""
+
+    def __init__(self, degrees, scale, position, compliance=1e-4):
+        R = eul2rotm_xyz(np.radians(np.asarray(degrees, dtype=np.float64)))
+        self.axes = R / np.linalg.norm(R, axis=1, keepdims=True)
+        self.halfExtent = np.asarray(scale, dtype=np.float64) / 2
+        self.center = np.asarray(position, dtype=np.float64)
+        self.compliance = float(compliance)
+
+    def finder(self):
+        f = L.ContactFinder()
+        f.kind = L.MLX_CONTACT_BOX
+        f.compliance = self.compliance
+        for a in range(3):
+            f.center[a] = self.center[a]
+            f.halfExtent[a] = self.halfExtent[a]
+        for k in range(9):
+            f.axes[k] = self.axes.reshape(-1)[k]
+        return f
+
+
+class SdfContactFinder:
+    """A collider given by its signed distance sampled on a regular grid:
+    values[i, j, k] at origin + spacing * (i, j, k) (the samples stand in for
+    gptoolbox's signed_distance query, 3d/contact/MeshSCD3D.m:139)."""
+
+    def __init__(self, values, origin, spacing, compliance=1e-4):
+        v = np.asarray(values, dtype=np.float64)
+        self.dims = v.shape
+        self.values = np.ascontiguousarray(v.ravel(order="F"))  # x fastest
+        self.origin = np.asarray(origin, dtype=np.float64)
+        self.spacing = float(spacing)
+        self.compliance = float(compliance)
+
+    def finder(self):
+        f = L.ContactFinder()
+        f.kind = L.MLX_CONTACT_SDF
+        f.compliance = self.compliance
+        f.sdf = self.values.ctypes.data_as(C.POINTER(C.c_double))
+        for a in range(3):
+            f.dims[a] = self.dims[a]
+            f.origin[a] = self.origin[a]
+        f.spacing = self.spacing
+        return f
+
+
+def finder_array(finders):
+    arr = (L.ContactFinder * len(finders))()
+    for i, f in enumerate(finders):
+        arr[i] = f.finder()
+    return arr
+
 
 class StepResult:
     def __init__(self, out, residuals):
@@ -148,7 +224,15 @@ class Simulation:
         self.state = st
         self.frame = 0
         self._settings = integrator.settings()
-        self._plane = contact.plane() if contact is not None else None
+        # one plane: mlx_integrate; a list of finders (planes, platform
+        # boxes, grid SDFs): mlx_integrate_finders
+        self._finders = None
+        self._plane = None
+        if isinstance(contact, (list, tuple)):
+            self._finders = finder_array(contact)
+            self._finder_objs = list(contact)
+        elif contact is not None:
+            self._plane = contact.plane()
         self._resid = np.zeros(residual_cap)
         self._out = L.LayerSolveOut()
         self._out.residualArray = L.ptr(self._resid)
@@ -157,12 +241,34 @@ class Simulation:
     def step(self):
         """One integrate() substep; returns a StepResult."""
         self.frame += 1
-        pl = C.byref(self._plane) if self._plane is not None else None
-        L.check("mlx_integrate", L.lib().mlx_integrate(self.state, C.byref(self._settings), self.h,
-                                                       self.frame, pl, C.byref(self._out)))
+        if self._finders is not None:
+            L.check("mlx_integrate_finders", L.lib().mlx_integrate_finders(
+                self.state, C.byref(self._settings), self.h, self.frame, self._finders,
+                len(self._finders), C.byref(self._out)))
+        else:
+            pl = C.byref(self._plane) if self._plane is not None else None
+            L.check("mlx_integrate", L.lib().mlx_integrate(self.state, C.byref(self._settings),
+                                                           self.h, self.frame, pl,
+                                                           C.byref(self._out)))
         self.integrator.runUntilResSmallerThan = self._settings.runUntilResSmallerThan
         n = min(self._out.residualCount, len(self._resid))
         return StepResult(self._out, self._resid[:n].copy())
+
+    def find_contacts(self, finders):
+        """The finders' contacts at the current positions: (vertex 0-based,
+        normals K x 3, points K x 3, compliance)."""
+        arr = finder_array(finders)
+        nV = self.mesh.nV
+        K = np.zeros(1, np.int32)
+        vert = np.zeros(nV, np.int32)
+        nrm = np.zeros(3 * nV)
+        pts = np.zeros(3 * nV)
+        cc = np.zeros(nV)
+        L.check("mlx_find_contacts", L.lib().mlx_find_contacts(
+            self.state, arr, len(finders), L.ptr(K, C.c_int32), nV, L.ptr(vert, C.c_int32),
+            L.ptr(nrm), L.ptr(pts), L.ptr(cc)))
+        k = int(K[0])
+        return vert[:k].copy(), nrm[:3 * k].reshape(k, 3), pts[:3 * k].reshape(k, 3), cc[:k].copy()
 
     def set_pinned_targets(self, pinnedx):
         """Prescribed positions of the pinned DOFs (pinnedDOFs order) for the

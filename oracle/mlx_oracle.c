@@ -953,6 +953,98 @@ int ora_plane_contacts(int dim, int nV, const double *p, const double *normal,
   return K;
 }
 
+/* PlatformContactDetector.findContacts (3d/contact/PlatformContactDetector.m:65-120):
+ * vertices inside the box, in ascending order; phi = -min(half extent -
+ * |offset . axis|) over the axes (the first on ties, Matlab min), normal the
+ * axis of that minimum signed by sign(offset . axis), collider point x -
+ * normal phi (contactInfo3D.m:34-38).  axes: 3 unit rows.  Pins: closed
+ * forms on axis-aligned and rotated boxes (tests). */
+int ora_box_contacts(int nV, const double *p, const double *center, const double *axes,
+                     const double *halfExtent, double compliance, int32_t *vertex,
+                     double *cnormal, double *cpoint, double *ccompliance) {
+  int K = 0;
+  for (int v = 0; v < nV; ++v) {
+    const double *x = p + 3 * v;
+    double d[3] = {x[0] - center[0], x[1] - center[1], x[2] - center[2]};
+    double dot[3];
+    for (int a = 0; a < 3; ++a)
+      dot[a] = d[0] * axes[3 * a] + d[1] * axes[3 * a + 1] + d[2] * axes[3 * a + 2];
+    if (!(fabs(dot[0]) <= halfExtent[0] && fabs(dot[1]) <= halfExtent[1] &&
+          fabs(dot[2]) <= halfExtent[2]))
+      continue;
+    double dist[3] = {halfExtent[0] - fabs(dot[0]), halfExtent[1] - fabs(dot[1]),
+                      halfExtent[2] - fabs(dot[2])};
+    int k = 0;
+    if (dist[1] < dist[k]) k = 1;
+    if (dist[2] < dist[k]) k = 2;
+    double phi = -dist[k];
+    double sg = dot[k] > 0 ? 1.0 : (dot[k] < 0 ? -1.0 : 0.0);
+    vertex[K] = v;
+    for (int a = 0; a < 3; ++a) {
+      double n = sg * axes[3 * k + a];
+      cnormal[3 * K + a] = n;
+      cpoint[3 * K + a] = x[a] - n * phi;
+    }
+    ccompliance[K] = compliance;
+    ++K;
+  }
+  return K;
+}
+
+/* Contacts against a signed distance sampled on a regular grid (values x
+ * fastest, sample (i,j,k) at origin + spacing (i,j,k)), in place of
+ * gptoolbox's signed_distance query of MeshSCD3D (3d/contact/MeshSCD3D.m:139,
+ * contacts where S < 0 with a finite normal): trilinear interpolation in
+ * the vertex's cell (none outside the grid), normal the unit gradient of the
+ * interpolant.  Pins: a sampled plane (trilinear is exact there) against the
+ * plane finder; a sampled sphere against its closed form (tests). */
+int ora_sdf_contacts(int nV, const double *p, const double *grid, const int32_t *dims,
+                     const double *origin, double spacing, double compliance, int32_t *vertex,
+                     double *cnormal, double *cpoint, double *ccompliance) {
+  int K = 0;
+  const long sx = 1, sy = dims[0], sz = (long)dims[0] * dims[1];
+  for (int v = 0; v < nV; ++v) {
+    const double *x = p + 3 * v;
+    int i0[3], out = 0;
+    double t[3];
+    for (int a = 0; a < 3; ++a) {
+      double g = (x[a] - origin[a]) / spacing;
+      if (!(g >= 0.0) || !(g <= (double)(dims[a] - 1))) { out = 1; break; }
+      int i = (int)floor(g);
+      if (i > dims[a] - 2) i = dims[a] - 2;
+      i0[a] = i;
+      t[a] = g - (double)i;
+    }
+    if (out) continue;
+    const double *c = grid + i0[0] * sx + i0[1] * sy + i0[2] * sz;
+    double c000 = c[0], c100 = c[sx], c010 = c[sy], c110 = c[sx + sy], c001 = c[sz],
+           c101 = c[sx + sz], c011 = c[sy + sz], c111 = c[sx + sy + sz];
+    double u = t[0], w = t[1], r = t[2];
+    double c00 = c000 + u * (c100 - c000), c10 = c010 + u * (c110 - c010);
+    double c01 = c001 + u * (c101 - c001), c11 = c011 + u * (c111 - c011);
+    double c0 = c00 + w * (c10 - c00), c1 = c01 + w * (c11 - c01);
+    double val = c0 + r * (c1 - c0);
+    if (!(val < 0.0)) continue;
+    double dx00 = c100 - c000, dx10 = c110 - c010, dx01 = c101 - c001, dx11 = c111 - c011;
+    double dx0 = dx00 + w * (dx10 - dx00), dx1 = dx01 + w * (dx11 - dx01);
+    double gx = (dx0 + r * (dx1 - dx0)) / spacing;
+    double dy0 = c10 - c00, dy1 = c11 - c01;
+    double gy = (dy0 + r * (dy1 - dy0)) / spacing;
+    double gz = (c1 - c0) / spacing;
+    double gn = sqrt(gx * gx + gy * gy + gz * gz);
+    if (!(gn > 0.0)) continue;
+    double n[3] = {gx / gn, gy / gn, gz / gn};
+    vertex[K] = v;
+    for (int a = 0; a < 3; ++a) {
+      cnormal[3 * K + a] = n[a];
+      cpoint[3 * K + a] = x[a] - n[a] * val;
+    }
+    ccompliance[K] = compliance;
+    ++K;
+  }
+  return K;
+}
+
 /* xpbdLayer2D.contactVelocityUpdate (xpbdLayer2D.m:577-603), shared by 3D
  * (xpbdLayer3D.m:740).  The e = 0 branch tests ||v|| <= 2*g*h with g < 0 and
  * so never fires for a downward gravity; it is kept. */

@@ -263,6 +263,14 @@ int ora_learn_warm_start3d(const ora_mesh *m, double h, const uint8_t *isEl,
                            const double *lam0, double *lambda, double *xi, const double *oldp,
                            const uint8_t *pinnedDof, const ora_contacts *c, double resistance,
                            double *res);
+/* Platform box and grid-sampled signed-distance contact finders (3D);
+ * return K and fill the contact arrays (capacity nV). */
+int ora_box_contacts(int nV, const double *p, const double *center, const double *axes,
+                     const double *halfExtent, double compliance, int32_t *vertex,
+                     double *cnormal, double *cpoint, double *ccompliance);
+int ora_sdf_contacts(int nV, const double *p, const double *grid, const int32_t *dims,
+                     const double *origin, double spacing, double compliance, int32_t *vertex,
+                     double *cnormal, double *cpoint, double *ccompliance);
 /* 2D rigid-layer steps on explicit body state (test hooks): bodies from
  * connectivity, boundary accumulation and contacts (xpbdLayer2D.m:296-444). */
 void ora_rigid_properties2d(int nV, int nb, const int32_t *vb, const double *p,

@@ -223,6 +223,54 @@ def plane_contacts(dim, p, normal, position, compliance, strict):
                 compliance=cc[:K].copy(), K=K)
 
 
+def box_contacts(p, center, axes, half, compliance):
+    nV = len(p) // 3
+    vert = np.zeros(nV, dtype=np.int32)
+    cn, cp, cc = np.zeros(3 * nV), np.zeros(3 * nV), np.zeros(nV)
+    K = lib().ora_box_contacts(nV, P(np.ascontiguousarray(p)), P(np.ascontiguousarray(center, dtype=np.float64)),
+                               P(np.ascontiguousarray(axes, dtype=np.float64).reshape(-1)),
+                               P(np.ascontiguousarray(half, dtype=np.float64)), C.c_double(compliance),
+                               P(vert, C.c_int32), P(cn), P(cp), P(cc))
+    return dict(vertex=vert[:K].copy(), normal=cn[:3 * K].copy(), point=cp[:3 * K].copy(),
+                compliance=cc[:K].copy(), K=K)
+
+
+def sdf_contacts(p, values_xfast, dims, origin, spacing, compliance):
+    nV = len(p) // 3
+    vert = np.zeros(nV, dtype=np.int32)
+    cn, cp, cc = np.zeros(3 * nV), np.zeros(3 * nV), np.zeros(nV)
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    K = lib().ora_sdf_contacts(nV, P(np.ascontiguousarray(p)),
+                               P(np.ascontiguousarray(values_xfast, dtype=np.float64)),
+                               P(d, C.c_int32), P(np.ascontiguousarray(origin, dtype=np.float64)),
+                               C.c_double(spacing), C.c_double(compliance), P(vert, C.c_int32),
+                               P(cn), P(cp), P(cc))
+    return dict(vertex=vert[:K].copy(), normal=cn[:3 * K].copy(), point=cp[:3 * K].copy(),
+                compliance=cc[:K].copy(), K=K)
+
+
+def finder_contacts(p, finders):
+    """The contacts of a list of 3D finders (plane, platform box, grid SDF;
+    the package's finder objects), concatenated in finder order."""
+    parts = []
+    for f in finders:
+        kind = type(f).__name__
+        if kind == "PlaneContactFinder":
+            parts.append(plane_contacts(3, p, f.normal, f.position, f.compliance, strict=False))
+        elif kind == "PlatformContactDetector":
+            parts.append(box_contacts(p, f.center, f.axes, f.halfExtent, f.compliance))
+        elif kind == "SdfContactFinder":
+            parts.append(sdf_contacts(p, f.values, f.dims, f.origin, f.spacing, f.compliance))
+        else:
+            raise TypeError(f"unknown contact finder {kind}")
+    out = dict(vertex=np.concatenate([c["vertex"] for c in parts]).astype(np.int32),
+               normal=np.concatenate([c["normal"] for c in parts]),
+               point=np.concatenate([c["point"] for c in parts]),
+               compliance=np.concatenate([c["compliance"] for c in parts]))
+    out["K"] = len(out["vertex"])
+    return out
+
+
 class OracleSim:
     """The oracle's integrate() loop for one mesh (simulate3D / simulate step
     order, contacts at the start of each step)."""
@@ -247,8 +295,11 @@ class OracleSim:
     def _contacts(self):
         if self.contact is None:
             return None, None
-        c = plane_contacts(self.dim, self.p, self.contact.normal, self.contact.position,
-                           self.contact.compliance, strict=(self.dim == 2))
+        if isinstance(self.contact, (list, tuple)):
+            c = finder_contacts(self.p, self.contact)
+        else:
+            c = plane_contacts(self.dim, self.p, self.contact.normal, self.contact.position,
+                               self.contact.compliance, strict=(self.dim == 2))
         self.last_contacts = c["K"]
         keep = c
         oc = OraContacts()
