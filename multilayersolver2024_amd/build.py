@@ -7,6 +7,7 @@ Compiles the C ABI library with hipcc for gfx950 only.  Run as
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -23,17 +24,21 @@ COMMON = [
 SOURCES = ["mlx_host.cpp", "mlx_solver.hip", "mlx_maps.hip", "mlx_learn.hip"]
 
 
+def _compile(src, verbose):
+    obj = os.path.join(os.path.dirname(LIB), src + ".o")
+    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    cmd = [HIPCC] + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
 def build(verbose=False):
+    """Compiles the sources side by side (one hipcc each), then links."""
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(os.path.dirname(LIB), src + ".o")
-        lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmd = [HIPCC] + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        objs = list(ex.map(lambda src: _compile(src, verbose), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:

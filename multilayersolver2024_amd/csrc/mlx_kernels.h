@@ -175,6 +175,23 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *sh, double *o
   __syncthreads();
 }
 
+// Workgroup sum of one double per thread in a fixed order: a butterfly
+// within each wavefront (lane 0's value), then the wavefront sums in
+// wavefront order.  Needs NT / 64 doubles of LDS; the result is valid in
+// every thread.
+template <int NT>
+__device__ __forceinline__ double wg_sum(double v, double *shw) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) shw[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) t += shw[w];
+  __syncthreads();
+  return t;
+}
+
 // M x = n for a 3x3 matrix by cofactors (x_i = sum_j cof_ji n_j / det).
 template <typename T>
 __device__ __forceinline__ void solve3_cof(const T (&M)[3][3], const T (&n)[3], T (&x)[3]) {
@@ -552,16 +569,64 @@ __device__ __forceinline__ void color_slots(const SweepParams<T> &P, int64_t s, 
   }
 }
 
+// Keeps a loaded value live up to this point, so the load cannot be moved
+// below a later branch (the empty asm emits no instruction).
 template <typename T>
+__device__ __forceinline__ void hold(const T &v) {
+  asm volatile("" ::"v"(v));
+}
+
+template <typename T>
+__device__ __forceinline__ void hold_tet(const TetIn<T> &in, const vec4_t<T> (&X)[4]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    hold(X[a].x), hold(X[a].y), hold(X[a].z), hold(X[a].w);
+    hold(in.O[a].x), hold(in.O[a].y), hold(in.O[a].z);
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) hold(in.D[k]);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) hold(in.lam[i]);
+  hold(in.vol), hold(in.tad), hold(in.tao), hold(in.tas), hold(in.g);
+}
+
+// One slot of one color.  The control word (the layer may have stopped),
+// the slot's elastic flag, its data and its vertices' positions do not
+// depend on one another, so all of them are loaded before any is tested:
+// a launch then waits for the longest of the load chains (slot ids ->
+// positions) instead of control word + flag + ids + positions in turn.  A
+// launch after the stop, or a rigid slot, loads and discards.
+// Stand-ins read when a launch has no control block or no elastic flags
+// (never written: global memory, so the selected address stays global).
+__device__ int kNoStop[2] = {0, 0};
+__device__ uint8_t kElastic = 1;
+
+template <typename T, int MAT>
+__device__ __forceinline__ void sweep_slot(const SweepParams<T> &P, const int64_t s) {
+  // addresses chosen without branches, so the loads issue together
+  const int *cw = P.ctl ? &P.ctl->stopSolve : kNoStop;
+  const uint8_t *fl = P.flags ? P.flags + s : &kElastic;
+  static_assert(offsetof(Ctl, endLayer) == offsetof(Ctl, stopSolve) + sizeof(int),
+                "stopSolve and endLayer are adjacent");
+  const int stopWord = cw[0] | cw[1];
+  const uint8_t elasticFlag = *fl;
+  TetIn<T> in;
+  tet_fetch<T, MAT>(P, s, in);
+  vec4_t<T> X[4];
+  tet_gather<T, false>(P, in, X);
+  hold_tet<T>(in, X);
+  if ((stopWord != 0) | (elasticFlag == 0)) return;
+  tet_solve_x<T, MAT>(P, s, in, X);
+}
+
+// One color of a layer, one thread per slot.  MAT is kMatMixed for a mesh
+// with NeoHookean elements (P.nh), else kMatStVK: chosen at launch, so the
+// kernel body has no branch between its loads.
+template <typename T, int MAT>
 __global__ __launch_bounds__(kBlock) void k_sweep3d(SweepParams<T> P) {
-  if (P.ctl && iteration_stopped(P.ctl)) return;
   const int64_t s = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.end) return;
-  if (P.flags && !P.flags[s]) return;
-  if (P.nh)
-    project_tet<T, kMatMixed>(P, s);
-  else
-    project_tet<T>(P, s);
+  sweep_slot<T, MAT>(P, s);
 }
 
 // Elastic slots of each color of a layer, compacted in ascending slot order
@@ -601,7 +666,9 @@ __global__ __launch_bounds__(kBlock) void k_color_lists(const int64_t *colorOff,
 // per color, in one workgroup: the compacted elastic slots of each color
 // (k_color_lists), colors separated by workgroup barriers (which wait for
 // the workgroup's stores; its waves share one L1).  Same projections in the
-// same color order as one k_sweep3d launch per color.
+// same color order as one k_sweep3d launch per color, through the same
+// slot routine (sweep_slot), so the arithmetic is the same instruction for
+// instruction and the states are bit-identical.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_sweep_wg(SweepParams<T> P, const int64_t *colorOff,
                                                  const int32_t *list, const int32_t *cnt,
@@ -612,9 +679,9 @@ __global__ __launch_bounds__(NT) void k_sweep_wg(SweepParams<T> P, const int64_t
     const int32_t *l = list + colorOff[c];
     for (int i = threadIdx.x; i < n; i += NT) {
       if (P.nh)
-        project_tet<T, kMatMixed>(P, l[i]);
+        sweep_slot<T, kMatMixed>(P, l[i]);
       else
-        project_tet<T>(P, l[i]);
+        sweep_slot<T, kMatStVK>(P, l[i]);
     }
     __syncthreads();
   }
@@ -1821,19 +1888,23 @@ __device__ __forceinline__ void monitor_update(Ctl &c, const MonitorParams &mp, 
 // of their slots (one thread per slot), the
 // rest the contact terms; each block stores its partial past the caches
 // (device-scope store), waits for it, and counts itself on *done.  The
-// block that counts last reads every partial the same way, sums them
-// exactly as k_iter_end does, runs the monitor and re-arms *done to 0 (so
-// a captured graph can replay the launch).  Every block returns at once
-// when the layer has stopped, leaving *done at 0.
+// block that counts last reads every partial the same way (thread t takes
+// partials t, t + 256, ...), sums them in a fixed order (wg_sum), runs the
+// monitor and re-arms *done to 0 (so a captured graph can replay the
+// launch).  Every block returns without writing when the layer has
+// stopped, leaving *done at 0.
 template <typename T, int DIM>
 __global__ __launch_bounds__(kBlock) void k_residual_tail(
     SweepParams<T> P, int64_t nSlots, const int32_t *lamSlot2d, int nE, ContactArrays c,
     double resistance, double h, const double *uniqAlpha, const double *uniqCount,
     double *partials, unsigned *done, Ctl *ctl, MonitorParams mp, double *resid, int residCap) {
-  if (iteration_stopped(ctl)) return;
   __shared__ double sh[kBlock];
   __shared__ int sLast;
   const int t = threadIdx.x;
+  // the control word is loaded beside the terms' inputs and tested once the
+  // term is formed (hold), so its latency overlaps theirs; nothing is
+  // written before the test
+  const int stopWord = ctl->stopSolve | ctl->endLayer;
   double acc = 0;
   if ((int)blockIdx.x < nE) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + t;
@@ -1844,11 +1915,11 @@ __global__ __launch_bounds__(kBlock) void k_residual_tail(
       acc = residual_contact_one<T, DIM>(c, i, P.x, resistance, h, uniqAlpha, uniqCount,
                                          c.counts[3]);
   }
-  double o[1];
-  double v1[1] = {acc};
-  block_sum<1>(v1, sh, o);
+  hold(acc);
+  if (stopWord) return;
+  const double part = wg_sum<kBlock>(acc, sh);
   if (t == 0) {
-    __hip_atomic_store(partials + blockIdx.x, o[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(partials + blockIdx.x, part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned prev =
         __hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     sLast = prev == gridDim.x - 1;
@@ -1859,13 +1930,11 @@ __global__ __launch_bounds__(kBlock) void k_residual_tail(
   double a2 = 0;
   for (int q = t; q < (int)gridDim.x; q += kBlock)
     a2 += __hip_atomic_load(partials + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  double o2[1];
-  double w1[1] = {a2};
-  block_sum<1>(w1, sh, o2);
+  const double total = wg_sum<kBlock>(a2, sh);
   if (t != 0) return;
   __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   ctl->iters += 1;
-  monitor_update(*ctl, mp, sqrt(o2[0]), resid, residCap);
+  monitor_update(*ctl, mp, sqrt(total), resid, residCap);
 }
 
 // End of one projectGS iteration: when residuals are monitored, sum the
@@ -2175,23 +2244,6 @@ struct BatchParams {
   int32_t *residualCount;
   const int32_t *sceneList;      // scene of each workgroup (null: workgroup i runs scene i)
 };
-
-// Workgroup sum of one double per thread in a fixed order: a butterfly
-// within each wavefront (lane 0's value), then the wavefront sums in
-// wavefront order.  Needs NT / 64 doubles of LDS; the result is valid in
-// every thread.
-template <int NT>
-__device__ __forceinline__ double wg_sum(double v, double *shw) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0) shw[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0;
-#pragma unroll
-  for (int w = 0; w < NT / 64; ++w) t += shw[w];
-  __syncthreads();
-  return t;
-}
 
 // k_batch_plain for a scene whose positions fit in LDS: x, y, z of its nv
 // vertices live in LDS (structure of arrays, pos[d * nv + local vertex])

@@ -372,8 +372,10 @@ static void enqueue_sweeps(mlx_mesh &m, SweepParams<T> P, const uint8_t *flags,
     // nothing: not launched
     if (flags && elasticPerColor && elasticPerColor[c] == 0) continue;
     const unsigned grid = (unsigned)((P.end - P.begin + nb - 1) / nb);
-    if (m.dim == 3)
-      hipLaunchKernelGGL(k_sweep3d<T>, dim3(grid), dim3(nb), 0, m.stream, P);
+    if (m.dim == 3 && P.nh)
+      hipLaunchKernelGGL((k_sweep3d<T, kMatMixed>), dim3(grid), dim3(nb), 0, m.stream, P);
+    else if (m.dim == 3)
+      hipLaunchKernelGGL((k_sweep3d<T, kMatStVK>), dim3(grid), dim3(nb), 0, m.stream, P);
     else
       hipLaunchKernelGGL(k_sweep2d<T>, dim3(grid), dim3(nb), 0, m.stream, P);
   }
