@@ -68,16 +68,18 @@ RESIDUAL_BYTES_PER_TET = {"fp32": 140, "fp64": 264}
 # the 12,872 sweep dispatches of the 81,920-tet beam run (64-thread
 # workgroups); a launch's traffic is that times its slots.  k_batch_plain:
 # per dispatch (one per substep of the 256 beams, 1,024 threads).
-TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 114.569 / 1024, "fetch_scale": 2.0,
+TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 113.488 / 1024, "fetch_scale": 2.0,
                                     "write": 55.865 / 1024, "write_scale": 1.0,
                                     "per": "thread slot"},
-               "k_batch_plain<float,1024>": {"fetch": 15272309.61, "fetch_scale": 2.0,
-                                             "write": 3361058.59, "write_scale": 1.0,
+               "k_batch_plain<float,1024>": {"fetch": 15441071.3, "fetch_scale": 2.0,
+                                             "write": 3376463.8, "write_scale": 1.0,
                                              "per": "dispatch"}}
 # Where the constants above come from.
-TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this bench (--graphs 0) "
-                  "at commit 3c06c62, scaled by the same counters of a 1 GiB device copy "
-                  "(FETCH x2.0, WRITE x1.0); summaries in profiles/hbm_traffic_*.md")
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this bench (--graphs 0; "
+                  "the sweep from the plain leg over 12,872 dispatches, the batched kernel from "
+                  "the plain batched leg with --batch-legs plain --batch-splits 0, 6 dispatches), "
+                  "scaled by the same counters of a 1 GiB device copy (FETCH x2.0, WRITE x1.0); "
+                  "commands and figures in profiles/hbm_traffic_*.md")
 
 
 def traffic_bytes(kernel, units=1.0):
@@ -114,14 +116,15 @@ def make_integrator(stop_rule=True, iterations=(2, 4, 14), hanging_stop=1000, ri
     return it
 
 
-def bench_batched(h, plane, steps, warmup, precision, nscenes=256, graph=True):
+def bench_batched(h, plane, steps, warmup, precision, nscenes=256, graph=True,
+                  splits=(0, 8000, 25000)):
     """Batched form: nscenes beams (tet counts log-uniform in [2000, 60000],
     random aspect ratios, per-scene E log-uniform in [1e4, 1e7]) integrated
     in one persistent launch per substep, plain method (one all-elastic
     layer, 20 iterations, the same 1e-4 relative-residual stop per scene).
     The block size and the scene split (target tets per workgroup; 0 = one
     workgroup per scene) are chosen by timing each candidate pair on the
-    same substep."""
+    same substep (split candidates other than 0 are cooperative launches)."""
     from multilayersolver2024_amd.batch import BatchSimulation, merge_meshes, random_beams
 
     merged = merge_meshes([Mesh(s) for s in random_beams(nscenes, seed=0)])
@@ -144,7 +147,7 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256, graph=True):
     frame0 = b.frame
     tune = {}
     for blk in (256, 512, 768, 1024):
-        for split in (0, 8000, 25000):
+        for split in splits:
             b.set_block(blk)
             b.set_split(split)
             b.set_state(p0, v0)
@@ -501,6 +504,11 @@ def main():
     ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
                     help="timed batched substeps replay a captured graph (1) or launch directly (0, for kernel-trace profiling)")
     ap.add_argument("--batch-warmup", type=int, default=6)
+    ap.add_argument("--batch-legs", choices=["both", "clusters", "plain"], default="both",
+                    help="which batched legs run: the cluster workload, the plain StVK one, or both")
+    ap.add_argument("--batch-splits", default="0,8000,25000",
+                    help="split candidates (tets per workgroup) of the plain batched autotune; "
+                         "0 = one workgroup per scene")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus must be 1 (multi-GPU sharding of the "
@@ -712,15 +720,18 @@ def main():
         }, **roof),
     }
     line.update(extra)
-    if not args.no_batched:
+    if not args.no_batched and args.batch_legs in ("both", "clusters"):
         log("batched leg (rigid clusters, NeoHookean half)")
         line["batched"] = bench_batched_clusters(h, plane, args.batch_steps, args.batch_warmup,
                                                  args.precision, graph=bool(args.batch_graph))
         log(f"batched: {line['batched']['ms_per_step']:.1f} ms/substep, "
             f"{line['batched']['value']:.3e} tet-proj/s")
+    if not args.no_batched and args.batch_legs in ("both", "plain"):
         log("batched leg (plain method, StVK-Voigt)")
+        splits = tuple(int(v) for v in args.batch_splits.split(",") if v.strip())
         line["batched_plain_stvk"] = bench_batched(h, plane, args.batch_steps, args.batch_warmup,
-                                                   args.precision, graph=bool(args.batch_graph))
+                                                   args.precision, graph=bool(args.batch_graph),
+                                                   splits=splits)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         cb, p_oracle_one = cpu_baseline(mesh, h, plane, snap_rep, runs=args.cpu_runs,

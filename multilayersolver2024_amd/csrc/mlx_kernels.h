@@ -270,11 +270,11 @@ __device__ __forceinline__ T mat_third(T m0, T m2, T ih2) {
   else return m0 < T(0) ? m2 : m2 * ih2;
 }
 
+// Everything but the vertex ids, which are in in.v already.
 template <typename T, int MAT = 0>
-__device__ __forceinline__ void tet_fetch(const SweepParams<T> &P, const int64_t s, TetIn<T> &in) {
+__device__ __forceinline__ void tet_fetch_rest(const SweepParams<T> &P, const int64_t s,
+                                               TetIn<T> &in) {
   const int64_t ld = P.ld;
-#pragma unroll
-  for (int a = 0; a < 4; ++a) in.v[a] = P.vid[a * ld + s];
 #pragma unroll
   for (int a = 0; a < 4; ++a) in.O[a] = P.xold[in.v[a]];
 #pragma unroll
@@ -287,6 +287,13 @@ __device__ __forceinline__ void tet_fetch(const SweepParams<T> &P, const int64_t
   in.g = P.mat[3 * ld + s] * P.ih;
 #pragma unroll
   for (int i = 0; i < 6; ++i) in.lam[i] = P.lam[i * ld + s];
+}
+
+template <typename T, int MAT = 0>
+__device__ __forceinline__ void tet_fetch(const SweepParams<T> &P, const int64_t s, TetIn<T> &in) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) in.v[a] = P.vid[a * P.ld + s];
+  tet_fetch_rest<T, MAT>(P, s, in);
 }
 
 // Material scalars of a slot.  StVK-Voigt: alpha_d = alpha(1,1),
@@ -576,46 +583,34 @@ __device__ __forceinline__ void hold(const T &v) {
   asm volatile("" ::"v"(v));
 }
 
-template <typename T>
-__device__ __forceinline__ void hold_tet(const TetIn<T> &in, const vec4_t<T> (&X)[4]) {
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    hold(X[a].x), hold(X[a].y), hold(X[a].z), hold(X[a].w);
-    hold(in.O[a].x), hold(in.O[a].y), hold(in.O[a].z);
-  }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) hold(in.D[k]);
-#pragma unroll
-  for (int i = 0; i < 6; ++i) hold(in.lam[i]);
-  hold(in.vol), hold(in.tad), hold(in.tao), hold(in.tas), hold(in.g);
-}
-
-// One slot of one color.  The control word (the layer may have stopped),
-// the slot's elastic flag, its data and its vertices' positions do not
-// depend on one another, so all of them are loaded before any is tested:
-// a launch then waits for the longest of the load chains (slot ids ->
-// positions) instead of control word + flag + ids + positions in turn.  A
-// launch after the stop, or a rigid slot, loads and discards.
-// Stand-ins read when a launch has no control block or no elastic flags
-// (never written: global memory, so the selected address stays global).
+// Stand-in read when a launch has no control block (never written: global
+// memory, so the selected address stays global).
 __device__ int kNoStop[2] = {0, 0};
-__device__ uint8_t kElastic = 1;
 
+// One slot of one color.  In a layer with rigid elements (P.flags set) a
+// rigid slot returns after its flag, so its data is never read.  The
+// control word (has the monitor stopped the layer?) and the slot's vertex
+// ids are loaded together and tested together, so the ids' latency, the
+// first link of the id -> position chain, overlaps the control word's; a
+// launch after the stop then returns having read only the ids.  The rest
+// of the slot data and the positions follow.
 template <typename T, int MAT>
 __device__ __forceinline__ void sweep_slot(const SweepParams<T> &P, const int64_t s) {
-  // addresses chosen without branches, so the loads issue together
+  if (P.flags && !P.flags[s]) return;
+  // address chosen without a branch, so the loads issue together
   const int *cw = P.ctl ? &P.ctl->stopSolve : kNoStop;
-  const uint8_t *fl = P.flags ? P.flags + s : &kElastic;
   static_assert(offsetof(Ctl, endLayer) == offsetof(Ctl, stopSolve) + sizeof(int),
                 "stopSolve and endLayer are adjacent");
   const int stopWord = cw[0] | cw[1];
-  const uint8_t elasticFlag = *fl;
   TetIn<T> in;
-  tet_fetch<T, MAT>(P, s, in);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) in.v[a] = P.vid[a * P.ld + s];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) hold(in.v[a]);
+  if (stopWord != 0) return;
+  tet_fetch_rest<T, MAT>(P, s, in);
   vec4_t<T> X[4];
   tet_gather<T, false>(P, in, X);
-  hold_tet<T>(in, X);
-  if ((stopWord != 0) | (elasticFlag == 0)) return;
   tet_solve_x<T, MAT>(P, s, in, X);
 }
 

@@ -234,11 +234,14 @@ def beam_family(seed, rng):
     return scenes.beam3d(nx, ny, nz, cell=0.02, z0=z0, seed=seed)
 
 
-def collect(seeds, substeps, per_substep, precision=L.MLX_FP32, log=print, base_rng=0):
+def collect(seeds, substeps, per_substep, precision=L.MLX_FP32, log=print, base_rng=0,
+            graphs=True):
     """Plain-solver trajectories (layers [98.4375, 87.5, 0], iterations
     [2, 4, 14], relative-residual stop 1e-4, ground contact, h = 1/60) of
     beams of the given seeds.  Returns X (n x 21 features), Y (n x 2
-    targets) and W (n weights), per_substep elements sampled per substep."""
+    targets) and W (n weights), per_substep elements sampled per substep.
+    graphs=False launches every kernel directly instead of replaying the
+    iteration graphs (same results; for kernel profilers)."""
     from .integrator import PlaneContactFinder, Simulation, XpbdLayer3D
     from .mesh import Mesh
 
@@ -259,6 +262,7 @@ def collect(seeds, substeps, per_substep, precision=L.MLX_FP32, log=print, base_
         it.useRigidContacts = False
         sim = Simulation(mesh, it, h, contact=PlaneContactFinder([0, 0, 1], [0, 0, 0]),
                          precision=precision)
+        sim.dmesh.set_option(L.MLX_OPT_GRAPHS, int(graphs))
         attach(sim, probe, flags=0)
         T = mesh.T0
         Dinv = rest_inverse(mesh.p, T)
@@ -368,13 +372,16 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--graphs", type=int, default=1, choices=[0, 1],
+                    help="replay iteration graphs while collecting (1) or launch directly (0)")
     ap.add_argument("--eval", type=int, default=0,
                     help="then compare plain and learned over this many substeps of the "
                          "81,920-tet beam")
     args = ap.parse_args(argv)
     lg = lambda *a: print("[learn]", *a, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.time()
-    X, Y, W = collect(range(1, args.scenes + 1), args.substeps, args.per_substep, log=lg)
+    X, Y, W = collect(range(1, args.scenes + 1), args.substeps, args.per_substep, log=lg,
+                      graphs=bool(args.graphs))
     lg(f"dataset: {len(X)} samples, {args.scenes * args.substeps} substeps, "
        f"{time.time() - t0:.1f} s")
     t1 = time.time()
