@@ -53,20 +53,84 @@ __device__ __forceinline__ void wg_sum_n(double (&v)[N], double *shw) {
   __syncthreads();
 }
 
+// Positions of a scene's vertices: in HBM (the vec4 array) or in LDS (x, y,
+// z of the scene's nv vertices as structure of arrays, from its first
+// vertex vb; the inverse mass, constant, from x_old).  The kernel body is
+// written once against load / store / store_comp.
+template <typename T>
+struct HbmPos {
+  static constexpr bool kLds = false;
+  vec4_t<T> *x;
+  __device__ __forceinline__ vec4_t<T> load(int64_t v) const { return x[v]; }
+  // with the inverse mass the caller already holds (ignored here)
+  __device__ __forceinline__ vec4_t<T> load_w(int64_t v, T) const { return x[v]; }
+  __device__ __forceinline__ void store(int64_t v, const vec4_t<T> &p) const { x[v] = p; }
+  __device__ __forceinline__ void store_comp(int64_t v, int c, T val) const {
+    reinterpret_cast<T *>(&x[v])[c] = val;
+  }
+};
+
+template <typename T>
+struct LdsPos {
+  static constexpr bool kLds = true;
+  T *pos;
+  int64_t vb;
+  int nv;
+  const vec4_t<T> *xold;
+  __device__ __forceinline__ vec4_t<T> load(int64_t v) const {
+    const int l = (int)(v - vb);
+    vec4_t<T> p;
+    p.x = pos[l];
+    p.y = pos[nv + l];
+    p.z = pos[2 * nv + l];
+    p.w = xold[v].w;
+    return p;
+  }
+  __device__ __forceinline__ vec4_t<T> load_w(int64_t v, T w) const {
+    const int l = (int)(v - vb);
+    vec4_t<T> p;
+    p.x = pos[l];
+    p.y = pos[nv + l];
+    p.z = pos[2 * nv + l];
+    p.w = w;
+    return p;
+  }
+  __device__ __forceinline__ void store(int64_t v, const vec4_t<T> &p) const {
+    const int l = (int)(v - vb);
+    pos[l] = p.x;
+    pos[nv + l] = p.y;
+    pos[2 * nv + l] = p.z;
+  }
+  __device__ __forceinline__ void store_comp(int64_t v, int c, T val) const {
+    pos[c * nv + (int)(v - vb)] = val;
+  }
+};
+
+// The kernel's static LDS, declared once at kernel scope (function-scope
+// __shared__ variables would be repeated for every instantiation of the
+// scene body), so the dynamic LDS left for positions is as large as it can
+// be: the largest sum is 7 doubles per wavefront.
+template <int NT>
+struct LayeredShared {
+  double shw[7 * (NT / 64)];
+  double sR[kMaxSceneBodies][9], sC[kMaxSceneBodies][3];
+  double sRunUntil;
+  int sStop, sStopSolve;
+};
+
 // makeRigidsFromConnectivity / matComputeRigidBodyProperties3D
 // (AdaptiveMesh3D.m:32-43, RigidBody3D.m:198-265) for body k of the scene,
 // as k_restrict does it, with the workgroup's sums.
-template <typename T, int NT>
-__device__ __forceinline__ void wg_restrict(const LayeredParams<T> &Lp, int k,
-                                            const vec4_t<T> *x, const double *mass,
-                                            double *shw) {
+template <typename T, int NT, typename Pos>
+__device__ __forceinline__ void wg_restrict(const LayeredParams<T> &Lp, int k, const Pos &x,
+                                            const double *mass, double *shw) {
   const int off = Lp.rigOff[k], n = Lp.rigOff[k + 1] - off;
   const T *rv = Lp.rv;
   double a[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int q = threadIdx.x; q < n; q += NT) {
     const int v = Lp.rigVert[off + q];
     const double m = mass[v];
-    const vec4_t<T> p = x[v];
+    const vec4_t<T> p = x.load(v);
     a[0] += m;
     a[1] += (double)p.x * m;
     a[2] += (double)p.y * m;
@@ -83,7 +147,7 @@ __device__ __forceinline__ void wg_restrict(const LayeredParams<T> &Lp, int k,
   for (int q = threadIdx.x; q < n; q += NT) {
     const int v = Lp.rigVert[off + q];
     const double m = mass[v];
-    const vec4_t<T> p = x[v];
+    const vec4_t<T> p = x.load(v);
     const double r[3] = {(double)p.x - com[0], (double)p.y - com[1], (double)p.z - com[2]};
     const double vd[3] = {(double)rv[3 * v] - cdot[0], (double)rv[3 * v + 1] - cdot[1],
                           (double)rv[3 * v + 2] - cdot[2]};
@@ -101,7 +165,17 @@ __device__ __forceinline__ void wg_restrict(const LayeredParams<T> &Lp, int k,
     Lp.ri[3 * v + 1] = r[1];
     Lp.ri[3 * v + 2] = r[2];
   }
-  wg_sum_n<NT, 12>(b, shw);
+  {
+    // two sums of six (each component is reduced on its own, so this is the
+    // 12-wide sum in half the LDS)
+    double lo[6], hi[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) lo[i] = b[i], hi[i] = b[6 + i];
+    wg_sum_n<NT, 6>(lo, shw);
+    wg_sum_n<NT, 6>(hi, shw);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) b[i] = lo[i], b[6 + i] = hi[i];
+  }
   if (threadIdx.x == 0) {
     Body &B = Lp.bodies[k];
     B.M = M;
@@ -138,16 +212,18 @@ __device__ __noinline__ void body_step_one(Body &Bd, double h) {
 // Constraints, xpbdLayer3D.m:381-420): its boundary vertices projected
 // against the body's pose, the sums applied by thread 0, the vertices
 // snapped to the new pose.
-template <typename T, int NT>
-__device__ __forceinline__ void wg_boundary(const LayeredParams<T> &Lp, int kb, vec4_t<T> *x,
+template <typename T, int NT, typename Pos>
+__device__ __forceinline__ void wg_boundary(const LayeredParams<T> &Lp, int kb, const Pos &x,
                                          const double *minv, double h, double *shw,
                                          double *sR, double *sC) {
   const Body &Bd = Lp.bodies[kb];
   const int off = Lp.bndOff[kb], n = Lp.bndOff[kb + 1] - off;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int q = threadIdx.x; q < n; q += NT)
-    boundary_project<T>(Bd, Lp.bndVert[off + q], x, minv, Lp.ri, Lp.lamB + off + q,
-                        Lp.boundaryCompliance, h, acc);
+  for (int q = threadIdx.x; q < n; q += NT) {
+    const int v = Lp.bndVert[off + q];
+    boundary_project_at<T>(Bd, v, x.load(v), minv, Lp.ri, Lp.lamB + off + q,
+                           Lp.boundaryCompliance, h, acc);
+  }
   wg_sum_n<NT, 6>(acc, shw);
   if (threadIdx.x == 0) {
     body_apply(Lp.bodies[kb], acc, acc + 3, h);
@@ -155,30 +231,40 @@ __device__ __forceinline__ void wg_boundary(const LayeredParams<T> &Lp, int kb, 
     for (int i = 0; i < 3; ++i) sC[i] = Lp.bodies[kb].COM[i];
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < n; q += NT) boundary_snap<T>(sR, sC, Lp.bndVert[off + q], Lp.ri, x);
+  for (int q = threadIdx.x; q < n; q += NT) {
+    // boundary_snap through the accessor
+    const int v = Lp.bndVert[off + q];
+    double xn[3];
+    rigid_pos(sR, Lp.ri + 3 * v, sC, xn);
+    vec4_t<T> p = x.load(v);
+    p.x = (T)xn[0];
+    p.y = (T)xn[1];
+    p.z = (T)xn[2];
+    x.store(v, p);
+  }
   __syncthreads();
 }
 
 // Rigid vertices of the scene: after the step, to R r_i + COM with residual
 // R v - (x_new - x)/h (stepped = true); after the layer, to R r_i + COM with
 // the residual rotated by R_projected R_stepped^T (xpbdLayer3D.m:217-241).
-template <typename T, int NT>
+template <typename T, int NT, typename Pos>
 __device__ __forceinline__ void wg_rigid_vertices(const LayeredParams<T> &Lp, int64_t vb,
-                                               int64_t ve, vec4_t<T> *x, double h,
+                                               int64_t ve, const Pos &x, double h,
                                                bool stepped) {
   T *rv = Lp.rv;
   for (int64_t v = vb + threadIdx.x; v < ve; v += NT) {
     const int bid = Lp.vbody[v];
     if (bid == 0) continue;
     const Body &Bd = Lp.bodies[bid - 1];
-    vec4_t<T> p = x[v];
+    vec4_t<T> p = x.load(v);
     double xn[3];
     rigid_pos(Bd.R, Lp.ri + 3 * v, Bd.COM, xn);
     const double prev[3] = {(double)p.x, (double)p.y, (double)p.z};
     p.x = (T)xn[0];
     p.y = (T)xn[1];
     p.z = (T)xn[2];
-    x[v] = p;
+    x.store(v, p);
     const double vr[3] = {(double)rv[3 * v], (double)rv[3 * v + 1], (double)rv[3 * v + 2]};
     const double zero[3] = {0, 0, 0};
     double o[3];
@@ -204,18 +290,21 @@ __device__ __forceinline__ void wg_rigid_vertices(const LayeredParams<T> &Lp, in
 // per layer is projectGS (xpbdLayer3D.m:315-341): colors (elastic slots
 // only), boundary accumulation of every body (rigidElasticBoundary-
 // AccumulationConstraints, :381-420), elastic contacts (:422-485; rigid
-// contacts are off in this path), pins, residual and monitor.
-template <typename T, int NT, int MAT>
+// contacts are off in this path), pins, residual and monitor.  Positions
+// through x (HbmPos or LdsPos); the same arithmetic in the same order
+// either way.
+template <typename T, int NT, int MAT, typename Pos>
 __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
                                                     const LayeredParams<T> &Lp, const int sc,
-                                                    double *shw) {
-  __shared__ int sStop, sStopSolve;
-  __shared__ double sRunUntil;
-  __shared__ double sR[kMaxSceneBodies][9], sC[kMaxSceneBodies][3];
+                                                    const Pos &x, LayeredShared<NT> &S) {
+  double *shw = S.shw;
+  int &sStop = S.sStop, &sStopSolve = S.sStopSolve;
+  double &sRunUntil = S.sRunUntil;
+  double(&sR)[kMaxSceneBodies][9] = S.sR;
+  double(&sC)[kMaxSceneBodies][3] = S.sC;
   const int tid = threadIdx.x;
   const int64_t vb = B.vert0[sc], ve = B.vert0[sc + 1];
   const int b0 = Lp.body0[sc], nbS = Lp.body0[sc + 1] - b0;
-  vec4_t<T> *x = B.P.x;
   T *rv = Lp.rv;
   const double *mass = Lp.mass;
   Ctl c;
@@ -252,14 +341,14 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
     // release of the elastic vertices (:138-139)
     for (int64_t v = vb + tid; v < ve; v += NT) {
       if (coarse && Lp.vbody[v] != 0) continue;
-      vec4_t<T> p = x[v];
+      vec4_t<T> p = x.load(v);
       p.x = p.x + T(h) * rv[3 * v];
       p.y = p.y + T(h) * rv[3 * v + 1];
       p.z = p.z + T(h) * rv[3 * v + 2];
       rv[3 * v] = T(0);
       rv[3 * v + 1] = T(0);
       rv[3 * v + 2] = T(0);
-      x[v] = p;
+      x.store(v, p);
     }
     __syncthreads();
     if (nb > 0) {
@@ -294,8 +383,20 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
     while (it < B.hardCap) {
       for (int k = 0; k < nc; ++k) {
         const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
-        for (int64_t s = a + tid; s < e; s += NT)
-          if (!P.flags || P.flags[s]) project_tet<T, MAT>(P, s);
+        for (int64_t s = a + tid; s < e; s += NT) {
+          if (P.flags && !P.flags[s]) continue;
+          if constexpr (!Pos::kLds) {
+            project_tet<T, MAT>(P, s);
+          } else {
+            TetIn<T> in;
+            tet_fetch<T, MAT>(P, s, in);
+            vec4_t<T> X[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) X[q] = x.load_w(in.v[q], in.O[q].w);
+            tet_solve_st<T, MAT>(P, s, in, X,
+                                 [&](int q, const vec4_t<T> &o) { x.store(in.v[q], o); });
+          }
+        }
         __syncthreads();
       }
       for (int k = 0; k < nb; ++k)
@@ -304,7 +405,7 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
         const int v = B.c.vert[i];
         if (nb > 0 && Lp.vbody[v] != 0) continue;  // rigid contact: not projected
         const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
-        vec4_t<T> p = x[v];
+        vec4_t<T> p = x.load(v);
         const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
         double phi = n[0] * (xv[0] - pc[0]) + n[1] * (xv[1] - pc[1]) + n[2] * (xv[2] - pc[2]);
         if (phi > 0) phi = 0;
@@ -316,25 +417,32 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
         p.y = (T)(xv[1] + mi * (dl * n[1]));
         p.z = (T)(xv[2] + mi * (dl * n[2]));
         B.c.lambda[i] += dl;
-        x[v] = p;
+        x.store(v, p);
       }
       __syncthreads();
       for (int64_t q = q0 + tid; q < q1; q += NT) {
         const int d = B.pinDof[q];
-        T *px = reinterpret_cast<T *>(&x[d / 3]);
-        px[d % 3] = B.pinVal[q];
+        x.store_comp(d / 3, d % 3, B.pinVal[q]);
       }
       __syncthreads();
       ++it;
       if (B.mp.computeResiduals) {
         double acc[1] = {0};
-        for (int64_t s = s0 + tid; s < s1; s += NT)
-          if (B.P.vid[s] >= 0) acc[0] += residual_tet<T, false, MAT>(B.P, s);
+        for (int64_t s = s0 + tid; s < s1; s += NT) {
+          if (B.P.vid[s] < 0) continue;
+          if constexpr (!Pos::kLds) {
+            acc[0] += residual_tet<T, false, MAT>(B.P, s);
+          } else {
+            ResIn<T> in;
+            res_fetch_pos<T, MAT>(B.P, s, in, [&](int v) { return x.load(v); });
+            acc[0] += res_compute<T, MAT>(in);
+          }
+        }
         const double Ks = (double)(k1 - k0);
         for (int64_t i = k0 + tid; i < k1; i += NT) {
           const int v = B.c.vert[i];
           const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
-          const vec4_t<T> p = x[v];
+          const vec4_t<T> p = x.load(v);
           double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]) +
                        n[2] * ((double)p.z - pc[2]);
           if (phi > 0) phi = 0;
@@ -374,14 +482,47 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
   }
 }
 
+template <typename T, int NT, int MAT>
+__device__ __forceinline__ void batch_scene_layered_at(const BatchParams<T> &B,
+                                                       const LayeredParams<T> &Lp, const int sc,
+                                                       T *lds, LayeredShared<NT> &shw) {
+  const int64_t vb = B.vert0[sc];
+  const int nv = (int)(B.vert0[sc + 1] - vb);
+  if (nv > B.ldsVerts) {
+    batch_scene_layered<T, NT, MAT>(B, Lp, sc, HbmPos<T>{B.P.x}, shw);
+    return;
+  }
+  // the scene's positions in LDS from the start to the end of the substep
+  for (int i = threadIdx.x; i < nv; i += NT) {
+    const vec4_t<T> p = B.P.x[vb + i];
+    lds[i] = p.x;
+    lds[nv + i] = p.y;
+    lds[2 * nv + i] = p.z;
+  }
+  __syncthreads();
+  batch_scene_layered<T, NT, MAT>(B, Lp, sc, LdsPos<T>{lds, vb, nv, B.P.xold}, shw);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nv; i += NT) {
+    vec4_t<T> p = B.P.x[vb + i];
+    p.x = lds[i];
+    p.y = lds[nv + i];
+    p.z = lds[2 * nv + i];
+    B.P.x[vb + i] = p;
+  }
+}
+
+// One scene per workgroup; a scene with at most B.ldsVerts vertices keeps
+// its positions in the dynamic LDS (3 values of T per vertex).
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_batch_layered(BatchParams<T> B, LayeredParams<T> Lp) {
-  __shared__ double shw[12 * (NT / 64)];
+  extern __shared__ __align__(16) unsigned char mlx_dyn_lds[];
+  __shared__ LayeredShared<NT> shw;
   const int sc = B.sceneList ? B.sceneList[blockIdx.x] : (int)blockIdx.x;
+  T *lds = reinterpret_cast<T *>(mlx_dyn_lds);
   if (B.sceneNH && B.sceneNH[sc])
-    batch_scene_layered<T, NT, kMatNH>(B, Lp, sc, shw);
+    batch_scene_layered_at<T, NT, kMatNH>(B, Lp, sc, lds, shw);
   else
-    batch_scene_layered<T, NT, kMatStVK>(B, Lp, sc, shw);
+    batch_scene_layered_at<T, NT, kMatStVK>(B, Lp, sc, lds, shw);
 }
 
 }  // namespace mlx

@@ -1148,14 +1148,14 @@ __device__ __forceinline__ void body_apply(Body &B, const double *dCOM, const do
 // rigidBoundaryConstraint.project (rigidBoundaryConstraint.m:4-35) for
 // boundary entry q (vertex v) of body B: updates its multiplier and adds the
 // impulse's dCOM and dOmega terms to acc.
+// boundary_project on the vertex's position p, loaded by the caller.
 template <typename T>
-__device__ __forceinline__ void boundary_project(const Body &B, int v, const vec4_t<T> *x,
-                                                 const double *minv, const double *ri,
-                                                 double *lamq, double compliance, double h,
-                                                 double (&acc)[6]) {
+__device__ __forceinline__ void boundary_project_at(const Body &B, int v, const vec4_t<T> &p,
+                                                    const double *minv, const double *ri,
+                                                    double *lamq, double compliance, double h,
+                                                    double (&acc)[6]) {
   double rx[3], r[3], dir[3], nn[3];
   rigid_pos(B.R, ri + 3 * v, B.COM, rx);
-  const vec4_t<T> p = x[v];
   const double xe[3] = {(double)p.x, (double)p.y, (double)p.z};
   for (int d = 0; d < 3; ++d) {
     r[d] = rx[d] - B.COM[d];
@@ -1177,6 +1177,14 @@ __device__ __forceinline__ void boundary_project(const Body &B, int v, const vec
     acc[d] += pp[d] * B.invM;
     acc[3 + d] += dom[d];
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void boundary_project(const Body &B, int v, const vec4_t<T> *x,
+                                                 const double *minv, const double *ri,
+                                                 double *lamq, double compliance, double h,
+                                                 double (&acc)[6]) {
+  boundary_project_at<T>(B, v, x[v], minv, ri, lamq, compliance, h, acc);
 }
 
 template <typename T>

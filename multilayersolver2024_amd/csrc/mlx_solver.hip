@@ -2807,7 +2807,7 @@ struct mlx_batch {
   int split = 0;     // target tets per workgroup (0: one workgroup per scene)
   int splitWGs = 0;  // workgroups of the last split launch
   int64_t splitFallbacks = 0;  // split launches redone unsplit (XCD placement differed)
-  int useLds = 1;              // scenes that fit keep their positions in LDS (k_batch_plain)
+  int useLds = 1;              // scenes that fit keep their positions in LDS (k_batch_plain, k_batch_layered)
   int ldsVerts = 0;            // vertex limit of the LDS path in the last launch
   bool tablesReady = false;    // static part of d_tables uploaded
   int useGraph = 1;            // substeps after frame 1 replay a captured graph
@@ -2866,25 +2866,18 @@ static void launch_batch(const BatchParams<T> &B, int n, size_t ldsBytes, hipStr
   hipLaunchKernelGGL((k_batch_plain<T, NT>), dim3(n), dim3(NT), ldsBytes, s, B);
 }
 
-template <typename T, int NT>
-static void launch_layered(const BatchParams<T> &B, const LayeredParams<T> &Lp, int n,
-                           hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_layered<T, NT>), dim3(n), dim3(NT), 0, s, B, Lp);
-}
-
 // Scenes with at most this many vertices keep their positions in LDS
 // (3 values of T per vertex), within the device's per-workgroup LDS less
-// the kernel's static LDS; 0 disables the LDS path.
-template <typename T, int NT>
-static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
+// the largest static LDS of the kernels fns; 0 disables the LDS path.
+template <int NF>
+static int lds_verts_for(const void *const (&fns)[NF], size_t elem, int64_t maxSceneVerts,
+                         int enabled, size_t *bytes) {
   *bytes = 0;
   if (!enabled) return 0;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-  const void *fns[2] = {reinterpret_cast<const void *>(&k_batch_plain<T, NT>),
-                        reinterpret_cast<const void *>(&k_batch_split<T, NT>)};
   size_t stat = 0;
   for (const void *fn : fns) {
     hipFuncAttributes fa;
@@ -2893,10 +2886,10 @@ static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
   }
   if (prop.sharedMemPerBlock <= stat + 1024) return 0;
   const size_t avail = prop.sharedMemPerBlock - stat - 256;
-  const int64_t fit = (int64_t)(avail / (3 * sizeof(T)));
+  const int64_t fit = (int64_t)(avail / (3 * elem));
   const int64_t nv = std::min<int64_t>(fit, maxSceneVerts);
   if (nv <= 0) return 0;
-  *bytes = (size_t)nv * 3 * sizeof(T);
+  *bytes = (size_t)nv * 3 * elem;
   if (*bytes > 65536)
     for (const void *fn : fns)
       if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)*bytes) !=
@@ -2905,6 +2898,29 @@ static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
         return 0;
       }
   return (int)nv;
+}
+
+template <typename T, int NT>
+static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
+  const void *const fns[2] = {reinterpret_cast<const void *>(&k_batch_plain<T, NT>),
+                              reinterpret_cast<const void *>(&k_batch_split<T, NT>)};
+  return lds_verts_for(fns, sizeof(T), maxSceneVerts, enabled, bytes);
+}
+
+template <typename T, int NT>
+static int layered_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
+  const void *const fns[1] = {reinterpret_cast<const void *>(&k_batch_layered<T, NT>)};
+  return lds_verts_for(fns, sizeof(T), maxSceneVerts, enabled, bytes);
+}
+
+// k_batch_layered over n scenes; scenes of at most maxSceneVerts vertices
+// that fit the LDS keep their positions there.
+template <typename T, int NT>
+static void launch_layered(BatchParams<T> B, const LayeredParams<T> &Lp, int n,
+                           int64_t maxSceneVerts, int useLds, hipStream_t s) {
+  size_t lb = 0;
+  B.ldsVerts = layered_lds_verts<T, NT>(maxSceneVerts, useLds, &lb);
+  hipLaunchKernelGGL((k_batch_layered<T, NT>), dim3(n), dim3(NT), lb, s, B, Lp);
 }
 
 // First contact of each scene: the number of flagged vertices before the
@@ -3128,8 +3144,8 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     const int n = b.bucketOff[i + 1] - b.bucketOff[i];
     const int blk = b.bucketBlock[i];
     if (b.layered) {
-      if (blk == 256) launch_layered<T, 256>(Bi, Lp, n, bs);
-      else launch_layered<T, 512>(Bi, Lp, n, bs);
+      if (blk == 256) launch_layered<T, 256>(Bi, Lp, n, b.bucketMaxVerts[i], b.useLds, bs);
+      else launch_layered<T, 512>(Bi, Lp, n, b.bucketMaxVerts[i], b.useLds, bs);
       return;
     }
     size_t lb = 0;
@@ -3155,10 +3171,10 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
       return;
     }
     if (b.layered) {
-      if (b.block == 256) launch_layered<T, 256>(B, Lp, ns, s);
-      else if (b.block == 768) launch_layered<T, 768>(B, Lp, ns, s);
-      else if (b.block == 1024) launch_layered<T, 1024>(B, Lp, ns, s);
-      else launch_layered<T, 512>(B, Lp, ns, s);
+      if (b.block == 256) launch_layered<T, 256>(B, Lp, ns, maxSceneVerts, b.useLds, s);
+      else if (b.block == 768) launch_layered<T, 768>(B, Lp, ns, maxSceneVerts, b.useLds, s);
+      else if (b.block == 1024) launch_layered<T, 1024>(B, Lp, ns, maxSceneVerts, b.useLds, s);
+      else launch_layered<T, 512>(B, Lp, ns, maxSceneVerts, b.useLds, s);
       return;
     }
     if (b.block == 256) launch_batch<T, 256>(B, ns, ldsBytes, s);

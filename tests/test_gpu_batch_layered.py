@@ -147,3 +147,37 @@ def test_layered_batch_fp32_full_workload_runs():
             assert proj > 0
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("precision,block", [(L.MLX_FP64, 256), (L.MLX_FP32, 512)])
+def test_layered_lds_positions_equal_hbm_positions(precision, block):
+    """Scenes whose positions stay in LDS for the whole two-layer substep
+    (restriction, rigid step, colors, boundary accumulation, contacts, pins,
+    residual, prolongation) run the same arithmetic in the same order as
+    with positions in HBM: bit-identical positions, velocities, lambdas,
+    iteration and contact counts, on uneven scenes with 1..3 clusters,
+    alternating materials, through touchdown."""
+    scs, masks = cluster_workload(8, seed=3, tets=(600, 6000))
+    for sc in scs:
+        sc["p"] = sc["p"] - np.array([0, 0, 0.045])
+    merged = merge_meshes([Mesh(s) for s in scs])
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    it = _integ(iterations=(3, 10), hanging=40)
+    a = BatchSimulation(merged, it, H, plane, precision=precision, block=block)
+    b = BatchSimulation(merged, copy.deepcopy(it), H, plane, precision=precision, block=block)
+    for s_ in (a, b):
+        s_.set_rigid(np.concatenate(masks))
+    a.set_lds(0)
+    b.set_lds(1)
+    try:
+        for k in range(6):
+            a.step()
+            b.step()
+            pa, va, la = a.download()
+            pb, vb, lb = b.download()
+            _say("lds", k, list(a.iters), list(b.iters), rel(pb, pa))
+            assert np.array_equal(a.iters, b.iters) and np.array_equal(a.contacts, b.contacts)
+            assert np.array_equal(pa, pb) and np.array_equal(va, vb) and np.array_equal(la, lb)
+    finally:
+        a.close()
+        b.close()
