@@ -6,8 +6,9 @@ reference pins these functions; the checks below come from the mathematics:
 finite differences of ||F||_F and det F, closed forms under scaling and
 rotation, the XPBD update rule against finite-difference gradients, and the
 rest state as a fixed point at the equilibrium multipliers.  The last test
-records the convergence behaviour that keeps the pair out of the GPU
-substep at h = 1/60."""
+records the convergence behaviour at the beam's step h = 1/60: the pair
+does not converge there (the NeoHookean half of the batched workload runs to
+hangingStop), and does with small steps."""
 import ctypes as C
 import math
 
