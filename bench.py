@@ -461,6 +461,7 @@ def set_options(sim, args):
     sim.dmesh.set_option(L.MLX_OPT_PERSIST_LOCAL, args.persist_local)
     sim.dmesh.set_option(L.MLX_OPT_SWEEP_BLOCK, args.sweep_block)
     sim.dmesh.set_option(L.MLX_OPT_SWEEP_WG, args.sweep_wg)
+    sim.dmesh.set_option(L.MLX_OPT_FLOW, args.flow)
 
 
 def rel(a, b):
@@ -500,6 +501,9 @@ def main():
     ap.add_argument("--sweep-wg", type=int, default=0,
                     help="sparse rigid layers sweep all colors in one workgroup (1) or launch "
                          "per color (0)")
+    ap.add_argument("--flow", type=int, default=0, choices=[0, 1],
+                    help="layers without rigid elements sweep their colors in the dataflow kernel "
+                         "(k_flow: per-vertex update counters instead of one launch per color)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
                     help="timed batched substeps replay a captured graph (1) or launch directly (0, for kernel-trace profiling)")
@@ -651,7 +655,17 @@ def main():
               "elements": elems.value, "bytes_per_unit": bpt,
               "avg_launch_us": 1e3 * ms.value / max(launches.value, 1),
               "achieved_GBs": (elems.value * bpt) / sweep_s / 1e9 if sweep_s > 0 else 0.0}
-    if plaunches.value > 0:
+    if plaunches.value > 0 and args.flow:
+        # k_flow: one launch = one iteration's sweep of the all-elastic
+        # layer (its colors, contacts and pins; the residual follows in
+        # k_residual_tail); one unit = one tet projection
+        pk_s = pms.value / 1e3
+        achieved = pelems.value * bpt / pk_s / 1e9
+        roof = {"kernel": f"k_flow<{ftype},0>", "bytes_per_unit": bpt,
+                "avg_launch_us": 1e3 * pms.value / plaunches.value, "launches": plaunches.value,
+                "elements_per_launch": pelems.value / plaunches.value,
+                "per_color_sweeps": sweeps}
+    elif plaunches.value > 0:
         # one unit = one element in one iteration: its projection plus its
         # residual evaluation (the monitor runs every iteration)
         bpu = bpt + RESIDUAL_BYTES_PER_TET[args.precision]
@@ -697,6 +711,7 @@ def main():
             "graphs": bool(args.graphs),
             "sweep_block": args.sweep_block, "sweep_wg": bool(args.sweep_wg),
             "persistent": bool(args.persistent),
+            "flow": bool(args.flow),
             "persist_local": bool(args.persist_local) if args.persistent else None,
             "colors": ncol,
             "mean_iterations_per_substep": w["mean_iterations"],

@@ -49,6 +49,7 @@ MLX_OPT_DEVICE_MAPS = 5
 MLX_OPT_PERSIST_LOCAL = 6
 MLX_OPT_SWEEP_BLOCK = 7
 MLX_OPT_SWEEP_WG = 8
+MLX_OPT_FLOW = 9
 
 _p = C.c_void_p
 _d = C.POINTER(C.c_double)

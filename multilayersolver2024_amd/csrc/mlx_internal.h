@@ -171,6 +171,14 @@ struct mlx_mesh {
   int persistLocal = 1;            // MLX_OPT_PERSIST_LOCAL: one XCD's workgroups, xcd_barrier
   int sweepBlock = 64;             // MLX_OPT_SWEEP_BLOCK: threads per workgroup of a color sweep
   int useSweepWg = 0;              // MLX_OPT_SWEEP_WG: one-workgroup sweeps for sparse layers
+  // dataflow sweep of layers without rigid elements (MLX_OPT_FLOW, k_flow)
+  int useFlow = 0;
+  bool flowReady = false;          // d_need / d_deg built for the mesh
+  mlx::DevBuf d_need;              // int32 4*ld: updates of a slot's vertex before its own
+  mlx::DevBuf d_deg;               // int32 nV: elements of each vertex
+  mlx::DevBuf d_vcnt;              // uint32 nV: updates applied in the layer
+  mlx::DevBuf d_flowsync;          // uint32 64: err | ... | barrier (15) | XCD group counters (16..32)
+  int32_t *h_flowerr = nullptr;    // pinned copy of the error word
   mlx::DevBuf d_elist;             // colorOff i64 | colorCnt i64 | count i32 | elastic slot lists i32
   mlx::DevBuf d_colors;            // int32 colorOff[numColors] | colorCnt[numColors]
   // layer maps on the device (mlx_maps.hip)

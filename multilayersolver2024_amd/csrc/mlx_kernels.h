@@ -2877,6 +2877,121 @@ __global__ __launch_bounds__(kBlock) void k_persist(PersistParams<T> Q) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// dataflow color sweep (one iteration of a layer without rigid elements)
+// ---------------------------------------------------------------------------
+
+// Per-vertex order of the element updates.  Gauss-Seidel over graph colors
+// fixes, for every vertex, the order in which its elements update it:
+// ascending color (the elements of one color share no vertex).  A slot's
+// update of its vertex a is therefore the need[a]-th update of that vertex
+// in the iteration (need = its elements of lower color).  With a counter of
+// applied updates per vertex, a slot may run as soon as each of its four
+// counters has reached base * deg + need (base = the layer's iterations
+// already run, deg = the vertex's elements): it then reads exactly the
+// positions the color-by-color order gives it, without waiting for whole
+// colors.  Same arithmetic as sweep_slot, so the results are bit-identical
+// to per-color launches.
+template <typename T>
+struct FlowParams {
+  SweepParams<T> P;              // P.ctl the layer's control block, P.flags null
+  const int32_t *colorOff;       // first slot of each color
+  const int32_t *colorCnt;       // elements of each color
+  int numColors;
+  const int32_t *need;           // 4 ld: updates of the slot's vertex a before its own
+  const int32_t *deg;            // nV: element updates of a vertex per iteration
+  unsigned *vcnt;                // nV: updates applied in the layer (zero at its start)
+  unsigned base;                 // iterations of the layer run by earlier launches
+  ContactArrays ca;
+  const int32_t *ecIdx;          // elastic contacts of the layer
+  int nec;
+  const double *minv;
+  double resistance, h;
+  const int32_t *pinDof;
+  const T *pinVal;
+  int64_t nPin;
+  unsigned *bar;                 // XCD barrier arrivals, zero at launch
+  int *err;                      // wait or barrier failure flag
+  unsigned *reg;                 // 17 XCD group counters, zero at launch
+  long spinLimit;                // polls of one wait before it fails
+};
+
+template <typename T, int MAT>
+__device__ __forceinline__ bool flow_slot(const FlowParams<T> &Q, const int64_t s) {
+  const SweepParams<T> &P = Q.P;
+  TetIn<T> in;
+  unsigned target[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) in.v[a] = P.vid[a * P.ld + s];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    target[a] = Q.base * (unsigned)Q.deg[in.v[a]] + (unsigned)Q.need[a * P.ld + s];
+  // the slot's data does not change in the launch: fetched while waiting
+  tet_fetch_rest<T, MAT>(P, s, in);
+  long spins = 0;
+  for (;;) {
+    unsigned c[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) c[a] = ldc<true>(Q.vcnt + in.v[a]);
+    if ((c[0] >= target[0]) & (c[1] >= target[1]) & (c[2] >= target[2]) & (c[3] >= target[3]))
+      break;
+    ++spins;
+    if (spins > Q.spinLimit || ((spins & 255) == 0 && ldc<true>(Q.err) != 0)) {
+      __hip_atomic_store(Q.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  vec4_t<T> X[4];
+  tet_gather<T, true>(P, in, X);
+  tet_solve_x<T, MAT>(P, s, in, X);
+  // the positions reach the L2 before the counters say so; readers load
+  // them past the vector L1 (ldc<true>)
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    __hip_atomic_fetch_add(Q.vcnt + in.v[a], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// One iteration's sweep of a layer without rigid elements on the
+// workgroups of one XCD (xcd_group; the others return): every thread takes
+// its slot of each color in color order and runs it when its vertices'
+// counters allow (flow_slot), with no barrier between colors; then, as
+// projectGS does after its colors, the elastic contacts (distinct
+// vertices) and the pins, each after a barrier of the group.  The residual
+// and the monitor follow in k_residual_tail.  Launched cooperatively, so
+// every workgroup is resident; every wait is bounded and a failed one sets
+// *err (the kernel then returns without finishing the sweep).
+template <typename T, int MAT>
+__global__ __launch_bounds__(kBlock) void k_flow(FlowParams<T> Q) {
+  int G = 0;
+  const int w = xcd_group(Q.reg, Q.err, &G);
+  if (w < 0) return;
+  const Ctl *ctl = Q.P.ctl;
+  if ((ctl->stopSolve | ctl->endLayer) != 0) return;  // every workgroup reads the same word
+  const int t = threadIdx.x;
+  const int64_t stride = (int64_t)G * kBlock, first = (int64_t)w * kBlock + t;
+  bool ok = true;
+  for (int c = 0; c < Q.numColors && ok; ++c) {
+    const int64_t off = Q.colorOff[c], end = off + Q.colorCnt[c];
+    for (int64_t s = off + first; s < end && ok; s += stride) ok = flow_slot<T, MAT>(Q, s);
+  }
+  unsigned nbar = 0;
+  xcd_arrive(Q.bar);
+  if (!xcd_wait(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
+  if (ldc<true>(Q.err) != 0) return;
+  for (int64_t q = first; q < Q.nec; q += stride)
+    contact_elastic_one<T, 3, true>(Q.ca, Q.ecIdx[q], Q.P.x, Q.minv, Q.resistance, Q.h);
+  if (Q.nPin > 0) {
+    xcd_arrive(Q.bar);
+    if (!xcd_wait(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
+    for (int64_t q = first; q < Q.nPin; q += stride) pin_one<T, 3>(Q.pinDof, Q.pinVal, Q.P.x, q);
+  }
+}
+
 }  // namespace mlx
 
 namespace mlx {

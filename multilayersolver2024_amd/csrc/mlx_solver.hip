@@ -538,6 +538,49 @@ int stage_upload(mlx_mesh &m, void *dst, const void *src, size_t bytes) {
   return MLX_OK;
 }
 
+// Per-slot update order of the dataflow sweep (k_flow), built once per
+// mesh: deg[v] = elements of vertex v; need[a ld + s] = elements of the
+// slot's vertex a with a lower color than the slot's (the updates that
+// vertex receives before this slot's in every iteration).
+static int flow_prepare(mlx_mesh &m) {
+  if (m.flowReady) return MLX_OK;
+  const int64_t nV = m.nV, nT = m.nT, ld = m.ld;
+  const int vpe = m.vpe;
+  std::vector<int32_t> deg(nV, 0), off(nV + 1, 0);
+  for (int64_t e = 0; e < nT; ++e)
+    for (int a = 0; a < vpe; ++a) deg[m.T[(size_t)e * vpe + a]]++;
+  for (int64_t v = 0; v < nV; ++v) off[v + 1] = off[v] + deg[v];
+  // colors of each vertex's elements
+  std::vector<int32_t> col(off[nV]), fill(off.begin(), off.end() - 1);
+  for (int64_t e = 0; e < nT; ++e)
+    for (int a = 0; a < vpe; ++a) {
+      const int32_t v = m.T[(size_t)e * vpe + a];
+      col[fill[v]++] = m.color[e];
+    }
+  std::vector<int32_t> need((size_t)vpe * ld, 0);
+  for (int64_t sl = 0; sl < ld; ++sl) {
+    const int32_t e = m.perm[sl];
+    if (e < 0) continue;
+    for (int a = 0; a < vpe; ++a) {
+      const int32_t v = m.T[(size_t)e * vpe + a];
+      int32_t n = 0;
+      for (int32_t q = off[v]; q < off[v + 1]; ++q) n += col[q] < m.color[e];
+      need[(size_t)a * ld + sl] = n;
+    }
+  }
+  MLX_TRY(m.d_need.reserve(sizeof(int32_t) * need.size()));
+  MLX_TRY(m.d_deg.reserve(sizeof(int32_t) * (size_t)nV));
+  MLX_TRY(m.d_vcnt.reserve(sizeof(uint32_t) * (size_t)nV));
+  MLX_TRY(m.d_flowsync.reserve(sizeof(uint32_t) * 64));
+  if (!m.h_flowerr) MLX_CHECK_HIP(hipHostMalloc((void **)&m.h_flowerr, sizeof(int32_t)));
+  MLX_CHECK_HIP(hipMemcpy(m.d_need.ptr, need.data(), sizeof(int32_t) * need.size(),
+                          hipMemcpyHostToDevice));
+  MLX_CHECK_HIP(hipMemcpy(m.d_deg.ptr, deg.data(), sizeof(int32_t) * (size_t)nV,
+                          hipMemcpyHostToDevice));
+  m.flowReady = true;
+  return MLX_OK;
+}
+
 static int read_ctl(mlx_mesh &m, Ctl &out) {
   MLX_CHECK_HIP(hipMemcpyAsync(m.h_ctl, m.d_ctl.ptr, sizeof(Ctl), hipMemcpyDeviceToHost, m.stream));
   MLX_CHECK_HIP(hipStreamSynchronize(m.stream));
@@ -788,6 +831,37 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       for (auto &e : m.prof.pev)
         if (!e) MLX_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   }
+  // dataflow sweep: per-slot order (once per mesh), the color table, the
+  // group size (one slot per thread per color) and the grid (that many
+  // workgroups on each of the 8 XCDs; one XCD's group runs)
+  int flowG = 0;
+  if (m.useFlow && m.dim == 3) {
+    MLX_TRY(flow_prepare(m));
+    std::vector<int32_t> ct(2 * (size_t)m.numColors);
+    int64_t maxCnt = 1;
+    for (int c = 0; c < m.numColors; ++c) {
+      ct[c] = (int32_t)m.colorOff[c];
+      ct[m.numColors + c] = (int32_t)m.colorCnt[c];
+      maxCnt = std::max<int64_t>(maxCnt, m.colorCnt[c]);
+    }
+    MLX_TRY(m.d_colors.reserve(sizeof(int32_t) * ct.size() + 4));
+    MLX_TRY(stage_upload(m, m.d_colors.ptr, ct.data(), sizeof(int32_t) * ct.size()));
+    flowG = (int)std::min<int64_t>(32, std::max<int64_t>(4, (maxCnt + kBlock - 1) / kBlock));
+    int perCU = 0, dev = 0;
+    const void *fn = m.hasNH ? reinterpret_cast<const void *>(&k_flow<T, kMatMixed>)
+                             : reinterpret_cast<const void *>(&k_flow<T, kMatStVK>);
+    MLX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, kBlock, 0));
+    MLX_CHECK_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    MLX_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
+    if ((int64_t)flowG * kXcds > (int64_t)perCU * prop.multiProcessorCount) {
+      set_error("dataflow sweep: %d workgroups do not fit the device", flowG * kXcds);
+      return MLX_ERR_STATE;
+    }
+    if (m.prof.on)
+      for (auto &e : m.prof.pev)
+        if (!e) MLX_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+  }
   res.projections = 0;
   res.layersRun = 0;
   Ctl hc_ctl;
@@ -893,6 +967,16 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
 
     hc_ctl.iters = 0;  // k_layer_begin cleared the device count
     const bool sweepWg = wgLayer[col] != 0;
+    // dataflow sweep (k_flow) for a layer without rigid elements: the
+    // per-vertex counters and the error word start at zero
+    const bool flow = flowG > 0 && !hasRigidEl && L.nb == 0 && !sweepWg;
+    unsigned flowBase = 0;
+    unsigned *flowSync = m.d_flowsync.as<unsigned>();
+    if (flow) {
+      hipLaunchKernelGGL(k_fill<unsigned>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV,
+                         m.d_vcnt.as<unsigned>(), 0u);
+      hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(kBlock), 0, st, (int64_t)64, flowSync, 0u);
+    }
     if (sweepWg) {
       char *eb = m.d_elist.as<char>();
       const size_t nc = (size_t)m.numColors;
@@ -903,8 +987,62 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
                          reinterpret_cast<int32_t *>(eb + 16 * nc));
     }
     int iterEnqueued = 0;
+    auto enqueue_flow = [&]() -> int {
+      FlowParams<T> Q;
+      Q.P = P;
+      Q.P.flags = nullptr;
+      Q.colorOff = m.d_colors.as<int32_t>();
+      Q.colorCnt = m.d_colors.as<int32_t>() + m.numColors;
+      Q.numColors = m.numColors;
+      Q.need = m.d_need.as<int32_t>();
+      Q.deg = m.d_deg.as<int32_t>();
+      Q.vcnt = m.d_vcnt.as<unsigned>();
+      Q.base = flowBase++;
+      Q.ca = ca;
+      Q.ecIdx = LL.ec_idx;
+      Q.nec = L.nec;
+      Q.minv = minv;
+      Q.resistance = sp.contactResistance;
+      Q.h = sp.h;
+      Q.pinDof = m.d_pinned.as<int32_t>();
+      Q.pinVal = m.d_pinnedx.as<T>();
+      Q.nPin = nPin;
+      Q.err = reinterpret_cast<int *>(flowSync);
+      Q.bar = flowSync + 15;
+      Q.reg = flowSync + 16;
+      Q.spinLimit = 1L << 22;
+      // the barrier and XCD-group counters start at zero in every launch
+      hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(kBlock), 0, st, (int64_t)18, flowSync + 15,
+                         0u);
+      void *args[] = {&Q};
+      const void *fn = m.hasNH ? reinterpret_cast<const void *>(&k_flow<T, kMatMixed>)
+                               : reinterpret_cast<const void *>(&k_flow<T, kMatStVK>);
+      const bool timed = m.prof.on && m.prof.pev[0] && m.prof.pev[1];
+      if (timed) MLX_CHECK_HIP(hipEventRecord(m.prof.pev[0], st));
+      MLX_CHECK_HIP(hipLaunchCooperativeKernel(fn, dim3(flowG * kXcds), dim3(kBlock), args, 0, st));
+      if (timed) {
+        MLX_CHECK_HIP(hipEventRecord(m.prof.pev[1], st));
+        // one launch is one iteration's sweep when the layer has not stopped
+        float ms = 0;
+        MLX_CHECK_HIP(hipEventSynchronize(m.prof.pev[1]));
+        MLX_CHECK_HIP(hipEventElapsedTime(&ms, m.prof.pev[0], m.prof.pev[1]));
+        Ctl c;
+        MLX_TRY(read_ctl(m, c));
+        if (!c.stopSolve && !c.endLayer) {
+          m.prof.pms += ms;
+          m.prof.plaunches += 1;
+          m.prof.piters += 1;
+          m.prof.pelements += nElastic[col];
+        }
+      }
+      return MLX_OK;
+    };
+    int flowErr = MLX_OK;  // first failure of a flow launch in this layer
     auto enqueue_iteration = [&]() {
-      if (sweepWg)
+      if (flow) {
+        const int r = enqueue_flow();
+        if (r != MLX_OK && flowErr == MLX_OK) flowErr = r;
+      } else if (sweepWg)
         enqueue_sweep_wg<T>(m, P);
       else
         enqueue_sweeps<T>(m, P, fl, perColor.data() + (size_t)m.numColors * col, iterEnqueued++);
@@ -925,7 +1063,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
         hipLaunchKernelGGL(k_boundary2d<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, x, minv, bodies,
                            ri, m.d_lamB.as<double>(), sp.boundaryCompliance / (sp.h * sp.h),
                            ctl);
-      if (L.nec > 0) {
+      if (L.nec > 0 && !flow) {
         if (m.dim == 3)
           hipLaunchKernelGGL((k_contacts_elastic<T, 3>), dim3(blocks_for((int64_t)contact_cap(m))), dim3(kBlock), 0,
                              st, LL, ca, x, minv, sp.contactResistance, sp.h, ctl);
@@ -939,7 +1077,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       if (m.dim == 2 && L.nrc > 0 && sp.useRigidContacts)
         hipLaunchKernelGGL(k_contacts_rigid2d<T>, dim3(L.nb), dim3(kBlock), 0, st, LL, ca, x,
                            bodies, ri, sp.contactResistance, sp.h, ctl);
-      if (nPin > 0) {
+      if (nPin > 0 && !flow) {
         if (m.dim == 3)
           hipLaunchKernelGGL((k_pins<T, 3>), dim3(blocks_for(nPin)), dim3(kBlock), 0, st, nPin,
                              m.d_pinned.as<int32_t>(), m.d_pinnedx.as<T>(), x, ctl);
@@ -1023,7 +1161,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     // Layers without rigid elements: the iterations run inside one
     // cooperative launch (k_persist) per call.
     // (StVK-Voigt meshes only: the persistent loop has no NeoHookean body)
-    const bool persist = m.usePersistent && !hasRigidEl && L.nb == 0 && !m.hasNH;
+    const bool persist = m.usePersistent && !hasRigidEl && L.nb == 0 && !m.hasNH && !flow;
     auto run_persistent = [&](int n) -> int {
       if (n <= 0) return MLX_OK;
       MLX_CHECK_HIP(hipMemsetAsync(m.d_sync.ptr, 0, 256, st));
@@ -1089,7 +1227,7 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     auto run_iterations = [&](int n) -> int {
       if (persist) return run_persistent(n);
       int left = n;
-      if (m.useGraphs && !m.prof.on && L.nb == 0) {
+      if (m.useGraphs && !m.prof.on && L.nb == 0 && !flow) {
         for (int G = 16; G >= 2; G /= 2) {
           if (G > m.graphIters) continue;
           while (left >= G) {
@@ -1135,6 +1273,15 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
           set_error("runUntil loop exceeded 1e7 iterations without converging (hangingStop < 0?)");
           return MLX_ERR_STATE;
         }
+      }
+    }
+    if (flow) {
+      if (flowErr != MLX_OK) return flowErr;
+      MLX_CHECK_HIP(hipMemcpyAsync(m.h_flowerr, flowSync, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      MLX_CHECK_HIP(hipStreamSynchronize(st));
+      if (*m.h_flowerr != 0) {
+        set_error("dataflow sweep: a wait or barrier of k_flow did not complete");
+        return MLX_ERR_STATE;
       }
     }
     if (m.dim == 2 && L.nb > 0)
@@ -1637,7 +1784,8 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_bodycnt, &m->d_seam, &m->d_elastic, &m->learn.d_lam0,
                          &m->learn.d_keyScale, &m->learn.d_save, &m->learn.d_dx,
                          &m->learn.d_inc, &m->learn.d_res, &m->learn.d_counts,
-                         &m->learn.d_feat, &m->d_finder};
+                         &m->learn.d_feat, &m->d_finder, &m->d_need, &m->d_deg,
+                         &m->d_vcnt, &m->d_flowsync};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)
@@ -1646,6 +1794,7 @@ int mlx_mesh_destroy(mlx_mesh *m) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (m->h_ctl) (void)hipHostFree(m->h_ctl);
   if (m->h_counts) (void)hipHostFree(m->h_counts);
+  if (m->h_flowerr) (void)hipHostFree(m->h_flowerr);
   if (m->h_stage) (void)hipHostFree(m->h_stage);
   for (char *p : m->stageRetired) (void)hipHostFree(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -2622,6 +2771,9 @@ int mlx_mesh_set_option(mlx_mesh *m, int32_t option, int64_t value) {
       return MLX_OK;
     case MLX_OPT_PERSIST_LOCAL:
       m->persistLocal = value != 0;
+      return MLX_OK;
+    case MLX_OPT_FLOW:
+      m->useFlow = value != 0;
       return MLX_OK;
     case MLX_OPT_SWEEP_WG:
       m->useSweepWg = value != 0;

@@ -1,0 +1,142 @@
+"""The dataflow sweep (MLX_OPT_FLOW, k_flow): in a layer without rigid
+elements each element of an iteration runs as soon as its own vertices have
+received their updates from the elements of lower color, with no barrier
+between colors.  Every vertex then sees its updates in the order of the
+per-color launches, the element arithmetic is the same routine, and the
+contacts, pins, residual and monitor are the same, so whole substeps are
+bit-identical to per-color launches: positions, velocities, lambdas,
+iteration counts and residual traces."""
+import copy
+
+import numpy as np
+import pytest
+
+from helpers import beam_case, rel
+from multilayersolver2024_amd import _lib as L
+from multilayersolver2024_amd import scenes
+from multilayersolver2024_amd.integrator import PlaneContactFinder, Simulation, XpbdLayer3D
+from multilayersolver2024_amd.mesh import Mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[flow]", *a, flush=True)
+
+
+def _pair(m, integ, plane, precision):
+    a = Simulation(m, integ, 1 / 60, plane, precision=precision)
+    b = Simulation(m, copy.deepcopy(integ), 1 / 60, plane, precision=precision)
+    a.dmesh.set_option(L.MLX_OPT_FLOW, 0)
+    b.dmesh.set_option(L.MLX_OPT_FLOW, 1)
+    return a, b
+
+
+def _compare(a, b, steps, tag):
+    kmax = 0
+    for k in range(steps):
+        ra, rb = a.step(), b.step()
+        pa, va, la, ka = a.download()
+        pb, vb, lb, kb = b.download()
+        kmax = max(kmax, ka)
+        _say(tag, k, ra.itersPerLayer, rb.itersPerLayer, ka, rel(pb, pa), rel(lb, la))
+        assert ra.itersPerLayer == rb.itersPerLayer and ka == kb
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb) and np.array_equal(la, lb)
+        assert np.array_equal(ra.residualArray, rb.residualArray)
+    return kmax
+
+
+@pytest.mark.parametrize("precision,size", [(L.MLX_FP64, "small"), (L.MLX_FP32, "small"),
+                                            (L.MLX_FP64, "large"), (L.MLX_FP32, "large")])
+def test_flow_sweep_equals_per_color_launches(precision, size):
+    """Layers [98.4375, 87.5, 0] (the coarse layers launch per color, the
+    all-elastic one runs k_flow), ground contacts, pinned end, relative
+    stop; the large beam (19,200 tets) has colors of ~480 tets over several
+    workgroups."""
+    if size == "small":
+        m, integ, plane = beam_case(nx=10, ny=4, nz=4, z0=0.004, pct=99.99)
+    else:
+        m, integ, plane = beam_case(nx=60, ny=8, nz=8, z0=0.004, pct=99.99)
+    a, b = _pair(m, integ, plane, precision)
+    try:
+        assert _compare(a, b, 8 if size == "small" else 5, f"layered {size}") > 0
+    finally:
+        a.close()
+        b.close()
+
+
+def test_flow_sweep_plain_method_equals_per_color_launches():
+    """A single all-elastic layer (every iteration through k_flow),
+    StVK-Voigt, FP64."""
+    m = Mesh(scenes.beam3d(12, 3, 3, seed=4, z0=0.004))
+    it = XpbdLayer3D()
+    it.Gravity = -9.81
+    it.iterations = [12]
+    it.computeResiduals = True
+    it.StopAtImprovementPercent = 99.99
+    it.hangingStop = 60
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    a, b = _pair(m, it, plane, L.MLX_FP64)
+    try:
+        _compare(a, b, 6, "plain")
+    finally:
+        a.close()
+        b.close()
+
+
+def test_flow_sweep_mixed_materials_match_per_color_launches():
+    """Every other element NeoHookean (the mixed-material slot routine), a
+    single all-elastic layer, FP64, at h = 1/6000 where the NeoHookean pair
+    converges.  The same updates in the same order; the two kernels may
+    round the NeoHookean terms differently in the last bit (at h = 1/60,
+    where the pair does not converge, such differences grow to O(1) within
+    tens of iterations in any two builds), so states are compared to 1e-12
+    and iteration counts exactly."""
+    sc = scenes.beam3d(12, 3, 3, seed=4, z0=0.004)
+    sc["neohookean"] = (np.arange(len(sc["t"])) % 2) == 1
+    m = Mesh(sc)
+    it = XpbdLayer3D()
+    it.Gravity = -9.81
+    it.iterations = [12]
+    it.computeResiduals = True
+    it.StopAtImprovementPercent = 99.99
+    it.hangingStop = 200
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    a = Simulation(m, it, 1 / 6000, plane, precision=L.MLX_FP64)
+    b = Simulation(m, copy.deepcopy(it), 1 / 6000, plane, precision=L.MLX_FP64)
+    a.dmesh.set_option(L.MLX_OPT_FLOW, 0)
+    b.dmesh.set_option(L.MLX_OPT_FLOW, 1)
+    try:
+        for k in range(6):
+            ra, rb = a.step(), b.step()
+            pa, va, la, ka = a.download()
+            pb, vb, lb, kb = b.download()
+            _say("mixed", k, ra.itersPerLayer, rb.itersPerLayer, ka, rel(pb, pa), rel(lb, la),
+                 "bit-identical" if np.array_equal(pa, pb) and np.array_equal(la, lb) else "")
+            assert ra.itersPerLayer == rb.itersPerLayer and ka == kb
+            assert rel(pb, pa) < 1e-12 and rel(lb, la) < 1e-12 and rel(vb, va) < 1e-10
+    finally:
+        a.close()
+        b.close()
+
+
+def test_flow_sweep_at_full_size_equals_per_color_launches():
+    """The 81,920-tet beam (64 x 16 x 16 x 5), FP32, layers [98.4375, 87.5,
+    0], through touchdown: every k_flow launch spans nine workgroups of one
+    XCD and ~2,050 tets per color."""
+    m = Mesh(scenes.beam3d(64, 16, 16, seed=0))
+    it = XpbdLayer3D()
+    it.Gravity = -9.81
+    it.setLayers([98.4375, 87.5, 0])
+    it.iterations = [2, 4, 14]
+    it.computeResiduals = True
+    it.StopAtImprovementPercent = 99.99
+    it.hangingStop = 1000
+    it.useRigidContacts = False
+    plane = PlaneContactFinder([0, 0, 1], [0, 0, 0])
+    a, b = _pair(m, it, plane, L.MLX_FP32)
+    try:
+        assert _compare(a, b, 8, "full") > 0
+    finally:
+        a.close()
+        b.close()
