@@ -576,13 +576,12 @@ int mlx_profile_read_persistent(mlx_mesh *mesh, double *ms, int64_t *launches,
 #define MLX_OPT_SWEEP_WG 8
 /* MLX_OPT_FLOW (default 0; 3D): in a layer without rigid elements each
  * iteration's color sweep, elastic contacts and pins run in one
- * cooperative launch on the workgroups of one XCD, where every element
- * waits only for its own vertices' earlier updates (a counter per vertex)
- * instead of for the whole previous color.  Same updates in the same order
- * per vertex as the per-color launches: bit-identical results (StVK-Voigt;
- * NeoHookean terms may round differently in the last bit).  Off by
- * default: on the 81,920-tet beam a launch takes 171 us for the 40 colors
- * (4.3 us per color against 3.5 us per graph-replayed color launch). */
+ * cooperative launch on the workgroups of one XCD, where every chunk of 64
+ * slots of a color waits only for the chunks that last updated its
+ * vertices instead of for the whole previous color.  Same updates in the
+ * same order per vertex as the per-color launches: bit-identical results.
+ * Off by default: on the 81,920-tet beam a launch takes 160-185 us for the
+ * 40 colors (about 4 us per color against 3.5 us per color launch). */
 #define MLX_OPT_FLOW 9
 int mlx_mesh_set_option(mlx_mesh *mesh, int32_t option, int64_t value);
 

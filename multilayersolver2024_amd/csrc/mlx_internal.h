@@ -174,9 +174,9 @@ struct mlx_mesh {
   // dataflow sweep of layers without rigid elements (MLX_OPT_FLOW, k_flow)
   int useFlow = 0;
   bool flowReady = false;          // d_need / d_deg built for the mesh
-  mlx::DevBuf d_need;              // int32 4*ld: updates of a slot's vertex before its own
-  mlx::DevBuf d_deg;               // int32 nV: elements of each vertex
-  mlx::DevBuf d_vcnt;              // uint32 nV: updates applied in the layer
+  mlx::DevBuf d_flowdeps;          // int32 chunkBase[numColors+1] | depOff[nChunks+1] | dep
+  mlx::DevBuf d_flowflags;         // uint32 nChunks: iterations each chunk has finished
+  int flowChunks = 0, flowDeps = 0;
   mlx::DevBuf d_flowsync;          // uint32 64: err | ... | barrier (15) | XCD group counters (16..32)
   int32_t *h_flowerr = nullptr;    // pinned copy of the error word
   mlx::DevBuf d_elist;             // colorOff i64 | colorCnt i64 | count i32 | elastic slot lists i32

@@ -250,6 +250,7 @@ struct TetIn {
   int v[4];
   T D[9];
   T vol, tad, tao, tas, g;
+  T m0, m1, m2, m3;  // the material scalars as stored (the solves scale them)
   T lam[6];
   vec4_t<T> O[4];
 };
@@ -281,10 +282,15 @@ __device__ __forceinline__ void tet_fetch_rest(const SweepParams<T> &P, const in
   for (int k = 0; k < 9; ++k) in.D[k] = P.dminv[k * ld + s];
   in.vol = P.vol[s];
   const T m0 = P.mat[s], m2 = P.mat[2 * ld + s];
+  const T m1 = P.mat[ld + s], m3 = P.mat[3 * ld + s];
+  in.m0 = m0;
+  in.m1 = m1;
+  in.m2 = m2;
+  in.m3 = m3;
   in.tad = m0 * P.ih2;
-  in.tao = P.mat[ld + s] * P.ih2;
+  in.tao = m1 * P.ih2;
   in.tas = mat_third<T, MAT>(m0, m2, P.ih2);
-  in.g = P.mat[3 * ld + s] * P.ih;
+  in.g = m3 * P.ih;
 #pragma unroll
   for (int i = 0; i < 6; ++i) in.lam[i] = P.lam[i * ld + s];
 }
@@ -365,8 +371,10 @@ __device__ __forceinline__ void tet_solve_nh_st(const SweepParams<T> &P, const i
     G[0][c] = -(G[1][c] + G[2][c] + G[3][c]);
   }
   const T w[4] = {X[0].w, X[1].w, X[2].w, X[3].w};
+  // the scaled material scalars are formed here, next to their uses, so
+  // every kernel that runs this solve rounds them the same way
   const T iv = T(1) / in.vol;
-  const T aD = iv * (-in.tad), aH = iv * in.tao, gam = in.tas;
+  const T aD = iv * (-(in.m0 * P.ih2)), aH = iv * (in.m1 * P.ih2), gam = in.m2;
   // vertex gradients: gD_a = F g_a / n, gH_a = cof(F) g_a; formed twice (for
   // the denominators, then for the displacements) rather than kept live
   auto grads = [&](int a, T (&gD)[3], T (&gH)[3]) {
@@ -413,7 +421,7 @@ __device__ __forceinline__ void tet_solve_st(const SweepParams<T> &P, const int6
     tet_solve_nh_st<T>(P, s, in, X, st);
     return;
   } else if constexpr (MAT == kMatMixed) {
-    if (in.tad < T(0)) {
+    if (in.m0 < T(0)) {
       tet_solve_nh_st<T>(P, s, in, X, st);
       return;
     }
@@ -425,8 +433,10 @@ __device__ __forceinline__ void tet_solve_st(const SweepParams<T> &P, const int6
 #pragma unroll
   for (int k = 0; k < 9; ++k) D[k] = in.D[k];
   const T vol = in.vol;
-  const T tad = in.tad, tao = in.tao, tas = in.tas;
-  const T g = in.g;
+  // the scaled material scalars are formed here, next to their uses, so
+  // every kernel that runs this solve rounds (and fuses) them the same way
+  const T tad = in.m0 * P.ih2, tao = in.m1 * P.ih2, tas = in.m2 * P.ih2;
+  const T g = in.m3 * P.ih;
 
   // edge vectors (current and displacement since x_old)
   const T e[3][3] = {{X1.x - X0.x, X1.y - X0.y, X1.z - X0.z},
@@ -2881,26 +2891,27 @@ __global__ __launch_bounds__(kBlock) void k_persist(PersistParams<T> Q) {
 // dataflow color sweep (one iteration of a layer without rigid elements)
 // ---------------------------------------------------------------------------
 
-// Per-vertex order of the element updates.  Gauss-Seidel over graph colors
-// fixes, for every vertex, the order in which its elements update it:
-// ascending color (the elements of one color share no vertex).  A slot's
-// update of its vertex a is therefore the need[a]-th update of that vertex
-// in the iteration (need = its elements of lower color).  With a counter of
-// applied updates per vertex, a slot may run as soon as each of its four
-// counters has reached base * deg + need (base = the layer's iterations
-// already run, deg = the vertex's elements): it then reads exactly the
-// positions the color-by-color order gives it, without waiting for whole
-// colors.  Same arithmetic as sweep_slot, so the results are bit-identical
-// to per-color launches.
+// Gauss-Seidel over graph colors fixes, for every vertex, the order in
+// which its elements update it (ascending color; the elements of one color
+// share no vertex) and nothing else.  Here the elements of a color are
+// taken in chunks of 64 consecutive slots, one per wavefront, and a chunk
+// may run as soon as the chunks holding the last lower-color updater of
+// each of its vertices have run (its dependences, precomputed per mesh:
+// waiting for the last updater of a vertex is enough, since that one waited
+// for the vertex's earlier updaters).  Every vertex then sees its updates
+// in the color order and the element arithmetic is sweep_slot's, so the
+// results are bit-identical to per-color launches.  A chunk's flag holds
+// the number of the layer's iterations it has finished.
 template <typename T>
 struct FlowParams {
   SweepParams<T> P;              // P.ctl the layer's control block, P.flags null
   const int32_t *colorOff;       // first slot of each color
   const int32_t *colorCnt;       // elements of each color
   int numColors;
-  const int32_t *need;           // 4 ld: updates of the slot's vertex a before its own
-  const int32_t *deg;            // nV: element updates of a vertex per iteration
-  unsigned *vcnt;                // nV: updates applied in the layer (zero at its start)
+  const int32_t *chunkBase;      // numColors + 1: first chunk of each color
+  const int32_t *depOff;         // nChunks + 1
+  const int32_t *dep;            // dependences of each chunk (chunk ids)
+  unsigned *flags;               // nChunks: iterations finished (zero at the layer's start)
   unsigned base;                 // iterations of the layer run by earlier launches
   ContactArrays ca;
   const int32_t *ecIdx;          // elastic contacts of the layer
@@ -2916,55 +2927,34 @@ struct FlowParams {
   long spinLimit;                // polls of one wait before it fails
 };
 
-template <typename T, int MAT>
-__device__ __forceinline__ bool flow_slot(const FlowParams<T> &Q, const int64_t s) {
-  const SweepParams<T> &P = Q.P;
-  TetIn<T> in;
-  unsigned target[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) in.v[a] = P.vid[a * P.ld + s];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-    target[a] = Q.base * (unsigned)Q.deg[in.v[a]] + (unsigned)Q.need[a * P.ld + s];
-  // the slot's data does not change in the launch: fetched while waiting
-  tet_fetch_rest<T, MAT>(P, s, in);
+// Whole wavefront: wait until every dependence of chunk g has finished
+// this iteration (flag > base).
+template <typename T>
+__device__ __forceinline__ bool flow_wait(const FlowParams<T> &Q, int g, int lane) {
+  const int d0 = Q.depOff[g], nd = Q.depOff[g + 1] - d0;
   long spins = 0;
   for (;;) {
-    unsigned c[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) c[a] = ldc<true>(Q.vcnt + in.v[a]);
-    if ((c[0] >= target[0]) & (c[1] >= target[1]) & (c[2] >= target[2]) & (c[3] >= target[3]))
-      break;
+    int ready = 1;
+    for (int q = lane; q < nd; q += 64) ready &= ldc<true>(Q.flags + Q.dep[d0 + q]) > Q.base;
+    if (__all(ready)) return true;
     ++spins;
     if (spins > Q.spinLimit || ((spins & 255) == 0 && ldc<true>(Q.err) != 0)) {
-      __hip_atomic_store(Q.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) __hip_atomic_store(Q.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
   }
-  vec4_t<T> X[4];
-  tet_gather<T, true>(P, in, X);
-  tet_solve_x<T, MAT>(P, s, in, X);
-  // the positions reach the L2 before the counters say so; readers load
-  // them past the vector L1 (ldc<true>)
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0);
-  asm volatile("" ::: "memory");
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-    __hip_atomic_fetch_add(Q.vcnt + in.v[a], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
 }
 
 // One iteration's sweep of a layer without rigid elements on the
-// workgroups of one XCD (xcd_group; the others return): every thread takes
-// its slot of each color in color order and runs it when its vertices'
-// counters allow (flow_slot), with no barrier between colors; then, as
-// projectGS does after its colors, the elastic contacts (distinct
-// vertices) and the pins, each after a barrier of the group.  The residual
-// and the monitor follow in k_residual_tail.  Launched cooperatively, so
-// every workgroup is resident; every wait is bounded and a failed one sets
-// *err (the kernel then returns without finishing the sweep).
+// workgroups of one XCD (xcd_group; the others return): every wavefront
+// takes its chunk of each color in color order and runs it when its
+// dependences allow, with no barrier between colors; then, as projectGS
+// does after its colors, the elastic contacts (distinct vertices) and the
+// pins, each after a barrier of the group.  The residual and the monitor
+// follow in k_residual_tail.  Launched cooperatively, so every workgroup
+// is resident; every wait is bounded and a failed one sets *err (the
+// kernel then returns without finishing the sweep).
 template <typename T, int MAT>
 __global__ __launch_bounds__(kBlock) void k_flow(FlowParams<T> Q) {
   int G = 0;
@@ -2972,17 +2962,41 @@ __global__ __launch_bounds__(kBlock) void k_flow(FlowParams<T> Q) {
   if (w < 0) return;
   const Ctl *ctl = Q.P.ctl;
   if ((ctl->stopSolve | ctl->endLayer) != 0) return;  // every workgroup reads the same word
-  const int t = threadIdx.x;
-  const int64_t stride = (int64_t)G * kBlock, first = (int64_t)w * kBlock + t;
+  const int t = threadIdx.x, lane = t & 63;
+  const SweepParams<T> &P = Q.P;
+  const int wfirst = w * (kBlock / 64) + (t >> 6), wstride = G * (kBlock / 64);
   bool ok = true;
   for (int c = 0; c < Q.numColors && ok; ++c) {
     const int64_t off = Q.colorOff[c], end = off + Q.colorCnt[c];
-    for (int64_t s = off + first; s < end && ok; s += stride) ok = flow_slot<T, MAT>(Q, s);
+    const int nch = (int)((Q.colorCnt[c] + 63) / 64);
+    for (int j = wfirst; j < nch && ok; j += wstride) {
+      const int g = Q.chunkBase[c] + j;
+      const int64_t s = off + 64 * (int64_t)j + lane;
+      const bool mine = s < end;
+      // the slot's data does not change in the launch: fetched before the wait
+      TetIn<T> in;
+      if (mine) tet_fetch<T, MAT>(P, s, in);
+      ok = flow_wait(Q, g, lane);
+      if (!ok) break;
+      if (mine) {
+        vec4_t<T> X[4];
+        tet_gather<T, true>(P, in, X);
+        tet_solve_x<T, MAT>(P, s, in, X);
+      }
+      // the chunk's positions reach the L2 before its flag says so;
+      // readers load them past the vector L1 (ldc<true>)
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_waitcnt(0);
+      asm volatile("" ::: "memory");
+      if (lane == 0)
+        __hip_atomic_store(Q.flags + g, Q.base + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   unsigned nbar = 0;
   xcd_arrive(Q.bar);
   if (!xcd_wait(Q.bar, ++nbar * (unsigned)G, Q.err)) return;
   if (ldc<true>(Q.err) != 0) return;
+  const int64_t stride = (int64_t)G * kBlock, first = (int64_t)w * kBlock + t;
   for (int64_t q = first; q < Q.nec; q += stride)
     contact_elastic_one<T, 3, true>(Q.ca, Q.ecIdx[q], Q.P.x, Q.minv, Q.resistance, Q.h);
   if (Q.nPin > 0) {

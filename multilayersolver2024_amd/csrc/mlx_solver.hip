@@ -538,45 +538,68 @@ int stage_upload(mlx_mesh &m, void *dst, const void *src, size_t bytes) {
   return MLX_OK;
 }
 
-// Per-slot update order of the dataflow sweep (k_flow), built once per
-// mesh: deg[v] = elements of vertex v; need[a ld + s] = elements of the
-// slot's vertex a with a lower color than the slot's (the updates that
-// vertex receives before this slot's in every iteration).
+// Chunk dependences of the dataflow sweep (k_flow), built once per mesh.
+// Chunk j of color c holds slots colorOff[c] + 64 j .. + 63 (one
+// wavefront); its dependences are the chunks holding, for each vertex of
+// its elements, the element of the largest color below the element's own
+// among the vertex's elements (the vertex's previous updater in the color
+// order), without repeats.
 static int flow_prepare(mlx_mesh &m) {
   if (m.flowReady) return MLX_OK;
-  const int64_t nV = m.nV, nT = m.nT, ld = m.ld;
-  const int vpe = m.vpe;
-  std::vector<int32_t> deg(nV, 0), off(nV + 1, 0);
+  const int64_t nV = m.nV, nT = m.nT;
+  const int vpe = m.vpe, nc = m.numColors;
+  std::vector<int32_t> chunkBase(nc + 1, 0);
+  for (int c = 0; c < nc; ++c)
+    chunkBase[c + 1] = chunkBase[c] + (int32_t)((m.colorCnt[c] + 63) / 64);
+  const int nChunks = chunkBase[nc];
+  auto chunk_of = [&](int32_t e) {
+    const int c = m.color[e] - 1;
+    const int64_t sl = m.slotOf[e];
+    return chunkBase[c] + (int32_t)((sl - m.colorOff[c]) / 64);
+  };
+  // elements of each vertex
+  std::vector<int32_t> off(nV + 1, 0);
   for (int64_t e = 0; e < nT; ++e)
-    for (int a = 0; a < vpe; ++a) deg[m.T[(size_t)e * vpe + a]]++;
-  for (int64_t v = 0; v < nV; ++v) off[v + 1] = off[v] + deg[v];
-  // colors of each vertex's elements
-  std::vector<int32_t> col(off[nV]), fill(off.begin(), off.end() - 1);
+    for (int a = 0; a < vpe; ++a) off[m.T[(size_t)e * vpe + a] + 1]++;
+  for (int64_t v = 0; v < nV; ++v) off[v + 1] += off[v];
+  std::vector<int32_t> el(off[nV]), fill(off.begin(), off.end() - 1);
   for (int64_t e = 0; e < nT; ++e)
+    for (int a = 0; a < vpe; ++a) el[fill[m.T[(size_t)e * vpe + a]]++] = (int32_t)e;
+  std::vector<std::vector<int32_t>> deps(nChunks);
+  for (int64_t e = 0; e < nT; ++e) {
+    const int32_t ce = m.color[e], g = chunk_of((int32_t)e);
     for (int a = 0; a < vpe; ++a) {
       const int32_t v = m.T[(size_t)e * vpe + a];
-      col[fill[v]++] = m.color[e];
-    }
-  std::vector<int32_t> need((size_t)vpe * ld, 0);
-  for (int64_t sl = 0; sl < ld; ++sl) {
-    const int32_t e = m.perm[sl];
-    if (e < 0) continue;
-    for (int a = 0; a < vpe; ++a) {
-      const int32_t v = m.T[(size_t)e * vpe + a];
-      int32_t n = 0;
-      for (int32_t q = off[v]; q < off[v + 1]; ++q) n += col[q] < m.color[e];
-      need[(size_t)a * ld + sl] = n;
+      int32_t best = -1, bestColor = 0;
+      for (int32_t q = off[v]; q < off[v + 1]; ++q) {
+        const int32_t f = el[q], cf = m.color[f];
+        if (cf < ce && cf > bestColor) {
+          bestColor = cf;
+          best = f;
+        }
+      }
+      if (best >= 0) deps[g].push_back(chunk_of(best));
     }
   }
-  MLX_TRY(m.d_need.reserve(sizeof(int32_t) * need.size()));
-  MLX_TRY(m.d_deg.reserve(sizeof(int32_t) * (size_t)nV));
-  MLX_TRY(m.d_vcnt.reserve(sizeof(uint32_t) * (size_t)nV));
+  std::vector<int32_t> buf;
+  buf.reserve((size_t)nc + 2 + nChunks + 8 * (size_t)nChunks);
+  buf.insert(buf.end(), chunkBase.begin(), chunkBase.end());
+  const size_t o_depOff = buf.size();
+  buf.resize(o_depOff + nChunks + 1, 0);
+  for (int g = 0; g < nChunks; ++g) {
+    std::sort(deps[g].begin(), deps[g].end());
+    deps[g].erase(std::unique(deps[g].begin(), deps[g].end()), deps[g].end());
+    buf[o_depOff + g + 1] = buf[o_depOff + g] + (int32_t)deps[g].size();
+  }
+  for (int g = 0; g < nChunks; ++g) buf.insert(buf.end(), deps[g].begin(), deps[g].end());
+  MLX_TRY(m.d_flowdeps.reserve(sizeof(int32_t) * buf.size()));
+  MLX_TRY(m.d_flowflags.reserve(sizeof(uint32_t) * (size_t)std::max(nChunks, 1)));
   MLX_TRY(m.d_flowsync.reserve(sizeof(uint32_t) * 64));
   if (!m.h_flowerr) MLX_CHECK_HIP(hipHostMalloc((void **)&m.h_flowerr, sizeof(int32_t)));
-  MLX_CHECK_HIP(hipMemcpy(m.d_need.ptr, need.data(), sizeof(int32_t) * need.size(),
+  MLX_CHECK_HIP(hipMemcpy(m.d_flowdeps.ptr, buf.data(), sizeof(int32_t) * buf.size(),
                           hipMemcpyHostToDevice));
-  MLX_CHECK_HIP(hipMemcpy(m.d_deg.ptr, deg.data(), sizeof(int32_t) * (size_t)nV,
-                          hipMemcpyHostToDevice));
+  m.flowChunks = nChunks;
+  m.flowDeps = (int)(buf.size() - o_depOff - nChunks - 1);
   m.flowReady = true;
   return MLX_OK;
 }
@@ -968,13 +991,13 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     hc_ctl.iters = 0;  // k_layer_begin cleared the device count
     const bool sweepWg = wgLayer[col] != 0;
     // dataflow sweep (k_flow) for a layer without rigid elements: the
-    // per-vertex counters and the error word start at zero
+    // chunk flags and the error word start at zero
     const bool flow = flowG > 0 && !hasRigidEl && L.nb == 0 && !sweepWg;
     unsigned flowBase = 0;
     unsigned *flowSync = m.d_flowsync.as<unsigned>();
     if (flow) {
-      hipLaunchKernelGGL(k_fill<unsigned>, dim3(blocks_for(nV)), dim3(kBlock), 0, st, nV,
-                         m.d_vcnt.as<unsigned>(), 0u);
+      hipLaunchKernelGGL(k_fill<unsigned>, dim3(blocks_for(m.flowChunks)), dim3(kBlock), 0, st,
+                         (int64_t)m.flowChunks, m.d_flowflags.as<unsigned>(), 0u);
       hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(kBlock), 0, st, (int64_t)64, flowSync, 0u);
     }
     if (sweepWg) {
@@ -994,9 +1017,10 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
       Q.colorOff = m.d_colors.as<int32_t>();
       Q.colorCnt = m.d_colors.as<int32_t>() + m.numColors;
       Q.numColors = m.numColors;
-      Q.need = m.d_need.as<int32_t>();
-      Q.deg = m.d_deg.as<int32_t>();
-      Q.vcnt = m.d_vcnt.as<unsigned>();
+      Q.chunkBase = m.d_flowdeps.as<int32_t>();
+      Q.depOff = Q.chunkBase + m.numColors + 1;
+      Q.dep = Q.depOff + m.flowChunks + 1;
+      Q.flags = m.d_flowflags.as<unsigned>();
       Q.base = flowBase++;
       Q.ca = ca;
       Q.ecIdx = LL.ec_idx;
@@ -1784,8 +1808,8 @@ int mlx_mesh_destroy(mlx_mesh *m) {
                          &m->d_bodycnt, &m->d_seam, &m->d_elastic, &m->learn.d_lam0,
                          &m->learn.d_keyScale, &m->learn.d_save, &m->learn.d_dx,
                          &m->learn.d_inc, &m->learn.d_res, &m->learn.d_counts,
-                         &m->learn.d_feat, &m->d_finder, &m->d_need, &m->d_deg,
-                         &m->d_vcnt, &m->d_flowsync};
+                         &m->learn.d_feat, &m->d_finder, &m->d_flowdeps, &m->d_flowflags,
+                         &m->d_flowsync};
   for (auto *b : bufs) b->release();
   for (auto e : m->prof.ev) (void)hipEventDestroy(e);
   for (auto e : m->prof.pev)

@@ -1,7 +1,7 @@
 """The dataflow sweep (MLX_OPT_FLOW, k_flow): in a layer without rigid
-elements each element of an iteration runs as soon as its own vertices have
-received their updates from the elements of lower color, with no barrier
-between colors.  Every vertex then sees its updates in the order of the
+elements each chunk of 64 slots of a color runs as soon as the chunks that
+last updated its vertices (in lower colors) have, with no barrier between
+colors.  Every vertex then sees its updates in the order of the
 per-color launches, the element arithmetic is the same routine, and the
 contacts, pins, residual and monitor are the same, so whole substeps are
 bit-identical to per-color launches: positions, velocities, lambdas,
@@ -87,11 +87,9 @@ def test_flow_sweep_plain_method_equals_per_color_launches():
 def test_flow_sweep_mixed_materials_match_per_color_launches():
     """Every other element NeoHookean (the mixed-material slot routine), a
     single all-elastic layer, FP64, at h = 1/6000 where the NeoHookean pair
-    converges.  The same updates in the same order; the two kernels may
-    round the NeoHookean terms differently in the last bit (at h = 1/60,
-    where the pair does not converge, such differences grow to O(1) within
-    tens of iterations in any two builds), so states are compared to 1e-12
-    and iteration counts exactly."""
+    converges (at h = 1/60 it does not, and any last-bit difference between
+    two builds grows to O(1) within tens of iterations).  Iteration counts
+    exactly, states to 1e-12; the printout says when they are bit-identical."""
     sc = scenes.beam3d(12, 3, 3, seed=4, z0=0.004)
     sc["neohookean"] = (np.arange(len(sc["t"])) % 2) == 1
     m = Mesh(sc)
