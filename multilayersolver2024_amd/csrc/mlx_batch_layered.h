@@ -53,59 +53,6 @@ __device__ __forceinline__ void wg_sum_n(double (&v)[N], double *shw) {
   __syncthreads();
 }
 
-// Positions of a scene's vertices: in HBM (the vec4 array) or in LDS (x, y,
-// z of the scene's nv vertices as structure of arrays, from its first
-// vertex vb; the inverse mass, constant, from x_old).  The kernel body is
-// written once against load / store / store_comp.
-template <typename T>
-struct HbmPos {
-  static constexpr bool kLds = false;
-  vec4_t<T> *x;
-  __device__ __forceinline__ vec4_t<T> load(int64_t v) const { return x[v]; }
-  // with the inverse mass the caller already holds (ignored here)
-  __device__ __forceinline__ vec4_t<T> load_w(int64_t v, T) const { return x[v]; }
-  __device__ __forceinline__ void store(int64_t v, const vec4_t<T> &p) const { x[v] = p; }
-  __device__ __forceinline__ void store_comp(int64_t v, int c, T val) const {
-    reinterpret_cast<T *>(&x[v])[c] = val;
-  }
-};
-
-template <typename T>
-struct LdsPos {
-  static constexpr bool kLds = true;
-  T *pos;
-  int64_t vb;
-  int nv;
-  const vec4_t<T> *xold;
-  __device__ __forceinline__ vec4_t<T> load(int64_t v) const {
-    const int l = (int)(v - vb);
-    vec4_t<T> p;
-    p.x = pos[l];
-    p.y = pos[nv + l];
-    p.z = pos[2 * nv + l];
-    p.w = xold[v].w;
-    return p;
-  }
-  __device__ __forceinline__ vec4_t<T> load_w(int64_t v, T w) const {
-    const int l = (int)(v - vb);
-    vec4_t<T> p;
-    p.x = pos[l];
-    p.y = pos[nv + l];
-    p.z = pos[2 * nv + l];
-    p.w = w;
-    return p;
-  }
-  __device__ __forceinline__ void store(int64_t v, const vec4_t<T> &p) const {
-    const int l = (int)(v - vb);
-    pos[l] = p.x;
-    pos[nv + l] = p.y;
-    pos[2 * nv + l] = p.z;
-  }
-  __device__ __forceinline__ void store_comp(int64_t v, int c, T val) const {
-    pos[c * nv + (int)(v - vb)] = val;
-  }
-};
-
 // The kernel's static LDS, declared once at kernel scope (function-scope
 // __shared__ variables would be repeated for every instantiation of the
 // scene body), so the dynamic LDS left for positions is as large as it can
@@ -114,9 +61,31 @@ template <int NT>
 struct LayeredShared {
   double shw[7 * (NT / 64)];
   double sR[kMaxSceneBodies][9], sC[kMaxSceneBodies][3];
+  Ctl c;  // the scene's stop rule, updated by thread 0 only
   double sRunUntil;
   int sStop, sStopSolve;
 };
+
+// Thread 0's part of wg_restrict: the body's mass, pose, inertia and
+// angular velocity from the workgroup's sums.  Not inlined: its pivoted
+// 3x3 solve in double would otherwise set the register budget of the whole
+// scene kernel, whose hot loops then spill.
+__device__ __noinline__ void restrict_finish(Body &B, double M, const double *com,
+                                             const double *cdot, double *b) {
+  B.M = M;
+  B.invM = 1.0 / M;
+  for (int d = 0; d < 3; ++d) {
+    B.COM[d] = com[d];
+    B.COMDOT[d] = cdot[d];
+  }
+  for (int i = 0; i < 9; ++i) {
+    B.I0[i] = b[i];
+    B.I[i] = b[i];
+    B.R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    B.Rstep[i] = B.R[i];
+  }
+  solve3_pivot(b, b + 9, B.omega);
+}
 
 // makeRigidsFromConnectivity / matComputeRigidBodyProperties3D
 // (AdaptiveMesh3D.m:32-43, RigidBody3D.m:198-265) for body k of the scene,
@@ -176,22 +145,7 @@ __device__ __forceinline__ void wg_restrict(const LayeredParams<T> &Lp, int k, c
 #pragma unroll
     for (int i = 0; i < 6; ++i) b[i] = lo[i], b[6 + i] = hi[i];
   }
-  if (threadIdx.x == 0) {
-    Body &B = Lp.bodies[k];
-    B.M = M;
-    B.invM = 1.0 / M;
-    for (int d = 0; d < 3; ++d) {
-      B.COM[d] = com[d];
-      B.COMDOT[d] = cdot[d];
-    }
-    for (int i = 0; i < 9; ++i) {
-      B.I0[i] = b[i];
-      B.I[i] = b[i];
-      B.R[i] = (i % 4 == 0) ? 1.0 : 0.0;
-      B.Rstep[i] = B.R[i];
-    }
-    solve3_pivot(b, b + 9, B.omega);
-  }
+  if (threadIdx.x == 0) restrict_finish(Lp.bodies[k], M, com, cdot, b);
 }
 
 // rigidSymplecticImperativeStep (RigidBody3D.m:174-179) of one body.
@@ -307,20 +261,9 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
   const int b0 = Lp.body0[sc], nbS = Lp.body0[sc + 1] - b0;
   T *rv = Lp.rv;
   const double *mass = Lp.mass;
-  Ctl c;
-  c.stopSolve = 0;
-  c.endLayer = 0;
-  c.iters = 0;
-  c.errorChangeDecreased = 0;
-  c.residualCount = 1;
-  c.stopLayer = -1;
-  c.isLastLayer = 0;
-  c.layer = 0;
-  c.runUntil = B.runUntil0;
-  c.setPercentStop = 0;
-  c.normresv = 0;
-  c.lastResidual = 0;
+  Ctl &c = S.c;
   if (tid == 0) {
+    ctl_start(c, B.runUntil0, 0.0, 0);
     sStop = 0;
     sStopSolve = 0;
     sRunUntil = B.runUntil0;
@@ -365,12 +308,14 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
         nr[0] += r * r;
       }
     wg_sum_n<NT, 1>(nr, shw);
-    c.normresv = sqrt(nr[0]);
-    c.endLayer = 0;
-    c.iters = 0;
-    c.layer = layer;
-    c.isLastLayer = !coarse;
-    c.setPercentStop = coarse ? B.stopPct : 0.0;
+    if (tid == 0) {
+      c.normresv = sqrt(nr[0]);
+      c.endLayer = 0;
+      c.iters = 0;
+      c.layer = layer;
+      c.isLastLayer = !coarse;
+      c.setPercentStop = coarse ? B.stopPct : 0.0;
+    }
     for (int64_t i = k0 + tid; i < k1; i += NT) B.c.lambda[i] = 0;
     if (nb > 0)
       for (int q = Lp.bndOff[b0] + tid; q < Lp.bndOff[b0 + nb]; q += NT) Lp.lamB[q] = 0;
@@ -383,20 +328,7 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
     while (it < B.hardCap) {
       for (int k = 0; k < nc; ++k) {
         const int64_t a = B.colorOff[cb + k], e = a + B.colorCnt[cb + k];
-        for (int64_t s = a + tid; s < e; s += NT) {
-          if (P.flags && !P.flags[s]) continue;
-          if constexpr (!Pos::kLds) {
-            project_tet<T, MAT>(P, s);
-          } else {
-            TetIn<T> in;
-            tet_fetch<T, MAT>(P, s, in);
-            vec4_t<T> X[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) X[q] = x.load_w(in.v[q], in.O[q].w);
-            tet_solve_st<T, MAT>(P, s, in, X,
-                                 [&](int q, const vec4_t<T> &o) { x.store(in.v[q], o); });
-          }
-        }
+        scene_color<T, MAT, kScenePairs<T, NT>>(P, a + tid, e, NT, x);
         __syncthreads();
       }
       for (int k = 0; k < nb; ++k)
@@ -427,17 +359,7 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
       __syncthreads();
       ++it;
       if (B.mp.computeResiduals) {
-        double acc[1] = {0};
-        for (int64_t s = s0 + tid; s < s1; s += NT) {
-          if (B.P.vid[s] < 0) continue;
-          if constexpr (!Pos::kLds) {
-            acc[0] += residual_tet<T, false, MAT>(B.P, s);
-          } else {
-            ResIn<T> in;
-            res_fetch_pos<T, MAT>(B.P, s, in, [&](int v) { return x.load(v); });
-            acc[0] += res_compute<T, MAT>(in);
-          }
-        }
+        double acc[1] = {scene_residual<T, MAT, kScenePairs<T, NT>>(B.P, s0 + tid, s1, NT, x)};
         const double Ks = (double)(k1 - k0);
         for (int64_t i = k0 + tid; i < k1; i += NT) {
           const int v = B.c.vert[i];

@@ -376,11 +376,15 @@ __device__ __forceinline__ void tet_solve_nh_st(const SweepParams<T> &P, const i
   const T iv = T(1) / in.vol;
   const T aD = iv * (-(in.m0 * P.ih2)), aH = iv * (in.m1 * P.ih2), gam = in.m2;
   // vertex gradients: gD_a = F g_a / n, gH_a = cof(F) g_a; formed twice (for
-  // the denominators, then for the displacements) rather than kept live
+  // the denominators, then for the displacements) rather than kept live.
+  // One reciprocal of n serves the 24 components: a correctly rounded
+  // division costs about ten instructions, and the scenes that run this
+  // solve to the iteration cap are bound by the CU's vector ALU.
+  const T rn = T(1) / n;
   auto grads = [&](int a, T (&gD)[3], T (&gH)[3]) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      gD[r] = (F[r][0] * G[a][0] + F[r][1] * G[a][1] + F[r][2] * G[a][2]) / n;
+      gD[r] = (F[r][0] * G[a][0] + F[r][1] * G[a][1] + F[r][2] * G[a][2]) * rn;
       gH[r] = PH[r][0] * G[a][0] + PH[r][1] * G[a][1] + PH[r][2] * G[a][2];
     }
   };
@@ -2257,6 +2261,162 @@ struct BatchParams {
   int32_t *residualCount;
   const int32_t *sceneList;      // scene of each workgroup (null: workgroup i runs scene i)
 };
+
+// Positions of a scene's vertices: in HBM (the vec4 array) or in LDS (x, y,
+// z of the scene's nv vertices as structure of arrays, from its first
+// vertex vb; the inverse mass, constant, from x_old).  The batched scene
+// bodies are written once against load / load_w / store / store_comp.
+template <typename T>
+struct HbmPos {
+  static constexpr bool kLds = false;
+  vec4_t<T> *x;
+  __device__ __forceinline__ vec4_t<T> load(int64_t v) const { return x[v]; }
+  // with the inverse mass the caller already holds (ignored here)
+  __device__ __forceinline__ vec4_t<T> load_w(int64_t v, T) const { return x[v]; }
+  __device__ __forceinline__ void store(int64_t v, const vec4_t<T> &p) const { x[v] = p; }
+  __device__ __forceinline__ void store_comp(int64_t v, int c, T val) const {
+    reinterpret_cast<T *>(&x[v])[c] = val;
+  }
+};
+
+template <typename T>
+struct LdsPos {
+  static constexpr bool kLds = true;
+  T *pos;
+  int64_t vb;
+  int nv;
+  const vec4_t<T> *xold;
+  __device__ __forceinline__ vec4_t<T> load(int64_t v) const {
+    const int l = (int)(v - vb);
+    vec4_t<T> p;
+    p.x = pos[l];
+    p.y = pos[nv + l];
+    p.z = pos[2 * nv + l];
+    p.w = xold[v].w;
+    return p;
+  }
+  __device__ __forceinline__ vec4_t<T> load_w(int64_t v, T w) const {
+    const int l = (int)(v - vb);
+    vec4_t<T> p;
+    p.x = pos[l];
+    p.y = pos[nv + l];
+    p.z = pos[2 * nv + l];
+    p.w = w;
+    return p;
+  }
+  __device__ __forceinline__ void store(int64_t v, const vec4_t<T> &p) const {
+    const int l = (int)(v - vb);
+    pos[l] = p.x;
+    pos[nv + l] = p.y;
+    pos[2 * nv + l] = p.z;
+  }
+  __device__ __forceinline__ void store_comp(int64_t v, int c, T val) const {
+    pos[c * nv + (int)(v - vb)] = val;
+  }
+};
+
+// Tets one thread of a scene's workgroup has in flight together: a
+// workgroup runs one scene on one CU, so with a single tet per thread its
+// waves wait out the memory latency of every pass.  In FP32 up to 512
+// threads (256 registers) the register file holds two tets' inputs per
+// thread; at 768 and 1,024 threads (168 and 128 registers), and in FP64,
+// one.
+template <typename T, int NT>
+constexpr bool kScenePairs = sizeof(T) == 4 && NT <= 512;
+
+// One tet of a scene: slot data, then the current positions.
+template <typename T, int MAT, typename Pos>
+__device__ __forceinline__ void scene_fetch(const SweepParams<T> &P, const int64_t s,
+                                            const Pos &x, TetIn<T> &in, vec4_t<T> (&X)[4]) {
+  tet_fetch<T, MAT>(P, s, in);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) X[q] = x.load_w(in.v[q], in.O[q].w);
+}
+
+template <typename T, int MAT, typename Pos>
+__device__ __forceinline__ void scene_solve(const SweepParams<T> &P, const int64_t s,
+                                            const Pos &x, const TetIn<T> &in,
+                                            const vec4_t<T> (&X)[4]) {
+  tet_solve_st<T, MAT>(P, s, in, X, [&](int q, const vec4_t<T> &o) { x.store(in.v[q], o); });
+}
+
+// One thread's slots s, s + step, ... < e of one color of a scene.  PAIRS:
+// two at a time, both tets' slot data and current positions loaded before
+// either is projected.  The tets of a color share no vertex, so no other
+// projection of the color writes these positions and the two projections
+// touch disjoint vertices: the same updates as one tet at a time.  With
+// P.flags set, slots whose flag is clear are skipped.
+template <typename T, int MAT, bool PAIRS, typename Pos>
+__device__ __forceinline__ void scene_color(const SweepParams<T> &P, int64_t s, const int64_t e,
+                                            const int64_t step, const Pos &x) {
+  if constexpr (!PAIRS) {
+    for (; s < e; s += step) {
+      if (P.flags && !P.flags[s]) continue;
+      TetIn<T> in;
+      vec4_t<T> X[4];
+      scene_fetch<T, MAT>(P, s, x, in, X);
+      scene_solve<T, MAT>(P, s, x, in, X);
+    }
+  } else {
+    for (; s < e; s += 2 * step) {
+      const int64_t s1 = s + step;
+      const bool oa = !P.flags || P.flags[s], ob = s1 < e && (!P.flags || P.flags[s1]);
+      TetIn<T> a, b;
+      vec4_t<T> Xa[4], Xb[4];
+      if (oa) scene_fetch<T, MAT>(P, s, x, a, Xa);
+      if (ob) scene_fetch<T, MAT>(P, s1, x, b, Xb);
+      if (oa) scene_solve<T, MAT>(P, s, x, a, Xa);
+      if (ob) scene_solve<T, MAT>(P, s1, x, b, Xb);
+    }
+  }
+}
+
+// Sum of the residual terms of one thread's real slots s, s + step, ... < e
+// of a scene, in slot order; PAIRS: two slots' inputs loaded before their
+// terms are computed.
+template <typename T, int MAT, bool PAIRS, typename Pos>
+__device__ __forceinline__ double scene_residual(const SweepParams<T> &P, int64_t s,
+                                                 const int64_t e, const int64_t step,
+                                                 const Pos &x) {
+  const auto ld = [&](int v) { return x.load(v); };
+  double acc = 0;
+  if constexpr (!PAIRS) {
+    for (; s < e; s += step)
+      if (P.vid[s] >= 0) {
+        ResIn<T> in;
+        res_fetch_pos<T, MAT>(P, s, in, ld);
+        acc += res_compute<T, MAT>(in);
+      }
+  } else {
+    for (; s < e; s += 2 * step) {
+      const int64_t s1 = s + step;
+      const bool oa = P.vid[s] >= 0, ob = s1 < e && P.vid[s1] >= 0;
+      ResIn<T> a, b;
+      if (oa) res_fetch_pos<T, MAT>(P, s, a, ld);
+      if (ob) res_fetch_pos<T, MAT>(P, s1, b, ld);
+      if (oa) acc += res_compute<T, MAT>(a);
+      if (ob) acc += res_compute<T, MAT>(b);
+    }
+  }
+  return acc;
+}
+
+// Control block of a scene's stop rule, in LDS: only thread 0 updates it
+// (monitor_update), so it occupies no registers of the other threads.
+__device__ __forceinline__ void ctl_start(Ctl &c, double runUntil, double pct, int isLast) {
+  c.stopSolve = 0;
+  c.endLayer = 0;
+  c.iters = 0;
+  c.errorChangeDecreased = 0;
+  c.residualCount = 1;
+  c.stopLayer = -1;
+  c.isLastLayer = isLast;
+  c.layer = 0;
+  c.runUntil = runUntil;
+  c.setPercentStop = pct;
+  c.normresv = 0;
+  c.lastResidual = 0;
+}
 
 // k_batch_plain for a scene whose positions fit in LDS: x, y, z of its nv
 // vertices live in LDS (structure of arrays, pos[d * nv + local vertex])
