@@ -503,7 +503,8 @@ def main():
                          "per color (0)")
     ap.add_argument("--flow", type=int, default=0, choices=[0, 1],
                     help="layers without rigid elements sweep their colors in the dataflow kernel "
-                         "(k_flow: per-vertex update counters instead of one launch per color)")
+                         "(k_flow: 64-slot chunks wait for the chunks that last updated their "
+                         "vertices instead of one launch per color)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
                     help="timed batched substeps replay a captured graph (1) or launch directly (0, for kernel-trace profiling)")
