@@ -1765,11 +1765,11 @@ __device__ __forceinline__ double residual_tet(const SweepParams<T> &P, const in
 }
 
 // Sum of residual_tet over the real slots s, s + step, ... < e of one
-// thread, four slots' inputs loaded before their terms are computed.
-template <typename T, bool COH>
+// thread, U slots' inputs loaded before their terms are computed.
+constexpr int kResPerThread = 2;  // slots per thread of k_residual_tail (a pair)
+template <typename T, bool COH, int MAT = 0, int U = 4>
 __device__ __forceinline__ double residual_slots(const SweepParams<T> &P, int64_t s,
                                                  const int64_t e, const int64_t step) {
-  constexpr int U = 4;
   double acc = 0;
   for (; s < e; s += U * step) {
     ResIn<T> in[U];
@@ -1778,12 +1778,27 @@ __device__ __forceinline__ double residual_slots(const SweepParams<T> &P, int64_
     for (int u = 0; u < U; ++u) {
       const int64_t su = s + u * step;
       ok[u] = su < e && P.vid[su] >= 0;
-      if (ok[u]) res_fetch<T, COH>(P, su, in[u]);
+      if (ok[u]) res_fetch<T, COH, MAT>(P, su, in[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (ok[u]) acc += res_compute<T>(in[u]);
+      if (ok[u]) acc += res_compute<T, MAT>(in[u]);
   }
+  return acc;
+}
+
+// Residual terms of slots s and s1 (> s) of one thread, both slots' inputs
+// loaded before either term is computed; summed in slot order.
+template <typename T, int MAT>
+__device__ __forceinline__ double residual_pair(const SweepParams<T> &P, const int64_t s,
+                                                const int64_t s1, const int64_t e) {
+  const bool oa = s < e && P.vid[s] >= 0, ob = s1 < e && P.vid[s1] >= 0;
+  ResIn<T> a, b;
+  if (oa) res_fetch<T, false, MAT>(P, s, a);
+  if (ob) res_fetch<T, false, MAT>(P, s1, b);
+  double acc = 0;
+  if (oa) acc += res_compute<T, MAT>(a);
+  if (ob) acc += res_compute<T, MAT>(b);
   return acc;
 }
 
@@ -1902,7 +1917,7 @@ __device__ __forceinline__ void monitor_update(Ctl &c, const MonitorParams &mp, 
 // The residual (simpleEvaluateAllConstraintResiduals, xpbdLayer3D.m:546-586;
 // 2D xpbdLayer2D.m:477-518), the monitor and the iteration count of one
 // projectGS iteration in one launch: blocks [0, nE) hold the element terms
-// of their slots (one thread per slot), the
+// of their slots (kResPerThread slots per thread), the
 // rest the contact terms; each block stores its partial past the caches
 // (device-scope store), waits for it, and counts itself on *done.  The
 // block that counts last reads every partial the same way (thread t takes
@@ -1924,8 +1939,16 @@ __global__ __launch_bounds__(kBlock) void k_residual_tail(
   const int stopWord = ctl->stopSolve | ctl->endLayer;
   double acc = 0;
   if ((int)blockIdx.x < nE) {
-    const int64_t s = (int64_t)blockIdx.x * kBlock + t;
-    if (s < nSlots && P.vid[s] >= 0) acc = residual_elem<T, DIM>(P, s, lamSlot2d);
+    // slots s, s + step (kResPerThread of them), in that order
+    const int64_t s = (int64_t)blockIdx.x * kBlock + t, step = (int64_t)nE * kBlock;
+    if constexpr (DIM == 3) {
+      static_assert(kResPerThread == 2, "k_residual_tail takes slot pairs");
+      acc = P.nh ? residual_pair<T, kMatMixed>(P, s, s + step, nSlots)
+                 : residual_pair<T, kMatStVK>(P, s, s + step, nSlots);
+    } else {
+      for (int64_t q = s; q < nSlots; q += step)
+        if (P.vid[q] >= 0) acc += residual_elem<T, DIM>(P, q, lamSlot2d);
+    }
   } else {
     const int i = ((int)blockIdx.x - nE) * kBlock + t;
     if (i < c.counts[0])

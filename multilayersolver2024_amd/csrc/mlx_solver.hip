@@ -818,6 +818,9 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
     set_error("scene too large for the residual partial buffer");
     return MLX_ERR_UNSUPPORTED;
   }
+  // k_residual_tail: kResPerThread slots per thread
+  const int64_t resSpan = (int64_t)kBlock * kResPerThread;
+  const int nTailElemBlocks = (int)((ld + resSpan - 1) / resSpan);
 
   const int64_t nPin = (int64_t)m.pinnedDOF.size();
   // persistent loop: color table on the device, barrier counter, grid size
@@ -1110,15 +1113,15 @@ static int layer_solve_dev(mlx_mesh &m, const HostMaps &maps, const SolveParams 
                              m.d_pinned.as<int32_t>(), m.d_pinnedx.as<T>(), x, ctl);
       }
       if (sp.computeResiduals) {
-        const int nBlk = nResElemBlocks + (K > 0 ? nResContactBlocks : 0);
+        const int nBlk = nTailElemBlocks + (K > 0 ? nResContactBlocks : 0);
         if (m.dim == 3)
           hipLaunchKernelGGL((k_residual_tail<T, 3>), dim3(nBlk), dim3(kBlock), 0, st, P, ld,
-                             (const int32_t *)nullptr, nResElemBlocks, ca, sp.contactResistance,
+                             (const int32_t *)nullptr, nTailElemBlocks, ca, sp.contactResistance,
                              sp.h, d_uniq, d_cnt, partials, m.d_count.as<unsigned>(), ctl, mp,
                              m.d_resid.as<double>(), residCap);
         else
           hipLaunchKernelGGL((k_residual_tail<T, 2>), dim3(nBlk), dim3(kBlock), 0, st, P, ld,
-                             (const int32_t *)d_lamSlot2d, nResElemBlocks, ca,
+                             (const int32_t *)d_lamSlot2d, nTailElemBlocks, ca,
                              sp.contactResistance, sp.h, d_uniq, d_cnt, partials,
                              m.d_count.as<unsigned>(), ctl, mp, m.d_resid.as<double>(),
                              residCap);
