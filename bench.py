@@ -73,13 +73,19 @@ TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 113.488 / 1024, "fetch_scale": 2.0,
                                     "per": "thread slot"},
                "k_batch_plain<float,1024>": {"fetch": 15441071.3, "fetch_scale": 2.0,
                                              "write": 3376463.8, "write_scale": 1.0,
-                                             "per": "dispatch"}}
+                                             "per": "dispatch"},
+               # the three bucket dispatches (512 threads each) of one substep
+               "k_batch_layered<float,512>": {"fetch": 92100579.1, "fetch_scale": 2.0,
+                                              "write": 14835501.3, "write_scale": 1.0,
+                                              "per": "substep"}}
 # Where the constants above come from.
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this bench (--graphs 0; "
-                  "the sweep from the plain leg over 12,872 dispatches, the batched kernel from "
-                  "the plain batched leg with --batch-legs plain --batch-splits 0, 6 dispatches), "
-                  "scaled by the same counters of a 1 GiB device copy (FETCH x2.0, WRITE x1.0); "
-                  "commands and figures in profiles/hbm_traffic_*.md")
+                  "the sweep from the plain leg over 12,872 dispatches, the plain batched kernel "
+                  "from the plain batched leg with --batch-legs plain --batch-splits 0, 6 "
+                  "dispatches, the cluster kernel from --batch-legs clusters --batch-graph 0, the "
+                  "4 timed substeps' 12 bucket dispatches), scaled by the same counters of a 1 GiB "
+                  "device copy (FETCH x2.0, WRITE x1.0); commands and figures in "
+                  "profiles/hbm_traffic_*.md")
 
 
 def traffic_bytes(kernel, units=1.0):
@@ -337,7 +343,11 @@ def bench_batched_clusters(h, plane, steps, warmup, precision, nscenes=256, grap
         "finite": bool(np.isfinite(p_end).all() and np.isfinite(v_end).all()),
         "roofline": {"bound": "hbm", "kernel": f"k_batch_layered<{ftype},*> (per bucket)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (traffic_bytes("k_batch_layered<float,512>")
+                                 if precision == "fp32" and all(t[0] == 512 for t in tuned)
+                                 else None),
+                     "traffic_per": "substep (the buckets' dispatches)",
                      "bytes_per_unit": f"{bpt} per projection + {rbytes} per residual evaluation",
                      "avg_launch_ms": kms / steps,
                      "measured_on": "the timed substeps rerun from the same state, one HIP event "
