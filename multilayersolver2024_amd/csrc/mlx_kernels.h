@@ -2288,7 +2288,8 @@ struct BatchParams {
 // Positions of a scene's vertices: in HBM (the vec4 array) or in LDS (x, y,
 // z of the scene's nv vertices as structure of arrays, from its first
 // vertex vb; the inverse mass, constant, from x_old).  The batched scene
-// bodies are written once against load / load_w / store / store_comp.
+// body with rigid clusters (batch_scene_layered) is written once against
+// load / load_w / store / store_comp.
 template <typename T>
 struct HbmPos {
   static constexpr bool kLds = false;
