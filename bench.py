@@ -261,7 +261,8 @@ def oracle_from_snapshot(O, mesh, h, plane, snap, integ):
     return osim
 
 
-def bench_batched_clusters(h, plane, steps, warmup, precision, nscenes=256, graph=True):
+def bench_batched_clusters(h, plane, steps, warmup, precision, nscenes=256, graph=True,
+                           model_path=None):
     """The batched workload as specified: nscenes beams (tet counts
     log-uniform in [2000, 60000], random aspect ratios, per-scene E
     log-uniform in [1e4, 1e7]), odd scenes NeoHookean, each with its own
@@ -321,9 +322,49 @@ def bench_batched_clusters(h, plane, steps, warmup, precision, nscenes=256, grap
         kproj += pr
         kms += k
         kres += int(np.dot(b.iters, nts))
-    b.close()
     its = np.array(its)
     nh = np.array([m.neohookean.any() for m in meshes])
+    # the learned form on the same workload: the committed network attached
+    # (mlx_batch_set_learned), the same window from the same state (one
+    # untimed substep captures the graph and seeds the multipliers the
+    # network reads), each scene's warm start judged by its equation residual
+    learned = None
+    if model_path and os.path.exists(model_path):
+        from multilayersolver2024_amd import learn
+        net = learn.Mlp.load(model_path)
+        b.set_learned(net, 2.0)
+        b.set_state(p0, v0)
+        b.frame = frame0
+        b.step()
+        b.learn_stats()
+        L.check("mlx_synchronize", L.lib().mlx_synchronize())
+        t0 = time.perf_counter()
+        lproj, lits = 0, []
+        for _ in range(steps):
+            pr, _ = b.step()
+            lproj += pr
+            lits.append(b.iters.copy())
+        L.check("mlx_synchronize", L.lib().mlx_synchronize())
+        lwall = time.perf_counter() - t0
+        kept, rej = b.learn_stats()
+        pl, vl, _ = b.download()
+        lits = np.array(lits)
+        b.set_learned(None)
+        net.close()
+        learned = {
+            "value": lproj / lwall, "unit": "tet-proj/s", "ms_per_step": 1e3 * lwall / steps,
+            "steps": steps,
+            "mean_iterations_stvk_scene_substep": float(lits[:, ~nh].mean()),
+            "mean_iterations_neohookean_scene_substep": float(lits[:, nh].mean()),
+            "warm_starts_kept": kept, "warm_starts_rejected": rej,
+            "finite": bool(np.isfinite(pl).all() and np.isfinite(vl).all()),
+            "model": os.path.relpath(model_path, ROOT),
+            "what": "the batched workload with the learned warm start of every scene's "
+                    "all-elastic layer (mlx_batch_set_learned), judged per scene; same window "
+                    "length from the post-warmup state, graph replays"}
+        log(f"learned batched: {learned['ms_per_step']:.1f} ms/substep, kept {kept}, "
+            f"rejected {rej}")
+    b.close()
     bpt = BYTES_PER_TET[precision]
     rbytes = RESIDUAL_BYTES_PER_TET[precision]
     achieved = (kproj * bpt + kres * rbytes) / (kms / 1e3) / 1e9
@@ -355,6 +396,7 @@ def bench_batched_clusters(h, plane, steps, warmup, precision, nscenes=256, grap
                                     "substep's small kernels)"},
         "note": "the NeoHookean pair does not converge at h = 1/60 (its scenes run to "
                 "hangingStop every substep); the StVK-Voigt scenes stop by the relative rule",
+        "learned": learned,
     }
 
 
@@ -748,10 +790,13 @@ def main():
     line.update(extra)
     if not args.no_batched and args.batch_legs in ("both", "clusters"):
         log("batched leg (rigid clusters, NeoHookean half)")
-        line["batched"] = bench_batched_clusters(h, plane, args.batch_steps, args.batch_warmup,
-                                                 args.precision, graph=bool(args.batch_graph))
+        line["batched"] = bench_batched_clusters(
+            h, plane, args.batch_steps, args.batch_warmup, args.precision,
+            graph=bool(args.batch_graph),
+            model_path=None if args.no_extra_legs else LEARNED_MODEL)
         log(f"batched: {line['batched']['ms_per_step']:.1f} ms/substep, "
             f"{line['batched']['value']:.3e} tet-proj/s")
+        line["learned_batched"] = line["batched"].pop("learned")
     if not args.no_batched and args.batch_legs in ("both", "plain"):
         log("batched leg (plain method, StVK-Voigt)")
         splits = tuple(int(v) for v in args.batch_splits.split(",") if v.strip())

@@ -652,6 +652,18 @@ int mlx_learn_outputs(mlx_mesh *mesh, double *lambda0, double *keyScale);
  * mesh with a network attached. */
 int mlx_learn_features(mlx_state *state, double h, float *features);
 
+/* The learned warm start on a batch with rigid clusters (the batched
+ * workload; no counterpart in the reference): the network runs on every
+ * slot at the start of each substep, and each scene's all-elastic layer
+ * starts from its lambda_0 and displacement as mlx_mesh_set_learned's
+ * MLX_LEARN_WARMSTART does for a single scene, judged per scene by the
+ * same equation residual (kept, or positions and multipliers of that scene
+ * restored exactly).  256 or 512 threads per scene.  Null detaches. */
+int mlx_batch_set_learned(mlx_batch *batch, mlx_mlp *model, float kappaMax);
+/* Warm starts kept and rejected over all scenes since the last call
+ * (then cleared). */
+int mlx_batch_learn_stats(mlx_batch *batch, int64_t *kept, int64_t *rejected);
+
 /* hipDeviceSynchronize on the library's device. */
 int mlx_synchronize(void);
 

@@ -38,6 +38,7 @@ EXPORTED = (
     "mlx_mlp_grad", "mlx_mlp_train", "mlx_mesh_set_learned", "mlx_learn_stats",
     "mlx_learn_outputs", "mlx_learn_features", "mlx_state_set", "mlx_state_get_oldp",
     "mlx_batch_set_rigid", "mlx_batch_autotune", "mlx_integrate_finders", "mlx_find_contacts",
+    "mlx_batch_set_learned", "mlx_batch_learn_stats",
 )
 MLX_LEARN_WARMSTART = 1
 MLX_LEARN_KEYS = 2
@@ -215,6 +216,8 @@ def lib():
         "mlx_state_set": [_p, _d, _d, _d, _d],
         "mlx_state_get_oldp": [_p, _d],
         "mlx_batch_set_rigid": [_p, _u8],
+        "mlx_batch_set_learned": [_p, _p, C.c_float],
+        "mlx_batch_learn_stats": [_p, _i64, _i64],
         "mlx_batch_autotune": [_p, C.POINTER(Settings), C.c_double, C.c_int32, C.POINTER(Plane),
                                _i32, _f, _i32],
         "mlx_integrate_finders": [_p, C.POINTER(Settings), C.c_double, C.c_int32,

@@ -126,6 +126,20 @@ class BatchSimulation:
         L.check("mlx_batch_set_rigid", L.lib().mlx_batch_set_rigid(
             self.handle, L.ptr(r, C.c_uint8) if r is not None else None))
 
+    def set_learned(self, mlp, kappa_max=2.0):
+        """Attach a network (learn.Mlp) for the learned warm start of every
+        scene's all-elastic layer (rigid clusters needed); None detaches.
+        The network must outlive the attachment."""
+        L.check("mlx_batch_set_learned", L.lib().mlx_batch_set_learned(
+            self.handle, mlp.handle if mlp is not None else None, float(kappa_max)))
+
+    def learn_stats(self):
+        """(warm starts kept, rejected) over all scenes since the last call."""
+        k, r = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        L.check("mlx_batch_learn_stats", L.lib().mlx_batch_learn_stats(
+            self.handle, L.ptr(k, C.c_int64), L.ptr(r, C.c_int64)))
+        return int(k[0]), int(r[0])
+
     def step(self):
         self.frame += 1
         out = L.BatchOut()

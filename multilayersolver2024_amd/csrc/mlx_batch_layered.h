@@ -11,6 +11,21 @@ namespace mlx {
 
 constexpr int kMaxSceneBodies = 8;
 
+// The learned warm start of every scene's all-elastic layer
+// (mlx_batch_set_learned): lambda_0 per slot from the network at the start
+// of the substep, and the scratch the scene's workgroup uses to apply and
+// judge it.
+template <typename T>
+struct BatchLearn {
+  const T *lam0;                // nc ld: lambda_0 (k_learn_infer)
+  T *dx;                        // 12 ld: displacement of (vertex a, slot s)
+  vec4_t<T> *xsave;             // nV: positions before the warm start
+  T *lamsave;                   // nc ld: multipliers before the warm start
+  const int32_t *incOff, *inc;  // incidences a ld + s of each vertex, ascending slot
+  const uint8_t *pinMask;       // 3 nV
+  unsigned long long *counts;   // warm starts kept, rejected
+};
+
 // The coarse layer's maps, fixed for the batch (mlx_batch_set_rigid).
 template <typename T>
 struct LayeredParams {
@@ -27,6 +42,7 @@ struct LayeredParams {
   int32_t *coarseIters;            // per scene: iterations the coarse layer ran
   int it0, it1;                    // iterations of the coarse and the elastic layer
   double boundaryCompliance;
+  BatchLearn<T> learn;             // used by the LEARN kernels only
 };
 
 // Workgroup sums of N doubles per thread in a fixed order (butterfly in each
@@ -247,7 +263,96 @@ __device__ __forceinline__ void wg_rigid_vertices(const LayeredParams<T> &Lp, in
 // contacts are off in this path), pins, residual and monitor.  Positions
 // through x (HbmPos or LdsPos); the same arithmetic in the same order
 // either way.
+// The learned warm start of a scene's all-elastic layer, in its workgroup
+// (the single-scene k_learn_* sequence on the scene's own slots, vertices
+// and contacts): the equation residual of the cold start, copies of the
+// positions and multipliers, lambda_0 and its displacement on every slot
+// whose multipliers are still zero (each vertex adds its incident
+// displacements in ascending slot order), the residual of the warm start,
+// and the copies restored when the warm start is worse.  Ends with a
+// workgroup barrier.
 template <typename T, int NT, int MAT, typename Pos>
+__device__ __forceinline__ void wg_warm_start(const BatchParams<T> &B, const LayeredParams<T> &Lp,
+                                              const int sc, const Pos &x, double *shw) {
+  const BatchLearn<T> &W = Lp.learn;
+  const SweepParams<T> &P = B.P;
+  const int tid = threadIdx.x;
+  const int64_t ld = P.ld;
+  const int64_t vb = B.vert0[sc], ve = B.vert0[sc + 1];
+  const int64_t s0 = B.slot0[sc], s1 = B.slot0[sc + 1];
+  const int64_t k0 = B.cont0[sc], k1 = B.cont0[sc + 1];
+  auto eq_residual = [&]() -> double {
+    double acc[1] = {0};
+    for (int64_t s = s0 + tid; s < s1; s += NT) {
+      if (P.vid[s] < 0) continue;
+      TetIn<T> in;
+      tet_fetch<T, MAT>(P, s, in);
+      vec4_t<T> X[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) X[q] = x.load_w(in.v[q], in.O[q].w);
+      acc[0] += xpbd_eq_residual_x<T>(in, X);
+    }
+    for (int64_t i = k0 + tid; i < k1; i += NT) {
+      const vec4_t<T> p = x.load(B.c.vert[i]);
+      const double *n = B.c.normal + 3 * i, *pc = B.c.point + 3 * i;
+      double phi = n[0] * ((double)p.x - pc[0]) + n[1] * ((double)p.y - pc[1]) +
+                   n[2] * ((double)p.z - pc[2]);
+      phi = phi < 0 ? phi : 0;
+      acc[0] += (B.contactResistance * phi) * (B.contactResistance * phi);
+    }
+    wg_sum_n<NT, 1>(acc, shw);
+    return sqrt(acc[0]);
+  };
+  const double cold = eq_residual();
+  for (int64_t v = vb + tid; v < ve; v += NT) W.xsave[v] = x.load(v);
+  for (int64_t s = s0 + tid; s < s1; s += NT)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) W.lamsave[i * ld + s] = P.lam[i * ld + s];
+  for (int64_t s = s0 + tid; s < s1; s += NT) {
+    bool fresh = P.vid[s] >= 0;
+    if (fresh)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) fresh = fresh && P.lam[i * ld + s] == T(0);
+    if (!fresh) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) W.dx[k * ld + s] = T(0);
+      continue;
+    }
+    vec4_t<T> X[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) X[q] = x.load(P.vid[q * ld + s]);
+    learn_disp_slot<T>(P, s, W.lam0, W.dx, X);
+  }
+  __syncthreads();
+  for (int64_t v = vb + tid; v < ve; v += NT) {
+    if (W.pinMask[3 * v]) continue;
+    T d[3] = {T(0), T(0), T(0)};
+    for (int q = W.incOff[v]; q < W.incOff[v + 1]; ++q) {
+      const int32_t e = W.inc[q];
+      const int64_t a = e / ld, s = e % ld;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) d[c] += W.dx[(3 * a + c) * ld + s];
+    }
+    vec4_t<T> p = x.load(v);
+    p.x += d[0];
+    p.y += d[1];
+    p.z += d[2];
+    x.store(v, p);
+  }
+  __syncthreads();
+  const double warm = eq_residual();
+  const bool keep = warm <= cold;  // false for a NaN warm residual
+  if (tid == 0) atomicAdd(W.counts + (keep ? 0 : 1), 1ull);
+  if (!keep) {
+    for (int64_t v = vb + tid; v < ve; v += NT) x.store(v, W.xsave[v]);
+    for (int64_t s = s0 + tid; s < s1; s += NT)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) P.lam[i * ld + s] = W.lamsave[i * ld + s];
+  }
+  __syncthreads();
+}
+
+template <typename T, int NT, int MAT, typename Pos, bool LEARN>
 __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
                                                     const LayeredParams<T> &Lp, const int sc,
                                                     const Pos &x, LayeredShared<NT> &S) {
@@ -321,6 +426,8 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
       for (int q = Lp.bndOff[b0] + tid; q < Lp.bndOff[b0 + nb]; q += NT) Lp.lamB[q] = 0;
     if (tid == 0) sStop = 0;
     __syncthreads();
+    if constexpr (LEARN)
+      if (!coarse) wg_warm_start<T, NT, MAT>(B, Lp, sc, x, shw);
     SweepParams<T> P = B.P;
     P.flags = nb > 0 ? Lp.flags0 : nullptr;
     const int iters = coarse ? Lp.it0 : Lp.it1;
@@ -404,14 +511,14 @@ __device__ __forceinline__ void batch_scene_layered(const BatchParams<T> &B,
   }
 }
 
-template <typename T, int NT, int MAT>
+template <typename T, int NT, int MAT, bool LEARN>
 __device__ __forceinline__ void batch_scene_layered_at(const BatchParams<T> &B,
                                                        const LayeredParams<T> &Lp, const int sc,
                                                        T *lds, LayeredShared<NT> &shw) {
   const int64_t vb = B.vert0[sc];
   const int nv = (int)(B.vert0[sc + 1] - vb);
   if (nv > B.ldsVerts) {
-    batch_scene_layered<T, NT, MAT>(B, Lp, sc, HbmPos<T>{B.P.x}, shw);
+    batch_scene_layered<T, NT, MAT, HbmPos<T>, LEARN>(B, Lp, sc, HbmPos<T>{B.P.x}, shw);
     return;
   }
   // the scene's positions in LDS from the start to the end of the substep
@@ -422,7 +529,8 @@ __device__ __forceinline__ void batch_scene_layered_at(const BatchParams<T> &B,
     lds[2 * nv + i] = p.z;
   }
   __syncthreads();
-  batch_scene_layered<T, NT, MAT>(B, Lp, sc, LdsPos<T>{lds, vb, nv, B.P.xold}, shw);
+  batch_scene_layered<T, NT, MAT, LdsPos<T>, LEARN>(B, Lp, sc, LdsPos<T>{lds, vb, nv, B.P.xold},
+                                                   shw);
   __syncthreads();
   for (int i = threadIdx.x; i < nv; i += NT) {
     vec4_t<T> p = B.P.x[vb + i];
@@ -434,17 +542,18 @@ __device__ __forceinline__ void batch_scene_layered_at(const BatchParams<T> &B,
 }
 
 // One scene per workgroup; a scene with at most B.ldsVerts vertices keeps
-// its positions in the dynamic LDS (3 values of T per vertex).
-template <typename T, int NT>
+// its positions in the dynamic LDS (3 values of T per vertex).  LEARN: the
+// all-elastic layer starts from the learned warm start (Lp.learn).
+template <typename T, int NT, bool LEARN = false>
 __global__ __launch_bounds__(NT) void k_batch_layered(BatchParams<T> B, LayeredParams<T> Lp) {
   extern __shared__ __align__(16) unsigned char mlx_dyn_lds[];
   __shared__ LayeredShared<NT> shw;
   const int sc = B.sceneList ? B.sceneList[blockIdx.x] : (int)blockIdx.x;
   T *lds = reinterpret_cast<T *>(mlx_dyn_lds);
   if (B.sceneNH && B.sceneNH[sc])
-    batch_scene_layered_at<T, NT, kMatNH>(B, Lp, sc, lds, shw);
+    batch_scene_layered_at<T, NT, kMatNH, LEARN>(B, Lp, sc, lds, shw);
   else
-    batch_scene_layered_at<T, NT, kMatStVK>(B, Lp, sc, lds, shw);
+    batch_scene_layered_at<T, NT, kMatStVK, LEARN>(B, Lp, sc, lds, shw);
 }
 
 }  // namespace mlx

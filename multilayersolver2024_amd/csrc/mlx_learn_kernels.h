@@ -154,12 +154,7 @@ __global__ void k_learn_keys(int64_t nT, const int32_t *slotOf, const double *ke
 // max(|C| - |alpha~ lambda|, 0), it grows when lambda overshoots, so it can
 // judge a warm start.
 template <typename T>
-__device__ __forceinline__ double xpbd_eq_residual(const SweepParams<T> &P, int64_t s) {
-  TetIn<T> in;
-  tet_fetch<T, kMatMixed>(P, s, in);
-  vec4_t<T> X[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) X[a] = P.x[in.v[a]];
+__device__ __forceinline__ double xpbd_eq_residual_x(const TetIn<T> &in, const vec4_t<T> (&X)[4]) {
   T D[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) D[k] = in.D[k];
@@ -212,6 +207,16 @@ __device__ __forceinline__ double xpbd_eq_residual(const SweepParams<T> &P, int6
   return acc;
 }
 
+template <typename T>
+__device__ __forceinline__ double xpbd_eq_residual(const SweepParams<T> &P, int64_t s) {
+  TetIn<T> in;
+  tet_fetch<T, kMatMixed>(P, s, in);
+  vec4_t<T> X[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) X[a] = P.x[in.v[a]];
+  return xpbd_eq_residual_x<T>(in, X);
+}
+
 // The warm-start judge in two launches: block partials of the squared
 // equation residuals of the elastic slots (P.flags, null = all) and of the
 // contacts (r phi)^2 (their multipliers are zero when a layer starts), then
@@ -254,31 +259,15 @@ __global__ __launch_bounds__(kBlock) void k_learn_res_final(const double *part, 
   if (threadIdx.x == 0) *out = sqrt(o[0]);
 }
 
-// Warm start of the elastic slots whose multipliers are still zero:
-// lambda takes lambda_0 and each vertex the matching displacement
-// (1/m_a) grad_a C(x)^T lambda_0 (XPBD keeps x = x~ + M^-1 grad C^T
-// lambda), written per (vertex a, slot) into dx (12 ld: component 3a + d of
-// slot s at (3a + d) ld + s).  Other slots write zeros.
+// The warm start of one fresh elastic slot with its vertices' current
+// positions X: lambda takes lambda_0, dx the displacement of each vertex.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_learn_disp(SweepParams<T> P, const T *lam0, T *dx) {
-  const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (s >= P.ld) return;
+__device__ __forceinline__ void learn_disp_slot(const SweepParams<T> &P, const int64_t s,
+                                                const T *lam0, T *dx, const vec4_t<T> (&X)[4]) {
   const int64_t ld = P.ld;
-  bool el = P.vid[s] >= 0 && (!P.flags || P.flags[s]);
-  if (el)
-#pragma unroll
-    for (int i = 0; i < 6; ++i) el = el && P.lam[i * ld + s] == T(0);
-  if (!el) {
-#pragma unroll
-    for (int k = 0; k < 12; ++k) dx[k * ld + s] = T(0);
-    return;
-  }
   T D[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) D[k] = P.dminv[k * ld + s];
-  vec4_t<T> X[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) X[a] = P.x[P.vid[a * ld + s]];
   T Fm[3][3];
   tet_F<T>(X, D, Fm);
   T G[4][3];
@@ -327,6 +316,31 @@ __global__ __launch_bounds__(kBlock) void k_learn_disp(SweepParams<T> P, const T
     for (int r = 0; r < 3; ++r)
       dx[(3 * a + r) * ld + s] =
           X[a].w * vol * (FS[r][0] * G[a][0] + FS[r][1] * G[a][1] + FS[r][2] * G[a][2]);
+}
+
+// Warm start of the elastic slots whose multipliers are still zero:
+// lambda takes lambda_0 and each vertex the matching displacement
+// (1/m_a) grad_a C(x)^T lambda_0 (XPBD keeps x = x~ + M^-1 grad C^T
+// lambda), written per (vertex a, slot) into dx (12 ld: component 3a + d of
+// slot s at (3a + d) ld + s).  Other slots write zeros.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_learn_disp(SweepParams<T> P, const T *lam0, T *dx) {
+  const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (s >= P.ld) return;
+  const int64_t ld = P.ld;
+  bool el = P.vid[s] >= 0 && (!P.flags || P.flags[s]);
+  if (el)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) el = el && P.lam[i * ld + s] == T(0);
+  if (!el) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) dx[k * ld + s] = T(0);
+    return;
+  }
+  vec4_t<T> X[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) X[a] = P.x[P.vid[a * ld + s]];
+  learn_disp_slot<T>(P, s, lam0, dx, X);
 }
 
 // Each unpinned vertex adds its incident (vertex a, slot s) displacements in

@@ -2816,11 +2816,9 @@ int mlx_mesh_set_option(mlx_mesh *m, int32_t option, int64_t value) {
   return MLX_ERR_ARG;
 }
 
-int mlx_mesh_set_learned(mlx_mesh *m, mlx_mlp *model, int32_t flags, float kappaMax) {
-  if (!m) {
-    set_error("mlx_mesh_set_learned: null mesh");
-    return MLX_ERR_ARG;
-  }
+// Attaches a network to a mesh (single scene or a batch's merged mesh):
+// the warm start's buffers and the vertex incidences.
+static int learn_attach(mlx_mesh *m, mlx_mlp *model, int32_t flags, float kappaMax) {
   mlx_mesh::Learn &L = m->learn;
   if (!model) {
     L.model = nullptr;
@@ -2828,8 +2826,8 @@ int mlx_mesh_set_learned(mlx_mesh *m, mlx_mlp *model, int32_t flags, float kappa
     L.ready = false;
     return MLX_OK;
   }
-  if (m->dim != 3 || m->nScenes != 1) {
-    set_error("mlx_mesh_set_learned: needs a 3D single-scene mesh");
+  if (m->dim != 3) {
+    set_error("mlx_mesh_set_learned: needs a 3D mesh");
     return MLX_ERR_UNSUPPORTED;
   }
   if ((flags & ~(MLX_LEARN_WARMSTART | MLX_LEARN_KEYS)) != 0 || !(kappaMax > 0) ||
@@ -2885,6 +2883,19 @@ int mlx_mesh_set_learned(mlx_mesh *m, mlx_mlp *model, int32_t flags, float kappa
   L.kappaMax = kappaMax;
   L.ready = false;
   return MLX_OK;
+}
+
+int mlx_mesh_set_learned(mlx_mesh *m, mlx_mlp *model, int32_t flags, float kappaMax) {
+  if (!m) {
+    set_error("mlx_mesh_set_learned: null mesh");
+    return MLX_ERR_ARG;
+  }
+  if (model && (m->dim != 3 || m->nScenes != 1)) {
+    set_error("mlx_mesh_set_learned: needs a 3D single-scene mesh (batches: "
+              "mlx_batch_set_learned)");
+    return MLX_ERR_UNSUPPORTED;
+  }
+  return learn_attach(m, model, flags, kappaMax);
 }
 
 int mlx_learn_stats(mlx_mesh *m, int64_t *kept, int64_t *rejected) {
@@ -3086,20 +3097,20 @@ static int batch_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
   return lds_verts_for(fns, sizeof(T), maxSceneVerts, enabled, bytes);
 }
 
-template <typename T, int NT>
+template <typename T, int NT, bool LEARN = false>
 static int layered_lds_verts(int64_t maxSceneVerts, int enabled, size_t *bytes) {
-  const void *const fns[1] = {reinterpret_cast<const void *>(&k_batch_layered<T, NT>)};
+  const void *const fns[1] = {reinterpret_cast<const void *>(&k_batch_layered<T, NT, LEARN>)};
   return lds_verts_for(fns, sizeof(T), maxSceneVerts, enabled, bytes);
 }
 
 // k_batch_layered over n scenes; scenes of at most maxSceneVerts vertices
 // that fit the LDS keep their positions there.
-template <typename T, int NT>
+template <typename T, int NT, bool LEARN = false>
 static void launch_layered(BatchParams<T> B, const LayeredParams<T> &Lp, int n,
                            int64_t maxSceneVerts, int useLds, hipStream_t s) {
   size_t lb = 0;
-  B.ldsVerts = layered_lds_verts<T, NT>(maxSceneVerts, useLds, &lb);
-  hipLaunchKernelGGL((k_batch_layered<T, NT>), dim3(n), dim3(NT), lb, s, B, Lp);
+  B.ldsVerts = layered_lds_verts<T, NT, LEARN>(maxSceneVerts, useLds, &lb);
+  hipLaunchKernelGGL((k_batch_layered<T, NT, LEARN>), dim3(n), dim3(NT), lb, s, B, Lp);
 }
 
 // First contact of each scene: the number of flagged vertices before the
@@ -3145,6 +3156,23 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
       set_error("a residual stop in the batched path needs computeResiduals and hangingStop >= 0");
       return MLX_ERR_UNSUPPORTED;
     }
+  }
+  // learned warm start of the all-elastic layer (mlx_batch_set_learned)
+  const bool learned = m.learn.model && (m.learn.flags & MLX_LEARN_WARMSTART);
+  if (learned) {
+    if (!b.layered) {
+      set_error("the batched learned warm start runs with rigid clusters (mlx_batch_set_rigid)");
+      return MLX_ERR_UNSUPPORTED;
+    }
+    if (!b.useBuckets && b.block != 256 && b.block != 512) {
+      set_error("the batched learned warm start runs at 256 or 512 threads per scene");
+      return MLX_ERR_UNSUPPORTED;
+    }
+    for (int i = 0; b.useBuckets && i < mlx_batch::kBuckets; ++i)
+      if (b.bucketBlock[i] != 256 && b.bucketBlock[i] != 512) {
+        set_error("the batched learned warm start runs at 256 or 512 threads per scene");
+        return MLX_ERR_UNSUPPORTED;
+      }
   }
   MLX_TRY(m.d_tmp.reserve(4096));
   int *flag = reinterpret_cast<int *>(m.d_tmp.as<char>() + 4000);
@@ -3204,6 +3232,8 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     hipLaunchKernelGGL((k_reset_oldp<T, 3>), dim3(blocks_for(std::max(nV, nPin))), dim3(kBlock), 0,
                        s, nV, flag, frame == 1, x, xo, nPin, m.d_pinned.as<int32_t>(),
                        m.d_pinnedx.as<T>());
+    // the network reads p, cache.oldp and the previous substep's multipliers
+    if (learned) MLX_TRY(learn_infer<T>(m, h, nullptr));
     if (planeOn) {
       hipLaunchKernelGGL((k_plane_flags<T, 3>), dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, x,
                          nrm[0], nrm[1], nrm[2], Dp, 0, fl);
@@ -3315,6 +3345,17 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     Lp.it0 = st->iterations[0];
     Lp.it1 = st->iterations[1];
     Lp.boundaryCompliance = st->boundaryCompliance;
+    if (learned) {
+      mlx_mesh::Learn &Ln = m.learn;
+      Lp.learn.lam0 = Ln.d_lam0.as<T>();
+      Lp.learn.dx = Ln.d_dx.as<T>();
+      Lp.learn.xsave = Ln.d_save.as<vec4_t<T>>();
+      Lp.learn.lamsave = reinterpret_cast<T *>(Lp.learn.xsave + nV);
+      Lp.learn.incOff = Ln.d_inc.as<int32_t>();
+      Lp.learn.inc = Lp.learn.incOff + nV + 1;
+      Lp.learn.pinMask = b.d_pinMask.as<uint8_t>();
+      Lp.learn.counts = Ln.d_counts.as<unsigned long long>();
+    }
   }
   // one bucket's launch: its scene list, block size and LDS share
   auto launch_bucket = [&](int i, hipStream_t bs) {
@@ -3323,8 +3364,13 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
     const int n = b.bucketOff[i + 1] - b.bucketOff[i];
     const int blk = b.bucketBlock[i];
     if (b.layered) {
-      if (blk == 256) launch_layered<T, 256>(Bi, Lp, n, b.bucketMaxVerts[i], b.useLds, bs);
-      else launch_layered<T, 512>(Bi, Lp, n, b.bucketMaxVerts[i], b.useLds, bs);
+      if (learned) {
+        if (blk == 256) launch_layered<T, 256, true>(Bi, Lp, n, b.bucketMaxVerts[i], b.useLds, bs);
+        else launch_layered<T, 512, true>(Bi, Lp, n, b.bucketMaxVerts[i], b.useLds, bs);
+      } else {
+        if (blk == 256) launch_layered<T, 256>(Bi, Lp, n, b.bucketMaxVerts[i], b.useLds, bs);
+        else launch_layered<T, 512>(Bi, Lp, n, b.bucketMaxVerts[i], b.useLds, bs);
+      }
       return;
     }
     size_t lb = 0;
@@ -3350,6 +3396,11 @@ static int batch_integrate_impl(mlx_batch &b, mlx_settings *st, double h, int fr
       return;
     }
     if (b.layered) {
+      if (learned) {  // 256 or 512 threads (checked above)
+        if (b.block == 256) launch_layered<T, 256, true>(B, Lp, ns, maxSceneVerts, b.useLds, s);
+        else launch_layered<T, 512, true>(B, Lp, ns, maxSceneVerts, b.useLds, s);
+        return;
+      }
       if (b.block == 256) launch_layered<T, 256>(B, Lp, ns, maxSceneVerts, b.useLds, s);
       else if (b.block == 768) launch_layered<T, 768>(B, Lp, ns, maxSceneVerts, b.useLds, s);
       else if (b.block == 1024) launch_layered<T, 1024>(B, Lp, ns, maxSceneVerts, b.useLds, s);
@@ -3719,6 +3770,29 @@ int mlx_batch_set_split(mlx_batch *b, int32_t tetsPerWorkgroup) {
   }
   b->split = tetsPerWorkgroup;
   return MLX_OK;
+}
+
+int mlx_batch_set_learned(mlx_batch *b, mlx_mlp *model, float kappaMax) {
+  if (!b) {
+    set_error("mlx_batch_set_learned: null batch");
+    return MLX_ERR_ARG;
+  }
+  MLX_CHECK_HIP(hipSetDevice(b->m->device));
+  // the captured substep changes
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  b->gsig.clear();
+  return learn_attach(b->m, model, MLX_LEARN_WARMSTART, kappaMax);
+}
+
+int mlx_batch_learn_stats(mlx_batch *b, int64_t *kept, int64_t *rejected) {
+  if (!b) {
+    set_error("mlx_batch_learn_stats: null batch");
+    return MLX_ERR_ARG;
+  }
+  return mlx_learn_stats(b->m, kept, rejected);
 }
 
 int mlx_batch_set_rigid(mlx_batch *b, const uint8_t *isRigidElement) {
