@@ -699,6 +699,28 @@ def main():
                      "(mlx_mesh_set_learned, MLX_LEARN_WARMSTART | MLX_LEARN_KEYS)")
             log(f"learned: {wl['ms_per_step']:.3f} ms/substep, {wl['mean_iterations']:.2f} "
                 f"iterations, kept {kept} rejected {rej}")
+        # the beam with NeoHookean tetrahedra (per-element E and nu from
+        # the same seed), same layers and stop rule, a short window after
+        # the same warmup
+        scN = scenes.beam3d(args.nx, args.ny, args.nz, seed=0, neohookean=True)
+        simN = Simulation(Mesh(scN), make_integrator(hanging_stop=args.hanging_stop), h, plane,
+                          precision=prec)
+        set_options(simN, args)
+        for _ in range(args.warmup):
+            simN.step()
+        nN = min(args.steps, 5)
+        wN = timed_from(simN, simN.snapshot(), nN)
+        pN = simN.download()[0]
+        simN.close()
+        extra["neohookean_beam"] = dict(
+            value=wN["value"], unit="tet-proj/s", ms_per_step=wN["ms_per_step"], steps=nN,
+            mean_iterations_per_substep=wN["mean_iterations"],
+            iterations_per_layer=wN["per_layer"], finite=bool(np.isfinite(pN).all()),
+            what="the same beam, layers and stop rule with NeoHookean tetrahedra (the "
+                 "NeoHookean material of the 3D beam configuration), a short window after the "
+                 "same warmup")
+        log(f"neohookean beam: {wN['ms_per_step']:.1f} ms/substep, "
+            f"{wN['mean_iterations']:.1f} iterations")
     sim.close()
     wall = w["wall"]
     bpt = BYTES_PER_TET[args.precision]
