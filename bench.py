@@ -68,15 +68,15 @@ RESIDUAL_BYTES_PER_TET = {"fp32": 140, "fp64": 264}
 # the 12,872 sweep dispatches of the 81,920-tet beam run (64-thread
 # workgroups); a launch's traffic is that times its slots.  k_batch_plain:
 # per dispatch (one per substep of the 256 beams, 1,024 threads).
-TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 113.488 / 1024, "fetch_scale": 2.0,
+TRAFFIC_KIB = {"k_sweep3d<float>": {"fetch": 114.394 / 1024, "fetch_scale": 2.0,
                                     "write": 55.865 / 1024, "write_scale": 1.0,
                                     "per": "thread slot"},
-               "k_batch_plain<float,1024>": {"fetch": 15441071.3, "fetch_scale": 2.0,
-                                             "write": 3376463.8, "write_scale": 1.0,
+               "k_batch_plain<float,1024>": {"fetch": 15498759.3, "fetch_scale": 2.0,
+                                             "write": 3375125.8, "write_scale": 1.0,
                                              "per": "dispatch"},
                # the three bucket dispatches (512 threads each) of one substep
-               "k_batch_layered<float,512>": {"fetch": 92100579.1, "fetch_scale": 2.0,
-                                              "write": 14835501.3, "write_scale": 1.0,
+               "k_batch_layered<float,512>": {"fetch": 92102241.8, "fetch_scale": 2.0,
+                                              "write": 14835576.8, "write_scale": 1.0,
                                               "per": "substep"}}
 # Where the constants above come from.
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this bench (--graphs 0; "
@@ -86,14 +86,44 @@ TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this 
                   "4 timed substeps' 12 bucket dispatches), scaled by the same counters of a 1 GiB "
                   "device copy (FETCH x2.0, WRITE x1.0); commands and figures in "
                   "profiles/hbm_traffic_*.md")
+# Digest of the kernel sources (multilayersolver2024_amd/csrc) the counters
+# above were read on.  When the sources differ, the constants may no longer
+# describe the kernels that run: the bench then reports no traffic figure
+# and says why, instead of a stale one.
+TRAFFIC_SOURCES_DIGEST = "20578410e67ea1e8"
+CSRC = os.path.join(ROOT, "multilayersolver2024_amd", "csrc")
+
+
+def kernel_sources_digest():
+    """SHA-256 (first 16 hex digits) over the names and bytes of the kernel
+    sources, in name order."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(CSRC)):
+        if name.endswith((".h", ".hip", ".cpp")):
+            h.update(name.encode())
+            with open(os.path.join(CSRC, name), "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def traffic_current():
+    return kernel_sources_digest() == TRAFFIC_SOURCES_DIGEST
+
+
+TRAFFIC_STALE = ("null: the kernel sources differ from those the counters were read on "
+                 "(kernel_sources_digest() != TRAFFIC_SOURCES_DIGEST); rerun the FETCH_SIZE and "
+                 "WRITE_SIZE passes of profiles/hbm_traffic_*.md")
 
 
 def traffic_bytes(kernel, units=1.0):
     """HBM bytes of one launch of `kernel` covering `units` (thread slots
-    for the sweep, 1 for a batched dispatch)."""
+    for the sweep, 1 for a batched dispatch); None when the kernel has no
+    counter reading or the sources changed since it was taken."""
     tk = TRAFFIC_KIB.get(kernel)
-    return (1024 * units * (tk["fetch"] * tk["fetch_scale"] + tk["write"] * tk["write_scale"])
-            if tk else None)
+    if tk is None or not traffic_current():
+        return None
+    return 1024 * units * (tk["fetch"] * tk["fetch_scale"] + tk["write"] * tk["write_scale"])
 
 
 def log(*a):
@@ -220,6 +250,7 @@ def bench_batched(h, plane, steps, warmup, precision, nscenes=256, graph=True,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_bytes(f"k_batch_plain<{ftype},{best[0]}>")
                      if best[1] == 0 and nscenes == 256 else None,
+                     "traffic_source": None if traffic_current() else TRAFFIC_STALE,
                      "bytes_per_unit": bpt, "unit_of_work": "element-iteration (projection + "
                      "residual evaluation)", "avg_launch_ms": kms / ksteps,
                      "measured_on": "the timed substeps rerun from the same state with the "
@@ -389,6 +420,7 @@ def bench_batched_clusters(h, plane, steps, warmup, precision, nscenes=256, grap
                                  if precision == "fp32" and all(t[0] == 512 for t in tuned)
                                  else None),
                      "traffic_per": "substep (the buckets' dispatches)",
+                     "traffic_source": None if traffic_current() else TRAFFIC_STALE,
                      "bytes_per_unit": f"{bpt} per projection + {rbytes} per residual evaluation",
                      "avg_launch_ms": kms / steps,
                      "measured_on": "the timed substeps rerun from the same state, one HIP event "
@@ -806,7 +838,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "traffic_source": TRAFFIC_SOURCE if traffic is not None else None,
+            "traffic_source": (TRAFFIC_SOURCE if traffic is not None
+                               else None if traffic_current() else TRAFFIC_STALE),
         }, **roof),
     }
     line.update(extra)
