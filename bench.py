@@ -21,7 +21,9 @@ graphs it needs) and is then timed from the same state; the other legs
 restart from that state too: `monitor_excluded` (the window with each
 substep's iteration counts fixed to the stop rule's and the residual monitor
 off, the time the reference reports), `fp64` (the same window in FP64) with
-`fp32_error_vs_fp64`, and `learned` (the committed network attached).
+`fp32_error_vs_fp64`, and `learned` (the committed network attached);
+`neohookean_beam` runs the same beam with NeoHookean tetrahedra over a
+short window after the same warmup.
 The roofline entry is for the element-sweep kernel: over --profile-steps
 substeps that follow the timed ones, one HIP event pair (no system fence)
 brackets the back-to-back color launches of every iteration on the mesh's
